@@ -1,0 +1,188 @@
+"""FP64 restatement of tomocupy's chunk preprocessing on the fourierrec path.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Citations: ``src/tomocupy/processing/proc_functions.py`` (``P``),
+``src/tomocupy/processing/remove_stripe.py`` (``R``).
+
+Wavelet taps are PyWavelets 1.1.1 values recorded in
+``tests/golden/wavelets.json`` (PyWavelets is a third-party dependency of the
+reference, ``setup.py:17``; it is not importable in the test interpreter).
+
+Parity status: ``darkflat_correction`` and ``minus_log`` are closed forms
+checked on hand-made cases (clamp, NaN and inf rules); the DWT analysis and
+synthesis (``afb1d``/``sfb1d``) are pinned against PyWavelets ``dwt2``/``idwt2``
+golden vectors and perfect reconstruction; the damping is a closed form
+(``tests/test_oracle_preproc.py``).
+"""
+
+import json
+import os
+
+import numpy as np
+
+_TAPS = None
+
+
+def wavelet_taps(name):
+    """``(dec_lo, dec_hi, rec_lo, rec_hi)`` of a PyWavelets wavelet, from the fixture."""
+    global _TAPS
+    if _TAPS is None:
+        here = os.path.dirname(os.path.abspath(__file__))
+        with open(os.path.join(here, '..', 'tests', 'golden', 'wavelets.json')) as f:
+            _TAPS = json.load(f)['taps']
+    t = _TAPS[name]
+    return tuple(np.array(t[k], dtype=np.float64) for k in ('dec_lo', 'dec_hi', 'rec_lo', 'rec_hi'))
+
+
+def darkflat_correction(data, dark, flat, bright_ratio=1.0, flat_linear=False):
+    """``ProcFunctions.darkflat_correction`` (P:55-74).
+
+    ``data (nproj, nz, n)``, ``dark (ndark, nz, n)``, ``flat (nflat, nz, n)``.
+    ``flat0 = mean(flat)/bright_ratio * (1+1e-5) - mean(dark)``;
+    ``res = (data - dark0)/flat0``.  With ``flat_linear`` (P:62-66) the flat is
+    interpolated per projection between the means of the first half and of the
+    second half (skipping the middle frame), as in the reference.
+    """
+    data = np.asarray(data, dtype=np.float64)
+    dark0 = np.asarray(dark, dtype=np.float64)
+    flat0 = np.asarray(flat, dtype=np.float64) / bright_ratio
+    if flat_linear:
+        nproj = data.shape[0]
+        f0 = np.mean(flat0[:flat0.shape[0] // 2], axis=0)
+        f1 = np.mean(flat0[flat0.shape[0] // 2 + 1:], axis=0)
+        v = np.linspace(0, 1, nproj)[:, None, None]
+        flat0 = (1 - v) * f0 + v * f1
+    else:
+        flat0 = np.mean(flat0, axis=0)
+    dark0 = np.mean(dark0, axis=0)
+    flat0 = flat0 * np.float64(np.float32(1 + 1e-5))
+    flat0 = flat0 - dark0
+    with np.errstate(divide='ignore', invalid='ignore'):
+        return (data - dark0) / flat0
+
+
+def minus_log(data):
+    """``ProcFunctions.minus_log`` (P:76-83): ``x<=0 -> 1``; ``-log``;
+    NaN -> 6, +inf -> 0, -inf -> 0.  Returns a new array."""
+    data = np.where(data <= 0, 1.0, data)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        data = -np.log(data)
+    return np.nan_to_num(data, nan=6.0, posinf=0.0, neginf=0.0)
+
+
+def dwt_coeff_len(N, L):
+    """PyWavelets ``dwt_coeff_len(N, L, 'symmetric')`` = ``floor((N+L-1)/2)``."""
+    return (N + L - 1) // 2
+
+
+def _reflect_index(idx, l):
+    """``_reflect(x, -0.5, l-0.5)`` on integer indices (R:51-68): half-sample
+    symmetric extension."""
+    minx, maxx = -0.5, l - 0.5
+    rng = maxx - minx
+    rng2 = 2 * rng
+    mod = np.fmod(idx - minx, rng2)
+    mod = np.where(mod < 0, mod + rng2, mod)
+    out = np.where(mod >= rng, rng2 - mod, mod) + minx
+    return out.astype(np.int64)
+
+
+def afb1d(x, h0, h1, axis):
+    """Analysis filter bank (R:91-129): symmetric pad, stride-2 correlation with
+    the reversed decomposition taps ``h0``/``h1``.  Returns ``(lo, hi)``."""
+    x = np.moveaxis(np.asarray(x, dtype=np.float64), axis, -1)
+    N = x.shape[-1]
+    L = len(h0)
+    outsize = dwt_coeff_len(N, L)
+    p = 2 * (outsize - 1) - N + L
+    idx = _reflect_index(np.arange(-(p // 2), N + (p + 1) // 2), N)
+    xp = x[..., idx]
+    lo = np.zeros(x.shape[:-1] + (outsize,))
+    hi = np.zeros_like(lo)
+    for j in range(L):
+        sl = xp[..., j:j + 2 * outsize:2]
+        lo += h0[j] * sl
+        hi += h1[j] * sl
+    return np.moveaxis(lo, -1, axis), np.moveaxis(hi, -1, axis)
+
+
+def sfb1d(lo, hi, g0, g1, axis):
+    """Synthesis filter bank (R:132-156): upsample-by-2 scatter of
+    ``g0*lo + g1*hi`` and crop of ``L-2`` samples on each side."""
+    lo = np.moveaxis(np.asarray(lo, dtype=np.float64), axis, -1)
+    hi = np.moveaxis(np.asarray(hi, dtype=np.float64), axis, -1)
+    H = lo.shape[-1]
+    L = len(g0)
+    size = (H - 1) * 2 + L
+    out = np.zeros(lo.shape[:-1] + (size,))
+    for i in range(L):
+        out[..., i:i + 2 * H:2] += g0[i] * lo + g1[i] * hi
+    return np.moveaxis(out[..., L - 2:size - (L - 2)], -1, axis)
+
+
+def dwt2(x, wname):
+    """One level of ``DWTForward.apply`` (R:158-207) on ``(..., H, W)``: rows
+    (axis -1) first, then columns (axis -2).  Returns ``ll, (lh, hl, hh)`` where
+    the first letter is the row (W) band and the second the column (H) band;
+    ``hl`` is the band the damping acts on (``yh[:, :, 1]``)."""
+    dec_lo, dec_hi, _, _ = wavelet_taps(wname)
+    h0, h1 = dec_lo[::-1], dec_hi[::-1]
+    lo, hi = afb1d(x, h0, h1, -1)
+    ll, lh = afb1d(lo, h0, h1, -2)
+    hl, hh = afb1d(hi, h0, h1, -2)
+    return ll, (lh, hl, hh)
+
+
+def idwt2(ll, bands, wname):
+    """One level of ``DWTInverse.apply`` (R:209-251): columns, then rows."""
+    _, _, rec_lo, rec_hi = wavelet_taps(wname)
+    lh, hl, hh = bands
+    lo = sfb1d(ll, lh, rec_lo, rec_hi, -2)
+    hi = sfb1d(hl, hh, rec_lo, rec_hi, -2)
+    return sfb1d(lo, hi, rec_lo, rec_hi, -1)
+
+
+def fw_damping(my, sigma):
+    """Damping of the ``rfft`` of the stripe band along angles (R:276-280):
+    ``-expm1(-yhat^2/(2 sigma^2))`` with ``yhat = ifftshift((arange(-my,my,2)+1)/2)``
+    truncated to ``my//2+1`` (``k+0.5`` for even ``my``, ``k`` for odd)."""
+    yhat = np.fft.ifftshift((np.arange(-my, my, 2) + 1) / 2)[:my // 2 + 1]
+    return -np.expm1(-yhat ** 2 / (2 * sigma ** 2))
+
+
+def remove_stripe_fw(data, sigma, wname, level):
+    """``remove_stripe_fw`` (R:253-293) on ``data (nproj, nz, ni)``, FP64."""
+    data = np.asarray(data, dtype=np.float64)
+    nproj, nz, ni = data.shape
+    nproj_pad = nproj + nproj // 8
+    st = (nproj_pad - nproj) // 2
+    sli = np.zeros((nz, nproj_pad, ni))
+    sli[:, st:st + nproj] = data.swapaxes(0, 1)
+    cc = []
+    for k in range(level):
+        sli, (lh, hl, hh) = dwt2(sli, wname)
+        my = hl.shape[-2]
+        f = np.fft.rfft(hl, axis=-2)
+        f *= fw_damping(my, sigma)[:, None]
+        hl = np.fft.irfft(f, my, axis=-2)
+        cc.append((lh, hl, hh))
+    for k in range(level)[::-1]:
+        h, w = cc[k][0].shape[-2:]
+        sli = sli[..., :h, :w]
+        sli = idwt2(sli, cc[k], wname)
+    return sli[:, st:st + nproj, :ni].swapaxes(0, 1)
+
+
+def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym16',
+               fw_level=7, take_minus_log=True, bright_ratio=1.0, flat_linear=False):
+    """``proc_sino`` then ``proc_proj`` then the swap to sinogram order
+    (P:140-198, ``rec.py:172-174``): darkflat -> [fw] -> [-log] -> ``(nz, nproj, n)``."""
+    res = darkflat_correction(data, dark, flat, bright_ratio, flat_linear)
+    if stripe == 'fw':
+        res = remove_stripe_fw(res, fw_sigma, fw_filter, fw_level)
+    elif stripe != 'none':
+        raise ValueError(stripe)
+    if take_minus_log:
+        res = minus_log(res)
+    return np.ascontiguousarray(res.swapaxes(0, 1))
