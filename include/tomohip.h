@@ -1,0 +1,200 @@
+/*
+ * tomohip.h -- C ABI of libtomohip.so, the MI355X (gfx950) implementation of
+ * tomocupy's fourierrec filtered-backprojection path.
+ *
+ * Conventions
+ *   - Every entry point returns TOMO_OK (0) or a negative TOMO_E* status and
+ *     never throws.  tomo_last_error() returns a thread-local message for the
+ *     last failure on the calling thread.
+ *   - Arguments are validated before any HIP call, so invalid sizes, dtypes
+ *     and null pointers are reported without touching a GPU.
+ *   - Array arguments are device pointers (hipMalloc'ed or torch tensors),
+ *     row-major and contiguous, unless a name says otherwise.  `stream` is a
+ *     hipStream_t passed as void* (NULL = the legacy default stream).
+ *   - A plan owns its scratch buffers and tables; one call per plan may be in
+ *     flight at a time (calls on one stream are ordered).  Distinct plans may
+ *     be used from distinct threads.
+ *   - dtype: TOMO_F32 (float32 storage and arithmetic), TOMO_F16 / TOMO_BF16
+ *     (16-bit storage of the sinogram/spectra, FP32 arithmetic).
+ */
+#ifndef TOMOHIP_H
+#define TOMOHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TOMO_OK = 0,
+  TOMO_EINVAL = -1,     /* invalid argument (checked before any HIP call) */
+  TOMO_EHIP = -2,       /* a HIP runtime call or kernel launch failed */
+  TOMO_ENOMEM = -3,     /* device or host allocation failed */
+  TOMO_EUNSUPPORTED = -4 /* valid request this build does not implement */
+};
+
+enum { TOMO_F32 = 0, TOMO_F16 = 1, TOMO_BF16 = 2 };
+
+/* Thread-local message describing the last failure on this thread ("" if none). */
+const char *tomo_last_error(void);
+/* Library version string. */
+const char *tomo_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* USFFT backprojector                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct tomo_fr_plan tomo_fr_plan;
+
+/*
+ * Replaces cfunc_fourierrec::cfunc_fourierrec(nproj, nz/2, n, theta)
+ * (src/cuda/cfunc_fourierrec.cu:4-50) together with FourierRec.__init__
+ * (src/tomocupy/reconstruction/fourierrec.py:49-59).
+ *   n        detector width = image size; even, >= 20
+ *   nproj    number of angles, >= 1
+ *   nz       number of REAL slices per call (the reference's ncz), >= 1;
+ *            odd nz is accepted and paired with an implicit zero slice
+ *   theta    device float32 (nproj,) angles in radians; copied at creation
+ *            (the reference keeps a borrowed pointer)
+ *   dtype    TOMO_F32 (this build); TOMO_F16/TOMO_BF16 return TOMO_EUNSUPPORTED
+ *   stream   stream used for the creation-time uploads (synchronised)
+ */
+int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
+              const float *theta, int dtype, void *stream);
+
+/*
+ * Replaces FourierRec.backprojection(obj, data, stream)
+ * (fourierrec.py:61-68) and cfunc_fourierrec::backprojection
+ * (cfunc_fourierrec.cu:67-93).
+ *   obj   output, device float32 (nz, n, n)
+ *   data  input filtered sinograms, device float32 (nz, nproj, n); NOT modified
+ *         (the reference's Python layer protects it with a packed copy)
+ * The real-pair packing data[c] + i*data[c+nz/2] and the unpacking are done
+ * inside the kernels.  Deterministic: no floating-point atomics.
+ */
+int fr_backprojection(tomo_fr_plan *plan, float *obj, const float *data, void *stream);
+
+/*
+ * Adjoint of fr_backprojection (USFFT forward projection); no counterpart in
+ * the reference's tomography path, structured like cfunc_usfft2d::fwd
+ * (src/cuda/cfunc_usfft2d.cu:53-70).  For real arrays
+ * sum(fr_backprojection(d) * o) == sum(d * fr_forward(o)) to rounding.
+ *   data  output, device float32 (nz, nproj, n)
+ *   obj   input, device float32 (nz, n, n); NOT modified
+ */
+int fr_forward(tomo_fr_plan *plan, float *data, const float *obj, void *stream);
+
+/* Replaces cfunc_fourierrec::free (cfunc_fourierrec.cu:56-65). NULL is a no-op. */
+void fr_destroy(tomo_fr_plan *plan);
+
+/* Plan sizes (the SWIG read-only attributes n, nproj, nz, cfunc_fourierrec.i:11-14). */
+int fr_get_sizes(const tomo_fr_plan *plan, int64_t *n, int64_t *nproj, int64_t *nz);
+
+/* ------------------------------------------------------------------------ */
+/* FBP filter                                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct tomo_flt_plan tomo_flt_plan;
+
+/*
+ * Replaces cfunc_filter::cfunc_filter(nproj, nz, ne) (src/cuda/cfunc_filter.cu:4-34)
+ * and FBPFilter.__init__ (src/tomocupy/reconstruction/fbp_filter.py:47-52).
+ *   n      unpadded detector width; even, >= 20
+ *   ne     padded FFT length (4n in float32, backproj_functions.py:50-54); ne >= n, even
+ *   nproj, nz  rows per call
+ */
+int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t nproj, int64_t nz,
+               int dtype);
+
+/*
+ * Replaces cfunc_filter::filter(g, w, stream) (cfunc_filter.cu:49-61, mulw
+ * kernels_filter.cuh:4-18) as called by FBPFilter.filter (fbp_filter.py:54-58).
+ *   g  device float32 (nz, nproj, ne), filtered in place: C2R(R2C(g) * w[z]),
+ *      unnormalised (the 1/ne is folded into the filter by calc_filter)
+ *   w  device complex64 (nz, ne/2+1) as interleaved float pairs
+ * The imaginary part of w at the Nyquist bin is ignored, as a C2R does.
+ */
+int flt_filter(tomo_flt_plan *plan, float *g, const float *w, void *stream);
+
+/*
+ * Replaces BackprojFunctions.fbp_filter_center(data, sht)
+ * (src/tomocupy/reconstruction/backproj_functions.py:93-109) in one pass:
+ * edge-replicate padding to ne, per-slice centre phase ramp
+ * w[z,k] = wfilter[k] * exp(-2 pi i (-center + sht[z] + n/2) k/ne) computed in
+ * FP64, filtering and cropping, without the (nz, nproj, ne) temporary.
+ *   data     device float32 (nz, nproj, n), filtered in place
+ *   wfilter  device float32 (ne/2+1,) from flt_calc_filter
+ *   center   rotation centre (pixels)
+ *   sht      device float32 (nz,) per-slice extra shift, or NULL for zeros
+ */
+int flt_filter_center(tomo_flt_plan *plan, float *data, const float *wfilter,
+                      double center, const float *sht, void *stream);
+
+void flt_destroy(tomo_flt_plan *plan);
+
+/*
+ * Replaces FBPFilter.calc_filter(name) (fbp_filter.py:60-88) with _wint
+ * (fbp_filter.py:90-118), evaluated on the host in FP64.  Host function; needs
+ * no GPU.  name: none, ramp, shepp, cosine, cosine2, hamming, hann, parzen.
+ *   out  host float64 (ne/2+1,) (the reference casts to float32 afterwards)
+ */
+int flt_calc_filter(int64_t ne, const char *name, double *out);
+
+/* ------------------------------------------------------------------------ */
+/* Chunk preprocessing                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct tomo_pre_plan tomo_pre_plan;
+
+/*
+ * Chunk preprocessing plan for raw uint16 chunks in tomocupy's layout:
+ * data (nproj, nz, n), dark (ndark, nz, n), flat (nflat, nz, n)
+ * (rec.py:80-84).  Owns the dark/flat mean scratch.
+ */
+int pre_create(tomo_pre_plan **plan, int64_t nproj, int64_t nz, int64_t n,
+               int64_t ndark, int64_t nflat);
+
+/*
+ * ProcFunctions.proc_sino + proc_proj + the swap to sinogram order
+ * (src/tomocupy/processing/proc_functions.py:55-83, 140-198; rec.py:172-174)
+ * without stripe removal:
+ *   out[z, p, x] = -log((data[p,z,x] - dark0[z,x]) / flat0[z,x])   (minus_log != 0)
+ * with dark0 = mean(dark), flat0 = mean(flat)/bright_ratio*(1+1e-5) - dark0,
+ * and minus_log's clamp/NaN rules.  flat_linear != 0 interpolates the flat per
+ * projection as proc_functions.py:62-66.
+ *   out  device float32 (nz, nproj, n)
+ */
+int pre_process(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark,
+                const uint16_t *flat, float *out, double bright_ratio, int flat_linear,
+                int minus_log, void *stream);
+
+/*
+ * Dark/flat correction only, in tomocupy's projection layout:
+ *   res (nproj, nz, n) float32 = (data - dark0)/flat0  (proc_functions.py:55-74)
+ */
+int pre_darkflat(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark,
+                 const uint16_t *flat, float *res, double bright_ratio, int flat_linear,
+                 void *stream);
+
+/*
+ * minus_log (proc_functions.py:76-83) fused with the swap to sinogram order:
+ *   out[z,p,x] = -log(res[p,z,x]) with x<=0 -> 0, NaN -> 6, +-inf -> 0
+ * (minus_log != 0), or a plain transpose (minus_log == 0).
+ */
+int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, float *out, int minus_log,
+                       void *stream);
+
+/*
+ * Fourier-wavelet stripe removal, remove_stripe_fw
+ * (src/tomocupy/processing/remove_stripe.py:253-293), in place on
+ *   res (nproj, nz, n) float32.
+ * wname: haar, db5, sym5, sym16 (config.py:178-182); level >= 1; sigma > 0.
+ */
+int pre_remove_stripe_fw(tomo_pre_plan *plan, float *res, double sigma, const char *wname,
+                         int level, void *stream);
+
+void pre_destroy(tomo_pre_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMOHIP_H */

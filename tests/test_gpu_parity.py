@@ -1,0 +1,190 @@
+"""GPU parity of the HIP fourierrec path against the FP64 oracle.
+
+Tolerance (float32 path): per-slice relative L2 error <= 1e-5 against the FP64
+oracle, for the backprojection, the filter and the full chain.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import filter_fp64 as ff
+from oracle import fourierrec_fp64 as fr
+from oracle import preproc_fp64 as pp
+from tomocupy_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+RTOL32 = 1e-5
+
+
+def _cuda(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def _rel_l2_per_slice(got, ref):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    num = np.sqrt(((got - ref) ** 2).reshape(len(ref), -1).sum(1))
+    den = np.sqrt((ref ** 2).reshape(len(ref), -1).sum(1))
+    return num / np.maximum(den, 1e-300)
+
+
+def _smooth_sinograms(rng, nz, nproj, n):
+    """Random sinograms with a Parzen-like roll-off (as filtered data has)."""
+    d = rng.standard_normal((nz, nproj, n))
+    f = np.fft.rfft(d, axis=-1)
+    k = np.arange(f.shape[-1]) / n
+    f *= np.clip(1 - 2 * k, 0, None) ** 3
+    return np.fft.irfft(f, n, axis=-1)
+
+
+@pytest.mark.parametrize("n,nproj,nz,start", [(64, 48, 4, 0.0), (96, 60, 3, 0.2),
+                                              (128, 90, 2, 0.0), (48, 30, 5, 0.7)])
+def test_backprojection_matches_oracle(n, nproj, nz, start):
+    from tomocupy_amd import FourierRec
+    print('case n=%d nproj=%d nz=%d' % (n, nproj, nz), flush=True)
+    rng = np.random.default_rng(n * 7 + nz)
+    theta = synth.angles(nproj, start)
+    data = _smooth_sinograms(rng, nz, nproj, n)
+    ref = fr.backprojection(data, theta)
+    rec = FourierRec(n, nproj, nz, _cuda(theta), 'float32')
+    d = _cuda(data)
+    d0 = d.clone()
+    obj = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    rec.backprojection(obj, d)
+    torch.cuda.synchronize()
+    err = _rel_l2_per_slice(obj.cpu().numpy(), ref)
+    print('rel-L2 per slice', err)
+    assert np.all(err <= RTOL32), err
+    assert torch.equal(d, d0), 'input data was modified'
+    rec.free()
+
+
+def test_backprojection_deterministic():
+    from tomocupy_amd import FourierRec
+    n, nproj, nz = 128, 120, 8
+    rng = np.random.default_rng(5)
+    theta = synth.angles(nproj)
+    d = _cuda(_smooth_sinograms(rng, nz, nproj, n))
+    rec = FourierRec(n, nproj, nz, _cuda(theta))
+    a = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    b = torch.zeros_like(a)
+    rec.backprojection(a, d)
+    rec.backprojection(b, d)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("n,nproj,nz", [(64, 40, 4), (96, 30, 3)])
+def test_forward_is_adjoint(n, nproj, nz):
+    from tomocupy_amd import FourierRec
+    rng = np.random.default_rng(11)
+    theta = synth.angles(nproj, 0.1)
+    rec = FourierRec(n, nproj, nz, _cuda(theta))
+    x = _cuda(rng.standard_normal((nz, n, n)))
+    y = _cuda(rng.standard_normal((nz, nproj, n)))
+    Ax = torch.zeros((nz, nproj, n), dtype=torch.float32, device='cuda')
+    Aty = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    rec.forward(Ax, x)
+    rec.backprojection(Aty, y)
+    torch.cuda.synchronize()
+    lhs = float((Ax.double() * y.double()).sum())
+    rhs = float((x.double() * Aty.double()).sum())
+    rel = abs(lhs - rhs) / (float(Ax.double().norm()) * float(y.double().norm()))
+    print('dot test', lhs, rhs, rel)
+    assert rel < 1e-5
+    # and against the FP64 oracle adjoint
+    ref = fr.forward(x.cpu().numpy().astype(np.float64), theta)
+    err = _rel_l2_per_slice(Ax.cpu().numpy(), ref)
+    assert np.all(err <= RTOL32), err
+
+
+@pytest.mark.parametrize("name,center,sht", [('parzen', 64.0, None), ('ramp', 63.5, [0.0, 1.5, -2.25]),
+                                             ('shepp', 70.25, None), ('hann', 64.0, [0.5, 0.0, 3.0])])
+def test_filter_center_matches_oracle(name, center, sht):
+    from tomocupy_amd import FBPFilter
+    n, nproj, nz = 128, 37, 3
+    ne = 4 * n
+    rng = np.random.default_rng(3)
+    data = rng.standard_normal((nz, nproj, n)) + 2.0
+    w64 = ff.calc_filter(ne, name)
+    shv = np.zeros(nz) if sht is None else np.array(sht)
+    ref = ff.fbp_filter_center(data, w64, ne, center, shv)
+    flt = FBPFilter(ne, nproj, nz, 'float32', n_unpadded=n)
+    wf = flt.calc_filter(name)
+    d = _cuda(data)
+    flt.filter_center(d, wf, center, None if sht is None else _cuda(shv))
+    torch.cuda.synchronize()
+    err = _rel_l2_per_slice(d.cpu().numpy(), ref)
+    print('filter rel-L2', err)
+    assert np.all(err <= RTOL32), err
+
+
+def test_filter_padded_matches_oracle():
+    from tomocupy_amd import FBPFilter
+    ne, nproj, nz = 256, 9, 2
+    rng = np.random.default_rng(4)
+    tmp = rng.standard_normal((nz, nproj, ne))
+    w = rng.standard_normal((nz, ne // 2 + 1)) + 1j * rng.standard_normal((nz, ne // 2 + 1))
+    ref = ff.filter_padded(tmp, w)
+    flt = FBPFilter(ne, nproj, nz, 'float32')
+    d = _cuda(tmp)
+    flt.filter(d, torch.as_tensor(w.astype(np.complex64)).cuda())
+    torch.cuda.synchronize()
+    err = _rel_l2_per_slice(d.cpu().numpy(), ref)
+    assert np.all(err <= RTOL32), err
+
+
+@pytest.mark.parametrize("flat_linear,bright", [(False, 1.0), (True, 1.0), (False, 10.0)])
+def test_preprocess_matches_oracle(flat_linear, bright):
+    from tomocupy_amd import ProcFunctions
+    n, nproj, nz = 96, 50, 4
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(1, n, theta, nz, ndark=4, nflat=6)
+    ref = pp.proc_chain(data, dark, flat, bright_ratio=bright, flat_linear=flat_linear)
+    pf = ProcFunctions(nproj, nz, n, 4, 6, bright_ratio=bright, flat_linear=flat_linear)
+    out = pf.proc_chunk(_cuda(data, torch.uint16), _cuda(dark, torch.uint16), _cuda(flat, torch.uint16))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    err = _rel_l2_per_slice(got, ref)
+    print('preprocess rel-L2', err)
+    assert np.all(err <= RTOL32), err
+    # dark/flat alone, projection order
+    res = pf.darkflat_correction(_cuda(data, torch.uint16), _cuda(dark, torch.uint16),
+                                 _cuda(flat, torch.uint16))
+    ref2 = pp.darkflat_correction(data, dark, flat, bright, flat_linear)
+    torch.cuda.synchronize()
+    assert np.allclose(res.cpu().numpy(), ref2, rtol=1e-5, atol=1e-6)
+
+
+def test_minus_log_rules_on_gpu():
+    from tomocupy_amd import ProcFunctions
+    pf = ProcFunctions(1, 1, 6, 1, 1)
+    x = _cuda(np.array([[[-1.0, 0.0, 1.0, np.e, np.nan, np.inf]]]))
+    out = pf.proc_proj_swap(x)
+    torch.cuda.synchronize()
+    assert np.allclose(out.cpu().numpy().ravel(), [0, 0, 0, -1, 6.0, 0.0], atol=1e-6)
+
+
+def test_config1_full_chain():
+    """Config-1 shape: n=512, nproj=720, nz=16, Parzen, dark/flat + -log, centre n/2."""
+    from tomocupy_amd import BackprojFunctions, ProcFunctions
+    n, nproj, nz = 512, 720, 16
+    print('config-1 chain', flush=True)
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(1, n, theta, nz)
+    sino = pp.proc_chain(data, dark, flat)
+    ne = 4 * n
+    filt = ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne, n / 2)
+    ref = fr.backprojection(filt, theta)
+    pf = ProcFunctions(nproj, nz, n, dark.shape[0], flat.shape[0])
+    bf = BackprojFunctions(n, nproj, nz, _cuda(theta), n / 2, 'float32', 'parzen')
+    out = pf.proc_chunk(_cuda(data, torch.uint16), _cuda(dark, torch.uint16), _cuda(flat, torch.uint16))
+    bf.fbp_filter_center(out)
+    obj = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    bf.cl_rec.backprojection(obj, out)
+    torch.cuda.synchronize()
+    err = _rel_l2_per_slice(obj.cpu().numpy(), ref)
+    print('config-1 rel-L2 max %.3g median %.3g' % (err.max(), np.median(err)))
+    assert np.all(err <= RTOL32), err

@@ -1,0 +1,51 @@
+"""Filter + backprojection driver with the reference's interface.
+
+Mirrors ``BackprojFunctions`` (``src/tomocupy/reconstruction/backproj_functions.py:46-109``)
+for the fourierrec algorithm.  The reference reads the process-global
+``args``/``params`` namespaces; this class takes the same values as explicit
+arguments (``n``, ``nproj``, ``ncz``, ``theta``, ``center``, ``dtype``,
+``fbp_filter``).
+"""
+import math
+
+from .fbp_filter import FBPFilter
+from .fourierrec import FourierRec
+
+
+def padded_length(n, dtype='float32'):
+    """``ne = 4n``, rounded up to a power of two for float16
+    (``backproj_functions.py:50-54``)."""
+    ne = 4 * n
+    if dtype == 'float16':
+        ne = 2 ** int(math.ceil(math.log2(ne)))
+    return ne
+
+
+class BackprojFunctions:
+    def __init__(self, n, nproj, ncz, theta, center, dtype='float32', fbp_filter='parzen',
+                 reconstruction_algorithm='fourierrec'):
+        if reconstruction_algorithm != 'fourierrec':
+            raise NotImplementedError('only the fourierrec algorithm is implemented')
+        self.n, self.nproj, self.ncz = int(n), int(nproj), int(ncz)
+        self.center = float(center)
+        self.ne = padded_length(self.n, dtype)
+        self.cl_rec = FourierRec(self.n, self.nproj, self.ncz, theta, dtype)
+        self.cl_filter = FBPFilter(self.ne, self.nproj, self.ncz, dtype, n_unpadded=self.n)
+        self.wfilter = self.cl_filter.calc_filter(fbp_filter)
+        self.pad = self.ne // 2 - self.n // 2
+
+    def fbp_filter_center(self, data, sht=0):
+        """FBP filtering of ``data (ncz, nproj, n)`` in place, with the rotation
+        centre shift; ``sht`` is a scalar or a ``(ncz,)`` float32 GPU tensor."""
+        import torch
+        if isinstance(sht, torch.Tensor):
+            shv = sht.to(torch.float32).contiguous()
+        elif sht == 0:
+            shv = None
+        else:
+            shv = torch.full((self.ncz,), float(sht), dtype=torch.float32, device=data.device)
+        return self.cl_filter.filter_center(data, self.wfilter, self.center, shv)
+
+    def free(self):
+        self.cl_rec.free()
+        self.cl_filter.free()

@@ -1,0 +1,194 @@
+// LDS-resident Stockham autosort FFT for sizes N = 2^a 3^b (N <= 16384 per
+// transform), executed cooperatively by a workgroup on 2^lgH interleaved
+// transforms held in LDS.
+//
+// Pass with radix R and current sub-transform length Ns (1, R1, R1*R2, ...):
+//   for j in [0, N/R):  k = j mod Ns
+//     a_r = x[j + r N/R] * W^(r k N/(Ns R))         (r = 0..R-1)
+//     b   = DFT_R(a)
+//     y[(j/Ns) Ns R + k + q Ns] = b_q
+// which leaves the output in natural order.  Every thread stages all of its
+// butterflies in registers before the barrier, so one LDS buffer suffices.
+// Twiddles come from a per-size table W_N^k = exp(-2 pi i k/N) computed on the
+// host in FP64; the inverse transform uses their conjugates.  Unnormalised in
+// both directions.
+#pragma once
+
+#include "common.h"
+
+namespace tomo {
+
+constexpr int kFftMaxRadices = 16;
+constexpr int kFftEPT = 16;  // complex elements staged per thread per pass (max)
+
+struct FftDesc {
+  int N;
+  int nr;
+  int r[kFftMaxRadices];
+  const float2 *tw;  // device, N entries
+};
+
+// Host-side plan: radix sequence + device twiddle table.
+struct FftPlan {
+  FftDesc d{};
+  DevBuf tw;
+  int init(int N);  // N = 2^a 3^b
+};
+
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  // multiply by (-i) for the forward transform, (+i) for the inverse
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void butterfly(float2 *a) {
+  if constexpr (R == 2) {
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  } else if constexpr (R == 3) {
+    const float s3 = 0.86602540378443864676f;  // sqrt(3)/2
+    float2 t1 = cadd(a[1], a[2]);
+    float2 t2 = csub(a[1], a[2]);
+    float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+    float2 r = cscale(mul_i<DIR>(t2), s3);
+    a[0] = cadd(a[0], t1);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  } else if constexpr (R == 4) {
+    float2 t0 = cadd(a[0], a[2]);
+    float2 t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]);
+    float2 t3 = mul_i<DIR>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  } else if constexpr (R == 8) {
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    butterfly<4, DIR>(e);
+    butterfly<4, DIR>(o);
+    const float h = 0.70710678118654752440f;
+    // w8^q for q = 1,2,3 with w8 = exp(DIR * i pi/4)
+    float2 o1 = make_float2(h * (o[1].x - DIR * o[1].y), h * (o[1].y + DIR * o[1].x));
+    float2 o2 = mul_i<DIR>(o[2]);
+    float2 o3 = make_float2(h * (-o[3].x - DIR * o[3].y), h * (-o[3].y + DIR * o[3].x));
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+  }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float2 *tw, int lgH,
+                                         int tid, int nt) {
+  constexpr int MAXB = kFftEPT / R;
+  const int H = 1 << lgH;
+  const int nbPer = N / R;
+  const int nb = nbPer << lgH;
+  const int twstep = N / (Ns * R);
+  const bool pow2 = (Ns & (Ns - 1)) == 0;
+  const int lgNs = pow2 ? __ffs(Ns) - 1 : 0;
+  const float invNs = 1.0f / (float)Ns;
+  // j -> (j / Ns, j % Ns); shifts when Ns is a power of two (all passes before
+  // the radix-3 ones), otherwise a float-reciprocal quotient with correction
+  auto split = [&](int j, int &jq, int &k) {
+    if (pow2) {
+      jq = j >> lgNs;
+      k = j & (Ns - 1);
+    } else {
+      jq = (int)((float)j * invNs);
+      k = j - jq * Ns;
+      if (k < 0) { jq -= 1; k += Ns; }
+      if (k >= Ns) { jq += 1; k -= Ns; }
+    }
+  };
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> lgH;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[((j + r * nbPer) << lgH) + t];
+      if (Ns > 1) {
+        int jq, k;
+        split(j, jq, k);
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * twstep];
+          if (DIR > 0) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      butterfly<R, DIR>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> lgH;
+      int jq, k;
+      split(j, jq, k);
+      const int o = jq * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[((o + r * Ns) << lgH) + t] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place transform of 2^lgH interleaved sequences of length d.N held in LDS:
+// element i of sequence t lives at buf[i * 2^lgH + t].  Requires
+// 2^lgH * N <= 15 * nt (16 * nt without radix-3 passes); see fft_threads().
+// The caller must __syncthreads() after filling buf; the function ends with a
+// barrier.
+template <int DIR>
+__device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt) {
+  int Ns = 1;
+  for (int s = 0; s < d.nr; ++s) {
+    const int R = d.r[s];
+    if (R == 8)
+      fft_pass<8, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    else if (R == 4)
+      fft_pass<4, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    else if (R == 2)
+      fft_pass<2, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    else
+      fft_pass<3, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    Ns *= R;
+  }
+}
+
+// Workgroup size for transforming `elems` complex values (all interleaved
+// sequences) with plan d: 16 values per thread for radices 2/4/8, 15 when a
+// radix-3 pass is present; at least 256 and a multiple of 64.  Returns -1 if
+// more than 1024 threads would be needed.
+inline int fft_threads(const FftDesc &d, int64_t elems) {
+  bool r3 = false;
+  for (int i = 0; i < d.nr; ++i) r3 |= (d.r[i] == 3);
+  const int64_t per = r3 ? 15 : 16;
+  int64_t t = (elems + per - 1) / per;
+  t = ((t + 63) / 64) * 64;
+  if (t > 1024) return -1;
+  return (int)(t < 256 ? 256 : t);
+}
+
+// True if N = 2^a 3^b with N >= 1.
+inline bool fft_size_ok(int64_t N) {
+  if (N < 1) return false;
+  while (N % 2 == 0) N /= 2;
+  while (N % 3 == 0) N /= 3;
+  return N == 1;
+}
+
+}  // namespace tomo

@@ -1,0 +1,322 @@
+// FBP filter (tomocupy cfunc_filter + FBPFilter + fbp_filter_center) for gfx950.
+//
+// Two rows of the same slice (same filter) are filtered together as one
+// complex row z = x1 + i*x2: FFT_ne(z) * W * IFFT_ne gives y1 + i*y2 because
+// the full-length filter W (W[k] = w[k] for k <= ne/2 with the Nyquist bin's
+// real part, W[ne-k] = conj(w[k])) is Hermitian, which is exactly the
+// R2C -> mulw -> C2R of cfunc_filter::filter (src/cuda/cfunc_filter.cu:49-61).
+// The edge-replicate padding of fbp_filter_center
+// (src/tomocupy/reconstruction/backproj_functions.py:97-100) is applied while
+// loading, so the (nz, nproj, ne) temporary of the reference never exists.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "fft_lds.h"
+
+using namespace tomo;
+
+namespace {
+
+// w[z][k] = wfilter[k] * exp(-2 pi i (-center + sht[z] + n/2) k / ne), FP64
+// phase (backproj_functions.py:103-104).
+__global__ void k_flt_weights(const float *__restrict__ wfilter, const float *__restrict__ sht,
+                              float2 *__restrict__ w, int ne, int n, int nz, double center) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int z = blockIdx.y;
+  const int nh = ne / 2 + 1;
+  if (k >= nh) return;
+  const double shift = -center + (sht ? (double)sht[z] : 0.0) + (double)(n / 2);
+  // reduce k*shift/ne modulo 1 before the trig for accuracy
+  double f = (double)k * shift / (double)ne;
+  f -= std::floor(f);
+  double s, c;
+  sincospi(-2.0 * f, &s, &c);
+  const double a = (double)wfilter[k];
+  w[(size_t)z * nh + k] = make_float2((float)(a * c), (float)(a * s));
+}
+
+// mode 0: rows of width ne already padded (cfunc_filter::filter);
+// mode 1: rows of width n, edge-replicate padded on load and cropped on store
+template <int MODE>
+__global__ void k_flt_rows(float *__restrict__ g, const float2 *__restrict__ w, FftDesc fd, int ne,
+                           int n, int nproj) {
+  extern __shared__ float2 sm[];
+  const int pp = blockIdx.x, z = blockIdx.y;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int p0 = 2 * pp, p1 = p0 + 1;
+  const bool has1 = p1 < nproj;
+  const int width = MODE == 0 ? ne : n;
+  float *r0 = g + ((size_t)z * nproj + p0) * width;
+  float *r1 = g + ((size_t)z * nproj + p1) * width;
+  const int pad = ne / 2 - n / 2;
+  for (int j = tid; j < ne; j += nt) {
+    int src = j;
+    if (MODE == 1) src = min(max(j - pad, 0), n - 1);
+    sm[j] = make_float2(r0[src], has1 ? r1[src] : 0.f);
+  }
+  __syncthreads();
+  lds_fft<-1>(sm, fd, 0, tid, nt);
+  const float2 *wz = w + (size_t)z * (ne / 2 + 1);
+  const int nh = ne / 2;
+  for (int k = tid; k < ne; k += nt) {
+    float2 W;
+    if (k == 0 || k == nh) {
+      W = make_float2(wz[k].x, 0.f);
+    } else if (k < nh) {
+      W = wz[k];
+    } else {
+      W = cconj(wz[ne - k]);
+    }
+    sm[k] = cmul(sm[k], W);
+  }
+  __syncthreads();
+  lds_fft<1>(sm, fd, 0, tid, nt);
+  if (MODE == 0) {
+    for (int j = tid; j < ne; j += nt) {
+      r0[j] = sm[j].x;
+      if (has1) r1[j] = sm[j].y;
+    }
+  } else {
+    for (int j = tid; j < n; j += nt) {
+      r0[j] = sm[pad + j].x;
+      if (has1) r1[j] = sm[pad + j].y;
+    }
+  }
+}
+
+
+// ---- filter design (fbp_filter.py:60-118), host FP64 ----------------------
+
+// Solve A X = B (n x n, n x m) by LU with partial pivoting; B overwritten by X.
+bool lu_solve(std::vector<double> &A, std::vector<double> &B, int n, int m) {
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > best) best = std::fabs(A[i * n + k]), p = i;
+    if (best == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+      for (int j = 0; j < m; ++j) std::swap(B[k * m + j], B[p * m + j]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      for (int j = 0; j < m; ++j) B[i * m + j] -= f * B[k * m + j];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i)
+    for (int j = 0; j < m; ++j) {
+      double s = B[i * m + j];
+      for (int q = i + 1; q < n; ++q) s -= A[i * n + q] * B[q * m + j];
+      B[i * m + j] = s / A[i * n + i];
+    }
+  return true;
+}
+
+// _wint(n, t) (fbp_filter.py:90-118): local order-n quadrature weights for the
+// integral of x*f(x), with the last 40 entries replaced by a linear ramp.
+bool wint(int n, const std::vector<double> &t, std::vector<double> &wn) {
+  const int N = (int)t.size();
+  std::vector<double> s(n);
+  // numpy.linspace(1e-40, 1, n): i*step + start, endpoint set exactly
+  const double step = (1.0 - 1e-40) / (double)(n - 1);
+  for (int i = 0; i < n; ++i) s[i] = (double)i * step + 1e-40;
+  s[n - 1] = 1.0;
+  // V[i][j] = exp(i*log(s_j)) = s_j^i ; u[i][j] = diff_j(exp((i+1) log s)/(i+1)), i < n+1
+  std::vector<double> V(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) V[i * n + j] = std::exp((double)i * std::log(s[j]));
+  std::vector<double> U((n + 1) * (n - 1));
+  for (int i = 0; i < n + 1; ++i) {
+    std::vector<double> e(n);
+    for (int j = 0; j < n; ++j) e[j] = std::exp((double)(i + 1) * std::log(s[j])) * (1.0 / (i + 1));
+    for (int j = 0; j < n - 1; ++j) U[i * (n - 1) + j] = e[j + 1] - e[j];
+  }
+  // iv = inv(V) (LU with partial pivoting against the identity, as LAPACK
+  // gesv), then W1 = iv @ U[1:n+1], W2 = iv @ U[0:n] (fbp_filter.py:97-101)
+  std::vector<double> iv(n * n, 0.0);
+  for (int i = 0; i < n; ++i) iv[i * n + i] = 1.0;
+  std::vector<double> A = V;
+  if (!lu_solve(A, iv, n, n)) return false;
+  std::vector<double> W1(n * (n - 1), 0.0), W2(n * (n - 1), 0.0);
+  for (int r = 0; r < n; ++r)
+    for (int k = 0; k < n - 1; ++k) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int q = 0; q < n; ++q) {
+        s1 += iv[r * n + q] * U[(q + 1) * (n - 1) + k];
+        s2 += iv[r * n + q] * U[q * (n - 1) + k];
+      }
+      W1[r * (n - 1) + k] = s1;
+      W2[r * (n - 1) + k] = s2;
+    }
+  std::vector<double> p;
+  for (int i = 1; i < n; ++i) p.push_back(1.0 / i);
+  for (int i = 0; i < N - 2 * (n - 1) - 1; ++i) p.push_back(1.0 / (n - 1));
+  for (int i = n - 1; i > 0; --i) p.push_back(1.0 / i);
+  std::vector<double> w(N, 0.0);
+  for (int j = 0; j < N - n + 1; ++j) {
+    const double h = t[j + n - 1] - t[j];
+    for (int k = 0; k < n - 1; ++k) {
+      const double pk = p[j + k];
+      for (int r = 0; r < n; ++r)
+        w[j + r] += pk * (h * h * W1[r * (n - 1) + k] + h * t[j] * W2[r * (n - 1) + k]);
+    }
+  }
+  wn = w;
+  for (int i = N - 40; i < N; ++i) wn[i] = w[N - 40] / (N - 40) * i;
+  return true;
+}
+
+}  // namespace
+
+struct tomo_flt_plan {
+  int64_t n = 0, ne = 0, nproj = 0, nz = 0;
+  FftPlan fft;
+  DevBuf w;  // (nz, ne/2+1) complex, filled by flt_filter_center
+  int threads = 256;
+  size_t lds = 0;
+};
+
+extern "C" int flt_calc_filter(int64_t ne, const char *name, double *out) {
+  clear_error();
+  TOMO_REQUIRE(name != nullptr && out != nullptr, "flt_calc_filter: NULL argument");
+  TOMO_REQUIRE(ne >= 80 && ne % 2 == 0, "flt_calc_filter: ne=%lld must be even and >= 80",
+               (long long)ne);
+  const std::string nm(name);
+  const int N = (int)(ne / 2 + 1);
+  std::vector<double> t(N);
+  for (int j = 0; j < N; ++j) t[j] = (double)j / (double)ne;
+  const double d = 0.5;
+  std::vector<double> wfa(N);
+  const char *known[] = {"none", "ramp", "shepp", "cosine", "cosine2", "hamming", "hann", "parzen"};
+  bool ok = false;
+  for (const char *k : known) ok |= (nm == k);
+  TOMO_REQUIRE(ok, "flt_calc_filter: unknown filter '%s'", name);
+  if (nm == "none") {
+    for (int j = 0; j < N; ++j) wfa[j] = (double)ne * 0.5;
+    wfa[0] *= 2;
+  } else {
+    std::vector<double> wi;
+    if (!wint(12, t, wi)) {
+      set_error("flt_calc_filter: singular quadrature system");
+      return TOMO_EINVAL;
+    }
+    for (int j = 0; j < N; ++j) {
+      const double x = t[j];
+      double win = 1.0;
+      if (nm == "shepp") {
+        const double a = x / (2 * d);
+        win = (a == 0.0 ? 1.0 : std::sin(M_PI * a) / (M_PI * a)) * (x / d <= 2);
+      } else if (nm == "cosine") {
+        win = std::cos(M_PI * x / (2 * d)) * (x / d <= 1);
+      } else if (nm == "cosine2") {
+        const double c = std::cos(M_PI * x / (2 * d));
+        win = c * c * (x / d <= 1);
+      } else if (nm == "hamming") {
+        win = (.54 + .46 * std::cos(M_PI * x / d)) * (x / d <= 1);
+      } else if (nm == "hann") {
+        win = (1 + std::cos(M_PI * x / d)) / 2.0 * (x / d <= 1);
+      } else if (nm == "parzen") {
+        win = std::pow(1 - x / d, 3) * (x / d <= 1);
+      }
+      wfa[j] = (double)ne * 0.5 * wi[j] * win;
+    }
+  }
+  for (int j = 0; j < N; ++j) wfa[j] = 2 * wfa[j] * (wfa[j] >= 0 ? 1.0 : 0.0);
+  wfa[0] *= 2;
+  for (int j = 0; j < N; ++j) out[j] = wfa[j] / (double)ne;
+  return TOMO_OK;
+}
+
+extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t nproj, int64_t nz,
+                          int dtype) {
+  clear_error();
+  TOMO_REQUIRE(plan != nullptr, "flt_create: plan pointer is NULL");
+  *plan = nullptr;
+  TOMO_REQUIRE(n >= 20 && n % 2 == 0, "flt_create: n=%lld must be even and >= 20", (long long)n);
+  TOMO_REQUIRE(ne >= n && ne % 2 == 0, "flt_create: ne=%lld must be even and >= n", (long long)ne);
+  TOMO_REQUIRE(fft_size_ok(ne) && ne <= 16384, "flt_create: ne=%lld must be 2^a 3^b <= 16384",
+               (long long)ne);
+  TOMO_REQUIRE(nproj >= 1 && nz >= 1, "flt_create: nproj and nz must be >= 1");
+  TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
+               "flt_create: unknown dtype %d", dtype);
+  if (dtype != TOMO_F32) {
+    set_error("flt_create: dtype %d is not implemented in this build (float32 only)", dtype);
+    return TOMO_EUNSUPPORTED;
+  }
+  auto *P = new (std::nothrow) tomo_flt_plan();
+  if (!P) {
+    set_error("flt_create: host allocation failed");
+    return TOMO_ENOMEM;
+  }
+  P->n = n;
+  P->ne = ne;
+  P->nproj = nproj;
+  P->nz = nz;
+  int rc = P->fft.init((int)ne);
+  if (!rc) rc = P->w.alloc(sizeof(float2) * nz * (ne / 2 + 1));
+  if (rc) {
+    flt_destroy(P);
+    return rc;
+  }
+  P->threads = fft_threads(P->fft.d, ne);
+  TOMO_REQUIRE(P->threads > 0, "flt_create: ne=%lld too large for one workgroup", (long long)ne);
+  P->lds = sizeof(float2) * ne;
+  TOMO_REQUIRE(P->lds <= max_lds_per_block(), "flt_create: ne=%lld needs %zu B of LDS",
+               (long long)ne, P->lds);
+  hipError_t e = hipFuncSetAttribute((const void *)k_flt_rows<0>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_flt_rows<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)P->lds);
+  if (e != hipSuccess) {
+    set_error("flt_create: hipFuncSetAttribute failed: %s", hipGetErrorString(e));
+    flt_destroy(P);
+    return TOMO_EHIP;
+  }
+  *plan = P;
+  return TOMO_OK;
+}
+
+extern "C" int flt_filter(tomo_flt_plan *P, float *g, const float *w, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "flt_filter: plan is NULL");
+  TOMO_REQUIRE(g != nullptr && w != nullptr, "flt_filter: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_flt_rows<0>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
+                     dim3(P->threads), P->lds, st, g, reinterpret_cast<const float2 *>(w), P->fft.d,
+                     (int)P->ne, (int)P->n, (int)P->nproj);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int flt_filter_center(tomo_flt_plan *P, float *data, const float *wfilter, double center,
+                                 const float *sht, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "flt_filter_center: plan is NULL");
+  TOMO_REQUIRE(data != nullptr && wfilter != nullptr, "flt_filter_center: NULL array");
+  TOMO_REQUIRE(std::isfinite(center), "flt_filter_center: center is not finite");
+  hipStream_t st = (hipStream_t)stream;
+  const int nh = (int)(P->ne / 2 + 1);
+  hipLaunchKernelGGL(k_flt_weights, dim3((unsigned)ceil_div(nh, 256), (unsigned)P->nz), dim3(256), 0,
+                     st, wfilter, sht, P->w.as<float2>(), (int)P->ne, (int)P->n, (int)P->nz, center);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_flt_rows<1>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
+                     dim3(P->threads), P->lds, st, data, P->w.as<float2>(), P->fft.d, (int)P->ne,
+                     (int)P->n, (int)P->nproj);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" void flt_destroy(tomo_flt_plan *P) {
+  if (!P) return;
+  P->fft.tw.release();
+  P->w.release();
+  delete P;
+}

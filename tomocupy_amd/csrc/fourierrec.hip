@@ -1,0 +1,580 @@
+// USFFT backprojector (tomocupy cfunc_fourierrec) and its adjoint for gfx950.
+//
+// Data flow of fr_backprojection for one call (nzc = ceil(nz/2) complex
+// slices, processed in groups of 4 so one sample carries 32 B of spectra):
+//   k_fr_prep    data (nz,nproj,n) f32 -> pair-pack, shifted n-point DFT,
+//                scale (4/n)*c0            -> G [grp][p*n+k][4] complex
+//   k_fr_grid    deterministic gather of the polar samples onto the
+//                periodic 2n x 2n grid: one wave per 8x8 cell block walks the
+//                block's sample list (built on the host in FP64) in a fixed
+//                order and accumulates in registers -> F [c][a][b]
+//   k_fr_rowfft  shifted inverse FFT along b, pruned to the n centre columns
+//                -> R [c][a][v']
+//   k_fr_colfft  shifted inverse FFT along a, pruned to the n rows divphi
+//                reads, deapodisation (FP64 tables), circle mask, unpacking
+//                -> obj (nz,n,n) f32
+// which is, step for step, cfunc_fourierrec::backprojection
+// (src/cuda/cfunc_fourierrec.cu:67-93): ifftshiftc/FFT1D/ifftshiftc (:78-80),
+// gather (:82) + wrap (:84) as one periodic gather, fftshiftc/FFT2D^-1/
+// fftshiftc (:86-89) as two pruned 1D passes, divphi (:91), circ (:92).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "fft_lds.h"
+
+using namespace tomo;
+
+namespace {
+
+constexpr int kGridWaves = 4;  // waves (= 8x8 blocks) per gather workgroup
+
+// Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
+__device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
+
+__global__ void k_fr_prep(const float *__restrict__ data, float2 *__restrict__ G, FftDesc fd,
+                          int n, int nproj, int nz, int nzc, float scale) {
+  extern __shared__ float2 sm[];
+  const int p = blockIdx.x, c = blockIdx.y;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  float2 *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
+  if (c >= nzc) {
+    for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = make_float2(0.f, 0.f);
+    return;
+  }
+  const float *r1 = data + ((size_t)c * nproj + p) * n;
+  const int z2 = c + nzc;
+  const float *r2 = data + ((size_t)z2 * nproj + p) * n;
+  const bool has2 = z2 < nz;
+  for (int j = tid; j < n; j += nt) {
+    const float s = shift_sign(j);
+    sm[j] = make_float2(s * r1[j], has2 ? s * r2[j] : 0.f);
+  }
+  __syncthreads();
+  lds_fft<-1>(sm, fd, 0, tid, nt);
+  for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[k], shift_sign(k) * scale);
+}
+
+__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
+    const float4 *__restrict__ geom, const int *__restrict__ lists, const int *__restrict__ offs,
+    const float2 *__restrict__ G, float2 *__restrict__ F, int n, int nblk, int nsamp, int nzc,
+    float K2) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = blockIdx.x * kGridWaves + wave;
+  const int grp = blockIdx.y;
+  if (blk >= nblk * nblk) return;
+  const int A = blk / nblk, B = blk - A * nblk;
+  const int a = A * 8 + (lane >> 3), b = B * 8 + (lane & 7);
+  const int n2 = 2 * n;
+  float2 acc0 = {0.f, 0.f}, acc1 = {0.f, 0.f}, acc2 = {0.f, 0.f}, acc3 = {0.f, 0.f};
+  const float4 *Gg = reinterpret_cast<const float4 *>(G + (size_t)grp * nsamp * 4);
+  const int e0 = offs[blk], e1 = offs[blk + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int s = __builtin_amdgcn_readfirstlane(lists[e]);
+    const float4 g = geom[s];  // du, dv, b0, b1
+    int d0 = b - n - (int)g.z;
+    d0 += (d0 < 0) ? n2 : 0;
+    d0 -= (d0 >= n2) ? n2 : 0;
+    int d1 = a - n - (int)g.w;
+    d1 += (d1 < 0) ? n2 : 0;
+    d1 -= (d1 >= n2) ? n2 : 0;
+    const float x0 = (float)(d0 - 4) - g.x;
+    const float x1 = (float)(d1 - 4) - g.y;
+    float w = exp2f(K2 * (x0 * x0 + x1 * x1));
+    w = ((unsigned)d0 <= 8u && (unsigned)d1 <= 8u) ? w : 0.f;
+    const float4 v01 = Gg[(size_t)s * 2];
+    const float4 v23 = Gg[(size_t)s * 2 + 1];
+    acc0.x += w * v01.x; acc0.y += w * v01.y;
+    acc1.x += w * v01.z; acc1.y += w * v01.w;
+    acc2.x += w * v23.x; acc2.y += w * v23.y;
+    acc3.x += w * v23.z; acc3.y += w * v23.w;
+  }
+  if (a < n2 && b < n2) {
+    const size_t plane = (size_t)n2 * n2;
+    const size_t o = (size_t)grp * 4 * plane + (size_t)a * n2 + b;
+    const int cbase = grp * 4;
+    if (cbase + 0 < nzc) F[o] = acc0;
+    if (cbase + 1 < nzc) F[o + plane] = acc1;
+    if (cbase + 2 < nzc) F[o + 2 * plane] = acc2;
+    if (cbase + 3 < nzc) F[o + 3 * plane] = acc3;
+  }
+}
+
+__global__ void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R, FftDesc fd,
+                            int n, int nzc) {
+  extern __shared__ float2 sm[];
+  const int a = blockIdx.x, c = blockIdx.y;
+  if (c >= nzc) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n2 = 2 * n;
+  const float2 *src = F + ((size_t)c * n2 + a) * n2;
+  for (int b = tid; b < n2; b += nt) {
+    const float2 x = src[b];
+    sm[b] = (b & 1) ? make_float2(-x.x, -x.y) : x;
+  }
+  __syncthreads();
+  lds_fft<1>(sm, fd, 0, tid, nt);
+  float2 *dst = R + ((size_t)c * n2 + a) * n;
+  const int h = n / 2;
+  for (int v = tid; v < n; v += nt) {
+    const float2 x = sm[v + h];
+    dst[v] = ((v + h) & 1) ? make_float2(-x.x, -x.y) : x;
+  }
+}
+
+__global__ void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ obj,
+                            const float *__restrict__ phix, const float *__restrict__ phiy,
+                            const int2 *__restrict__ circ, FftDesc fd, int n, int nz, int nzc,
+                            int lgCB) {
+  extern __shared__ float2 sm[];  // [a][t]: CB interleaved columns
+  const int cb = blockIdx.x, c = blockIdx.y;
+  if (c >= nzc) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int CB = 1 << lgCB;
+  const int n2 = 2 * n, v0 = cb * CB;
+  const float2 *src = R + (size_t)c * n2 * n;
+  for (int idx = tid; idx < CB * n2; idx += nt) {
+    const int a = idx >> lgCB, t = idx & (CB - 1);
+    float2 x = (v0 + t < n) ? src[(size_t)a * n + v0 + t] : make_float2(0.f, 0.f);
+    sm[idx] = (a & 1) ? make_float2(-x.x, -x.y) : x;
+  }
+  __syncthreads();
+  lds_fft<1>(sm, fd, lgCB, tid, nt);
+  const int z2 = c + nzc;
+  const int h = n / 2;
+  for (int idx = tid; idx < CB * n; idx += nt) {
+    const int ty = idx >> lgCB, t = idx & (CB - 1);
+    const int tx = v0 + t;
+    if (tx >= n) continue;
+    const int u = ty + 1 + h;
+    float2 x = sm[(u << lgCB) + t];
+    const int2 ci = circ[ty];
+    const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
+    obj[((size_t)c * n + ty) * n + tx] = x.x * ph;
+    if (z2 < nz) obj[((size_t)z2 * n + ty) * n + tx] = x.y * ph;
+  }
+}
+
+// ---- adjoint (forward projection) kernels --------------------------------
+
+__global__ void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
+                                const float *__restrict__ phix, const float *__restrict__ phiy,
+                                const int2 *__restrict__ circ, FftDesc fd, int n, int nz,
+                                int nzc, int lgCB) {
+  extern __shared__ float2 sm[];  // [a][t]: CB interleaved columns
+  const int cb = blockIdx.x, c = blockIdx.y;
+  if (c >= nzc) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int CB = 1 << lgCB;
+  const int n2 = 2 * n, v0 = cb * CB, h = n / 2;
+  const int z2 = c + nzc;
+  for (int idx = tid; idx < CB * n2; idx += nt) sm[idx] = make_float2(0.f, 0.f);
+  __syncthreads();
+  for (int idx = tid; idx < CB * n; idx += nt) {
+    const int ty = idx >> lgCB, t = idx & (CB - 1);
+    const int tx = v0 + t;
+    if (tx >= n) continue;
+    const int u = ty + 1 + h;
+    const int2 ci = circ[ty];
+    const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
+    const float re = obj[((size_t)c * n + ty) * n + tx];
+    const float im = (z2 < nz) ? obj[((size_t)z2 * n + ty) * n + tx] : 0.f;
+    sm[(u << lgCB) + t] = make_float2(re * ph, im * ph);
+  }
+  __syncthreads();
+  lds_fft<-1>(sm, fd, lgCB, tid, nt);
+  float2 *dst = R + (size_t)c * n2 * n;
+  for (int idx = tid; idx < CB * n2; idx += nt) {
+    const int a = idx >> lgCB, t = idx & (CB - 1);
+    if (v0 + t >= n) continue;
+    const float2 x = sm[idx];
+    dst[(size_t)a * n + v0 + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
+  }
+}
+
+__global__ void k_fr_rowfft_adj(const float2 *__restrict__ R, float2 *__restrict__ F, FftDesc fd,
+                                int n, int nzc) {
+  extern __shared__ float2 sm[];
+  const int a = blockIdx.x, c = blockIdx.y;
+  if (c >= nzc) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n2 = 2 * n, h = n / 2;
+  const float2 *src = R + ((size_t)c * n2 + a) * n;
+  for (int v = tid; v < n2; v += nt) {
+    float2 x = make_float2(0.f, 0.f);
+    if (v >= h && v < h + n) {
+      x = src[v - h];
+      if (v & 1) x = make_float2(-x.x, -x.y);
+    }
+    sm[v] = x;
+  }
+  __syncthreads();
+  lds_fft<-1>(sm, fd, 0, tid, nt);
+  float2 *dst = F + ((size_t)c * n2 + a) * n2;
+  for (int b = tid; b < n2; b += nt) {
+    const float2 x = sm[b];
+    dst[b] = (b & 1) ? make_float2(-x.x, -x.y) : x;
+  }
+}
+
+// Interpolation of the periodic grid at the polar samples (gather^T); one
+// thread per sample, taps summed in a fixed order.
+__global__ void k_fr_interp(const float4 *__restrict__ geom, const float2 *__restrict__ F,
+                            float2 *__restrict__ G, int n, int nsamp, int nzc, float K2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int grp = blockIdx.y;
+  if (s >= nsamp) return;
+  const int n2 = 2 * n;
+  const float4 g = geom[s];
+  const int c0 = grp * 4;
+  float2 acc[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+  const size_t plane = (size_t)n2 * n2;
+  const float2 *Fg = F + (size_t)c0 * plane;
+  float ex[9];
+#pragma unroll
+  for (int i0 = 0; i0 < 9; ++i0) {
+    const float x0 = (float)(i0 - 4) - g.x;
+    ex[i0] = x0 * x0;
+  }
+  for (int i1 = 0; i1 < 9; ++i1) {
+    int a = n + (int)g.w + i1;
+    a += (a < 0) ? n2 : 0;
+    a -= (a >= n2) ? n2 : 0;
+    const float x1 = (float)(i1 - 4) - g.y;
+#pragma unroll
+    for (int i0 = 0; i0 < 9; ++i0) {
+      int b = n + (int)g.z + i0;
+      b += (b < 0) ? n2 : 0;
+      b -= (b >= n2) ? n2 : 0;
+      const float w = exp2f(K2 * (ex[i0] + x1 * x1));
+      const size_t o = (size_t)a * n2 + b;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c0 + c < nzc) {
+          const float2 v = Fg[o + c * plane];
+          acc[c].x += w * v.x;
+          acc[c].y += w * v.y;
+        }
+      }
+    }
+  }
+  float2 *out = G + ((size_t)grp * nsamp + s) * 4;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c] = acc[c];
+}
+
+__global__ void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ data, FftDesc fd,
+                              int n, int nproj, int nz, int nzc, float scale) {
+  extern __shared__ float2 sm[];
+  const int p = blockIdx.x, c = blockIdx.y;
+  if (c >= nzc) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const float2 *in = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
+  for (int k = tid; k < n; k += nt) sm[k] = cscale(in[(size_t)k * 4], shift_sign(k) * scale);
+  __syncthreads();
+  lds_fft<1>(sm, fd, 0, tid, nt);
+  const int z2 = c + nzc;
+  float *r1 = data + ((size_t)c * nproj + p) * n;
+  float *r2 = data + ((size_t)z2 * nproj + p) * n;
+  for (int j = tid; j < n; j += nt) {
+    const float2 x = cscale(sm[j], shift_sign(j));
+    r1[j] = x.x;
+    if (z2 < nz) r2[j] = x.y;
+  }
+}
+
+}  // namespace
+
+struct tomo_fr_plan {
+  int64_t n = 0, nproj = 0, nz = 0, nzc = 0, ngrp = 0;
+  int nblk = 0;
+  int64_t nsamp = 0;
+  FftPlan fft1, fft2;
+  DevBuf geom, lists, offs, phix, phiy, circ;
+  DevBuf G, F, R;
+  int colCB = 1, lgCB = 0, colThreads = 256, prepThreads = 256, rowThreads = 256;
+  size_t colLds = 0;
+  float scale1 = 0.f, K2 = 0.f;
+};
+
+namespace {
+
+int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
+  const int64_t n = P->n, nproj = P->nproj, n2 = 2 * n;
+  const double mu = -std::log(1e-3) / (2.0 * (double)n * (double)n);
+  const int m = 4;
+  const double c0 = M_PI / (mu * 4.0 * (double)n * (double)n);
+  const double C1n = -M_PI * M_PI / (mu * 4.0 * (double)n * (double)n);
+  P->scale1 = (float)(4.0 / (double)n * c0);
+  P->K2 = (float)(C1n / std::log(2.0));
+
+  // polar sample geometry (kernels_fourierrec.cuh:23-36, :86-98), FP64
+  const int64_t S = nproj * n;
+  P->nsamp = S;
+  std::vector<float4> geom(S);
+  std::vector<int> b0v(S), b1v(S);
+  for (int64_t p = 0; p < nproj; ++p) {
+    const double th = (double)theta[p];
+    const double ct = std::cos(th), st = std::sin(th);
+    for (int64_t k = 0; k < n; ++k) {
+      const double t = (double)(k - n / 2) / (double)n;
+      double x = t * ct, y = -t * st;
+      if (x >= 0.5) x = 0.5 - 1e-5;
+      if (y >= 0.5) y = 0.5 - 1e-5;
+      const double u = (double)(2 * n) * x, v = (double)(2 * n) * y;
+      const double fu = std::floor(u), fv = std::floor(v);
+      const int64_t s = p * n + k;
+      b0v[s] = (int)fu - m;
+      b1v[s] = (int)fv - m;
+      geom[s] = make_float4((float)(u - fu), (float)(v - fv), (float)b0v[s], (float)b1v[s]);
+    }
+  }
+  // 8x8 block lists of the periodic grid, samples in increasing order
+  const int nb = (int)ceil_div(n2, 8);
+  P->nblk = nb;
+  auto blocks_of = [&](int base, int *out) {  // distinct blocks of rows base..base+8 (mod 2n)
+    int cnt = 0;
+    for (int i = 0; i < 9; ++i) {
+      int r = (int)((n + base + i) % n2);
+      if (r < 0) r += (int)n2;
+      const int bk = r / 8;
+      bool seen = false;
+      for (int q = 0; q < cnt; ++q) seen |= (out[q] == bk);
+      if (!seen) out[cnt++] = bk;
+    }
+    return cnt;
+  };
+  std::vector<int> counts((size_t)nb * nb + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int> fill;
+    std::vector<int> list;
+    if (pass == 1) {
+      fill.assign(counts.begin(), counts.end() - 1);
+      list.resize(counts.back());
+    }
+    for (int64_t s = 0; s < S; ++s) {
+      int br[4], bc[4];
+      const int nr = blocks_of(b1v[s], br), nc = blocks_of(b0v[s], bc);
+      for (int i = 0; i < nr; ++i)
+        for (int j = 0; j < nc; ++j) {
+          const size_t blk = (size_t)br[i] * nb + bc[j];
+          if (pass == 0)
+            counts[blk]++;
+          else
+            list[fill[blk]++] = (int)s;
+        }
+    }
+    if (pass == 0) {
+      int acc = 0;
+      for (size_t i = 0; i <= (size_t)nb * nb; ++i) {
+        const int c = counts[i];
+        counts[i] = acc;
+        acc += (i < (size_t)nb * nb) ? c : 0;
+      }
+    } else {
+      int rc = P->lists.alloc(sizeof(int) * std::max<size_t>(1, list.size()));
+      if (rc) return rc;
+      if (!list.empty())
+        TOMO_HIP(hipMemcpy(P->lists.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+    }
+  }
+  int rc = P->offs.alloc(sizeof(int) * counts.size());
+  if (rc) return rc;
+  TOMO_HIP(hipMemcpy(P->offs.p, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
+  rc = P->geom.alloc(sizeof(float4) * S);
+  if (rc) return rc;
+  TOMO_HIP(hipMemcpy(P->geom.p, geom.data(), sizeof(float4) * S, hipMemcpyHostToDevice));
+
+  // deapodisation (kernels_fourierrec.cuh:14-15), FP64, separable in x and y;
+  // the 1/nproj and (1 - n%4) factors ride on the y table
+  std::vector<float> px(n), py(n);
+  std::vector<int2> circ(n);
+  for (int64_t t = 0; t < n; ++t) {
+    const double d = (double)t / (double)n - 0.5;
+    px[t] = (float)std::exp(mu * (double)n * (double)n * d * d);
+    py[t] = (float)(std::exp(mu * (double)n * (double)n * d * d) / (double)nproj *
+                    (double)(1 - n % 4));
+  }
+  // circle mask 4x^2 + 4y^2 < 1 (kernels_fourierrec.cuh:161-174), FP64, as a
+  // per-row column interval
+  for (int64_t ty = 0; ty < n; ++ty) {
+    const double yy = (double)(ty - n / 2) / (double)n;
+    int lo = (int)n, hi = 0;
+    for (int64_t tx = 0; tx < n; ++tx) {
+      const double xx = (double)(tx - n / 2) / (double)n;
+      if (4.0 * xx * xx + 4.0 * yy * yy < 1.0) {
+        lo = std::min<int>(lo, (int)tx);
+        hi = std::max<int>(hi, (int)tx + 1);
+      }
+    }
+    if (lo >= hi) lo = hi = 0;
+    circ[ty] = make_int2(lo, hi);
+  }
+  if ((rc = P->phix.alloc(sizeof(float) * n))) return rc;
+  if ((rc = P->phiy.alloc(sizeof(float) * n))) return rc;
+  if ((rc = P->circ.alloc(sizeof(int2) * n))) return rc;
+  TOMO_HIP(hipMemcpy(P->phix.p, px.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  TOMO_HIP(hipMemcpy(P->phiy.p, py.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  TOMO_HIP(hipMemcpy(P->circ.p, circ.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
+  return TOMO_OK;
+}
+
+
+}  // namespace
+
+extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
+                         const float *theta, int dtype, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(plan != nullptr, "fr_create: plan pointer is NULL");
+  *plan = nullptr;
+  TOMO_REQUIRE(n >= 20 && n % 2 == 0, "fr_create: n=%lld must be even and >= 20", (long long)n);
+  TOMO_REQUIRE(n <= 4096, "fr_create: n=%lld exceeds the supported maximum 4096", (long long)n);
+  TOMO_REQUIRE(fft_size_ok(n), "fr_create: n=%lld must be of the form 2^a 3^b", (long long)n);
+  TOMO_REQUIRE(nproj >= 1 && nproj <= (1 << 20), "fr_create: nproj=%lld out of range", (long long)nproj);
+  TOMO_REQUIRE(nz >= 1 && nz <= (1 << 16), "fr_create: nz=%lld out of range", (long long)nz);
+  TOMO_REQUIRE(theta != nullptr, "fr_create: theta is NULL");
+  TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
+               "fr_create: unknown dtype %d", dtype);
+  TOMO_REQUIRE(nproj * n < (int64_t)1 << 31, "fr_create: nproj*n too large");
+  if (dtype != TOMO_F32) {
+    set_error("fr_create: dtype %d is not implemented in this build (float32 only)", dtype);
+    return TOMO_EUNSUPPORTED;
+  }
+  auto *P = new (std::nothrow) tomo_fr_plan();
+  if (!P) {
+    set_error("fr_create: host allocation failed");
+    return TOMO_ENOMEM;
+  }
+  P->n = n;
+  P->nproj = nproj;
+  P->nz = nz;
+  P->nzc = (nz + 1) / 2;
+  P->ngrp = ceil_div(P->nzc, 4);
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<float> th(nproj);
+  int rc = TOMO_OK;
+  auto fail = [&](int code) {
+    fr_destroy(P);
+    return code;
+  };
+  {
+    hipError_t e = hipMemcpyAsync(th.data(), theta, sizeof(float) * nproj, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_error("fr_create: reading theta failed: %s", hipGetErrorString(e));
+      return fail(TOMO_EHIP);
+    }
+  }
+  if ((rc = build_tables(P, th))) return fail(rc);
+  if ((rc = P->fft1.init((int)n))) return fail(rc);
+  if ((rc = P->fft2.init((int)(2 * n)))) return fail(rc);
+  const int64_t n2 = 2 * n;
+  const size_t plane = (size_t)n2 * n2;
+  if ((rc = P->G.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * P->nsamp))) return fail(rc);
+  if ((rc = P->F.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * plane))) return fail(rc);
+  if ((rc = P->R.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * n2 * n))) return fail(rc);
+  // column pass: CB columns of length 2n in LDS
+  const size_t maxlds = std::min<size_t>(max_lds_per_block(), 131072);
+  int CB = 1, lg = 0;
+  while (CB * 2 <= 16 && (size_t)(CB * 2) * n2 * sizeof(float2) <= maxlds &&
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) > 0)
+    CB *= 2, lg += 1;
+  P->colCB = CB;
+  P->lgCB = lg;
+  P->colLds = (size_t)CB * n2 * sizeof(float2);
+  P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2);
+  P->prepThreads = fft_threads(P->fft1.d, n);
+  P->rowThreads = fft_threads(P->fft2.d, n2);
+  if (P->colThreads < 0 || P->prepThreads < 0 || P->rowThreads < 0) {
+    set_error("fr_create: n=%lld needs more than 1024 threads per FFT workgroup", (long long)n);
+    return fail(TOMO_EINVAL);
+  }
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft_adj,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
+  const int rowLds = (int)(n2 * sizeof(float2));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft_adj,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
+  *plan = P;
+  return TOMO_OK;
+}
+
+extern "C" int fr_backprojection(tomo_fr_plan *P, float *obj, const float *data, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "fr_backprojection: plan is NULL");
+  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_backprojection: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
+  const int ng = (int)P->ngrp;
+  const int n2 = 2 * n;
+  hipLaunchKernelGGL(k_fr_prep, dim3(nproj, ng * 4), dim3(P->prepThreads), n * sizeof(float2), st,
+                     data, P->G.as<float2>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fr_grid, dim3((unsigned)ceil_div((int64_t)P->nblk * P->nblk, kGridWaves), ng),
+                     dim3(64 * kGridWaves), 0, st, P->geom.as<float4>(), P->lists.as<int>(),
+                     P->offs.as<int>(), P->G.as<float2>(), P->F.as<float2>(), n, P->nblk,
+                     (int)P->nsamp, nzc, P->K2);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fr_rowfft, dim3(n2, ng * 4), dim3(P->rowThreads), n2 * sizeof(float2), st,
+                     P->F.as<float2>(), P->R.as<float2>(), P->fft2.d, n, nzc);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fr_colfft, dim3((unsigned)ceil_div(n, P->colCB), ng * 4), dim3(P->colThreads),
+                     P->colLds, st, P->R.as<float2>(), obj, P->phix.as<float>(), P->phiy.as<float>(),
+                     P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int fr_forward(tomo_fr_plan *P, float *data, const float *obj, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "fr_forward: plan is NULL");
+  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_forward: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
+  const int ng = (int)P->ngrp;
+  const int n2 = 2 * n;
+  hipLaunchKernelGGL(k_fr_colfft_adj, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
+                     dim3(P->colThreads), P->colLds, st, obj, P->R.as<float2>(), P->phix.as<float>(),
+                     P->phiy.as<float>(), P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fr_rowfft_adj, dim3(n2, ng * 4), dim3(P->rowThreads), n2 * sizeof(float2),
+                     st, P->R.as<float2>(), P->F.as<float2>(), P->fft2.d, n, nzc);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fr_interp, dim3((unsigned)ceil_div(P->nsamp, 256), ng), dim3(256), 0, st,
+                     P->geom.as<float4>(), P->F.as<float2>(), P->G.as<float2>(), n, (int)P->nsamp, nzc,
+                     P->K2);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fr_prep_adj, dim3(nproj, ng * 4), dim3(P->prepThreads), n * sizeof(float2), st,
+                     P->G.as<float2>(), data, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" void fr_destroy(tomo_fr_plan *P) {
+  if (!P) return;
+  P->geom.release();
+  P->lists.release();
+  P->offs.release();
+  P->phix.release();
+  P->phiy.release();
+  P->circ.release();
+  P->G.release();
+  P->F.release();
+  P->R.release();
+  P->fft1.tw.release();
+  P->fft2.tw.release();
+  delete P;
+}
+
+extern "C" int fr_get_sizes(const tomo_fr_plan *P, int64_t *n, int64_t *nproj, int64_t *nz) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "fr_get_sizes: plan is NULL");
+  if (n) *n = P->n;
+  if (nproj) *nproj = P->nproj;
+  if (nz) *nz = P->nz;
+  return TOMO_OK;
+}

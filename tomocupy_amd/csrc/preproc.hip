@@ -1,0 +1,198 @@
+// Chunk preprocessing for gfx950: dark/flat correction, minus-log and the swap
+// from projection order (nproj, nz, n) to sinogram order (nz, nproj, n)
+// (tomocupy processing/proc_functions.py:55-83, 140-198; rec.py:172-174).
+#include <cmath>
+#include <string>
+
+#include "common.h"
+
+using namespace tomo;
+
+namespace {
+
+// dark0 = mean(dark); flat means (one, or two halves for flat_linear)
+// (proc_functions.py:55-72).  Means accumulate in FP64.
+__global__ void k_pre_means(const uint16_t *__restrict__ dark, const uint16_t *__restrict__ flat,
+                            float *__restrict__ dark0, float *__restrict__ flatA,
+                            float *__restrict__ flatB, int zn, int ndark, int nflat,
+                            float bright, int flat_linear) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= zn) return;
+  double sd = 0.0;
+  for (int f = 0; f < ndark; ++f) sd += (double)dark[(size_t)f * zn + i];
+  const float d0 = (float)(sd / ndark);
+  dark0[i] = d0;
+  if (!flat_linear) {
+    double sf = 0.0;
+    for (int f = 0; f < nflat; ++f) sf += (double)((float)flat[(size_t)f * zn + i] / bright);
+    flatA[i] = (float)(sf / nflat);
+  } else {
+    const int h = nflat / 2;
+    double sa = 0.0, sb = 0.0;
+    for (int f = 0; f < h; ++f) sa += (double)((float)flat[(size_t)f * zn + i] / bright);
+    for (int f = h + 1; f < nflat; ++f) sb += (double)((float)flat[(size_t)f * zn + i] / bright);
+    flatA[i] = (float)(sa / h);
+    flatB[i] = (float)(sb / (nflat - h - 1));
+  }
+}
+
+__device__ __forceinline__ float darkflat(float x, float d0, float fa, float fb, float v,
+                                          int flat_linear) {
+  const float eps = 1.0f + 1e-5f;  // float32(1 + 1e-5), proc_functions.py:71
+  float f0 = flat_linear ? (1.0f - v) * fa + v * fb : fa;
+  f0 = f0 * eps - d0;
+  return (x - d0) / f0;
+}
+
+__device__ __forceinline__ float minus_log(float x) {
+  // x <= 0 -> 1; -log; NaN -> 6; +-inf -> 0 (proc_functions.py:76-83)
+  if (x <= 0.f) x = 1.f;
+  float r = -logf(x);
+  if (isnan(r)) r = 6.0f;
+  if (isinf(r)) r = 0.0f;
+  return r;
+}
+
+// One workgroup per (p, z) detector row: read data[p][z][:], write out[z][p][:].
+template <int MODE>  // 0: darkflat (+ optional -log) + swap; 1: darkflat only, projection order
+__global__ void k_pre_rows(const uint16_t *__restrict__ data, const float *__restrict__ dark0,
+                           const float *__restrict__ flatA, const float *__restrict__ flatB,
+                           float *__restrict__ out, int nproj, int nz, int n, int flat_linear,
+                           int take_log) {
+  const int p = blockIdx.x, z = blockIdx.y;
+  const uint16_t *src = data + ((size_t)p * nz + z) * n;
+  float *dst = MODE == 0 ? out + ((size_t)z * nproj + p) * n : out + ((size_t)p * nz + z) * n;
+  const float v = nproj > 1 ? (float)((double)p / (double)(nproj - 1)) : 0.f;
+  const float *d0 = dark0 + (size_t)z * n;
+  const float *fa = flatA + (size_t)z * n;
+  const float *fb = flatB + (size_t)z * n;
+  for (int x = threadIdx.x; x < n; x += blockDim.x) {
+    float r = darkflat((float)src[x], d0[x], fa[x], flat_linear ? fb[x] : 0.f, v, flat_linear);
+    if (MODE == 0 && take_log) r = minus_log(r);
+    dst[x] = r;
+  }
+}
+
+__global__ void k_pre_log_swap(const float *__restrict__ res, float *__restrict__ out, int nproj,
+                               int nz, int n, int take_log) {
+  const int p = blockIdx.x, z = blockIdx.y;
+  const float *src = res + ((size_t)p * nz + z) * n;
+  float *dst = out + ((size_t)z * nproj + p) * n;
+  for (int x = threadIdx.x; x < n; x += blockDim.x) {
+    const float r = src[x];
+    dst[x] = take_log ? minus_log(r) : r;
+  }
+}
+
+}  // namespace
+
+struct tomo_pre_plan {
+  int64_t nproj = 0, nz = 0, n = 0, ndark = 0, nflat = 0;
+  DevBuf dark0, flatA, flatB;
+};
+
+extern "C" int pre_create(tomo_pre_plan **plan, int64_t nproj, int64_t nz, int64_t n,
+                          int64_t ndark, int64_t nflat) {
+  clear_error();
+  TOMO_REQUIRE(plan != nullptr, "pre_create: plan pointer is NULL");
+  *plan = nullptr;
+  TOMO_REQUIRE(nproj >= 1 && nz >= 1 && n >= 1, "pre_create: sizes must be >= 1");
+  TOMO_REQUIRE(ndark >= 1 && nflat >= 1, "pre_create: need at least one dark and one flat frame");
+  TOMO_REQUIRE(nproj <= 65535 && nz <= 65535, "pre_create: nproj and nz must be <= 65535");
+  auto *P = new (std::nothrow) tomo_pre_plan();
+  if (!P) {
+    set_error("pre_create: host allocation failed");
+    return TOMO_ENOMEM;
+  }
+  P->nproj = nproj;
+  P->nz = nz;
+  P->n = n;
+  P->ndark = ndark;
+  P->nflat = nflat;
+  int rc = P->dark0.alloc(sizeof(float) * nz * n);
+  if (!rc) rc = P->flatA.alloc(sizeof(float) * nz * n);
+  if (!rc) rc = P->flatB.alloc(sizeof(float) * nz * n);
+  if (rc) {
+    pre_destroy(P);
+    return rc;
+  }
+  *plan = P;
+  return TOMO_OK;
+}
+
+static int launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *flat,
+                        double bright_ratio, int flat_linear, hipStream_t st) {
+  TOMO_REQUIRE(bright_ratio > 0 && std::isfinite(bright_ratio), "bright_ratio must be > 0");
+  TOMO_REQUIRE(!flat_linear || P->nflat >= 3, "flat_linear needs at least 3 flat frames");
+  const int zn = (int)(P->nz * P->n);
+  hipLaunchKernelGGL(k_pre_means, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st, dark, flat,
+                     P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), zn,
+                     (int)P->ndark, (int)P->nflat, (float)bright_ratio, flat_linear);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
+                           const uint16_t *flat, float *out, double bright_ratio, int flat_linear,
+                           int minus_log, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "pre_process: plan is NULL");
+  TOMO_REQUIRE(data && dark && flat && out, "pre_process: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pre_rows<0>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+                     data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), out,
+                     (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, minus_log);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
+                            const uint16_t *flat, float *res, double bright_ratio, int flat_linear,
+                            void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "pre_darkflat: plan is NULL");
+  TOMO_REQUIRE(data && dark && flat && res, "pre_darkflat: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pre_rows<1>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+                     data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), res,
+                     (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, 0);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, float *out, int minus_log,
+                                  void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "pre_minus_log_swap: plan is NULL");
+  TOMO_REQUIRE(res && out, "pre_minus_log_swap: NULL array");
+  TOMO_REQUIRE(res != out, "pre_minus_log_swap: in-place swap is not supported");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pre_log_swap, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+                     res, out, (int)P->nproj, (int)P->nz, (int)P->n, minus_log);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, const char *wname,
+                                    int level, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr && res != nullptr && wname != nullptr,
+               "pre_remove_stripe_fw: NULL argument");
+  (void)sigma;
+  (void)level;
+  (void)stream;
+  set_error("pre_remove_stripe_fw: not implemented in this build");
+  return TOMO_EUNSUPPORTED;
+}
+
+extern "C" void pre_destroy(tomo_pre_plan *P) {
+  if (!P) return;
+  P->dark0.release();
+  P->flatA.release();
+  P->flatB.release();
+  delete P;
+}

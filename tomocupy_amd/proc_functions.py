@@ -1,0 +1,107 @@
+"""Chunk preprocessing with the reference's interface.
+
+Mirrors ``ProcFunctions`` (``src/tomocupy/processing/proc_functions.py:46-198``)
+for the steps on the fourierrec path: ``darkflat_correction``, ``minus_log``,
+``proc_sino`` (dark/flat, stripe removal) and ``proc_proj`` (minus log).
+Settings the reference reads from the global ``args`` are constructor
+arguments.  ``proc_chunk`` is the fused form used by the conveyor: raw uint16
+chunk -> dark/flat -> [stripe removal] -> minus log -> sinogram order.
+"""
+import ctypes
+
+from . import _lib
+
+
+class ProcFunctions:
+    def __init__(self, nproj, ncz, n, ndark, nflat, dtype='float32', minus_log=True,
+                 bright_ratio=1.0, flat_linear=False, remove_stripe_method='none',
+                 fw_sigma=1.0, fw_filter='sym16', fw_level=7):
+        if remove_stripe_method not in ('none', 'fw'):
+            raise NotImplementedError('remove_stripe_method %r is not implemented'
+                                      % remove_stripe_method)
+        self.nproj, self.ncz, self.n = int(nproj), int(ncz), int(n)
+        self.ndark, self.nflat = int(ndark), int(nflat)
+        self.dtype = dtype
+        self.take_minus_log = bool(minus_log)
+        self.bright_ratio = float(bright_ratio)
+        self.flat_linear = bool(flat_linear)
+        self.remove_stripe_method = remove_stripe_method
+        self.fw_sigma, self.fw_filter, self.fw_level = float(fw_sigma), fw_filter, int(fw_level)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().pre_create(ctypes.byref(h), self.nproj, self.ncz, self.n,
+                                         self.ndark, self.nflat), 'pre_create')
+        self._h = h
+
+    def _shapes(self):
+        import torch
+        return ((self.nproj, self.ncz, self.n), (self.ndark, self.ncz, self.n),
+                (self.nflat, self.ncz, self.n), torch.uint16)
+
+    def darkflat_correction(self, data, dark, flat, res=None):
+        import torch
+        sd, sk, sf, u16 = self._shapes()
+        if res is None:
+            res = torch.empty(sd, dtype=torch.float32, device=data.device)
+        _lib.check(_lib.lib().pre_darkflat(
+            self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
+            _lib.dev_ptr(flat, sf, u16, 'flat'), _lib.dev_ptr(res, sd, torch.float32, 'res'),
+            self.bright_ratio, int(self.flat_linear), _lib.stream_ptr(None)), 'pre_darkflat')
+        return res
+
+    def remove_stripe_fw(self, res):
+        import torch
+        sd = (self.nproj, self.ncz, self.n)
+        _lib.check(_lib.lib().pre_remove_stripe_fw(
+            self._h, _lib.dev_ptr(res, sd, torch.float32, 'res'), self.fw_sigma,
+            self.fw_filter.encode(), self.fw_level, _lib.stream_ptr(None)), 'pre_remove_stripe_fw')
+        return res
+
+    def proc_sino(self, data, dark, flat, res=None):
+        """dark/flat correction, then stripe removal (``proc_functions.py:140-161``)."""
+        res = self.darkflat_correction(data, dark, flat, res)
+        if self.remove_stripe_method == 'fw':
+            self.remove_stripe_fw(res)
+        return res
+
+    def proc_proj_swap(self, data, out=None):
+        """minus log (``proc_functions.py:163-198``) fused with the swap to
+        sinogram order (``rec.py:174``): ``(nproj, ncz, n) -> (ncz, nproj, n)``."""
+        import torch
+        sd = (self.nproj, self.ncz, self.n)
+        if out is None:
+            out = torch.empty((self.ncz, self.nproj, self.n), dtype=torch.float32,
+                              device=data.device)
+        _lib.check(_lib.lib().pre_minus_log_swap(
+            self._h, _lib.dev_ptr(data, sd, torch.float32, 'data'),
+            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), torch.float32, 'out'),
+            int(self.take_minus_log), _lib.stream_ptr(None)), 'pre_minus_log_swap')
+        return out
+
+    def proc_chunk(self, data, dark, flat, out=None, res=None):
+        """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, n)``."""
+        import torch
+        sd, sk, sf, u16 = self._shapes()
+        if out is None:
+            out = torch.empty((self.ncz, self.nproj, self.n), dtype=torch.float32,
+                              device=data.device)
+        if self.remove_stripe_method == 'fw':
+            res = self.proc_sino(data, dark, flat, res)
+            return self.proc_proj_swap(res, out)
+        _lib.check(_lib.lib().pre_process(
+            self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
+            _lib.dev_ptr(flat, sf, u16, 'flat'),
+            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), torch.float32, 'out'),
+            self.bright_ratio, int(self.flat_linear), int(self.take_minus_log),
+            _lib.stream_ptr(None)), 'pre_process')
+        return out
+
+    def free(self):
+        h, self._h = getattr(self, '_h', None), None
+        if h is not None and h.value:
+            _lib.lib().pre_destroy(h)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
