@@ -1,0 +1,239 @@
+#!/usr/bin/env python
+"""Throughput of the fourierrec reconstruction path on one MI355X.
+
+Workload (default): n=2048 detector columns, nproj=1500 angles in [0, pi),
+nz=2048 slices processed in chunks of 8 over 3 HIP streams, raw uint16 counts
+with 10 dark and 20 flat frames and column-gain stripes -> dark/flat ->
+[Fourier-wavelet stripe removal] -> -log -> ramp filter (centre 1024.5) ->
+USFFT backprojection.  All inputs are resident in HBM before the timed region.
+One step = the whole nz-slice volume.  Prints one JSON line.
+
+``--gpus N`` with N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "slices/s at 2048^2 x1500 proj, FP32 vs FP16 at rel-L2 vs FP64; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_damp', 'k_fw_syn_cols',
+           'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_fr_prep',
+           'k_fr_grid', 'k_fr_rowfft', 'k_fr_colfft']
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--n', type=int, default=2048)
+    ap.add_argument('--nproj', type=int, default=1500)
+    ap.add_argument('--nz', type=int, default=2048)
+    ap.add_argument('--chunk', type=int, default=8)
+    ap.add_argument('--streams', type=int, default=3)
+    ap.add_argument('--dtype', default='float32')
+    ap.add_argument('--stripe', default='fw', choices=['fw', 'none'])
+    ap.add_argument('--filter', default='ramp')
+    ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
+    ap.add_argument('--distinct', type=int, default=8,
+                    help='distinct synthetic slices, tiled over nz')
+    ap.add_argument('--cpu-baseline', default='on', choices=['on', 'off'])
+    ap.add_argument('--cpu-workers', type=int, default=0, help='0 = min(16, cpu count)')
+    ap.add_argument('--no-prof', action='store_true', help='do not time kernels with events')
+    return ap.parse_args()
+
+
+def algorithmic_bytes(name, a, s):
+    """External I/O of one launch of a kernel at storage precision (bytes)."""
+    n, nproj, ch = a.n, a.nproj, a.chunk
+    nzc = (ch + 1) // 2
+    grp = 4  # complex slices per gridding group
+    if name == 'k_fr_grid':      # read the polar spectra, write the 2n x 2n grid
+        return grp * (nproj * n * 8 + (2 * n) ** 2 * 8)
+    if name == 'k_fr_rowfft':    # read the grid rows, write the pruned half
+        return nzc * ((2 * n) ** 2 * 8 + 2 * n * n * 8)
+    if name == 'k_fr_colfft':    # read the pruned rows, write two real slices per complex slice
+        return nzc * (2 * n * n * 8) + ch * n * n * 4
+    if name == 'k_fr_prep':      # read the real sinograms, write the complex spectra
+        return ch * nproj * n * 4 + grp * nproj * n * 8
+    if name == 'k_flt_rows':     # read + write the sinograms in place
+        return 2 * ch * nproj * n * 4
+    if name == 'k_pre_rows':     # read uint16 counts, write float32 sinograms
+        return ch * nproj * n * (2 + 4)
+    return None
+
+
+def host_data(a):
+    from tomocupy_amd import synth
+    theta = synth.angles(a.nproj)
+    t0 = time.time()
+    d = min(a.distinct, a.chunk)
+    data, dark, flat, _ = synth.raw_counts(2, a.n, theta, d, ndark=10, nflat=20, stripes=True)
+    reps = -(-a.chunk // d)
+    data = np.concatenate([data] * reps, axis=1)[:, :a.chunk]
+    dark = np.concatenate([dark] * reps, axis=1)[:, :a.chunk]
+    flat = np.concatenate([flat] * reps, axis=1)[:, :a.chunk]
+    print('# synthetic data: %d distinct slices in %.1fs' % (d, time.time() - t0),
+          file=sys.stderr, flush=True)
+    return theta, data, dark, flat
+
+
+def device_data(a, torch, data, dark, flat):
+    nchunks = -(-a.nz // a.chunk)
+    t0 = time.time()
+    g = torch.from_numpy(data).cuda()
+    raw = torch.empty((nchunks,) + tuple(g.shape), dtype=torch.uint16, device='cuda')
+    raw[:] = g  # every chunk at its own address (no cache reuse across chunks)
+    dk = torch.from_numpy(dark).cuda().expand((nchunks,) + dark.shape).contiguous()
+    fl = torch.from_numpy(flat).cuda().expand((nchunks,) + flat.shape).contiguous()
+    torch.cuda.synchronize()
+    print('# tiled to %d chunks on the GPU in %.1fs' % (nchunks, time.time() - t0),
+          file=sys.stderr, flush=True)
+    return raw, dk, fl
+
+
+def cpu_baseline(a, data, dark, flat, theta, center):
+    """FP64 oracle chain on the host: a process pool over complex slice pairs."""
+    import multiprocessing as mp
+    workers = a.cpu_workers or min(16, os.cpu_count() or 1)
+    pairs = workers
+    sl = []
+    for i in range(pairs):
+        z = (2 * i) % data.shape[1]
+        idx = [z, (z + 1) % data.shape[1]]
+        sl.append((data[:, idx], dark[:, idx], flat[:, idx]))
+    args = [(s, theta, center, a) for s in sl]
+    # runs before this process initialises the GPU, so forked workers are plain CPU processes
+    ctx = mp.get_context('fork')
+    t0 = time.time()
+    with ctx.Pool(workers) as pool:
+        pool.map(_cpu_pair, args)
+    dt = time.time() - t0
+    return {'value': round(2 * pairs / dt, 3), 'unit': 'slices/s', 'cores': workers,
+            'kind': 'port',
+            'sample': '%d complex slice pairs (%d slices) of the same workload through the FP64 '
+                      'oracle (dark/flat, %s-log, %s filter, USFFT backprojection), one pair '
+                      'per process, %.1fs wall' % (pairs, 2 * pairs,
+                                                   'fw stripe removal, ' if a.stripe == 'fw' else '',
+                                                   a.filter, dt)}
+
+
+def _cpu_pair(arg):
+    (data, dark, flat), theta, center, a = arg
+    os.environ['OMP_NUM_THREADS'] = '1'
+    from oracle import filter_fp64 as ff
+    from oracle import fourierrec_fp64 as fr
+    from oracle import preproc_fp64 as pp
+    sino = pp.proc_chain(data, dark, flat, stripe=a.stripe)
+    ne = 4 * a.n
+    filt = ff.fbp_filter_center(sino, ff.calc_filter(ne, a.filter), ne, center)
+    fr.backprojection(filt, theta)
+    return 0
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one GPU only (this build has no multi-GPU path); got --gpus %d'
+              % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    center = a.center if a.center is not None else a.n / 2 + 0.5
+    theta, h_data, h_dark, h_flat = host_data(a)
+    cpu = None
+    if a.cpu_baseline == 'on':
+        print('# cpu baseline...', file=sys.stderr, flush=True)
+        cpu = cpu_baseline(a, h_data, h_dark, h_flat, theta, center)
+        print('# cpu baseline %s' % cpu, file=sys.stderr, flush=True)
+    import torch
+    from tomocupy_amd import BackprojFunctions, ProcFunctions, _lib
+    raw, dark, flat = device_data(a, torch, h_data, h_dark, h_flat)
+    nchunks = raw.shape[0]
+    th = torch.from_numpy(theta).cuda()
+    streams = [torch.cuda.Stream() for _ in range(a.streams)]
+    pfs, bfs, sinos = [], [], []
+    t0 = time.time()
+    for s in streams:
+        with torch.cuda.stream(s):
+            pfs.append(ProcFunctions(a.nproj, a.chunk, a.n, dark.shape[1],
+                                     flat.shape[1], a.dtype, remove_stripe_method=a.stripe,
+                                     fw_sigma=1.0, fw_filter='sym16', fw_level=7))
+            bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, a.dtype, a.filter))
+            sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=torch.float32, device='cuda'))
+    rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=torch.float32, device='cuda')
+    torch.cuda.synchronize()
+    print('# plans ready in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
+
+    def step():
+        for k in range(nchunks):
+            i = k % len(streams)
+            with torch.cuda.stream(streams[i]):
+                out = pfs[i].proc_chunk(raw[k], dark[k], flat[k], out=sinos[i])
+                bfs[i].fbp_filter_center(out)
+                bfs[i].cl_rec.backprojection(rec[k * a.chunk:(k + 1) * a.chunk], out)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.prof_reset()
+    if not a.no_prof:
+        _lib.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step()
+        torch.cuda.synchronize()
+        print('# step %d done' % i, file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    _lib.prof_enable(False)
+    dt = (t1 - t0) / a.steps
+    slices = a.nz
+    value = slices / dt
+
+    kern = {}
+    for k in KERNELS:
+        ms, cnt = _lib.prof_read(k)
+        if cnt:
+            kern[k] = {'avg_ms': ms / cnt, 'launches': cnt, 'total_ms_per_step': ms / a.steps}
+    roof = None
+    if kern:
+        dom = max(kern, key=lambda k: kern[k]['total_ms_per_step'])
+        ab = algorithmic_bytes(dom, a, None)
+        if ab is not None:
+            ach = ab / (kern[dom]['avg_ms'] * 1e-3) / 1e9
+            traffic = None
+            tf = os.path.join(ROOT, 'profiles', 'traffic.json')
+            if os.path.exists(tf):
+                tj = json.load(open(tf))
+                key = '%s@n%d_nproj%d_chunk%d' % (dom, a.n, a.nproj, a.chunk)
+                traffic = tj.get(key)
+            roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1),
+                    'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
+                    'traffic': traffic, 'algorithmic_bytes_per_launch': ab}
+    line = {
+        'metric': METRIC, 'value': round(value, 2), 'unit': 'slices/s', 'n_gpus': 1,
+        'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(dt * 1e3, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': {'float32': 'f32', 'float16': 'f16', 'bfloat16': 'bf16'}[a.dtype],
+        'data': 'synthetic: %d distinct seeded ellipse-phantom slices (uint16 counts, 10 dark, '
+                '20 flat, column-gain stripes) tiled over nz' % min(a.distinct, a.chunk),
+        'config': {'workload': 'fourierrec full chunk pipeline: n=%d nproj=%d nz=%d chunk=%d '
+                               'streams=%d stripe=%s filter=%s centre=%g'
+                               % (a.n, a.nproj, a.nz, a.chunk, a.streams, a.stripe, a.filter,
+                                  center),
+                   'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk,
+                   'streams': a.streams, 'stripe': a.stripe, 'filter': a.filter,
+                   'timed_with_kernel_events': not a.no_prof},
+        'roofline': roof, 'cpu_baseline': cpu, 'kernels': kern,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

@@ -41,6 +41,17 @@ const char *tomo_last_error(void);
 /* Library version string. */
 const char *tomo_version(void);
 
+/*
+ * Kernel timing for measurement (no counterpart in the reference, which has no
+ * profiling hooks).  While enabled, each kernel launch is bracketed by two HIP
+ * events on its own stream; tomo_prof_read() waits for the recorded events and
+ * returns the accumulated milliseconds and launch count of one kernel by name
+ * (e.g. "k_fr_grid").  tomo_prof_reset() clears the accumulators.
+ */
+int tomo_prof_enable(int on);
+int tomo_prof_reset(void);
+int tomo_prof_read(const char *kernel, double *total_ms, int64_t *launches);
+
 /* ------------------------------------------------------------------------ */
 /* USFFT backprojector                                                      */
 /* ------------------------------------------------------------------------ */

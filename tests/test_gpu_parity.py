@@ -188,3 +188,119 @@ def test_config1_full_chain():
     err = _rel_l2_per_slice(obj.cpu().numpy(), ref)
     print('config-1 rel-L2 max %.3g median %.3g' % (err.max(), np.median(err)))
     assert np.all(err <= RTOL32), err
+
+
+@pytest.mark.parametrize("wname,level,sigma,shape", [('sym16', 7, 1.0, (180, 3, 256)),
+                                                     ('db5', 4, 2.0, (97, 2, 130)),
+                                                     ('haar', 3, 1.0, (64, 2, 64)),
+                                                     ('sym5', 5, 1.5, (120, 1, 100))])
+def test_remove_stripe_fw_matches_oracle(wname, level, sigma, shape):
+    from tomocupy_amd import ProcFunctions
+    nproj, nz, n = shape
+    print('fw case', wname, level, sigma, shape, flush=True)
+    rng = np.random.default_rng(level)
+    yy = np.arange(nproj)[:, None, None]
+    xx = np.arange(n)[None, None, :]
+    data = 1 + 0.2 * np.sin(yy / 7.0) * np.cos(xx / 5.0) + 0.05 * rng.standard_normal((nproj, nz, n))
+    data += 0.1 * (rng.random((1, nz, n)) < 0.05)  # column stripes
+    ref = pp.remove_stripe_fw(data, sigma, wname, level)
+    pf = ProcFunctions(nproj, nz, n, 1, 1, remove_stripe_method='fw', fw_sigma=sigma,
+                       fw_filter=wname, fw_level=level)
+    d = _cuda(data)
+    pf.remove_stripe_fw(d)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    err = _rel_l2_per_slice(got.swapaxes(0, 1), ref.swapaxes(0, 1))
+    print('fw rel-L2', err)
+    assert np.all(err <= RTOL32), err
+
+
+def test_config1_chain_with_fw():
+    """Config-1 shape with Fourier-wavelet stripe removal (sym16, level 7,
+    sigma 1), column stripes and the ramp filter at centre n/2 + 0.5, on a
+    well-exposed phantom (attenuation x0.25, no photon starvation)."""
+    from tomocupy_amd import BackprojFunctions, ProcFunctions
+    n, nproj, nz = 512, 720, 4
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(2, n, theta, nz, stripes=True, atten=0.25)
+    sino = pp.proc_chain(data, dark, flat, stripe='fw')
+    ne = 4 * n
+    filt = ff.fbp_filter_center(sino, ff.calc_filter(ne, 'ramp'), ne, n / 2 + 0.5)
+    ref = fr.backprojection(filt, theta)
+    pf = ProcFunctions(nproj, nz, n, dark.shape[0], flat.shape[0], remove_stripe_method='fw')
+    bf = BackprojFunctions(n, nproj, nz, _cuda(theta), n / 2 + 0.5, 'float32', 'ramp')
+    out = pf.proc_chunk(_cuda(data, torch.uint16), _cuda(dark, torch.uint16), _cuda(flat, torch.uint16))
+    bf.fbp_filter_center(out)
+    obj = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    bf.cl_rec.backprojection(obj, out)
+    torch.cuda.synchronize()
+    err = _rel_l2_per_slice(obj.cpu().numpy(), ref)
+    print('config-1+fw rel-L2 max %.3g median %.3g' % (err.max(), np.median(err)))
+    assert np.all(err <= RTOL32), err
+
+
+def test_starved_chain_with_fw_stagewise():
+    """Photon-starved counts (the default attenuation): after fw, pixels with
+    near-zero transmission make minus_log's ``x <= 0 -> 1`` clamp flip on
+    rounding-level differences, so the chain is ill-conditioned end to end.
+    Parity is checked stage by stage instead: the fw output before the log
+    against the oracle, then -log + filter + backprojection of that same
+    float32 input against the oracle."""
+    from tomocupy_amd import BackprojFunctions, ProcFunctions
+    n, nproj, nz = 512, 720, 4
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(2, n, theta, nz, stripes=True)
+    pf = ProcFunctions(nproj, nz, n, dark.shape[0], flat.shape[0], remove_stripe_method='fw')
+    res = pf.proc_sino(_cuda(data, torch.uint16), _cuda(dark, torch.uint16), _cuda(flat, torch.uint16))
+    torch.cuda.synchronize()
+    res_np = res.cpu().numpy()
+    ref_res = pp.remove_stripe_fw(pp.darkflat_correction(data, dark, flat), 1.0, 'sym16', 7)
+    err = _rel_l2_per_slice(res_np.swapaxes(0, 1), ref_res.swapaxes(0, 1))
+    print('starved fw pre-log rel-L2', err)
+    assert np.all(err <= RTOL32), err
+    out = pf.proc_proj_swap(res)
+    bf = BackprojFunctions(n, nproj, nz, _cuda(theta), n / 2 + 0.5, 'float32', 'ramp')
+    bf.fbp_filter_center(out)
+    obj = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    bf.cl_rec.backprojection(obj, out)
+    torch.cuda.synchronize()
+    sino = np.ascontiguousarray(pp.minus_log(res_np.astype(np.float64)).swapaxes(0, 1))
+    ne = 4 * n
+    ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'ramp'), ne, n / 2 + 0.5),
+                            theta)
+    err = _rel_l2_per_slice(obj.cpu().numpy(), ref)
+    print('starved post-fw chain rel-L2', err)
+    assert np.all(err <= RTOL32), err
+
+
+def test_conveyor_recon_all_matches_oracle():
+    """GPURec.recon_all (3 streams, pinned double buffers, writer threads) over
+    3 chunks of 4 slices with a partial last chunk, against the oracle chain
+    applied chunk by chunk (the pairing is within a chunk)."""
+    from tomocupy_amd.rec import GPURec
+    n, nproj, nz, ncz = 128, 90, 10, 4
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(3, n, theta, nz, ndark=3, nflat=5)
+    rec = GPURec(n, nproj, nz, ncz, _cuda(theta), n / 2, 3, 5, fbp_filter='parzen',
+                 max_write_threads=2)
+    out = np.full((nz, n, n), np.nan, np.float32)
+
+    def read(k):
+        s = slice(k * ncz, min(nz, (k + 1) * ncz))
+        return data[:, s], dark[:, s], flat[:, s]
+
+    def write(k, r):
+        out[k * ncz:k * ncz + len(r)] = r
+
+    rec.recon_all(read, write)
+    ne = 4 * n
+    w = ff.calc_filter(ne, 'parzen')
+    for k in range(3):
+        s = slice(k * ncz, min(nz, (k + 1) * ncz))
+        sino = pp.proc_chain(data[:, s], dark[:, s], flat[:, s])
+        lz = sino.shape[0]
+        sino = np.concatenate([sino, np.zeros((ncz - lz, nproj, n))])
+        ref = fr.backprojection(ff.fbp_filter_center(sino, w, ne, n / 2), theta)[:lz]
+        err = _rel_l2_per_slice(out[s], ref)
+        print('chunk', k, err)
+        assert np.all(err <= RTOL32), err

@@ -25,6 +25,10 @@ _c_d = ctypes.c_double
 SIGNATURES = {
     'tomo_last_error': (ctypes.c_char_p, []),
     'tomo_version': (ctypes.c_char_p, []),
+    'tomo_prof_enable': (_c_int, [_c_int]),
+    'tomo_prof_reset': (_c_int, []),
+    'tomo_prof_read': (_c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(_c_i64)]),
     'fr_create': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_p]),
     'fr_backprojection': (_c_int, [_c_p, _c_p, _c_p, _c_p]),
     'fr_forward': (_c_int, [_c_p, _c_p, _c_p, _c_p]),
@@ -126,3 +130,20 @@ def calc_filter(ne, name):
     out = np.zeros(ne // 2 + 1, dtype=np.float64)
     check(lib().flt_calc_filter(ne, name.encode(), out.ctypes.data), 'flt_calc_filter')
     return out
+
+
+def prof_enable(on=True):
+    lib().tomo_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    lib().tomo_prof_reset()
+
+
+def prof_read(kernel):
+    """(total milliseconds, launches) of one kernel since the last reset."""
+    ms = ctypes.c_double()
+    cnt = _c_i64()
+    check(lib().tomo_prof_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(cnt)),
+          'tomo_prof_read')
+    return ms.value, cnt.value

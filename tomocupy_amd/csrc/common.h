@@ -79,4 +79,29 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Largest dynamic LDS a workgroup may use on the current device.
 size_t max_lds_per_block();
 
+// ---- kernel timing (tomo_prof_*) -----------------------------------------
+// When enabled, every launch wrapped in a ProfScope is bracketed by two HIP
+// events recorded on the launch stream; durations are collected lazily by
+// tomo_prof_read().
+bool prof_enabled();
+void prof_push(const char *name, hipEvent_t a, hipEvent_t b);
+
+struct ProfScope {
+  const char *name;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(const char *nm, hipStream_t s) : name(nm), st(s) {
+    if (prof_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, st);
+    else
+      a = b = nullptr;
+  }
+  ~ProfScope() {
+    if (a && b) {
+      (void)hipEventRecord(b, st);
+      prof_push(name, a, b);
+    }
+  }
+};
+
 }  // namespace tomo

@@ -1,6 +1,10 @@
 // Status handling, version string and FFT plan construction for libtomohip.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 
 #include "common.h"
 #include "fft_lds.h"
@@ -32,8 +36,9 @@ int FftPlan::init(int N) {
   d.N = N;
   d.nr = 0;
   int m = N;
-  // radix 8 first (fewest passes), then 4, 2, then 3s
-  while (m % 8 == 0 && m > 8) { d.r[d.nr++] = 8; m /= 8; }
+  // radix 16 first (fewest passes), then 8, 4, 2, then 3s
+  while (m % 16 == 0) { d.r[d.nr++] = 16; m /= 16; }
+  while (m % 8 == 0) { d.r[d.nr++] = 8; m /= 8; }
   while (m % 4 == 0) { d.r[d.nr++] = 4; m /= 4; }
   while (m % 2 == 0) { d.r[d.nr++] = 2; m /= 2; }
   while (m % 3 == 0) { d.r[d.nr++] = 3; m /= 3; }
@@ -49,7 +54,61 @@ int FftPlan::init(int N) {
   return TOMO_OK;
 }
 
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> g_prof_pending;
+static std::map<std::string, std::pair<double, int64_t>> g_prof_acc;
+
+bool prof_enabled() { return g_prof_on; }
+
+void prof_push(const char *name, hipEvent_t a, hipEvent_t b) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_pending.emplace_back(name, a, b);
+}
+
+static int prof_drain() {
+  for (auto &e : g_prof_pending) {
+    hipEvent_t a = std::get<1>(e), b = std::get<2>(e);
+    float ms = 0.f;
+    hipError_t err = hipEventSynchronize(b);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+    if (err == hipSuccess) {
+      auto &acc = g_prof_acc[std::get<0>(e)];
+      acc.first += ms;
+      acc.second += 1;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  g_prof_pending.clear();
+  return TOMO_OK;
+}
+
 }  // namespace tomo
+
+extern "C" int tomo_prof_enable(int on) {
+  std::lock_guard<std::mutex> lk(tomo::g_prof_mu);
+  tomo::g_prof_on = on != 0;
+  return TOMO_OK;
+}
+
+extern "C" int tomo_prof_reset(void) {
+  std::lock_guard<std::mutex> lk(tomo::g_prof_mu);
+  tomo::prof_drain();
+  tomo::g_prof_acc.clear();
+  return TOMO_OK;
+}
+
+extern "C" int tomo_prof_read(const char *kernel, double *total_ms, int64_t *launches) {
+  tomo::clear_error();
+  TOMO_REQUIRE(kernel != nullptr, "tomo_prof_read: kernel name is NULL");
+  std::lock_guard<std::mutex> lk(tomo::g_prof_mu);
+  tomo::prof_drain();
+  auto it = tomo::g_prof_acc.find(kernel);
+  if (total_ms) *total_ms = it == tomo::g_prof_acc.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == tomo::g_prof_acc.end() ? 0 : it->second.second;
+  return TOMO_OK;
+}
 
 extern "C" const char *tomo_last_error(void) { return tomo::g_err; }
 extern "C" const char *tomo_version(void) { return "tomohip 0.1.0 (gfx950)"; }

@@ -83,8 +83,52 @@ __device__ __forceinline__ void butterfly(float2 *a) {
     a[6] = csub(e[2], o2);
     a[3] = cadd(e[3], o3);
     a[7] = csub(e[3], o3);
+  } else if constexpr (R == 16) {
+    // 16 = 4 x 4 in place: radix-4 over r2 for each r1 (results q2 back in
+    // slots r1 + 4 q2), internal twiddles w16^(r1 q2), radix-4 over r1 for each
+    // q2 (results q1 in slots 4 q2 + q1).  Output q = q2 + 4 q1 therefore sits
+    // in slot out_slot<16>(q) = 4 (q & 3) + (q >> 2).
+#pragma unroll
+    for (int r1 = 0; r1 < 4; ++r1) {
+      float2 t[4] = {a[r1], a[r1 + 4], a[r1 + 8], a[r1 + 12]};
+      butterfly<4, DIR>(t);
+      a[r1] = t[0];
+      a[r1 + 4] = t[1];
+      a[r1 + 8] = t[2];
+      a[r1 + 12] = t[3];
+    }
+    // w16^m = exp(DIR * 2 pi i m / 16), m = r1*q2 in {1,2,3,4,6,9}
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float h = 0.70710678118654752440f;
+    const float2 w1 = make_float2(c1, DIR * s1), w2 = make_float2(h, DIR * h);
+    const float2 w3 = make_float2(s1, DIR * c1);
+    const float2 w6 = make_float2(-h, DIR * h), w9 = make_float2(-c1, -DIR * s1);
+    a[1 + 4 * 1] = cmul(a[1 + 4 * 1], w1);
+    a[1 + 4 * 2] = cmul(a[1 + 4 * 2], w2);
+    a[1 + 4 * 3] = cmul(a[1 + 4 * 3], w3);
+    a[2 + 4 * 1] = cmul(a[2 + 4 * 1], w2);
+    a[2 + 4 * 2] = mul_i<DIR>(a[2 + 4 * 2]);  // w16^4 = exp(DIR i pi/2) = DIR * i
+    a[2 + 4 * 3] = cmul(a[2 + 4 * 3], w6);
+    a[3 + 4 * 1] = cmul(a[3 + 4 * 1], w3);
+    a[3 + 4 * 2] = cmul(a[3 + 4 * 2], w6);
+    a[3 + 4 * 3] = cmul(a[3 + 4 * 3], w9);
+#pragma unroll
+    for (int q2 = 0; q2 < 4; ++q2) butterfly<4, DIR>(a + 4 * q2);
   }
 }
+
+// Register slot holding output q of butterfly<R>: the identity except for the
+// in-place radix-16 butterfly.
+template <int R>
+__device__ __forceinline__ constexpr int out_slot(int q) {
+  return R == 16 ? 4 * (q & 3) + (q >> 2) : q;
+}
+
+// LDS index of element i of a transform: one pad slot per 16 elements, which
+// breaks the power-of-two strides of the early Stockham passes across banks.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+// LDS elements needed for a padded transform of length N.
+__host__ __device__ inline int lds_padded_len(int N) { return N + (N >> 4); }
 
 template <int R, int DIR>
 __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float2 *tw, int lgH,
@@ -117,13 +161,13 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
     if (g < nb) {
       const int t = g & (H - 1), j = g >> lgH;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = buf[((j + r * nbPer) << lgH) + t];
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(lds_pad(j + r * nbPer) << lgH) + t];
       if (Ns > 1) {
         int jq, k;
         split(j, jq, k);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[r * k * twstep];
+          float2 w = tw[(unsigned)(r * k * twstep)];
           if (DIR > 0) w.y = -w.y;
           v[q][r] = cmul(v[q][r], w);
         }
@@ -141,14 +185,14 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
       split(j, jq, k);
       const int o = jq * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[((o + r * Ns) << lgH) + t] = v[q][r];
+      for (int r = 0; r < R; ++r) buf[(lds_pad(o + r * Ns) << lgH) + t] = v[q][out_slot<R>(r)];
     }
   }
   __syncthreads();
 }
 
 // In-place transform of 2^lgH interleaved sequences of length d.N held in LDS:
-// element i of sequence t lives at buf[i * 2^lgH + t].  Requires
+// element i of sequence t lives at buf[lds_pad(i) * 2^lgH + t].  Requires
 // 2^lgH * N <= 15 * nt (16 * nt without radix-3 passes); see fft_threads().
 // The caller must __syncthreads() after filling buf; the function ends with a
 // barrier.
@@ -157,7 +201,9 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
   int Ns = 1;
   for (int s = 0; s < d.nr; ++s) {
     const int R = d.r[s];
-    if (R == 8)
+    if (R == 16)
+      fft_pass<16, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    else if (R == 8)
       fft_pass<8, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 4)
       fft_pass<4, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);

@@ -55,7 +55,7 @@ __global__ void k_flt_rows(float *__restrict__ g, const float2 *__restrict__ w, 
   for (int j = tid; j < ne; j += nt) {
     int src = j;
     if (MODE == 1) src = min(max(j - pad, 0), n - 1);
-    sm[j] = make_float2(r0[src], has1 ? r1[src] : 0.f);
+    sm[lds_pad(j)] = make_float2(r0[src], has1 ? r1[src] : 0.f);
   }
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
@@ -70,19 +70,21 @@ __global__ void k_flt_rows(float *__restrict__ g, const float2 *__restrict__ w, 
     } else {
       W = cconj(wz[ne - k]);
     }
-    sm[k] = cmul(sm[k], W);
+    sm[lds_pad(k)] = cmul(sm[lds_pad(k)], W);
   }
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
   if (MODE == 0) {
     for (int j = tid; j < ne; j += nt) {
-      r0[j] = sm[j].x;
-      if (has1) r1[j] = sm[j].y;
+      const float2 y = sm[lds_pad(j)];
+      r0[j] = y.x;
+      if (has1) r1[j] = y.y;
     }
   } else {
     for (int j = tid; j < n; j += nt) {
-      r0[j] = sm[pad + j].x;
-      if (has1) r1[j] = sm[pad + j].y;
+      const float2 y = sm[lds_pad(pad + j)];
+      r0[j] = y.x;
+      if (has1) r1[j] = y.y;
     }
   }
 }
@@ -267,7 +269,7 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   }
   P->threads = fft_threads(P->fft.d, ne);
   TOMO_REQUIRE(P->threads > 0, "flt_create: ne=%lld too large for one workgroup", (long long)ne);
-  P->lds = sizeof(float2) * ne;
+  P->lds = sizeof(float2) * lds_padded_len((int)ne);
   TOMO_REQUIRE(P->lds <= max_lds_per_block(), "flt_create: ne=%lld needs %zu B of LDS",
                (long long)ne, P->lds);
   hipError_t e = hipFuncSetAttribute((const void *)k_flt_rows<0>,
@@ -289,9 +291,12 @@ extern "C" int flt_filter(tomo_flt_plan *P, float *g, const float *w, void *stre
   TOMO_REQUIRE(P != nullptr, "flt_filter: plan is NULL");
   TOMO_REQUIRE(g != nullptr && w != nullptr, "flt_filter: NULL array");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_flt_rows<0>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
-                     dim3(P->threads), P->lds, st, g, reinterpret_cast<const float2 *>(w), P->fft.d,
-                     (int)P->ne, (int)P->n, (int)P->nproj);
+  {
+    ProfScope ps_("k_flt_rows", st);
+    hipLaunchKernelGGL(k_flt_rows<0>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
+                       dim3(P->threads), P->lds, st, g, reinterpret_cast<const float2 *>(w), P->fft.d,
+                       (int)P->ne, (int)P->n, (int)P->nproj);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
@@ -304,12 +309,18 @@ extern "C" int flt_filter_center(tomo_flt_plan *P, float *data, const float *wfi
   TOMO_REQUIRE(std::isfinite(center), "flt_filter_center: center is not finite");
   hipStream_t st = (hipStream_t)stream;
   const int nh = (int)(P->ne / 2 + 1);
-  hipLaunchKernelGGL(k_flt_weights, dim3((unsigned)ceil_div(nh, 256), (unsigned)P->nz), dim3(256), 0,
-                     st, wfilter, sht, P->w.as<float2>(), (int)P->ne, (int)P->n, (int)P->nz, center);
+  {
+    ProfScope ps_("k_flt_weights", st);
+    hipLaunchKernelGGL(k_flt_weights, dim3((unsigned)ceil_div(nh, 256), (unsigned)P->nz), dim3(256), 0,
+                       st, wfilter, sht, P->w.as<float2>(), (int)P->ne, (int)P->n, (int)P->nz, center);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_flt_rows<1>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
-                     dim3(P->threads), P->lds, st, data, P->w.as<float2>(), P->fft.d, (int)P->ne,
-                     (int)P->n, (int)P->nproj);
+  {
+    ProfScope ps_("k_flt_rows", st);
+    hipLaunchKernelGGL(k_flt_rows<1>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
+                       dim3(P->threads), P->lds, st, data, P->w.as<float2>(), P->fft.d, (int)P->ne,
+                       (int)P->n, (int)P->nproj);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }

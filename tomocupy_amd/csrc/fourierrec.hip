@@ -5,9 +5,10 @@
 //   k_fr_prep    data (nz,nproj,n) f32 -> pair-pack, shifted n-point DFT,
 //                scale (4/n)*c0            -> G [grp][p*n+k][4] complex
 //   k_fr_grid    deterministic gather of the polar samples onto the
-//                periodic 2n x 2n grid: one wave per 8x8 cell block walks the
-//                block's sample list (built on the host in FP64) in a fixed
-//                order and accumulates in registers -> F [c][a][b]
+//                periodic 2n x 2n grid as 16x16-tile matrix products on the
+//                matrix cores (tile sample lists built on the host in FP64),
+//                + k_fr_grid_reduce for tiles split over several waves
+//                                              -> F [c][a][b]
 //   k_fr_rowfft  shifted inverse FFT along b, pruned to the n centre columns
 //                -> R [c][a][v']
 //   k_fr_colfft  shifted inverse FFT along a, pruned to the n rows divphi
@@ -28,7 +29,8 @@ using namespace tomo;
 
 namespace {
 
-constexpr int kGridWaves = 4;  // waves (= 8x8 blocks) per gather workgroup
+constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
+constexpr int kGridChunk = 512; // max list entries per work item (multiple of 4)
 
 // Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
@@ -49,56 +51,110 @@ __global__ void k_fr_prep(const float *__restrict__ data, float2 *__restrict__ G
   const bool has2 = z2 < nz;
   for (int j = tid; j < n; j += nt) {
     const float s = shift_sign(j);
-    sm[j] = make_float2(s * r1[j], has2 ? s * r2[j] : 0.f);
+    sm[lds_pad(j)] = make_float2(s * r1[j], has2 ? s * r2[j] : 0.f);
   }
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
-  for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[k], shift_sign(k) * scale);
+  for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[lds_pad(k)], shift_sign(k) * scale);
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Gridding as a matrix product on the matrix cores.  For one 16x16 cell tile
+// (rows a, columns b of the periodic 2n x 2n grid) and four polar samples s:
+//   F[a][b] += sum_s (E1_s(a) * G_s) * E0_s(b)
+// with the separable Gaussian factors E1 (rows) and E0 (columns): A = E1*G is
+// 16 x 4, B = E0 is 4 x 16, one v_mfma_f32_16x16x4f32 per slice and re/im
+// part.  Lane l holds A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+r][l%16] in
+// register r.  A wave walks one work item (a tile's sample list or a bounded
+// chunk of it) in a fixed order; chunks of heavy tiles land in private
+// partial buffers that k_fr_grid_reduce sums in chunk order, so the result is
+// bitwise reproducible.
 __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
-    const float4 *__restrict__ geom, const int *__restrict__ lists, const int *__restrict__ offs,
-    const float2 *__restrict__ G, float2 *__restrict__ F, int n, int nblk, int nsamp, int nzc,
-    float K2) {
+    const int4 *__restrict__ items, int nitems, const int2 *__restrict__ ent,
+    const float4 *__restrict__ geom, const float2 *__restrict__ G, float2 *__restrict__ F,
+    float *__restrict__ part, int n, int ntile, int nslots, int nsamp, int nzc, float K2) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int blk = blockIdx.x * kGridWaves + wave;
+  const int it = blockIdx.x * kGridWaves + wave;
   const int grp = blockIdx.y;
-  if (blk >= nblk * nblk) return;
-  const int A = blk / nblk, B = blk - A * nblk;
-  const int a = A * 8 + (lane >> 3), b = B * 8 + (lane & 7);
-  const int n2 = 2 * n;
-  float2 acc0 = {0.f, 0.f}, acc1 = {0.f, 0.f}, acc2 = {0.f, 0.f}, acc3 = {0.f, 0.f};
+  if (it >= nitems) return;
+  const int4 item = items[it];
+  const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
+  const int k = lane >> 4, i = lane & 15;
   const float4 *Gg = reinterpret_cast<const float4 *>(G + (size_t)grp * nsamp * 4);
-  const int e0 = offs[blk], e1 = offs[blk + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int s = __builtin_amdgcn_readfirstlane(lists[e]);
-    const float4 g = geom[s];  // du, dv, b0, b1
-    int d0 = b - n - (int)g.z;
-    d0 += (d0 < 0) ? n2 : 0;
-    d0 -= (d0 >= n2) ? n2 : 0;
-    int d1 = a - n - (int)g.w;
-    d1 += (d1 < 0) ? n2 : 0;
-    d1 -= (d1 >= n2) ? n2 : 0;
-    const float x0 = (float)(d0 - 4) - g.x;
-    const float x1 = (float)(d1 - 4) - g.y;
-    float w = exp2f(K2 * (x0 * x0 + x1 * x1));
-    w = ((unsigned)d0 <= 8u && (unsigned)d1 <= 8u) ? w : 0.f;
-    const float4 v01 = Gg[(size_t)s * 2];
-    const float4 v23 = Gg[(size_t)s * 2 + 1];
-    acc0.x += w * v01.x; acc0.y += w * v01.y;
-    acc1.x += w * v01.z; acc1.y += w * v01.w;
-    acc2.x += w * v23.x; acc2.y += w * v23.y;
-    acc3.x += w * v23.z; acc3.y += w * v23.w;
+  floatx4 acc[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int e = ebeg; e < ebeg + cnt; e += 4) {
+    const int2 en = ent[e + k];
+    float e0v = 0.f, e1v = 0.f;
+    float4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
+    if (en.x >= 0) {
+      const float4 g = geom[en.x];
+      const int rb0 = (int)(signed char)(en.y & 0xff);
+      const int rb1 = (int)(signed char)((en.y >> 8) & 0xff);
+      const int d0 = i - rb0, d1 = i - rb1;
+      const float x0 = (float)(d0 - 4) - g.x, x1 = (float)(d1 - 4) - g.y;
+      e0v = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
+      e1v = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
+      va = Gg[(size_t)en.x * 2];
+      vb = Gg[(size_t)en.x * 2 + 1];
+    }
+    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.x, e0v, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.y, e0v, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.z, e0v, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.w, e0v, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.x, e0v, acc[4], 0, 0, 0);
+    acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.y, e0v, acc[5], 0, 0, 0);
+    acc[6] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.z, e0v, acc[6], 0, 0, 0);
+    acc[7] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.w, e0v, acc[7], 0, 0, 0);
   }
-  if (a < n2 && b < n2) {
+  const int j = lane & 15, ib = 4 * (lane >> 4);
+  if (slot < 0) {
+    const int n2 = 2 * n;
+    const int TA = tile / ntile, TB = tile - TA * ntile;
+    const int b = 16 * TB + j;
     const size_t plane = (size_t)n2 * n2;
-    const size_t o = (size_t)grp * 4 * plane + (size_t)a * n2 + b;
-    const int cbase = grp * 4;
-    if (cbase + 0 < nzc) F[o] = acc0;
-    if (cbase + 1 < nzc) F[o + plane] = acc1;
-    if (cbase + 2 < nzc) F[o + 2 * plane] = acc2;
-    if (cbase + 3 < nzc) F[o + 3 * plane] = acc3;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (grp * 4 + c >= nzc) continue;
+      float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = 16 * TA + ib + r;
+        if (a < n2 && b < n2) Fc[(size_t)a * n2 + b] = make_float2(acc[2 * c][r], acc[2 * c + 1][r]);
+      }
+    }
+  } else {
+    float *P = part + ((size_t)grp * nslots + slot) * 2048;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) P[m * 256 + (ib + r) * 16 + j] = acc[m][r];
   }
+}
+
+// Sums the partial tiles of heavily loaded tiles in chunk order.
+__global__ void k_fr_grid_reduce(const int4 *__restrict__ red, const float *__restrict__ part,
+                                 float2 *__restrict__ F, int n, int ntile, int nslots, int nzc) {
+  const int4 r = red[blockIdx.x];
+  const int grp = blockIdx.y;
+  const int cell = threadIdx.x;  // 256 threads = 16 x 16 cells
+  const int n2 = 2 * n;
+  const int TA = r.x / ntile, TB = r.x - TA * ntile;
+  const int a = 16 * TA + (cell >> 4), b = 16 * TB + (cell & 15);
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int q = 0; q < r.z; ++q) {
+    const float *P = part + ((size_t)grp * nslots + r.y + q) * 2048;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] += P[m * 256 + cell];
+  }
+  if (a >= n2 || b >= n2) return;
+  const size_t plane = (size_t)n2 * n2;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    if (grp * 4 + c < nzc)
+      F[(size_t)(grp * 4 + c) * plane + (size_t)a * n2 + b] = make_float2(acc[2 * c], acc[2 * c + 1]);
 }
 
 __global__ void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R, FftDesc fd,
@@ -111,14 +167,14 @@ __global__ void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R
   const float2 *src = F + ((size_t)c * n2 + a) * n2;
   for (int b = tid; b < n2; b += nt) {
     const float2 x = src[b];
-    sm[b] = (b & 1) ? make_float2(-x.x, -x.y) : x;
+    sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
   }
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
   float2 *dst = R + ((size_t)c * n2 + a) * n;
   const int h = n / 2;
   for (int v = tid; v < n; v += nt) {
-    const float2 x = sm[v + h];
+    const float2 x = sm[lds_pad(v + h)];
     dst[v] = ((v + h) & 1) ? make_float2(-x.x, -x.y) : x;
   }
 }
@@ -137,7 +193,7 @@ __global__ void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ ob
   for (int idx = tid; idx < CB * n2; idx += nt) {
     const int a = idx >> lgCB, t = idx & (CB - 1);
     float2 x = (v0 + t < n) ? src[(size_t)a * n + v0 + t] : make_float2(0.f, 0.f);
-    sm[idx] = (a & 1) ? make_float2(-x.x, -x.y) : x;
+    sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
   }
   __syncthreads();
   lds_fft<1>(sm, fd, lgCB, tid, nt);
@@ -148,7 +204,7 @@ __global__ void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ ob
     const int tx = v0 + t;
     if (tx >= n) continue;
     const int u = ty + 1 + h;
-    float2 x = sm[(u << lgCB) + t];
+    float2 x = sm[(lds_pad(u) << lgCB) + t];
     const int2 ci = circ[ty];
     const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
     obj[((size_t)c * n + ty) * n + tx] = x.x * ph;
@@ -169,7 +225,7 @@ __global__ void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restric
   const int CB = 1 << lgCB;
   const int n2 = 2 * n, v0 = cb * CB, h = n / 2;
   const int z2 = c + nzc;
-  for (int idx = tid; idx < CB * n2; idx += nt) sm[idx] = make_float2(0.f, 0.f);
+  for (int idx = tid; idx < CB * lds_padded_len(n2); idx += nt) sm[idx] = make_float2(0.f, 0.f);
   __syncthreads();
   for (int idx = tid; idx < CB * n; idx += nt) {
     const int ty = idx >> lgCB, t = idx & (CB - 1);
@@ -180,7 +236,7 @@ __global__ void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restric
     const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
     const float re = obj[((size_t)c * n + ty) * n + tx];
     const float im = (z2 < nz) ? obj[((size_t)z2 * n + ty) * n + tx] : 0.f;
-    sm[(u << lgCB) + t] = make_float2(re * ph, im * ph);
+    sm[(lds_pad(u) << lgCB) + t] = make_float2(re * ph, im * ph);
   }
   __syncthreads();
   lds_fft<-1>(sm, fd, lgCB, tid, nt);
@@ -188,7 +244,7 @@ __global__ void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restric
   for (int idx = tid; idx < CB * n2; idx += nt) {
     const int a = idx >> lgCB, t = idx & (CB - 1);
     if (v0 + t >= n) continue;
-    const float2 x = sm[idx];
+    const float2 x = sm[(lds_pad(a) << lgCB) + t];
     dst[(size_t)a * n + v0 + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
   }
 }
@@ -207,13 +263,13 @@ __global__ void k_fr_rowfft_adj(const float2 *__restrict__ R, float2 *__restrict
       x = src[v - h];
       if (v & 1) x = make_float2(-x.x, -x.y);
     }
-    sm[v] = x;
+    sm[lds_pad(v)] = x;
   }
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
   float2 *dst = F + ((size_t)c * n2 + a) * n2;
   for (int b = tid; b < n2; b += nt) {
-    const float2 x = sm[b];
+    const float2 x = sm[lds_pad(b)];
     dst[b] = (b & 1) ? make_float2(-x.x, -x.y) : x;
   }
 }
@@ -271,14 +327,14 @@ __global__ void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ 
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const float2 *in = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
-  for (int k = tid; k < n; k += nt) sm[k] = cscale(in[(size_t)k * 4], shift_sign(k) * scale);
+  for (int k = tid; k < n; k += nt) sm[lds_pad(k)] = cscale(in[(size_t)k * 4], shift_sign(k) * scale);
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
   const int z2 = c + nzc;
   float *r1 = data + ((size_t)c * nproj + p) * n;
   float *r2 = data + ((size_t)z2 * nproj + p) * n;
   for (int j = tid; j < n; j += nt) {
-    const float2 x = cscale(sm[j], shift_sign(j));
+    const float2 x = cscale(sm[lds_pad(j)], shift_sign(j));
     r1[j] = x.x;
     if (z2 < nz) r2[j] = x.y;
   }
@@ -288,12 +344,13 @@ __global__ void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ 
 
 struct tomo_fr_plan {
   int64_t n = 0, nproj = 0, nz = 0, nzc = 0, ngrp = 0;
-  int nblk = 0;
   int64_t nsamp = 0;
   FftPlan fft1, fft2;
-  DevBuf geom, lists, offs, phix, phiy, circ;
+  DevBuf geom, ent, items, red, part, phix, phiy, circ;
+  int ntile = 0, nitems = 0, nred = 0, nslots = 0;
   DevBuf G, F, R;
   int colCB = 1, lgCB = 0, colThreads = 256, prepThreads = 256, rowThreads = 256;
+  int rowLds = 0, prepLds = 0;
   size_t colLds = 0;
   float scale1 = 0.f, K2 = 0.f;
 };
@@ -330,60 +387,87 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
       geom[s] = make_float4((float)(u - fu), (float)(v - fv), (float)b0v[s], (float)b1v[s]);
     }
   }
-  // 8x8 block lists of the periodic grid, samples in increasing order
-  const int nb = (int)ceil_div(n2, 8);
-  P->nblk = nb;
-  auto blocks_of = [&](int base, int *out) {  // distinct blocks of rows base..base+8 (mod 2n)
+  // 16x16 tile lists of the periodic grid: entries {sample, footprint start
+  // relative to the tile (columns | rows << 8, each in [-8, 16))}, samples in
+  // increasing order, each list padded to a multiple of 4 with {-1, 0}
+  const int nt = (int)ceil_div(n2, 16);
+  P->ntile = nt;
+  auto tiles_of = [&](int base, int *tl, int *rel) {  // distinct tiles of rows base..base+8
     int cnt = 0;
-    for (int i = 0; i < 9; ++i) {
-      int r = (int)((n + base + i) % n2);
+    for (int q = 0; q < 9; ++q) {
+      int r = (int)((n + base + q) % n2);
       if (r < 0) r += (int)n2;
-      const int bk = r / 8;
+      const int t = r / 16;
       bool seen = false;
-      for (int q = 0; q < cnt; ++q) seen |= (out[q] == bk);
-      if (!seen) out[cnt++] = bk;
+      for (int u = 0; u < cnt; ++u) seen |= (tl[u] == t);
+      if (!seen) {
+        int v = (int)(((n + base - 16 * t) % n2 + n2) % n2);
+        if (v >= n2 - 8) v -= (int)n2;
+        tl[cnt] = t;
+        rel[cnt] = v;
+        cnt++;
+      }
     }
     return cnt;
   };
-  std::vector<int> counts((size_t)nb * nb + 1, 0);
-  for (int pass = 0; pass < 2; ++pass) {
-    std::vector<int> fill;
-    std::vector<int> list;
-    if (pass == 1) {
-      fill.assign(counts.begin(), counts.end() - 1);
-      list.resize(counts.back());
-    }
-    for (int64_t s = 0; s < S; ++s) {
-      int br[4], bc[4];
-      const int nr = blocks_of(b1v[s], br), nc = blocks_of(b0v[s], bc);
-      for (int i = 0; i < nr; ++i)
-        for (int j = 0; j < nc; ++j) {
-          const size_t blk = (size_t)br[i] * nb + bc[j];
-          if (pass == 0)
-            counts[blk]++;
-          else
-            list[fill[blk]++] = (int)s;
+  const size_t ntiles = (size_t)nt * nt;
+  std::vector<int64_t> counts(ntiles + 1, 0);
+  for (int64_t s = 0; s < S; ++s) {
+    int tr[4], rr[4], tc[4], rc2[4];
+    const int nr = tiles_of(b1v[s], tr, rr), nc = tiles_of(b0v[s], tc, rc2);
+    for (int u = 0; u < nr; ++u)
+      for (int v = 0; v < nc; ++v) counts[(size_t)tr[u] * nt + tc[v]]++;
+  }
+  std::vector<int64_t> off(ntiles + 1, 0);
+  for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + 3) / 4 * 4;
+  std::vector<int2> ent(std::max<int64_t>(1, off[ntiles]), make_int2(-1, 0));
+  std::vector<int64_t> fill(off.begin(), off.end() - 1);
+  for (int64_t s = 0; s < S; ++s) {
+    int tr[4], rr[4], tc[4], rc2[4];
+    const int nr = tiles_of(b1v[s], tr, rr), nc = tiles_of(b0v[s], tc, rc2);
+    for (int u = 0; u < nr; ++u)
+      for (int v = 0; v < nc; ++v) {
+        if (rr[u] < -8 || rr[u] >= 16 || rc2[v] < -8 || rc2[v] >= 16) {
+          set_error("fr_create: internal tile offset out of range");
+          return TOMO_EINVAL;
         }
-    }
-    if (pass == 0) {
-      int acc = 0;
-      for (size_t i = 0; i <= (size_t)nb * nb; ++i) {
-        const int c = counts[i];
-        counts[i] = acc;
-        acc += (i < (size_t)nb * nb) ? c : 0;
+        const size_t t = (size_t)tr[u] * nt + tc[v];
+        ent[fill[t]++] = make_int2((int)s, (rc2[v] & 0xff) | ((rr[u] & 0xff) << 8));
       }
+  }
+  TOMO_REQUIRE(off[ntiles] < ((int64_t)1 << 31), "fr_create: tile lists too large");
+  // work items: whole tiles, or chunks of kGridChunk entries for heavy tiles
+  std::vector<int4> items, red;
+  int nslots = 0;
+  for (size_t t = 0; t < ntiles; ++t) {
+    const int64_t L = off[t + 1] - off[t];
+    if (L <= kGridChunk) {
+      items.push_back(make_int4((int)t, (int)off[t], (int)L, -1));
     } else {
-      int rc = P->lists.alloc(sizeof(int) * std::max<size_t>(1, list.size()));
-      if (rc) return rc;
-      if (!list.empty())
-        TOMO_HIP(hipMemcpy(P->lists.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+      const int nch = (int)ceil_div(L, kGridChunk);
+      red.push_back(make_int4((int)t, nslots, nch, 0));
+      for (int q = 0; q < nch; ++q)
+        items.push_back(make_int4((int)t, (int)(off[t] + (int64_t)q * kGridChunk),
+                                  (int)std::min<int64_t>(kGridChunk, L - (int64_t)q * kGridChunk),
+                                  nslots + q));
+      nslots += nch;
     }
   }
-  int rc = P->offs.alloc(sizeof(int) * counts.size());
+  // longest items first (results do not depend on the order)
+  std::stable_sort(items.begin(), items.end(), [](const int4 &x, const int4 &y) { return x.z > y.z; });
+  P->nitems = (int)items.size();
+  P->nred = (int)red.size();
+  P->nslots = nslots;
+  int rc = P->ent.alloc(sizeof(int2) * ent.size());
   if (rc) return rc;
-  TOMO_HIP(hipMemcpy(P->offs.p, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice));
-  rc = P->geom.alloc(sizeof(float4) * S);
-  if (rc) return rc;
+  TOMO_HIP(hipMemcpy(P->ent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+  if ((rc = P->items.alloc(sizeof(int4) * std::max<size_t>(1, items.size())))) return rc;
+  TOMO_HIP(hipMemcpy(P->items.p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+  if (!red.empty()) {
+    if ((rc = P->red.alloc(sizeof(int4) * red.size()))) return rc;
+    TOMO_HIP(hipMemcpy(P->red.p, red.data(), sizeof(int4) * red.size(), hipMemcpyHostToDevice));
+  }
+  if ((rc = P->geom.alloc(sizeof(float4) * S))) return rc;
   TOMO_HIP(hipMemcpy(P->geom.p, geom.data(), sizeof(float4) * S, hipMemcpyHostToDevice));
 
   // deapodisation (kernels_fourierrec.cuh:14-15), FP64, separable in x and y;
@@ -472,17 +556,21 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   const int64_t n2 = 2 * n;
   const size_t plane = (size_t)n2 * n2;
   if ((rc = P->G.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * P->nsamp))) return fail(rc);
+  if ((rc = P->part.alloc(sizeof(float) * 2048 * (size_t)P->ngrp * std::max(1, P->nslots))))
+    return fail(rc);
   if ((rc = P->F.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * plane))) return fail(rc);
   if ((rc = P->R.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * n2 * n))) return fail(rc);
   // column pass: CB columns of length 2n in LDS
-  const size_t maxlds = std::min<size_t>(max_lds_per_block(), 131072);
+  const size_t maxlds = max_lds_per_block();
+  const size_t col1 = (size_t)lds_padded_len((int)n2) * sizeof(float2);
   int CB = 1, lg = 0;
-  while (CB * 2 <= 16 && (size_t)(CB * 2) * n2 * sizeof(float2) <= maxlds &&
+  while (CB * 2 <= 16 && (size_t)(CB * 2) * col1 <= maxlds &&
          fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) > 0)
     CB *= 2, lg += 1;
   P->colCB = CB;
   P->lgCB = lg;
-  P->colLds = (size_t)CB * n2 * sizeof(float2);
+  P->colLds = (size_t)CB * col1;
+  TOMO_REQUIRE(P->colLds <= maxlds, "fr_create: 2n=%lld exceeds the LDS of one workgroup", (long long)n2);
   P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2);
   P->prepThreads = fft_threads(P->fft1.d, n);
   P->rowThreads = fft_threads(P->fft2.d, n2);
@@ -494,7 +582,9 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft_adj,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
-  const int rowLds = (int)(n2 * sizeof(float2));
+  const int rowLds = (int)(lds_padded_len((int)n2) * sizeof(float2));
+  P->rowLds = rowLds;
+  P->prepLds = (int)(lds_padded_len((int)n) * sizeof(float2));
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft,
                                hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft_adj,
@@ -511,20 +601,39 @@ extern "C" int fr_backprojection(tomo_fr_plan *P, float *obj, const float *data,
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   const int n2 = 2 * n;
-  hipLaunchKernelGGL(k_fr_prep, dim3(nproj, ng * 4), dim3(P->prepThreads), n * sizeof(float2), st,
-                     data, P->G.as<float2>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  {
+    ProfScope ps_("k_fr_prep", st);
+    hipLaunchKernelGGL(k_fr_prep, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
+                       data, P->G.as<float2>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fr_grid, dim3((unsigned)ceil_div((int64_t)P->nblk * P->nblk, kGridWaves), ng),
-                     dim3(64 * kGridWaves), 0, st, P->geom.as<float4>(), P->lists.as<int>(),
-                     P->offs.as<int>(), P->G.as<float2>(), P->F.as<float2>(), n, P->nblk,
-                     (int)P->nsamp, nzc, P->K2);
+  {
+    ProfScope ps_("k_fr_grid", st);
+    hipLaunchKernelGGL(k_fr_grid, dim3((unsigned)ceil_div(P->nitems, kGridWaves), ng),
+                       dim3(64 * kGridWaves), 0, st, P->items.as<int4>(), P->nitems,
+                       P->ent.as<int2>(), P->geom.as<float4>(), P->G.as<float2>(),
+                       P->F.as<float2>(), P->part.as<float>(), n, P->ntile, P->nslots,
+                       (int)P->nsamp, nzc, P->K2);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fr_rowfft, dim3(n2, ng * 4), dim3(P->rowThreads), n2 * sizeof(float2), st,
-                     P->F.as<float2>(), P->R.as<float2>(), P->fft2.d, n, nzc);
+  if (P->nred > 0) {
+    ProfScope ps_("k_fr_grid_reduce", st);
+    hipLaunchKernelGGL(k_fr_grid_reduce, dim3(P->nred, ng), dim3(256), 0, st, P->red.as<int4>(),
+                       P->part.as<float>(), P->F.as<float2>(), n, P->ntile, P->nslots, nzc);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fr_colfft, dim3((unsigned)ceil_div(n, P->colCB), ng * 4), dim3(P->colThreads),
-                     P->colLds, st, P->R.as<float2>(), obj, P->phix.as<float>(), P->phiy.as<float>(),
-                     P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+  {
+    ProfScope ps_("k_fr_rowfft", st);
+    hipLaunchKernelGGL(k_fr_rowfft, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds, st,
+                       P->F.as<float2>(), P->R.as<float2>(), P->fft2.d, n, nzc);
+  }
+  TOMO_LAUNCH_CHECK();
+  {
+    ProfScope ps_("k_fr_colfft", st);
+    hipLaunchKernelGGL(k_fr_colfft, dim3((unsigned)ceil_div(n, P->colCB), ng * 4), dim3(P->colThreads),
+                       P->colLds, st, P->R.as<float2>(), obj, P->phix.as<float>(), P->phiy.as<float>(),
+                       P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
@@ -537,19 +646,31 @@ extern "C" int fr_forward(tomo_fr_plan *P, float *data, const float *obj, void *
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   const int n2 = 2 * n;
-  hipLaunchKernelGGL(k_fr_colfft_adj, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
-                     dim3(P->colThreads), P->colLds, st, obj, P->R.as<float2>(), P->phix.as<float>(),
-                     P->phiy.as<float>(), P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+  {
+    ProfScope ps_("k_fr_colfft_adj", st);
+    hipLaunchKernelGGL(k_fr_colfft_adj, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
+                       dim3(P->colThreads), P->colLds, st, obj, P->R.as<float2>(), P->phix.as<float>(),
+                       P->phiy.as<float>(), P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fr_rowfft_adj, dim3(n2, ng * 4), dim3(P->rowThreads), n2 * sizeof(float2),
-                     st, P->R.as<float2>(), P->F.as<float2>(), P->fft2.d, n, nzc);
+  {
+    ProfScope ps_("k_fr_rowfft_adj", st);
+    hipLaunchKernelGGL(k_fr_rowfft_adj, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds,
+                       st, P->R.as<float2>(), P->F.as<float2>(), P->fft2.d, n, nzc);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fr_interp, dim3((unsigned)ceil_div(P->nsamp, 256), ng), dim3(256), 0, st,
-                     P->geom.as<float4>(), P->F.as<float2>(), P->G.as<float2>(), n, (int)P->nsamp, nzc,
-                     P->K2);
+  {
+    ProfScope ps_("k_fr_interp", st);
+    hipLaunchKernelGGL(k_fr_interp, dim3((unsigned)ceil_div(P->nsamp, 256), ng), dim3(256), 0, st,
+                       P->geom.as<float4>(), P->F.as<float2>(), P->G.as<float2>(), n, (int)P->nsamp, nzc,
+                       P->K2);
+  }
   TOMO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fr_prep_adj, dim3(nproj, ng * 4), dim3(P->prepThreads), n * sizeof(float2), st,
-                     P->G.as<float2>(), data, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  {
+    ProfScope ps_("k_fr_prep_adj", st);
+    hipLaunchKernelGGL(k_fr_prep_adj, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
+                       P->G.as<float2>(), data, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
@@ -557,8 +678,10 @@ extern "C" int fr_forward(tomo_fr_plan *P, float *data, const float *obj, void *
 extern "C" void fr_destroy(tomo_fr_plan *P) {
   if (!P) return;
   P->geom.release();
-  P->lists.release();
-  P->offs.release();
+  P->ent.release();
+  P->items.release();
+  P->red.release();
+  P->part.release();
   P->phix.release();
   P->phiy.release();
   P->circ.release();

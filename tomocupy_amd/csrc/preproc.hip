@@ -5,6 +5,7 @@
 #include <string>
 
 #include "common.h"
+#include "preproc.h"
 
 using namespace tomo;
 
@@ -86,10 +87,6 @@ __global__ void k_pre_log_swap(const float *__restrict__ res, float *__restrict_
 
 }  // namespace
 
-struct tomo_pre_plan {
-  int64_t nproj = 0, nz = 0, n = 0, ndark = 0, nflat = 0;
-  DevBuf dark0, flatA, flatB;
-};
 
 extern "C" int pre_create(tomo_pre_plan **plan, int64_t nproj, int64_t nz, int64_t n,
                           int64_t ndark, int64_t nflat) {
@@ -125,9 +122,12 @@ static int launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *
   TOMO_REQUIRE(bright_ratio > 0 && std::isfinite(bright_ratio), "bright_ratio must be > 0");
   TOMO_REQUIRE(!flat_linear || P->nflat >= 3, "flat_linear needs at least 3 flat frames");
   const int zn = (int)(P->nz * P->n);
-  hipLaunchKernelGGL(k_pre_means, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st, dark, flat,
-                     P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), zn,
-                     (int)P->ndark, (int)P->nflat, (float)bright_ratio, flat_linear);
+  {
+    ProfScope ps_("k_pre_means", st);
+    hipLaunchKernelGGL(k_pre_means, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st, dark, flat,
+                       P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), zn,
+                       (int)P->ndark, (int)P->nflat, (float)bright_ratio, flat_linear);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
@@ -141,9 +141,12 @@ extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_
   hipStream_t st = (hipStream_t)stream;
   int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pre_rows<0>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
-                     data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), out,
-                     (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, minus_log);
+  {
+    ProfScope ps_("k_pre_rows", st);
+    hipLaunchKernelGGL(k_pre_rows<0>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+                       data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), out,
+                       (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, minus_log);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
@@ -157,9 +160,12 @@ extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16
   hipStream_t st = (hipStream_t)stream;
   int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pre_rows<1>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
-                     data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), res,
-                     (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, 0);
+  {
+    ProfScope ps_("k_pre_rows", st);
+    hipLaunchKernelGGL(k_pre_rows<1>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+                       data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), res,
+                       (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, 0);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
@@ -171,22 +177,13 @@ extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, float *out
   TOMO_REQUIRE(res && out, "pre_minus_log_swap: NULL array");
   TOMO_REQUIRE(res != out, "pre_minus_log_swap: in-place swap is not supported");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_pre_log_swap, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
-                     res, out, (int)P->nproj, (int)P->nz, (int)P->n, minus_log);
+  {
+    ProfScope ps_("k_pre_log_swap", st);
+    hipLaunchKernelGGL(k_pre_log_swap, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+                       res, out, (int)P->nproj, (int)P->nz, (int)P->n, minus_log);
+  }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
-}
-
-extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, const char *wname,
-                                    int level, void *stream) {
-  clear_error();
-  TOMO_REQUIRE(P != nullptr && res != nullptr && wname != nullptr,
-               "pre_remove_stripe_fw: NULL argument");
-  (void)sigma;
-  (void)level;
-  (void)stream;
-  set_error("pre_remove_stripe_fw: not implemented in this build");
-  return TOMO_EUNSUPPORTED;
 }
 
 extern "C" void pre_destroy(tomo_pre_plan *P) {
@@ -194,5 +191,19 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->dark0.release();
   P->flatA.release();
   P->flatB.release();
+  for (auto &l : P->lv) {
+    l.tw.release();
+    l.gk.release();
+    l.ll.release();
+    l.lh.release();
+    l.hl.release();
+    l.hh.release();
+  }
+  P->rowlo.release();
+  P->rowhi.release();
+  P->synlo.release();
+  P->synhi.release();
+  P->syn[0].release();
+  P->syn[1].release();
   delete P;
 }

@@ -1,0 +1,166 @@
+"""Chunk conveyor with the reference's structure (``src/tomocupy/rec.py:62-208``).
+
+``GPURec.recon_all(read_chunk, write_chunk)`` reconstructs a volume chunk by
+chunk with three HIP streams: stream 1 uploads chunk k from pinned host
+memory, stream 2 processes chunk k-1 (fused preprocessing, FBP filter,
+backprojection), stream 3 downloads chunk k-2 into a pinned buffer that a
+writer thread hands to ``write_chunk`` -- the conveyor of ``rec.py:161-205``.
+Settings the reference reads from its global ``args``/``params`` are
+constructor arguments.  ``recon_resident`` runs the same per-chunk work on a
+volume that is already in GPU memory.
+
+Deviation: the reference leaves stale slices from an earlier chunk in the
+tail of a partial last chunk (``rec.py:188-190``), where they are paired with
+valid slices in the complex packing; here the tail is reset to an empty
+sinogram (data 0, dark 0, flat 1), so valid slices never pair with stale data.
+"""
+import threading
+
+import numpy as np
+
+from .backproj_functions import BackprojFunctions
+from .proc_functions import ProcFunctions
+
+
+class _WRThread:
+    """Writer thread slot (``utils.py:123-145``)."""
+
+    def __init__(self):
+        self.thread = None
+        self.done = threading.Event()
+        self.done.set()
+        self.error = None
+
+    def run(self, fun, args):
+        self.done.clear()
+
+        def wrapper():
+            try:
+                fun(*args)
+            except BaseException as e:  # surfaced by join()
+                self.error = e
+            finally:
+                self.done.set()
+
+        self.thread = threading.Thread(target=wrapper, daemon=True)
+        self.thread.start()
+
+    def busy(self):
+        return not self.done.is_set()
+
+    def join(self):
+        if self.thread is not None:
+            self.thread.join()
+        if self.error is not None:
+            err, self.error = self.error, None
+            raise err
+
+
+def _free_slot(slots):
+    """Index of an idle writer slot (``utils.py:148-154``)."""
+    while True:
+        for i, t in enumerate(slots):
+            if not t.busy():
+                return i
+        slots[0].done.wait(timeout=0.01)
+
+
+class GPURec:
+    def __init__(self, n, nproj, nz, ncz, theta, center, ndark, nflat, dtype='float32',
+                 fbp_filter='parzen', remove_stripe_method='none', fw_sigma=1.0,
+                 fw_filter='sym16', fw_level=7, minus_log=True, bright_ratio=1.0,
+                 flat_linear=False, max_write_threads=8):
+        import torch
+        if ncz == 1:
+            ncz = 2  # fourierrec pairs two slices (reader.py:113-115)
+        self.n, self.nproj, self.nz, self.ncz = int(n), int(nproj), int(nz), int(ncz)
+        self.ndark, self.nflat = int(ndark), int(nflat)
+        self.nzchunk = -(-self.nz // self.ncz)
+        self.lzchunk = [min(self.ncz, self.nz - k * self.ncz) for k in range(self.nzchunk)]
+        self.pf = ProcFunctions(self.nproj, self.ncz, self.n, self.ndark, self.nflat, dtype,
+                                minus_log=minus_log, bright_ratio=bright_ratio,
+                                flat_linear=flat_linear, remove_stripe_method=remove_stripe_method,
+                                fw_sigma=fw_sigma, fw_filter=fw_filter, fw_level=fw_level)
+        self.bf = BackprojFunctions(self.n, self.nproj, self.ncz, theta, center, dtype, fbp_filter)
+        self.stream1 = torch.cuda.Stream()
+        self.stream2 = torch.cuda.Stream()
+        self.stream3 = torch.cuda.Stream()
+        self.max_write_threads = int(max_write_threads)
+        self._sino = torch.empty((self.ncz, self.nproj, self.n), dtype=torch.float32,
+                                 device='cuda')
+
+    def _process(self, data, dark, flat, rec):
+        out = self.pf.proc_chunk(data, dark, flat, out=self._sino)
+        self.bf.fbp_filter_center(out)
+        self.bf.cl_rec.backprojection(rec, out)
+
+    def recon_resident(self, raw, dark, flat, rec):
+        """Reconstruct a GPU-resident volume: ``raw (nzchunk, nproj, ncz, n)`` uint16,
+        ``dark``/``flat`` ``(nzchunk, ndark|nflat, ncz, n)``, into
+        ``rec (nzchunk*ncz, n, n)`` float32, on stream 2."""
+        import torch
+        with torch.cuda.stream(self.stream2):
+            for k in range(self.nzchunk):
+                self._process(raw[k], dark[k], flat[k], rec[k * self.ncz:(k + 1) * self.ncz])
+        return rec
+
+    def recon_all(self, read_chunk, write_chunk):
+        """Conveyor over host data.
+
+        ``read_chunk(k)`` returns ``(data, dark, flat)`` numpy uint16 arrays of
+        shapes ``(nproj, lz, n)``, ``(ndark, lz, n)``, ``(nflat, lz, n)`` with
+        ``lz = lzchunk[k]``; ``write_chunk(k, rec)`` receives a ``(lz, n, n)``
+        float32 numpy view that is valid until it returns.
+        """
+        import torch
+        ncz, nproj, n = self.ncz, self.nproj, self.n
+        pin = dict(pin_memory=True)
+        item_pinned = {
+            'data': torch.zeros((2, nproj, ncz, n), dtype=torch.uint16, **pin),
+            'dark': torch.zeros((2, self.ndark, ncz, n), dtype=torch.uint16, **pin),
+            'flat': torch.ones((2, self.nflat, ncz, n), dtype=torch.uint16, **pin),
+        }
+        item_gpu = {k: torch.empty_like(v, device='cuda') for k, v in item_pinned.items()}
+        item_gpu['flat'].copy_(item_pinned['flat'])
+        rec_pinned = torch.zeros((self.max_write_threads, ncz, n, n), dtype=torch.float32, **pin)
+        rec_gpu = torch.zeros((2, ncz, n, n), dtype=torch.float32, device='cuda')
+        writers = [_WRThread() for _ in range(self.max_write_threads)]
+        nzchunk = self.nzchunk
+        for k in range(nzchunk + 2):
+            if 0 < k < nzchunk + 1:
+                with torch.cuda.stream(self.stream2):
+                    i = (k - 1) % 2
+                    self._process(item_gpu['data'][i], item_gpu['dark'][i], item_gpu['flat'][i],
+                                  rec_gpu[i])
+            ithread = None
+            if k > 1:
+                with torch.cuda.stream(self.stream3):
+                    ithread = _free_slot(writers)
+                    # chunk k-2 was finished on stream 2 before the previous step ended
+                    rec_pinned[ithread].copy_(rec_gpu[(k - 2) % 2], non_blocking=True)
+            if k < nzchunk:
+                data, dark, flat = read_chunk(k)
+                lz = self.lzchunk[k]
+                item_pinned['data'][k % 2, :, :lz].copy_(torch.from_numpy(np.ascontiguousarray(data)))
+                item_pinned['dark'][k % 2, :, :lz].copy_(torch.from_numpy(np.ascontiguousarray(dark)))
+                item_pinned['flat'][k % 2, :, :lz].copy_(torch.from_numpy(np.ascontiguousarray(flat)))
+                if lz < ncz:
+                    item_pinned['data'][k % 2, :, lz:].zero_()
+                    item_pinned['dark'][k % 2, :, lz:].zero_()
+                    item_pinned['flat'][k % 2, :, lz:].copy_(
+                        torch.ones((self.nflat, ncz - lz, n), dtype=torch.int32).to(torch.uint16))
+                with torch.cuda.stream(self.stream1):
+                    for key in ('data', 'dark', 'flat'):
+                        item_gpu[key][k % 2].copy_(item_pinned[key][k % 2], non_blocking=True)
+            self.stream3.synchronize()
+            if k > 1:
+                lz = self.lzchunk[k - 2]
+                writers[ithread].run(write_chunk, (k - 2, rec_pinned[ithread, :lz].numpy()))
+            self.stream1.synchronize()
+            self.stream2.synchronize()
+        for t in writers:
+            t.join()
+
+    def free(self):
+        self.pf.free()
+        self.bf.free()

@@ -69,19 +69,20 @@ def angles(nproj, start=0.0):
     return (start + np.arange(nproj) * np.pi / nproj).astype(np.float32)
 
 
-def sinograms(cfg, n, theta, nz, z0=0):
-    """Analytic sinograms ``(nz, nproj, n)`` FP64 for slices ``z0..z0+nz-1``."""
-    return np.stack([sinogram(ellipses(n, 1000 * cfg + z), theta, n)
-                     for z in range(z0, z0 + nz)])
+def sinograms(cfg, n, theta, nz, z0=0, atten=1.0):
+    """Analytic sinograms ``(nz, nproj, n)`` FP64 for slices ``z0..z0+nz-1``;
+    ``atten`` scales every attenuation coefficient."""
+    return atten * np.stack([sinogram(ellipses(n, 1000 * cfg + z), theta, n)
+                             for z in range(z0, z0 + nz)])
 
 
-def raw_counts(cfg, n, theta, nz, ndark=10, nflat=20, stripes=False, z0=0):
+def raw_counts(cfg, n, theta, nz, ndark=10, nflat=20, stripes=False, z0=0, atten=1.0):
     """Detector data in tomocupy's chunk layout.
 
     Returns ``data (nproj, nz, n)``, ``dark (ndark, nz, n)``, ``flat (nflat, nz, n)``
     as uint16, plus the analytic sinograms ``(nz, nproj, n)`` they were made from.
     """
-    sino = sinograms(cfg, n, theta, nz, z0)
+    sino = sinograms(cfg, n, theta, nz, z0, atten)
     nproj = len(theta)
     data = np.empty((nproj, nz, n), np.uint16)
     dark = np.empty((ndark, nz, n), np.uint16)
