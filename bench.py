@@ -23,9 +23,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "slices/s at 2048^2 x1500 proj, FP32 vs FP16 at rel-L2 vs FP64; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_damp', 'k_fw_syn_cols',
-           'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_fr_prep',
-           'k_fr_grid', 'k_fr_rowfft', 'k_fr_colfft']
+KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_dft', 'k_fw_notch', 'k_fw_syn_cols',
+           'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_flt_kernel2n',
+           'k_fr_prep', 'k_fr_filtprep',
+           'k_fr_grid', 'k_fr_grid_reduce', 'k_fr_rowfft', 'k_fr_colfft']
 
 
 def parse():
@@ -61,7 +62,7 @@ def algorithmic_bytes(name, a, s):
         return nzc * ((2 * n) ** 2 * 8 + 2 * n * n * 8)
     if name == 'k_fr_colfft':    # read the pruned rows, write two real slices per complex slice
         return nzc * (2 * n * n * 8) + ch * n * n * 4
-    if name == 'k_fr_prep':      # read the real sinograms, write the complex spectra
+    if name in ('k_fr_prep', 'k_fr_filtprep'):  # read the real sinograms, write the spectra
         return ch * nproj * n * 4 + grp * nproj * n * 8
     if name == 'k_flt_rows':     # read + write the sinograms in place
         return 2 * ch * nproj * n * 4
@@ -175,8 +176,7 @@ def main():
             i = k % len(streams)
             with torch.cuda.stream(streams[i]):
                 out = pfs[i].proc_chunk(raw[k], dark[k], flat[k], out=sinos[i])
-                bfs[i].fbp_filter_center(out)
-                bfs[i].cl_rec.backprojection(rec[k * a.chunk:(k + 1) * a.chunk], out)
+                bfs[i].fbp_filter_backprojection(rec[k * a.chunk:(k + 1) * a.chunk], out)
 
     for _ in range(a.warmup):
         step()

@@ -56,6 +56,7 @@ int tomo_prof_read(const char *kernel, double *total_ms, int64_t *launches);
 /* USFFT backprojector                                                      */
 /* ------------------------------------------------------------------------ */
 typedef struct tomo_fr_plan tomo_fr_plan;
+typedef struct tomo_flt_plan tomo_flt_plan;
 
 /*
  * Replaces cfunc_fourierrec::cfunc_fourierrec(nproj, nz/2, n, theta)
@@ -86,6 +87,21 @@ int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
 int fr_backprojection(tomo_fr_plan *plan, float *obj, const float *data, void *stream);
 
 /*
+ * fbp_filter_center followed by backprojection in one pass over the data
+ * (backproj_functions.py:93-109 then fourierrec.py:61-68, as rec.py:175-176
+ * calls them), with `flt` created for the same n, nproj, nz.  The
+ * edge-replicated filter of length ne is applied exactly as a length-2n
+ * circular convolution plus two edge terms; the filtered sinograms are never
+ * written to memory.  Unlike flt_filter_center, `data` is NOT modified.
+ *   obj      output, device float32 (nz, n, n)
+ *   data     device float32 (nz, nproj, n) sinograms (after -log)
+ *   wfilter, center, sht  as flt_filter_center
+ */
+int fr_filter_backprojection(tomo_fr_plan *plan, tomo_flt_plan *flt, float *obj,
+                             const float *data, const float *wfilter, double center,
+                             const float *sht, void *stream);
+
+/*
  * Adjoint of fr_backprojection (USFFT forward projection); no counterpart in
  * the reference's tomography path, structured like cfunc_usfft2d::fwd
  * (src/cuda/cfunc_usfft2d.cu:53-70).  For real arrays
@@ -104,7 +120,6 @@ int fr_get_sizes(const tomo_fr_plan *plan, int64_t *n, int64_t *nproj, int64_t *
 /* ------------------------------------------------------------------------ */
 /* FBP filter                                                               */
 /* ------------------------------------------------------------------------ */
-typedef struct tomo_flt_plan tomo_flt_plan;
 
 /*
  * Replaces cfunc_filter::cfunc_filter(nproj, nz, ne) (src/cuda/cfunc_filter.cu:4-34)

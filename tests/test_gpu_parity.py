@@ -304,3 +304,28 @@ def test_conveyor_recon_all_matches_oracle():
         err = _rel_l2_per_slice(out[s], ref)
         print('chunk', k, err)
         assert np.all(err <= RTOL32), err
+
+
+@pytest.mark.parametrize("n,nproj,nz,name,center,sht", [
+    (128, 90, 4, 'parzen', 64.0, None), (96, 60, 3, 'ramp', 47.5, [0.0, 1.25, -2.0]),
+    (256, 100, 2, 'shepp', 130.25, None), (64, 33, 5, 'hann', 30.0, [0.5, 0.0, 2.0, -1.0, 0.25])])
+def test_fused_filter_backprojection_matches_oracle(n, nproj, nz, name, center, sht):
+    from tomocupy_amd import BackprojFunctions
+    print('fused case', n, nproj, nz, name, center, flush=True)
+    rng = np.random.default_rng(n + nz)
+    theta = synth.angles(nproj, 0.05)
+    data = rng.standard_normal((nz, nproj, n)) + np.linspace(0, 2, n)
+    shv = np.zeros(nz) if sht is None else np.array(sht, dtype=np.float64)
+    ne = 4 * n
+    ref = fr.backprojection(ff.fbp_filter_center(data, ff.calc_filter(ne, name), ne, center, shv),
+                            theta)
+    bf = BackprojFunctions(n, nproj, nz, _cuda(theta), center, 'float32', name)
+    d = _cuda(data)
+    d0 = d.clone()
+    obj = torch.zeros((nz, n, n), dtype=torch.float32, device='cuda')
+    bf.fbp_filter_backprojection(obj, d, 0 if sht is None else _cuda(shv))
+    torch.cuda.synchronize()
+    err = _rel_l2_per_slice(obj.cpu().numpy(), ref)
+    print('fused rel-L2', err)
+    assert np.all(err <= RTOL32), err
+    assert torch.equal(d, d0)

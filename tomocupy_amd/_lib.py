@@ -32,6 +32,7 @@ SIGNATURES = {
     'fr_create': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_p]),
     'fr_backprojection': (_c_int, [_c_p, _c_p, _c_p, _c_p]),
     'fr_forward': (_c_int, [_c_p, _c_p, _c_p, _c_p]),
+    'fr_filter_backprojection': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_p, _c_p]),
     'fr_destroy': (None, [_c_p]),
     'fr_get_sizes': (_c_int, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
                               ctypes.POINTER(_c_i64)]),

@@ -46,6 +46,26 @@ class BackprojFunctions:
             shv = torch.full((self.ncz,), float(sht), dtype=torch.float32, device=data.device)
         return self.cl_filter.filter_center(data, self.wfilter, self.center, shv)
 
+    def fbp_filter_backprojection(self, obj, data, sht=0, stream=None):
+        """``fbp_filter_center`` then ``backprojection`` in one pass
+        (``rec.py:175-176``): ``obj (ncz, n, n)`` <- filtered backprojection of
+        ``data (ncz, nproj, n)``.  ``data`` is not modified."""
+        import torch
+        from . import _lib
+        po = _lib.dev_ptr(obj, (self.ncz, self.n, self.n), torch.float32, 'obj')
+        pd = _lib.dev_ptr(data, (self.ncz, self.nproj, self.n), torch.float32, 'data')
+        ps = None
+        if isinstance(sht, torch.Tensor):
+            shv = sht.to(torch.float32).contiguous()
+            ps = _lib.dev_ptr(shv, (self.ncz,), torch.float32, 'sht')
+        elif sht != 0:
+            shv = torch.full((self.ncz,), float(sht), dtype=torch.float32, device=data.device)
+            ps = shv.data_ptr()
+        _lib.check(_lib.lib().fr_filter_backprojection(
+            self.cl_rec._handle(), self.cl_filter._h, po, pd, self.wfilter.data_ptr(),
+            self.center, ps, _lib.stream_ptr(stream)), 'fr_filter_backprojection')
+        return obj
+
     def free(self):
         self.cl_rec.free()
         self.cl_filter.free()

@@ -130,6 +130,45 @@ __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 // LDS elements needed for a padded transform of length N.
 __host__ __device__ inline int lds_padded_len(int N) { return N + (N >> 4); }
 
+// a[r] *= w^r for r = 1..R-1 with w = W_N^base (conjugated for the inverse).
+// Powers of two of w come from the FP64-built table; the others are products
+// of at most three of them (relative error a few float32 ulps).
+template <int R, int DIR>
+__device__ __forceinline__ void twiddle(float2 *a, const float2 *__restrict__ tw, unsigned base) {
+  auto ld = [&](unsigned m) {
+    float2 w = tw[base * m];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+  };
+  if constexpr (R == 2) {
+    a[1] = cmul(a[1], ld(1));
+  } else if constexpr (R == 3) {
+    a[1] = cmul(a[1], ld(1));
+    a[2] = cmul(a[2], ld(2));
+  } else if constexpr (R == 4) {
+    const float2 w1 = ld(1), w2 = ld(2);
+    a[1] = cmul(a[1], w1);
+    a[2] = cmul(a[2], w2);
+    a[3] = cmul(a[3], cmul(w2, w1));
+  } else {
+    float2 w[R];
+    w[1] = ld(1);
+    w[2] = ld(2);
+    w[4] = ld(4);
+    w[3] = cmul(w[2], w[1]);
+    w[5] = cmul(w[4], w[1]);
+    w[6] = cmul(w[4], w[2]);
+    w[7] = cmul(w[4], w[3]);
+    if constexpr (R == 16) {
+      w[8] = ld(8);
+#pragma unroll
+      for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+  }
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float2 *tw, int lgH,
                                          int tid, int nt) {
@@ -165,12 +204,7 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
       if (Ns > 1) {
         int jq, k;
         split(j, jq, k);
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          float2 w = tw[(unsigned)(r * k * twstep)];
-          if (DIR > 0) w.y = -w.y;
-          v[q][r] = cmul(v[q][r], w);
-        }
+        twiddle<R, DIR>(v[q], tw, (unsigned)(k * twstep));
       }
       butterfly<R, DIR>(v[q]);
     }

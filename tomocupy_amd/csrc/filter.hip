@@ -15,6 +15,7 @@
 
 #include "common.h"
 #include "fft_lds.h"
+#include "internal.h"
 
 using namespace tomo;
 
@@ -89,6 +90,109 @@ __global__ void k_flt_rows(float *__restrict__ g, const float2 *__restrict__ w, 
   }
 }
 
+
+// ---- fused path: the padded filter as a 2n circular convolution ------------
+//
+// With h = C2R(w_z) (length ne, real) the reference computes, for detector
+// sample j in [0, n) of an edge-replicated row x,
+//   y[j] = sum_{i<n} h[(j-i) mod ne] x[i] + x[0] HL[j] + x[n-1] HR[j],
+//   HL[j] = sum_{d=j+1}^{j+pad} h[d],  HR[j] = sum_{d=pad+j+1}^{ne-n+j} h[d]
+// (pad = ne/2 - n/2).  The first sum only sees lags |j-i| < n, so it is the
+// circular convolution of length 2n of the zero-padded row with h2 (h wrapped
+// to lags (-n, n), h2[n] = 0): FFT_2n, multiply by H2 = FFT_2n(h2)/(2n), and
+// the unnormalised IFFT_2n.
+// One workgroup per slice builds H2, HL, HR; the window sums use FP64 prefix
+// sums of h.
+__global__ __launch_bounds__(1024) void k_flt_kernel2n(
+    const float *__restrict__ wfilter, const float *__restrict__ sht, float2 *__restrict__ H2,
+    float *__restrict__ HL, float *__restrict__ HR, double *__restrict__ P, FftDesc fde,
+    FftDesc fd2, int ne, int n, double center) {
+  extern __shared__ float2 sm[];
+  const int z = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int nh = ne / 2;
+  const double shift = -center + (sht ? (double)sht[z] : 0.0) + (double)(n / 2);
+  for (int k = tid; k < ne; k += nt) {
+    const int kk = k <= nh ? k : ne - k;
+    double f = (double)kk * shift / (double)ne;
+    f -= std::floor(f);
+    double sn, cs;
+    sincospi(-2.0 * f, &sn, &cs);
+    const double a = (double)wfilter[kk];
+    float2 W = make_float2((float)(a * cs), (float)(a * sn));
+    if (k == 0 || k == nh) W.y = 0.f;
+    if (k > nh) W.y = -W.y;
+    sm[lds_pad(k)] = W;
+  }
+  __syncthreads();
+  lds_fft<1>(sm, fde, 0, tid, nt);  // h[d] = Re sm[d] (unnormalised C2R)
+  // FP64 prefix sums P[d] = sum_{e<d} h[e], d = 0..ne: per-thread chunk sums,
+  // then a serial scan of the chunk totals
+  __shared__ double csum[1024];
+  const int chunk = (ne + nt - 1) / nt;
+  const int d0 = tid * chunk, d1 = min(ne, d0 + chunk);
+  double acc = 0.0;
+  for (int d = d0; d < d1; ++d) acc += (double)sm[lds_pad(d)].x;
+  csum[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    double run = 0.0;
+    for (int t = 0; t < nt; ++t) {
+      const double c = csum[t];
+      csum[t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  double *Pz = P + (size_t)z * (ne + 1);
+  acc = csum[tid];
+  for (int d = d0; d < d1; ++d) {
+    Pz[d] = acc;
+    acc += (double)sm[lds_pad(d)].x;
+  }
+  if (d1 == ne && d0 < d1) Pz[ne] = acc;
+  __syncthreads();  // the global prefix sums are visible to the workgroup
+  const int pad = nh - n / 2;
+  for (int j = tid; j < n; j += nt) {
+    HL[(size_t)z * n + j] = (float)(Pz[pad + j + 1] - Pz[j + 1]);
+    HR[(size_t)z * n + j] = (float)(Pz[j - n + ne + 1] - Pz[pad + j + 1]);
+  }
+  // h2 (lags -(n-1)..(n-1) wrapped to length 2n) -> H2 = FFT_2n(h2)
+  float hv[8];
+  const int n2 = 2 * n;
+  // read h for this thread's h2 entries before the LDS buffer is reused
+  int cnt = 0;
+  for (int d = tid; d < n2 && cnt < 8; d += nt, ++cnt) {
+    float v = 0.f;
+    if (d < n) v = sm[lds_pad(d)].x;
+    else if (d > n) v = sm[lds_pad(ne - n2 + d)].x;
+    hv[cnt] = v;
+  }
+  __syncthreads();
+  cnt = 0;
+  for (int d = tid; d < n2 && cnt < 8; d += nt, ++cnt) sm[lds_pad(d)] = make_float2(hv[cnt], 0.f);
+  __syncthreads();
+  lds_fft<-1>(sm, fd2, 0, tid, nt);
+  // 1/(2n) of the unnormalised inverse transform that follows rides on H2
+  const float inv2n = 1.0f / (float)n2;
+  for (int k = tid; k < n2; k += nt) H2[(size_t)z * n2 + k] = cscale(sm[lds_pad(k)], inv2n);
+}
+
+}  // namespace
+
+int tomo::flt_prepare_2n(tomo_flt_plan *P, const float *wfilter, double center, const float *sht,
+                         hipStream_t st) {
+  TOMO_REQUIRE(P->fft2n.d.N == 2 * P->n && P->H2.p != nullptr,
+               "fused filter path not available for n=%lld, ne=%lld", (long long)P->n,
+               (long long)P->ne);
+  ProfScope ps_("k_flt_kernel2n", st);
+  hipLaunchKernelGGL(k_flt_kernel2n, dim3((unsigned)P->nz), dim3(P->kthreads), P->klds, st, wfilter,
+                     sht, P->H2.as<float2>(), P->HL.as<float>(), P->HR.as<float>(),
+                     P->P.as<double>(), P->fft.d, P->fft2n.d, (int)P->ne, (int)P->n, center);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+namespace {
 
 // ---- filter design (fbp_filter.py:60-118), host FP64 ----------------------
 
@@ -177,13 +281,6 @@ bool wint(int n, const std::vector<double> &t, std::vector<double> &wn) {
 
 }  // namespace
 
-struct tomo_flt_plan {
-  int64_t n = 0, ne = 0, nproj = 0, nz = 0;
-  FftPlan fft;
-  DevBuf w;  // (nz, ne/2+1) complex, filled by flt_filter_center
-  int threads = 256;
-  size_t lds = 0;
-};
 
 extern "C" int flt_calc_filter(int64_t ne, const char *name, double *out) {
   clear_error();
@@ -267,11 +364,39 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
     flt_destroy(P);
     return rc;
   }
+  // fused 2n path (needs 2n <= 8192 and ne >= 2n so that lags |d| < n fit)
+  if (2 * n <= 8192 && ne >= 2 * n && ne <= 16384 && !rc) {
+    rc = P->fft2n.init((int)(2 * n));
+    if (!rc) rc = P->H2.alloc(sizeof(float2) * nz * 2 * n);
+    if (!rc) rc = P->HL.alloc(sizeof(float) * nz * n);
+    if (!rc) rc = P->HR.alloc(sizeof(float) * nz * n);
+    if (!rc) rc = P->P.alloc(sizeof(double) * nz * (ne + 1));
+    if (rc) {
+      flt_destroy(P);
+      return rc;
+    }
+    P->kthreads = fft_threads(P->fft.d, ne);
+    P->klds = sizeof(float2) * lds_padded_len((int)ne);
+    if (!(P->kthreads > 0 && P->kthreads * 8 >= 2 * n)) {
+      set_error("flt_create: ne=%lld too large for the fused filter setup", (long long)ne);
+      flt_destroy(P);
+      return TOMO_EINVAL;
+    }
+    hipError_t e2 = hipFuncSetAttribute((const void *)k_flt_kernel2n,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->klds);
+    if (e2 != hipSuccess) {
+      set_error("flt_create: hipFuncSetAttribute failed: %s", hipGetErrorString(e2));
+      flt_destroy(P);
+      return TOMO_EHIP;
+    }
+  }
   P->threads = fft_threads(P->fft.d, ne);
-  TOMO_REQUIRE(P->threads > 0, "flt_create: ne=%lld too large for one workgroup", (long long)ne);
   P->lds = sizeof(float2) * lds_padded_len((int)ne);
-  TOMO_REQUIRE(P->lds <= max_lds_per_block(), "flt_create: ne=%lld needs %zu B of LDS",
-               (long long)ne, P->lds);
+  if (P->threads <= 0 || P->lds > max_lds_per_block()) {
+    set_error("flt_create: ne=%lld too large for one workgroup", (long long)ne);
+    flt_destroy(P);
+    return TOMO_EINVAL;
+  }
   hipError_t e = hipFuncSetAttribute((const void *)k_flt_rows<0>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds);
   if (e == hipSuccess)
@@ -328,6 +453,11 @@ extern "C" int flt_filter_center(tomo_flt_plan *P, float *data, const float *wfi
 extern "C" void flt_destroy(tomo_flt_plan *P) {
   if (!P) return;
   P->fft.tw.release();
+  P->fft2n.tw.release();
   P->w.release();
+  P->H2.release();
+  P->HL.release();
+  P->HR.release();
+  P->P.release();
   delete P;
 }

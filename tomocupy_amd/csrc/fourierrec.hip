@@ -20,10 +20,12 @@
 // fftshiftc (:86-89) as two pruned 1D passes, divphi (:91), circ (:92).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "common.h"
 #include "fft_lds.h"
+#include "internal.h"
 
 using namespace tomo;
 
@@ -35,7 +37,7 @@ constexpr int kGridChunk = 512; // max list entries per work item (multiple of 4
 // Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
 
-__global__ void k_fr_prep(const float *__restrict__ data, float2 *__restrict__ G, FftDesc fd,
+__global__ __launch_bounds__(512) void k_fr_prep(const float *__restrict__ data, float2 *__restrict__ G, FftDesc fd,
                           int n, int nproj, int nz, int nzc, float scale) {
   extern __shared__ float2 sm[];
   const int p = blockIdx.x, c = blockIdx.y;
@@ -58,6 +60,70 @@ __global__ void k_fr_prep(const float *__restrict__ data, float2 *__restrict__ G
   for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[lds_pad(k)], shift_sign(k) * scale);
 }
 
+// Fused FBP filter + pair packing + shifted detector FFT (the fused form of
+// fbp_filter_center and the first stage of the backprojection).  Per angle p
+// and complex slice c: z = x1 + i*x2 zero-padded to 2n, FFT_2n, multiply by
+// the two slices' wrapped filter spectra (Y = Z (H1+H2)/2 + conj(Z(-k))
+// (H1-H2)/2, exact for real rows), IFFT_2n, add the edge-replication terms
+// x[0] HL + x[n-1] HR per slice, then the shifted n-point DFT of k_fr_prep.
+__global__ __launch_bounds__(512) void k_fr_filtprep(
+    const float *__restrict__ data, float2 *__restrict__ G, const float2 *__restrict__ H2,
+    const float *__restrict__ HL, const float *__restrict__ HR, FftDesc fd2, FftDesc fd1, int n,
+    int nproj, int nz, int nzc, float scale) {
+  extern __shared__ float2 sm[];
+  __shared__ float edge[4];
+  const int p = blockIdx.x, c = blockIdx.y;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  float2 *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
+  if (c >= nzc) {
+    for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = make_float2(0.f, 0.f);
+    return;
+  }
+  const int n2 = 2 * n;
+  const int z2 = c + nzc;
+  const bool has2 = z2 < nz;
+  const int zb = has2 ? z2 : c;
+  const float *r1 = data + ((size_t)c * nproj + p) * n;
+  const float *r2 = data + ((size_t)zb * nproj + p) * n;
+  for (int j = tid; j < n2; j += nt)
+    sm[lds_pad(j)] = j < n ? make_float2(r1[j], has2 ? r2[j] : 0.f) : make_float2(0.f, 0.f);
+  if (tid == 0) {
+    edge[0] = r1[0];
+    edge[1] = r1[n - 1];
+    edge[2] = has2 ? r2[0] : 0.f;
+    edge[3] = has2 ? r2[n - 1] : 0.f;
+  }
+  __syncthreads();
+  lds_fft<-1>(sm, fd2, 0, tid, nt);
+  const float2 *Ha = H2 + (size_t)c * n2;
+  const float2 *Hb = H2 + (size_t)zb * n2;
+  for (int k = tid; k <= n; k += nt) {
+    const int km = (n2 - k) % n2;
+    const float2 zk = sm[lds_pad(k)], zm = sm[lds_pad(km)];
+    const float2 ak = Ha[k], bk = Hb[k], am = Ha[km], bm = Hb[km];
+    const float2 sk = cscale(cadd(ak, bk), 0.5f), dk = cscale(csub(ak, bk), 0.5f);
+    const float2 sm_ = cscale(cadd(am, bm), 0.5f), dm = cscale(csub(am, bm), 0.5f);
+    const float2 yk = cadd(cmul(zk, sk), cmul(cconj(zm), dk));
+    const float2 ym = cadd(cmul(zm, sm_), cmul(cconj(zk), dm));
+    sm[lds_pad(k)] = yk;
+    if (km != k) sm[lds_pad(km)] = ym;
+  }
+  __syncthreads();
+  lds_fft<1>(sm, fd2, 0, tid, nt);
+  const float *hl1 = HL + (size_t)c * n, *hr1 = HR + (size_t)c * n;
+  const float *hl2 = HL + (size_t)zb * n, *hr2 = HR + (size_t)zb * n;
+  const float e0 = edge[0], e1 = edge[1], e2 = edge[2], e3 = edge[3];
+  for (int j = tid; j < n; j += nt) {
+    float2 y = sm[lds_pad(j)];
+    y.x += e0 * hl1[j] + e1 * hr1[j];
+    y.y += e2 * hl2[j] + e3 * hr2[j];
+    sm[lds_pad(j)] = cscale(y, shift_sign(j));
+  }
+  __syncthreads();
+  lds_fft<-1>(sm, fd1, 0, tid, nt);
+  for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[lds_pad(k)], shift_sign(k) * scale);
+}
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Gridding as a matrix product on the matrix cores.  For one 16x16 cell tile
@@ -71,9 +137,9 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // partial buffers that k_fr_grid_reduce sums in chunk order, so the result is
 // bitwise reproducible.
 __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
-    const int4 *__restrict__ items, int nitems, const int2 *__restrict__ ent,
-    const float4 *__restrict__ geom, const float2 *__restrict__ G, float2 *__restrict__ F,
-    float *__restrict__ part, int n, int ntile, int nslots, int nsamp, int nzc, float K2) {
+    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
+    const float2 *__restrict__ G, float2 *__restrict__ F, float *__restrict__ part, int n,
+    int ntile, int nslots, int nsamp, int nzc, float K2) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int it = blockIdx.x * kGridWaves + wave;
   const int grp = blockIdx.y;
@@ -85,29 +151,45 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int e = ebeg; e < ebeg + cnt; e += 4) {
-    const int2 en = ent[e + k];
+  // entry {sample, footprint offsets (cols | rows << 8), du bits, dv bits};
+  // loads run one step ahead of the matrix products
+  const int eend = ebeg + cnt;
+  int4 en = cnt > 0 ? ent[ebeg + k] : make_int4(-1, 0, 0, 0);
+  float4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
+  if (en.x >= 0) {
+    va = Gg[(size_t)en.x * 2];
+    vb = Gg[(size_t)en.x * 2 + 1];
+  }
+  for (int e = ebeg; e < eend; e += 4) {
+    const int4 cur = en;
+    const float4 ca = va, cb = vb;
+    if (e + 4 < eend) {
+      en = ent[e + 4 + k];
+      if (en.x >= 0) {
+        va = Gg[(size_t)en.x * 2];
+        vb = Gg[(size_t)en.x * 2 + 1];
+      } else {
+        va = vb = float4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
     float e0v = 0.f, e1v = 0.f;
-    float4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
-    if (en.x >= 0) {
-      const float4 g = geom[en.x];
-      const int rb0 = (int)(signed char)(en.y & 0xff);
-      const int rb1 = (int)(signed char)((en.y >> 8) & 0xff);
+    if (cur.x >= 0) {
+      const int rb0 = (int)(signed char)(cur.y & 0xff);
+      const int rb1 = (int)(signed char)((cur.y >> 8) & 0xff);
       const int d0 = i - rb0, d1 = i - rb1;
-      const float x0 = (float)(d0 - 4) - g.x, x1 = (float)(d1 - 4) - g.y;
+      const float x0 = (float)(d0 - 4) - __int_as_float(cur.z);
+      const float x1 = (float)(d1 - 4) - __int_as_float(cur.w);
       e0v = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
       e1v = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
-      va = Gg[(size_t)en.x * 2];
-      vb = Gg[(size_t)en.x * 2 + 1];
     }
-    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.x, e0v, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.y, e0v, acc[1], 0, 0, 0);
-    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.z, e0v, acc[2], 0, 0, 0);
-    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * va.w, e0v, acc[3], 0, 0, 0);
-    acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.x, e0v, acc[4], 0, 0, 0);
-    acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.y, e0v, acc[5], 0, 0, 0);
-    acc[6] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.z, e0v, acc[6], 0, 0, 0);
-    acc[7] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * vb.w, e0v, acc[7], 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.x, e0v, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.y, e0v, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.z, e0v, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.w, e0v, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.x, e0v, acc[4], 0, 0, 0);
+    acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.y, e0v, acc[5], 0, 0, 0);
+    acc[6] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.z, e0v, acc[6], 0, 0, 0);
+    acc[7] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.w, e0v, acc[7], 0, 0, 0);
   }
   const int j = lane & 15, ib = 4 * (lane >> 4);
   if (slot < 0) {
@@ -157,7 +239,7 @@ __global__ void k_fr_grid_reduce(const int4 *__restrict__ red, const float *__re
       F[(size_t)(grp * 4 + c) * plane + (size_t)a * n2 + b] = make_float2(acc[2 * c], acc[2 * c + 1]);
 }
 
-__global__ void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R, FftDesc fd,
+__global__ __launch_bounds__(512) void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R, FftDesc fd,
                             int n, int nzc) {
   extern __shared__ float2 sm[];
   const int a = blockIdx.x, c = blockIdx.y;
@@ -179,7 +261,7 @@ __global__ void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R
   }
 }
 
-__global__ void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ obj,
+__global__ __launch_bounds__(512) void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ obj,
                             const float *__restrict__ phix, const float *__restrict__ phiy,
                             const int2 *__restrict__ circ, FftDesc fd, int n, int nz, int nzc,
                             int lgCB) {
@@ -214,7 +296,7 @@ __global__ void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ ob
 
 // ---- adjoint (forward projection) kernels --------------------------------
 
-__global__ void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
+__global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
                                 const float *__restrict__ phix, const float *__restrict__ phiy,
                                 const int2 *__restrict__ circ, FftDesc fd, int n, int nz,
                                 int nzc, int lgCB) {
@@ -249,7 +331,7 @@ __global__ void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restric
   }
 }
 
-__global__ void k_fr_rowfft_adj(const float2 *__restrict__ R, float2 *__restrict__ F, FftDesc fd,
+__global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict__ R, float2 *__restrict__ F, FftDesc fd,
                                 int n, int nzc) {
   extern __shared__ float2 sm[];
   const int a = blockIdx.x, c = blockIdx.y;
@@ -320,7 +402,7 @@ __global__ void k_fr_interp(const float4 *__restrict__ geom, const float2 *__res
   for (int c = 0; c < 4; ++c) out[c] = acc[c];
 }
 
-__global__ void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ data, FftDesc fd,
+__global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ data, FftDesc fd,
                               int n, int nproj, int nz, int nzc, float scale) {
   extern __shared__ float2 sm[];
   const int p = blockIdx.x, c = blockIdx.y;
@@ -420,7 +502,7 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   }
   std::vector<int64_t> off(ntiles + 1, 0);
   for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + 3) / 4 * 4;
-  std::vector<int2> ent(std::max<int64_t>(1, off[ntiles]), make_int2(-1, 0));
+  std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, 0, 0, 0));
   std::vector<int64_t> fill(off.begin(), off.end() - 1);
   for (int64_t s = 0; s < S; ++s) {
     int tr[4], rr[4], tc[4], rc2[4];
@@ -432,7 +514,11 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
           return TOMO_EINVAL;
         }
         const size_t t = (size_t)tr[u] * nt + tc[v];
-        ent[fill[t]++] = make_int2((int)s, (rc2[v] & 0xff) | ((rr[u] & 0xff) << 8));
+        float du = geom[s].x, dv = geom[s].y;
+        int dub, dvb;
+        std::memcpy(&dub, &du, 4);
+        std::memcpy(&dvb, &dv, 4);
+        ent[fill[t]++] = make_int4((int)s, (rc2[v] & 0xff) | ((rr[u] & 0xff) << 8), dub, dvb);
       }
   }
   TOMO_REQUIRE(off[ntiles] < ((int64_t)1 << 31), "fr_create: tile lists too large");
@@ -458,9 +544,9 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   P->nitems = (int)items.size();
   P->nred = (int)red.size();
   P->nslots = nslots;
-  int rc = P->ent.alloc(sizeof(int2) * ent.size());
+  int rc = P->ent.alloc(sizeof(int4) * ent.size());
   if (rc) return rc;
-  TOMO_HIP(hipMemcpy(P->ent.p, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+  TOMO_HIP(hipMemcpy(P->ent.p, ent.data(), sizeof(int4) * ent.size(), hipMemcpyHostToDevice));
   if ((rc = P->items.alloc(sizeof(int4) * std::max<size_t>(1, items.size())))) return rc;
   TOMO_HIP(hipMemcpy(P->items.p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
   if (!red.empty()) {
@@ -565,7 +651,8 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   const size_t col1 = (size_t)lds_padded_len((int)n2) * sizeof(float2);
   int CB = 1, lg = 0;
   while (CB * 2 <= 16 && (size_t)(CB * 2) * col1 <= maxlds &&
-         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) > 0)
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) > 0 &&
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) <= 512)
     CB *= 2, lg += 1;
   P->colCB = CB;
   P->lgCB = lg;
@@ -574,7 +661,8 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2);
   P->prepThreads = fft_threads(P->fft1.d, n);
   P->rowThreads = fft_threads(P->fft2.d, n2);
-  if (P->colThreads < 0 || P->prepThreads < 0 || P->rowThreads < 0) {
+  if (P->colThreads < 0 || P->prepThreads < 0 || P->rowThreads < 0 || P->colThreads > 512 ||
+      P->prepThreads > 512 || P->rowThreads > 512) {
     set_error("fr_create: n=%lld needs more than 1024 threads per FFT workgroup", (long long)n);
     return fail(TOMO_EINVAL);
   }
@@ -593,25 +681,16 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   return TOMO_OK;
 }
 
-extern "C" int fr_backprojection(tomo_fr_plan *P, float *obj, const float *data, void *stream) {
-  clear_error();
-  TOMO_REQUIRE(P != nullptr, "fr_backprojection: plan is NULL");
-  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_backprojection: NULL array");
-  hipStream_t st = (hipStream_t)stream;
-  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
+// Gridding, the two pruned FFT passes and the output stage, from G.
+static int fr_tail(tomo_fr_plan *P, float *obj, hipStream_t st) {
+  const int n = (int)P->n, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   const int n2 = 2 * n;
-  {
-    ProfScope ps_("k_fr_prep", st);
-    hipLaunchKernelGGL(k_fr_prep, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
-                       data, P->G.as<float2>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
-  }
-  TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_grid", st);
     hipLaunchKernelGGL(k_fr_grid, dim3((unsigned)ceil_div(P->nitems, kGridWaves), ng),
                        dim3(64 * kGridWaves), 0, st, P->items.as<int4>(), P->nitems,
-                       P->ent.as<int2>(), P->geom.as<float4>(), P->G.as<float2>(),
+                       P->ent.as<int4>(), P->G.as<float2>(),
                        P->F.as<float2>(), P->part.as<float>(), n, P->ntile, P->nslots,
                        (int)P->nsamp, nzc, P->K2);
   }
@@ -636,6 +715,51 @@ extern "C" int fr_backprojection(tomo_fr_plan *P, float *obj, const float *data,
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
+}
+
+extern "C" int fr_backprojection(tomo_fr_plan *P, float *obj, const float *data, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "fr_backprojection: plan is NULL");
+  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_backprojection: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
+  const int ng = (int)P->ngrp;
+  {
+    ProfScope ps_("k_fr_prep", st);
+    hipLaunchKernelGGL(k_fr_prep, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
+                       data, P->G.as<float2>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  }
+  TOMO_LAUNCH_CHECK();
+  return fr_tail(P, obj, st);
+}
+
+extern "C" int fr_filter_backprojection(tomo_fr_plan *P, tomo_flt_plan *F, float *obj,
+                                        const float *data, const float *wfilter, double center,
+                                        const float *sht, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr && F != nullptr, "fr_filter_backprojection: plan is NULL");
+  TOMO_REQUIRE(obj != nullptr && data != nullptr && wfilter != nullptr,
+               "fr_filter_backprojection: NULL array");
+  TOMO_REQUIRE(std::isfinite(center), "fr_filter_backprojection: center is not finite");
+  TOMO_REQUIRE(F->n == P->n && F->nproj == P->nproj && F->nz == P->nz,
+               "fr_filter_backprojection: filter plan (n=%lld nproj=%lld nz=%lld) does not match "
+               "(n=%lld nproj=%lld nz=%lld)", (long long)F->n, (long long)F->nproj,
+               (long long)F->nz, (long long)P->n, (long long)P->nproj, (long long)P->nz);
+  TOMO_REQUIRE(F->H2.p != nullptr, "fr_filter_backprojection: filter plan has no fused path "
+               "(needs ne >= 2n and 2n <= 8192)");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = flt_prepare_2n(F, wfilter, center, sht, st);
+  if (rc) return rc;
+  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
+  const int ng = (int)P->ngrp;
+  {
+    ProfScope ps_("k_fr_filtprep", st);
+    hipLaunchKernelGGL(k_fr_filtprep, dim3(nproj, ng * 4), dim3(P->rowThreads), P->rowLds, st,
+                       data, P->G.as<float2>(), F->H2.as<float2>(), F->HL.as<float>(),
+                       F->HR.as<float>(), F->fft2n.d, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  }
+  TOMO_LAUNCH_CHECK();
+  return fr_tail(P, obj, st);
 }
 
 extern "C" int fr_forward(tomo_fr_plan *P, float *data, const float *obj, void *stream) {

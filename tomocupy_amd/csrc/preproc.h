@@ -29,5 +29,5 @@ struct tomo_pre_plan {
   double fw_sigma = 0.0;
   int fw_L = 0;
   std::vector<tomo::FwLevel> lv;
-  tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2];
+  tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2], notch;
 };

@@ -205,5 +205,6 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->synhi.release();
   P->syn[0].release();
   P->syn[1].release();
+  P->notch.release();
   delete P;
 }

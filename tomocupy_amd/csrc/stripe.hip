@@ -75,51 +75,72 @@ __global__ void k_fw_rows(const float *__restrict__ src, float *__restrict__ lo,
   }
 }
 
-// Analysis along the angle axis (columns) of both row bands.
-__global__ void k_fw_cols(const float *__restrict__ lo, const float *__restrict__ hi,
-                          float *__restrict__ ll, float *__restrict__ lh, float *__restrict__ hl,
-                          float *__restrict__ hh, Taps t, int H, int W, int Ho) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  const int oh = blockIdx.y, z = blockIdx.z;
-  if (w >= W) return;
+// Analysis along the angle axis (columns) of both row bands.  A workgroup
+// produces kColOut output rows of 64 columns; the 2*kColOut+L-2 input rows
+// it needs (reflected at the borders) are staged once in LDS.
+constexpr int kColOut = 32;
+__global__ __launch_bounds__(256) void k_fw_cols(const float *__restrict__ lo,
+                                                 const float *__restrict__ hi,
+                                                 float *__restrict__ ll, float *__restrict__ lh,
+                                                 float *__restrict__ hl, float *__restrict__ hh,
+                                                 Taps t, int H, int W, int Ho) {
+  __shared__ float2 xs[2 * kColOut + 30][64];
+  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + lane;
+  const int oh0 = blockIdx.y * kColOut, z = blockIdx.z;
   const int L = t.L;
   const int pad = (2 * (Ho - 1) - H + L) / 2;
-  const float *plo = lo + (size_t)z * H * W + w;
-  const float *phi = hi + (size_t)z * H * W + w;
-  float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-  for (int j = 0; j < L; ++j) {
-    const int r = refl(2 * oh + j - pad, H);
-    const float x = plo[(size_t)r * W], y = phi[(size_t)r * W];
-    a0 += t.a[j] * x;
-    a1 += t.b[j] * x;
-    b0 += t.a[j] * y;
-    b1 += t.b[j] * y;
+  const int nin = 2 * kColOut + L - 2;
+  const int base = 2 * oh0 - pad;
+  const bool live = w < W;
+  const float *plo = lo + (size_t)z * H * W + (live ? w : 0);
+  const float *phi = hi + (size_t)z * H * W + (live ? w : 0);
+  for (int i = sub; i < nin; i += 4) {
+    const int r = refl(base + i, H);
+    xs[i][lane] = live ? make_float2(plo[(size_t)r * W], phi[(size_t)r * W]) : make_float2(0.f, 0.f);
   }
-  const size_t o = ((size_t)z * Ho + oh) * W + w;
-  ll[o] = a0;
-  lh[o] = a1;
-  hl[o] = b0;
-  hh[o] = b1;
+  __syncthreads();
+  if (!live) return;
+  for (int q = sub; q < kColOut; q += 4) {
+    const int oh = oh0 + q;
+    if (oh >= Ho) break;
+    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    for (int j = 0; j < L; ++j) {
+      const float2 v = xs[2 * q + j][lane];
+      a0 += t.a[j] * v.x;
+      a1 += t.b[j] * v.x;
+      b0 += t.a[j] * v.y;
+      b1 += t.b[j] * v.y;
+    }
+    const size_t o = ((size_t)z * Ho + oh) * W + w;
+    ll[o] = a0;
+    lh[o] = a1;
+    hl[o] = b0;
+    hh[o] = b1;
+  }
 }
 
 // Notch along angles of the stripe band: x_t -= sum_k gk_k Re(X_k e^{2 pi i k t/my}),
-// X_k = sum_t x_t e^{-2 pi i k t/my}.  A workgroup takes 64 band columns;
-// its 4 waves split the rows, partial X_k are combined through LDS in a fixed
-// order, then every thread corrects its rows.
+// X_k = sum_t x_t e^{-2 pi i k t/my}.  Pass 1 (k_fw_dft): partial X_k over
+// blocks of kNotchRows rows for 64 columns, the 4 waves' partials combined in
+// LDS in a fixed order.  Pass 2 (k_fw_notch): sum the row-block partials in
+// order and correct the block's rows.
+constexpr int kNotchRows = 64;
 template <int KM>
-__global__ __launch_bounds__(256) void k_fw_damp(float *__restrict__ hl,
-                                                 const float2 *__restrict__ tw,
-                                                 const float *__restrict__ gk, int my, int mx,
-                                                 int nk) {
-  __shared__ float2 part[4][KM][64];
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int w = blockIdx.x * 64 + lane, z = blockIdx.y;
+__global__ __launch_bounds__(256) void k_fw_dft(const float *__restrict__ hl,
+                                                const float2 *__restrict__ tw,
+                                                float2 *__restrict__ part, int my, int mx, int nk) {
+  __shared__ float2 red[4][KM][64];
+  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + lane, rb = blockIdx.y, z = blockIdx.z;
+  const int nrb = gridDim.y;
   const bool live = w < mx;
-  float *col = hl + (size_t)z * my * mx + (live ? w : 0);
+  const float *col = hl + (size_t)z * my * mx + (live ? w : 0);
   float xr[KM], xi[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) xr[k] = xi[k] = 0.f;
-  for (int t = grp; t < my; t += 4) {
+  const int t1 = min(my, (rb + 1) * kNotchRows);
+  for (int t = rb * kNotchRows + sub; t < t1; t += 4) {
     const float x = live ? col[(size_t)t * mx] : 0.f;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
@@ -131,18 +152,48 @@ __global__ __launch_bounds__(256) void k_fw_damp(float *__restrict__ hl,
     }
   }
 #pragma unroll
-  for (int k = 0; k < KM; ++k) part[grp][k][lane] = make_float2(xr[k], xi[k]);
+  for (int k = 0; k < KM; ++k) red[sub][k][lane] = make_float2(xr[k], xi[k]);
   __syncthreads();
+  if (sub != 0 || !live) return;
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
-    const float2 p0 = part[0][k][lane], p1 = part[1][k][lane];
-    const float2 p2 = part[2][k][lane], p3 = part[3][k][lane];
-    const float g = k < nk ? gk[k] : 0.f;
-    xr[k] = ((p0.x + p1.x) + (p2.x + p3.x)) * g;
-    xi[k] = ((p0.y + p1.y) + (p2.y + p3.y)) * g;
+    if (k < nk) {
+      const float2 p0 = red[0][k][lane], p1 = red[1][k][lane];
+      const float2 p2 = red[2][k][lane], p3 = red[3][k][lane];
+      part[(((size_t)z * nrb + rb) * KM + k) * mx + w] =
+          make_float2((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
+    }
   }
-  if (!live) return;
-  for (int t = grp; t < my; t += 4) {
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void k_fw_notch(float *__restrict__ hl,
+                                                  const float2 *__restrict__ tw,
+                                                  const float *__restrict__ gk,
+                                                  const float2 *__restrict__ part, int my, int mx,
+                                                  int nk) {
+  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + lane, rb = blockIdx.y, z = blockIdx.z;
+  const int nrb = gridDim.y;
+  if (w >= mx) return;
+  float xr[KM], xi[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    float sr = 0.f, si = 0.f;
+    if (k < nk) {
+      for (int q = 0; q < nrb; ++q) {
+        const float2 p = part[(((size_t)z * nrb + q) * KM + k) * mx + w];
+        sr += p.x;
+        si += p.y;
+      }
+    }
+    const float g = k < nk ? gk[k] : 0.f;
+    xr[k] = sr * g;
+    xi[k] = si * g;
+  }
+  float *col = hl + (size_t)z * my * mx + w;
+  const int t1 = min(my, (rb + 1) * kNotchRows);
+  for (int t = rb * kNotchRows + sub; t < t1; t += 4) {
     float corr = 0.f;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
@@ -157,53 +208,84 @@ __global__ __launch_bounds__(256) void k_fw_damp(float *__restrict__ hl,
 
 // Synthesis along angles: lo = sfb(ll, lh), hi = sfb(hl, hh); output rows
 // t' in [0, 2H-L+2) (the L-2 crop of sfb1d).  ll has its own pitch (llH, llW)
-// because it is the cropped reconstruction of the deeper level.
-__global__ void k_fw_syn_cols(const float *__restrict__ ll, int llH, int llW,
-                              const float *__restrict__ lh, const float *__restrict__ hl,
-                              const float *__restrict__ hh, float *__restrict__ lo,
-                              float *__restrict__ hi, Taps g, int H, int W, int Hs) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  const int tp = blockIdx.y, z = blockIdx.z;
-  if (w >= W) return;
+// because it is the cropped reconstruction of the deeper level.  A workgroup
+// produces kSynOut output rows of 64 columns from kSynOut/2 + L/2 band rows
+// staged in LDS.
+constexpr int kSynOut = 64;
+__global__ __launch_bounds__(256) void k_fw_syn_cols(const float *__restrict__ ll, int llH, int llW,
+                                                     const float *__restrict__ lh,
+                                                     const float *__restrict__ hl,
+                                                     const float *__restrict__ hh,
+                                                     float *__restrict__ lo, float *__restrict__ hi,
+                                                     Taps g, int H, int W, int Hs) {
+  __shared__ float4 xs[kSynOut / 2 + 17][64];
+  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + lane;
+  const int tp0 = blockIdx.y * kSynOut, z = blockIdx.z;
   const int L = g.L;
-  const int t = tp + L - 2;
-  float a = 0.f, b = 0.f;
-  for (int j = (t & 1); j < L; j += 2) {
-    const int i = (t - j) >> 1;
-    if (i < 0 || i >= H) continue;
-    const size_t ob = ((size_t)z * H + i) * W + w;
-    a += g.a[j] * ll[((size_t)z * llH + i) * llW + w] + g.b[j] * lh[ob];
-    b += g.a[j] * hl[ob] + g.b[j] * hh[ob];
+  // band rows i with 2i + j = t for t in [tp0+L-2, tp0+kSynOut+L-3], j in [0, L)
+  const int i0 = (tp0 - 1) >> 1;  // floor((tp0 + L - 2 - (L - 1)) / 2)
+  const int nin = kSynOut / 2 + L / 2 + 1;
+  const bool live = w < W;
+  for (int q = sub; q < nin; q += 4) {
+    const int i = i0 + q;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live && i >= 0 && i < H) {
+      const size_t ob = ((size_t)z * H + i) * W + w;
+      v = make_float4(ll[((size_t)z * llH + i) * llW + w], lh[ob], hl[ob], hh[ob]);
+    }
+    xs[q][lane] = v;
   }
-  const size_t o = ((size_t)z * Hs + tp) * W + w;
-  lo[o] = a;
-  hi[o] = b;
+  __syncthreads();
+  if (!live) return;
+  for (int q = sub; q < kSynOut; q += 4) {
+    const int tp = tp0 + q;
+    if (tp >= Hs) break;
+    const int t = tp + L - 2;
+    float a = 0.f, b = 0.f;
+    for (int j = (t & 1); j < L; j += 2) {
+      const int i = (t - j) >> 1;
+      const float4 v = xs[i - i0][lane];
+      a += g.a[j] * v.x + g.b[j] * v.y;
+      b += g.a[j] * v.z + g.b[j] * v.w;
+    }
+    const size_t o = ((size_t)z * Hs + tp) * W + w;
+    lo[o] = a;
+    hi[o] = b;
+  }
 }
 
-// Synthesis along the detector axis.  LAST writes rows [st, st+nproj) and
-// columns [0, n) straight into the chunk layout res[p][z][x].
+// Synthesis along the detector axis, one workgroup per output row with both
+// input rows staged in LDS.  LAST writes rows [st, st+nproj) and columns
+// [0, n) straight into the chunk layout res[p][z][x].
 template <int LAST>
-__global__ void k_fw_syn_rows(const float *__restrict__ lo, const float *__restrict__ hi,
-                              float *__restrict__ out, Taps g, int Hs, int W, int Ws, int nz,
-                              int nproj, int st, int n) {
-  const int tp = blockIdx.x * blockDim.x + threadIdx.x;
-  const int h = blockIdx.y, z = blockIdx.z;
-  if (tp >= Ws) return;
-  if (LAST && (h < st || h >= st + nproj || tp >= n)) return;
-  const int L = g.L;
-  const int t = tp + L - 2;
+__global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ lo,
+                                                     const float *__restrict__ hi,
+                                                     float *__restrict__ out, Taps g, int Hs, int W,
+                                                     int Ws, int nz, int nproj, int st, int n) {
+  extern __shared__ float2 rs[];
+  const int h = blockIdx.x, z = blockIdx.y;
+  if (LAST && (h < st || h >= st + nproj)) return;
   const float *pl = lo + ((size_t)z * Hs + h) * W;
   const float *ph = hi + ((size_t)z * Hs + h) * W;
-  float a = 0.f;
-  for (int j = (t & 1); j < L; j += 2) {
-    const int i = (t - j) >> 1;
-    if (i < 0 || i >= W) continue;
-    a += g.a[j] * pl[i] + g.b[j] * ph[i];
+  for (int i = threadIdx.x; i < W; i += blockDim.x) rs[i] = make_float2(pl[i], ph[i]);
+  __syncthreads();
+  const int L = g.L;
+  const int nout = LAST ? min(Ws, n) : Ws;
+  for (int tp = threadIdx.x; tp < nout; tp += blockDim.x) {
+    const int t = tp + L - 2;
+    float a = 0.f;
+    for (int j = (t & 1); j < L; j += 2) {
+      const int i = (t - j) >> 1;
+      if (i < 0 || i >= W) continue;
+      const float2 v = rs[i];
+      a += g.a[j] * v.x + g.b[j] * v.y;
+    }
+    if (LAST)
+      out[((size_t)(h - st) * nz + z) * n + tp] = a;
+    else
+      out[((size_t)z * Hs + h) * Ws + tp] = a;
   }
-  if (LAST)
-    out[((size_t)(h - st) * nz + z) * n + tp] = a;
-  else
-    out[((size_t)z * Hs + h) * Ws + tp] = a;
 }
 
 const WaveletTaps *find_wavelet(const char *name) {
@@ -226,7 +308,7 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
   const int L = wv->len;
   const int64_t nz = P->nz;
   int H = (int)(P->nproj + P->nproj / 8), W = (int)P->n;
-  size_t rowmax = 0, synmax = 0, outmax = 0;
+  size_t rowmax = 0, synmax = 0, outmax = 0, partmax = 0;
   for (int k = 0; k < level; ++k) {
     FwLevel &l = P->lv[k];
     l.H = H;
@@ -269,6 +351,7 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
       TOMO_HIP(hipMemcpy(l.tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
       TOMO_HIP(hipMemcpy(l.gk.p, g.data(), sizeof(float) * l.nk, hipMemcpyHostToDevice));
     }
+    partmax = std::max(partmax, (size_t)ceil_div(l.Ho, kNotchRows) * kMaxNotch * l.Wo);
     const size_t Hs = 2 * (size_t)l.Ho - L + 2, Ws = 2 * (size_t)l.Wo - L + 2;
     synmax = std::max(synmax, Hs * l.Wo);
     outmax = std::max(outmax, Hs * Ws);
@@ -281,7 +364,8 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
       (rc = P->synlo.alloc(sizeof(float) * nz * synmax)) ||
       (rc = P->synhi.alloc(sizeof(float) * nz * synmax)) ||
       (rc = P->syn[0].alloc(sizeof(float) * nz * outmax)) ||
-      (rc = P->syn[1].alloc(sizeof(float) * nz * outmax)))
+      (rc = P->syn[1].alloc(sizeof(float) * nz * outmax)) ||
+      (rc = P->notch.alloc(sizeof(float2) * nz * partmax)))
     return rc;
   P->fw_name = wv->name;
   P->fw_levels = level;
@@ -332,24 +416,35 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
     TOMO_LAUNCH_CHECK();
     {
       ProfScope ps_("k_fw_cols", st);
-      hipLaunchKernelGGL(k_fw_cols, dim3((unsigned)ceil_div(l.Wo, 256), l.Ho, nz), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_fw_cols, dim3((unsigned)ceil_div(l.Wo, 64), (unsigned)ceil_div(l.Ho, kColOut), nz),
+                         dim3(256), 0, st,
                          P->rowlo.as<float>(), P->rowhi.as<float>(), l.ll.as<float>(),
                          l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(), ana, l.H, l.Wo,
                          l.Ho);
     }
     TOMO_LAUNCH_CHECK();
     if (l.nk > 0) {
-      ProfScope ps_("k_fw_damp", st);
-      const dim3 g((unsigned)ceil_div(l.Wo, 64), nz);
+      const dim3 g((unsigned)ceil_div(l.Wo, 64), (unsigned)ceil_div(l.Ho, kNotchRows), nz);
+      float2 *part = P->notch.as<float2>();
+#define TOMO_NOTCH(KM)                                                                        \
+  do {                                                                                        \
+    {                                                                                         \
+      ProfScope ps_("k_fw_dft", st);                                                          \
+      hipLaunchKernelGGL(k_fw_dft<KM>, g, dim3(256), 0, st, l.hl.as<float>(),                 \
+                         l.tw.as<float2>(), part, l.Ho, l.Wo, l.nk);                          \
+    }                                                                                         \
+    TOMO_LAUNCH_CHECK();                                                                      \
+    ProfScope ps_("k_fw_notch", st);                                                          \
+    hipLaunchKernelGGL(k_fw_notch<KM>, g, dim3(256), 0, st, l.hl.as<float>(),                 \
+                       l.tw.as<float2>(), l.gk.as<float>(), part, l.Ho, l.Wo, l.nk);          \
+  } while (0)
       if (l.nk <= 8)
-        hipLaunchKernelGGL(k_fw_damp<8>, g, dim3(256), 0, st, l.hl.as<float>(), l.tw.as<float2>(),
-                           l.gk.as<float>(), l.Ho, l.Wo, l.nk);
+        TOMO_NOTCH(8);
       else if (l.nk <= 16)
-        hipLaunchKernelGGL(k_fw_damp<16>, g, dim3(256), 0, st, l.hl.as<float>(), l.tw.as<float2>(),
-                           l.gk.as<float>(), l.Ho, l.Wo, l.nk);
+        TOMO_NOTCH(16);
       else
-        hipLaunchKernelGGL(k_fw_damp<32>, g, dim3(256), 0, st, l.hl.as<float>(), l.tw.as<float2>(),
-                           l.gk.as<float>(), l.Ho, l.Wo, l.nk);
+        TOMO_NOTCH(32);
+#undef TOMO_NOTCH
     }
     TOMO_LAUNCH_CHECK();
   }
@@ -363,7 +458,8 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
     const int Hs = 2 * Hb - L + 2, Ws = 2 * Wb - L + 2;
     {
       ProfScope ps_("k_fw_syn_cols", st);
-      hipLaunchKernelGGL(k_fw_syn_cols, dim3((unsigned)ceil_div(Wb, 256), Hs, nz), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_fw_syn_cols, dim3((unsigned)ceil_div(Wb, 64), (unsigned)ceil_div(Hs, kSynOut), nz),
+                         dim3(256), 0, st,
                          cur, curH, curW, l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(),
                          P->synlo.as<float>(), P->synhi.as<float>(), syn, Hb, Wb, Hs);
     }
@@ -371,14 +467,13 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
     float *out = P->syn[k & 1].as<float>();
     {
       ProfScope ps_("k_fw_syn_rows", st);
+      const size_t rl = sizeof(float2) * Wb;
       if (k == 0)
-        hipLaunchKernelGGL(k_fw_syn_rows<1>, dim3((unsigned)ceil_div(Ws, 256), Hs, nz), dim3(256), 0,
-                           st, P->synlo.as<float>(), P->synhi.as<float>(), res, syn, Hs, Wb, Ws, nz,
-                           nproj, st0, n);
+        hipLaunchKernelGGL(k_fw_syn_rows<1>, dim3(Hs, nz), dim3(256), rl, st, P->synlo.as<float>(),
+                           P->synhi.as<float>(), res, syn, Hs, Wb, Ws, nz, nproj, st0, n);
       else
-        hipLaunchKernelGGL(k_fw_syn_rows<0>, dim3((unsigned)ceil_div(Ws, 256), Hs, nz), dim3(256), 0,
-                           st, P->synlo.as<float>(), P->synhi.as<float>(), out, syn, Hs, Wb, Ws, nz,
-                           nproj, st0, n);
+        hipLaunchKernelGGL(k_fw_syn_rows<0>, dim3(Hs, nz), dim3(256), rl, st, P->synlo.as<float>(),
+                           P->synhi.as<float>(), out, syn, Hs, Wb, Ws, nz, nproj, st0, n);
     }
     TOMO_LAUNCH_CHECK();
     cur = out;

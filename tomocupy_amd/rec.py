@@ -91,8 +91,7 @@ class GPURec:
 
     def _process(self, data, dark, flat, rec):
         out = self.pf.proc_chunk(data, dark, flat, out=self._sino)
-        self.bf.fbp_filter_center(out)
-        self.bf.cl_rec.backprojection(rec, out)
+        self.bf.fbp_filter_backprojection(rec, out)
 
     def recon_resident(self, raw, dark, flat, rec):
         """Reconstruct a GPU-resident volume: ``raw (nzchunk, nproj, ncz, n)`` uint16,
