@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "slices/s at 2048^2 x1500 proj, FP32 vs FP16 at rel-L2 vs FP64; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_dft', 'k_fw_notch', 'k_fw_syn_cols',
+KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_xsum', 'k_fw_syn_cols',
            'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_flt_kernel2n',
            'k_fr_prep', 'k_fr_filtprep',
            'k_fr_grid', 'k_fr_grid_reduce', 'k_fr_rowfft', 'k_fr_colfft']
@@ -48,11 +48,48 @@ def parse():
     ap.add_argument('--cpu-baseline', default='on', choices=['on', 'off'])
     ap.add_argument('--cpu-workers', type=int, default=0, help='0 = min(16, cpu count)')
     ap.add_argument('--no-prof', action='store_true', help='do not time kernels with events')
+    ap.add_argument('--copy-calibration-mb', type=float, default=0.0,
+                    help='run one device-to-device copy of this size before the timed steps '
+                         '(a known-size kernel for HBM counter calibration)')
     return ap.parse_args()
+
+
+def fw_levels(a, L=32, level=7):
+    """Band shapes (H, W, Ho, Wo, Hs, Ws) of the sym16 Fourier-wavelet levels."""
+    H, W = a.nproj + a.nproj // 8, a.n
+    out = []
+    for _ in range(level):
+        Ho, Wo = (H + L - 1) // 2, (W + L - 1) // 2
+        out.append((H, W, Ho, Wo, 2 * Ho - L + 2, 2 * Wo - L + 2))
+        H, W = Ho, Wo
+    return out
+
+
+def fw_bytes(name, a):
+    """Mean external float32 I/O per launch of a Fourier-wavelet kernel (one
+    launch per level and chunk, levels launched equally often)."""
+    ch = a.chunk
+    per = []
+    for k, (H, W, Ho, Wo, Hs, Ws) in enumerate(fw_levels(a)):
+        if name == 'k_fw_rows':      # read the rows (level 0: the unpadded chunk), write lo+hi
+            rd = a.nproj * a.n if k == 0 else H * W
+            per.append(4 * ch * (rd + 2 * H * Wo))
+        elif name == 'k_fw_cols':    # read lo+hi, write the four bands
+            per.append(4 * ch * (2 * H * Wo + 4 * Ho * Wo))
+        elif name == 'k_fw_syn_cols':  # read the four bands, write lo+hi
+            per.append(4 * ch * (4 * Ho * Wo + 2 * Hs * Wo))
+        elif name == 'k_fw_syn_rows':  # read lo+hi, write the level (last: the chunk rows)
+            wr = a.nproj * a.n if k == 0 else Hs * Ws
+            per.append(4 * ch * (2 * Hs * Wo + wr))
+        else:
+            return None
+    return sum(per) // len(per)
 
 
 def algorithmic_bytes(name, a, s):
     """External I/O of one launch of a kernel at storage precision (bytes)."""
+    if name.startswith('k_fw_') and a.stripe == 'fw':
+        return fw_bytes(name, a)
     n, nproj, ch = a.n, a.nproj, a.chunk
     nzc = (ch + 1) // 2
     grp = 4  # complex slices per gridding group
@@ -178,6 +215,13 @@ def main():
                 out = pfs[i].proc_chunk(raw[k], dark[k], flat[k], out=sinos[i])
                 bfs[i].fbp_filter_backprojection(rec[k * a.chunk:(k + 1) * a.chunk], out)
 
+    if a.copy_calibration_mb > 0:
+        nb = int(a.copy_calibration_mb * 1e6) // 4
+        src = torch.ones(nb, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()

@@ -169,10 +169,10 @@ __device__ __forceinline__ void twiddle(float2 *a, const float2 *__restrict__ tw
   }
 }
 
-template <int R, int DIR>
+template <int R, int DIR, int EPT>
 __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float2 *tw, int lgH,
                                          int tid, int nt) {
-  constexpr int MAXB = kFftEPT / R;
+  constexpr int MAXB = EPT / R;
   const int H = 1 << lgH;
   const int nbPer = N / R;
   const int nb = nbPer << lgH;
@@ -230,33 +230,42 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
 // 2^lgH * N <= 15 * nt (16 * nt without radix-3 passes); see fft_threads().
 // The caller must __syncthreads() after filling buf; the function ends with a
 // barrier.
-template <int DIR>
+template <int DIR, int EPT = kFftEPT>
 __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt) {
   int Ns = 1;
   for (int s = 0; s < d.nr; ++s) {
     const int R = d.r[s];
     if (R == 16)
-      fft_pass<16, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass<16, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 8)
-      fft_pass<8, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass<8, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 4)
-      fft_pass<4, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass<4, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 2)
-      fft_pass<2, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass<2, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else
-      fft_pass<3, DIR>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass<3, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     Ns *= R;
   }
+}
+
+// Maps a flattened workgroup id to a logical one such that the workgroups the
+// dispatcher sends to one XCD (round robin over 8 XCDs) receive consecutive
+// logical ids: neighbouring tiles then share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int id, int total) {
+  const int q = total / 8, r = total % 8;
+  const int x = id % 8, i = id / 8;
+  return x * q + min(x, r) + i;
 }
 
 // Workgroup size for transforming `elems` complex values (all interleaved
 // sequences) with plan d: 16 values per thread for radices 2/4/8, 15 when a
 // radix-3 pass is present; at least 256 and a multiple of 64.  Returns -1 if
 // more than 1024 threads would be needed.
-inline int fft_threads(const FftDesc &d, int64_t elems) {
+inline int fft_threads(const FftDesc &d, int64_t elems, int ept = kFftEPT) {
   bool r3 = false;
   for (int i = 0; i < d.nr; ++i) r3 |= (d.r[i] == 3);
-  const int64_t per = r3 ? 15 : 16;
+  const int64_t per = r3 ? (ept / 3) * 3 : ept;
   int64_t t = (elems + per - 1) / per;
   t = ((t + 63) / 64) * 64;
   if (t > 1024) return -1;

@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
 constexpr int kGridChunk = 512; // max list entries per work item (multiple of 4)
+constexpr int kColEPT = 32;     // FFT values per thread in the column pass
 
 // Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
@@ -266,7 +267,9 @@ __global__ __launch_bounds__(512) void k_fr_colfft(const float2 *__restrict__ R,
                             const int2 *__restrict__ circ, FftDesc fd, int n, int nz, int nzc,
                             int lgCB) {
   extern __shared__ float2 sm[];  // [a][t]: CB interleaved columns
-  const int cb = blockIdx.x, c = blockIdx.y;
+  // column blocks that share cache lines go to the same XCD
+  const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int cb = L % gridDim.x, c = L / gridDim.x;
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int CB = 1 << lgCB;
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(512) void k_fr_colfft(const float2 *__restrict__ R,
     sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
   }
   __syncthreads();
-  lds_fft<1>(sm, fd, lgCB, tid, nt);
+  lds_fft<1, kColEPT>(sm, fd, lgCB, tid, nt);
   const int z2 = c + nzc;
   const int h = n / 2;
   for (int idx = tid; idx < CB * n; idx += nt) {
@@ -301,7 +304,8 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
                                 const int2 *__restrict__ circ, FftDesc fd, int n, int nz,
                                 int nzc, int lgCB) {
   extern __shared__ float2 sm[];  // [a][t]: CB interleaved columns
-  const int cb = blockIdx.x, c = blockIdx.y;
+  const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int cb = L % gridDim.x, c = L / gridDim.x;
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int CB = 1 << lgCB;
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
     sm[(lds_pad(u) << lgCB) + t] = make_float2(re * ph, im * ph);
   }
   __syncthreads();
-  lds_fft<-1>(sm, fd, lgCB, tid, nt);
+  lds_fft<-1, kColEPT>(sm, fd, lgCB, tid, nt);
   float2 *dst = R + (size_t)c * n2 * n;
   for (int idx = tid; idx < CB * n2; idx += nt) {
     const int a = idx >> lgCB, t = idx & (CB - 1);
@@ -651,14 +655,14 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   const size_t col1 = (size_t)lds_padded_len((int)n2) * sizeof(float2);
   int CB = 1, lg = 0;
   while (CB * 2 <= 16 && (size_t)(CB * 2) * col1 <= maxlds &&
-         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) > 0 &&
-         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2) <= 512)
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2, kColEPT) > 0 &&
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2, kColEPT) <= 512)
     CB *= 2, lg += 1;
   P->colCB = CB;
   P->lgCB = lg;
   P->colLds = (size_t)CB * col1;
   TOMO_REQUIRE(P->colLds <= maxlds, "fr_create: 2n=%lld exceeds the LDS of one workgroup", (long long)n2);
-  P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2);
+  P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2, kColEPT);
   P->prepThreads = fft_threads(P->fft1.d, n);
   P->rowThreads = fft_threads(P->fft2.d, n2);
   if (P->colThreads < 0 || P->prepThreads < 0 || P->rowThreads < 0 || P->colThreads > 512 ||

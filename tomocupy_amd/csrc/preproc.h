@@ -13,6 +13,7 @@ namespace tomo {
 struct FwLevel {
   int H = 0, W = 0, Ho = 0, Wo = 0;
   int nk = 0;
+  size_t xoff = 0;  // offset of this level's notch coefficients in notchX (float2)
   DevBuf tw;   // (nk, Ho) float2: cos/sin(2 pi k t / Ho)
   DevBuf gk;   // (nk,) float: c_k g_k / Ho
   DevBuf ll, lh, hl, hh;  // (nz, Ho, Wo) each
@@ -29,5 +30,5 @@ struct tomo_pre_plan {
   double fw_sigma = 0.0;
   int fw_L = 0;
   std::vector<tomo::FwLevel> lv;
-  tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2], notch;
+  tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2], notch, notchX;
 };

@@ -206,5 +206,6 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->syn[0].release();
   P->syn[1].release();
   P->notch.release();
+  P->notchX.release();
   delete P;
 }

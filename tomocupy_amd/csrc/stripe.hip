@@ -39,13 +39,21 @@ __device__ __forceinline__ int refl(int m, int N) {
   return mm >= N ? P - 1 - mm : mm;
 }
 
-// Analysis along the detector axis (rows).  MODE 0 reads the chunk layout
-// res[p][z][x] with the zero angle padding; MODE 1 reads a band (z, H, W).
+__device__ __forceinline__ int refl_fast(int m, int N) {
+  return (unsigned)m < (unsigned)N ? m : refl(m, N);
+}
+
+// Analysis along the detector axis (rows), one workgroup per (row, slice).
+// MODE 0 reads the chunk layout res[p][z][x] with the zero angle padding;
+// MODE 1 reads a band (z, H, W).  The reflected row x is staged in LDS and
+// read as pairs (x[2k], x[2k+1]): lo[o] = sum_m a[2m] x[2(o+m)] + a[2m+1] x[2(o+m)+1].
 template <int MODE>
-__global__ void k_fw_rows(const float *__restrict__ src, float *__restrict__ lo,
-                          float *__restrict__ hi, Taps t, int H, int W, int Wo, int nz, int nproj,
-                          int st) {
-  extern __shared__ float xs[];
+__global__ __launch_bounds__(256) void k_fw_rows(const float *__restrict__ src,
+                                                 float *__restrict__ lo, float *__restrict__ hi,
+                                                 Taps t, int H, int W, int Wo, int nz, int nproj,
+                                                 int st) {
+  extern __shared__ float2 xs2[];
+  float *xs = reinterpret_cast<float *>(xs2);
   const int h = blockIdx.x, z = blockIdx.y;
   const int L = t.L;
   const int pad = (2 * (Wo - 1) - W + L) / 2;
@@ -59,16 +67,30 @@ __global__ void k_fw_rows(const float *__restrict__ src, float *__restrict__ lo,
   } else {
     row = src + ((size_t)z * H + h) * W;
   }
-  for (int i = threadIdx.x; i < np; i += blockDim.x) xs[i] = zero ? 0.f : row[refl(i - pad, W)];
+  constexpr int U = 8;
+  for (int i0 = threadIdx.x; i0 < np; i0 += U * 256) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * 256;
+      v[u] = (!zero && i < np) ? row[refl_fast(i - pad, W)] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * 256 < np) xs[i0 + u * 256] = v[u];
+  }
   __syncthreads();
   float *plo = lo + ((size_t)z * H + h) * Wo;
   float *phi = hi + ((size_t)z * H + h) * Wo;
-  for (int o = threadIdx.x; o < Wo; o += blockDim.x) {
+  const int L2 = L >> 1;
+  for (int o = threadIdx.x; o < Wo; o += 256) {
     float a = 0.f, b = 0.f;
-    for (int j = 0; j < L; ++j) {
-      const float x = xs[2 * o + j];
-      a += t.a[j] * x;
-      b += t.b[j] * x;
+    for (int m = 0; m < L2; ++m) {
+      const float2 x = xs2[o + m];
+      a = fmaf(t.a[2 * m], x.x, a);
+      a = fmaf(t.a[2 * m + 1], x.y, a);
+      b = fmaf(t.b[2 * m], x.x, b);
+      b = fmaf(t.b[2 * m + 1], x.y, b);
     }
     plo[o] = a;
     phi[o] = b;
@@ -76,188 +98,226 @@ __global__ void k_fw_rows(const float *__restrict__ src, float *__restrict__ lo,
 }
 
 // Analysis along the angle axis (columns) of both row bands.  A workgroup
-// produces kColOut output rows of 64 columns; the 2*kColOut+L-2 input rows
-// it needs (reflected at the borders) are staged once in LDS.
+// produces `ot` (<= kColOut) output rows of 64 columns; the 2*ot+L-2 input
+// rows it needs (reflected at the borders) are staged once in LDS, with every
+// load of a thread issued before the first LDS store.  It also accumulates the
+// partial DFT X_k (k < nk) of its rows of the stripe band hl along angles,
+// written per row block for the notch (k_fw_xsum).
 constexpr int kColOut = 32;
-__global__ __launch_bounds__(256) void k_fw_cols(const float *__restrict__ lo,
-                                                 const float *__restrict__ hi,
-                                                 float *__restrict__ ll, float *__restrict__ lh,
-                                                 float *__restrict__ hl, float *__restrict__ hh,
-                                                 Taps t, int H, int W, int Ho) {
-  __shared__ float2 xs[2 * kColOut + 30][64];
+constexpr int kColIn = 2 * kColOut + 30;  // taps L <= 32
+template <int KM>
+__global__ __launch_bounds__(256) void k_fw_cols(
+    const float *__restrict__ lo, const float *__restrict__ hi, float *__restrict__ ll,
+    float *__restrict__ lh, float *__restrict__ hl, float *__restrict__ hh, Taps t, int H, int W,
+    int Ho, int ot, const float2 *__restrict__ tw, float2 *__restrict__ part, int nk) {
+  __shared__ float2 xs[kColIn][64];
+  static_assert(2 * KM <= kColIn, "partial-sum reduction must fit the staging LDS");
   const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
   const int w = blockIdx.x * 64 + lane;
-  const int oh0 = blockIdx.y * kColOut, z = blockIdx.z;
+  const int rb = blockIdx.y, oh0 = rb * ot, z = blockIdx.z;
+  const int nrb = gridDim.y;
   const int L = t.L;
   const int pad = (2 * (Ho - 1) - H + L) / 2;
-  const int nin = 2 * kColOut + L - 2;
+  const int nin = 2 * ot + L - 2;
   const int base = 2 * oh0 - pad;
   const bool live = w < W;
   const float *plo = lo + (size_t)z * H * W + (live ? w : 0);
   const float *phi = hi + (size_t)z * H * W + (live ? w : 0);
-  for (int i = sub; i < nin; i += 4) {
-    const int r = refl(base + i, H);
-    xs[i][lane] = live ? make_float2(plo[(size_t)r * W], phi[(size_t)r * W]) : make_float2(0.f, 0.f);
+  constexpr int kIt = (kColIn + 3) / 4;
+  float2 v[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = sub + 4 * it;
+    v[it] = make_float2(0.f, 0.f);
+    if (live && i < nin) {
+      const size_t r = (size_t)refl_fast(base + i, H) * W;
+      v[it] = make_float2(plo[r], phi[r]);
+    }
   }
+#pragma unroll
+  for (int it = 0; it < kIt; ++it)
+    if (sub + 4 * it < nin) xs[sub + 4 * it][lane] = v[it];
   __syncthreads();
-  if (!live) return;
-  for (int q = sub; q < kColOut; q += 4) {
+  float xr[KM], xi[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) xr[k] = xi[k] = 0.f;
+  for (int q = sub; q < ot; q += 4) {
     const int oh = oh0 + q;
     if (oh >= Ho) break;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
     for (int j = 0; j < L; ++j) {
-      const float2 v = xs[2 * q + j][lane];
-      a0 += t.a[j] * v.x;
-      a1 += t.b[j] * v.x;
-      b0 += t.a[j] * v.y;
-      b1 += t.b[j] * v.y;
+      const float2 x = xs[2 * q + j][lane];
+      a0 = fmaf(t.a[j], x.x, a0);
+      a1 = fmaf(t.b[j], x.x, a1);
+      b0 = fmaf(t.a[j], x.y, b0);
+      b1 = fmaf(t.b[j], x.y, b1);
     }
-    const size_t o = ((size_t)z * Ho + oh) * W + w;
-    ll[o] = a0;
-    lh[o] = a1;
-    hl[o] = b0;
-    hh[o] = b1;
-  }
-}
-
-// Notch along angles of the stripe band: x_t -= sum_k gk_k Re(X_k e^{2 pi i k t/my}),
-// X_k = sum_t x_t e^{-2 pi i k t/my}.  Pass 1 (k_fw_dft): partial X_k over
-// blocks of kNotchRows rows for 64 columns, the 4 waves' partials combined in
-// LDS in a fixed order.  Pass 2 (k_fw_notch): sum the row-block partials in
-// order and correct the block's rows.
-constexpr int kNotchRows = 64;
-template <int KM>
-__global__ __launch_bounds__(256) void k_fw_dft(const float *__restrict__ hl,
-                                                const float2 *__restrict__ tw,
-                                                float2 *__restrict__ part, int my, int mx, int nk) {
-  __shared__ float2 red[4][KM][64];
-  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
-  const int w = blockIdx.x * 64 + lane, rb = blockIdx.y, z = blockIdx.z;
-  const int nrb = gridDim.y;
-  const bool live = w < mx;
-  const float *col = hl + (size_t)z * my * mx + (live ? w : 0);
-  float xr[KM], xi[KM];
-#pragma unroll
-  for (int k = 0; k < KM; ++k) xr[k] = xi[k] = 0.f;
-  const int t1 = min(my, (rb + 1) * kNotchRows);
-  for (int t = rb * kNotchRows + sub; t < t1; t += 4) {
-    const float x = live ? col[(size_t)t * mx] : 0.f;
+    // wave-uniform row: the twiddles come through scalar loads
+    const int ohu = __builtin_amdgcn_readfirstlane(oh);
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
       if (k < nk) {
-        const float2 c = tw[k * my + t];
-        xr[k] += x * c.x;
-        xi[k] -= x * c.y;
+        const float2 c = tw[k * Ho + ohu];
+        xr[k] = fmaf(b0, c.x, xr[k]);
+        xi[k] = fmaf(-b0, c.y, xi[k]);
       }
     }
+    if (live) {
+      const size_t o = ((size_t)z * Ho + oh) * W + w;
+      ll[o] = a0;
+      lh[o] = a1;
+      hl[o] = b0;
+      hh[o] = b1;
+    }
   }
+  if (nk == 0) return;
+  // reduce the four row groups' partial sums through the (now free) staging
+  // LDS: groups 2,3 hand to 0,1, then 1 hands to 0
+  float2 *red = &xs[0][0];
+  __syncthreads();
+  if (sub >= 2) {
 #pragma unroll
-  for (int k = 0; k < KM; ++k) red[sub][k][lane] = make_float2(xr[k], xi[k]);
+    for (int k = 0; k < KM; ++k) red[((sub - 2) * KM + k) * 64 + lane] = make_float2(xr[k], xi[k]);
+  }
+  __syncthreads();
+  if (sub < 2) {
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const float2 p = red[(sub * KM + k) * 64 + lane];
+      xr[k] += p.x;
+      xi[k] += p.y;
+    }
+  }
+  __syncthreads();
+  if (sub == 1) {
+#pragma unroll
+    for (int k = 0; k < KM; ++k) red[k * 64 + lane] = make_float2(xr[k], xi[k]);
+  }
   __syncthreads();
   if (sub != 0 || !live) return;
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
     if (k < nk) {
-      const float2 p0 = red[0][k][lane], p1 = red[1][k][lane];
-      const float2 p2 = red[2][k][lane], p3 = red[3][k][lane];
-      part[(((size_t)z * nrb + rb) * KM + k) * mx + w] =
-          make_float2((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
+      const float2 p = red[k * 64 + lane];
+      part[(((size_t)z * nrb + rb) * KM + k) * W + w] = make_float2(xr[k] + p.x, xi[k] + p.y);
     }
   }
 }
 
+// Notch coefficients of the stripe band: X_k = (sum over row blocks, in
+// order, of the partial DFTs) * gk_k, gk_k = c_k g_k / my.  One thread per
+// (column, k, slice).
 template <int KM>
-__global__ __launch_bounds__(256) void k_fw_notch(float *__restrict__ hl,
-                                                  const float2 *__restrict__ tw,
-                                                  const float *__restrict__ gk,
-                                                  const float2 *__restrict__ part, int my, int mx,
-                                                  int nk) {
+__global__ __launch_bounds__(256) void k_fw_xsum(const float2 *__restrict__ part,
+                                                 const float *__restrict__ gk,
+                                                 float2 *__restrict__ X, int W, int nrb) {
+  const int w = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, z = blockIdx.z;
+  if (w >= W) return;
+  const float2 *p = part + ((size_t)z * nrb * KM + k) * W + w;
+  const size_t stride = (size_t)KM * W;
+  float sr = 0.f, si = 0.f;
+#pragma unroll 8
+  for (int q = 0; q < nrb; ++q) {
+    const float2 v = p[q * stride];
+    sr += v.x;
+    si += v.y;
+  }
+  const float s = gk[k];
+  X[((size_t)z * KM + k) * W + w] = make_float2(sr * s, si * s);
+}
+
+// Synthesis along angles: lo = sfb(ll, lh), hi = sfb(hl', hh); output rows
+// t' in [0, 2H-L+2) (the L-2 crop of sfb1d).  ll has its own pitch (llH, llW)
+// because it is the cropped reconstruction of the deeper level.  hl' is the
+// notched stripe band, corrected while it is staged:
+//   hl'[i] = hl[i] - sum_k Re(X_k e^{2 pi i k i/H}).
+// With t = t' + L - 2 even (L even), outputs t and t+1 read the same band
+// rows i = t/2 - m, m < L/2, through the even and the odd taps, so a thread
+// produces output rows in pairs.  A workgroup produces `ot` (<= kSynOut) rows
+// of 64 columns from ot/2 + L/2 - 1 band rows staged in LDS.
+constexpr int kSynOut = 64;
+constexpr int kSynIn = kSynOut / 2 + 15;  // taps L <= 32
+template <int KM>
+__global__ __launch_bounds__(256) void k_fw_syn_cols(
+    const float *__restrict__ ll, int llH, int llW, const float *__restrict__ lh,
+    const float *__restrict__ hl, const float *__restrict__ hh, float *__restrict__ lo,
+    float *__restrict__ hi, Taps g, int H, int W, int Hs, int ot, const float2 *__restrict__ tw,
+    const float2 *__restrict__ X, int nk) {
+  __shared__ float4 xs[kSynIn][64];
   const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
-  const int w = blockIdx.x * 64 + lane, rb = blockIdx.y, z = blockIdx.z;
-  const int nrb = gridDim.y;
-  if (w >= mx) return;
+  const int w = blockIdx.x * 64 + lane;
+  const int tp0 = blockIdx.y * ot, z = blockIdx.z;
+  const int L2 = g.L >> 1;
+  const int i0 = tp0 >> 1;
+  const int nin = ot / 2 + L2 - 1;
+  const bool live = w < W;
   float xr[KM], xi[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
-    float sr = 0.f, si = 0.f;
-    if (k < nk) {
-      for (int q = 0; q < nrb; ++q) {
-        const float2 p = part[(((size_t)z * nrb + q) * KM + k) * mx + w];
-        sr += p.x;
-        si += p.y;
-      }
-    }
-    const float g = k < nk ? gk[k] : 0.f;
-    xr[k] = sr * g;
-    xi[k] = si * g;
+    const float2 x = (live && k < nk) ? X[((size_t)z * KM + k) * W + w] : make_float2(0.f, 0.f);
+    xr[k] = x.x;
+    xi[k] = x.y;
   }
-  float *col = hl + (size_t)z * my * mx + w;
-  const int t1 = min(my, (rb + 1) * kNotchRows);
-  for (int t = rb * kNotchRows + sub; t < t1; t += 4) {
-    float corr = 0.f;
+  constexpr int kIt = (kSynIn + 3) / 4;
+  float4 v[kIt];
 #pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < nk) {
-        const float2 c = tw[k * my + t];
-        corr += xr[k] * c.x - xi[k] * c.y;
-      }
-    }
-    col[(size_t)t * mx] -= corr;
-  }
-}
-
-// Synthesis along angles: lo = sfb(ll, lh), hi = sfb(hl, hh); output rows
-// t' in [0, 2H-L+2) (the L-2 crop of sfb1d).  ll has its own pitch (llH, llW)
-// because it is the cropped reconstruction of the deeper level.  A workgroup
-// produces kSynOut output rows of 64 columns from kSynOut/2 + L/2 band rows
-// staged in LDS.
-constexpr int kSynOut = 64;
-__global__ __launch_bounds__(256) void k_fw_syn_cols(const float *__restrict__ ll, int llH, int llW,
-                                                     const float *__restrict__ lh,
-                                                     const float *__restrict__ hl,
-                                                     const float *__restrict__ hh,
-                                                     float *__restrict__ lo, float *__restrict__ hi,
-                                                     Taps g, int H, int W, int Hs) {
-  __shared__ float4 xs[kSynOut / 2 + 17][64];
-  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
-  const int w = blockIdx.x * 64 + lane;
-  const int tp0 = blockIdx.y * kSynOut, z = blockIdx.z;
-  const int L = g.L;
-  // band rows i with 2i + j = t for t in [tp0+L-2, tp0+kSynOut+L-3], j in [0, L)
-  const int i0 = (tp0 - 1) >> 1;  // floor((tp0 + L - 2 - (L - 1)) / 2)
-  const int nin = kSynOut / 2 + L / 2 + 1;
-  const bool live = w < W;
-  for (int q = sub; q < nin; q += 4) {
-    const int i = i0 + q;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (live && i >= 0 && i < H) {
+  for (int it = 0; it < kIt; ++it) {
+    const int q = sub + 4 * it, i = i0 + q;
+    v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live && q < nin && i < H) {
       const size_t ob = ((size_t)z * H + i) * W + w;
-      v = make_float4(ll[((size_t)z * llH + i) * llW + w], lh[ob], hl[ob], hh[ob]);
+      v[it] = make_float4(ll[((size_t)z * llH + i) * llW + w], lh[ob], hl[ob], hh[ob]);
     }
-    xs[q][lane] = v;
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int q = sub + 4 * it;
+    if (q < nin) {
+      const int iu = __builtin_amdgcn_readfirstlane(i0 + q);
+      if (iu < H) {
+        float c = v[it].z;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          if (k < nk) {
+            const float2 e = tw[k * H + iu];
+            c -= xr[k] * e.x - xi[k] * e.y;
+          }
+        }
+        v[it].z = c;
+      }
+      xs[q][lane] = v[it];
+    }
   }
   __syncthreads();
   if (!live) return;
-  for (int q = sub; q < kSynOut; q += 4) {
-    const int tp = tp0 + q;
+  for (int pq = sub; pq < ot / 2; pq += 4) {
+    const int tp = tp0 + 2 * pq;
     if (tp >= Hs) break;
-    const int t = tp + L - 2;
-    float a = 0.f, b = 0.f;
-    for (int j = (t & 1); j < L; j += 2) {
-      const int i = (t - j) >> 1;
-      const float4 v = xs[i - i0][lane];
-      a += g.a[j] * v.x + g.b[j] * v.y;
-      b += g.a[j] * v.z + g.b[j] * v.w;
+    float a0 = 0.f, b0 = 0.f, a1 = 0.f, b1 = 0.f;
+    for (int m = 0; m < L2; ++m) {
+      const float4 x = xs[pq + L2 - 1 - m][lane];
+      const float ge = g.a[2 * m], he = g.b[2 * m], go = g.a[2 * m + 1], ho = g.b[2 * m + 1];
+      a0 = fmaf(ge, x.x, a0);
+      a0 = fmaf(he, x.y, a0);
+      b0 = fmaf(ge, x.z, b0);
+      b0 = fmaf(he, x.w, b0);
+      a1 = fmaf(go, x.x, a1);
+      a1 = fmaf(ho, x.y, a1);
+      b1 = fmaf(go, x.z, b1);
+      b1 = fmaf(ho, x.w, b1);
     }
     const size_t o = ((size_t)z * Hs + tp) * W + w;
-    lo[o] = a;
-    hi[o] = b;
+    lo[o] = a0;
+    lo[o + W] = a1;
+    hi[o] = b0;
+    hi[o + W] = b1;
   }
 }
 
 // Synthesis along the detector axis, one workgroup per output row with both
-// input rows staged in LDS.  LAST writes rows [st, st+nproj) and columns
-// [0, n) straight into the chunk layout res[p][z][x].
+// input rows staged in LDS; outputs in pairs as in k_fw_syn_cols.  LAST writes
+// rows [st, st+nproj) and columns [0, n) straight into the chunk layout
+// res[p][z][x].
 template <int LAST>
 __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ lo,
                                                      const float *__restrict__ hi,
@@ -268,23 +328,33 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
   if (LAST && (h < st || h >= st + nproj)) return;
   const float *pl = lo + ((size_t)z * Hs + h) * W;
   const float *ph = hi + ((size_t)z * Hs + h) * W;
-  for (int i = threadIdx.x; i < W; i += blockDim.x) rs[i] = make_float2(pl[i], ph[i]);
-  __syncthreads();
-  const int L = g.L;
-  const int nout = LAST ? min(Ws, n) : Ws;
-  for (int tp = threadIdx.x; tp < nout; tp += blockDim.x) {
-    const int t = tp + L - 2;
-    float a = 0.f;
-    for (int j = (t & 1); j < L; j += 2) {
-      const int i = (t - j) >> 1;
-      if (i < 0 || i >= W) continue;
-      const float2 v = rs[i];
-      a += g.a[j] * v.x + g.b[j] * v.y;
+  constexpr int U = 4;
+  for (int i0 = threadIdx.x; i0 < W; i0 += U * 256) {
+    float2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * 256;
+      v[u] = i < W ? make_float2(pl[i], ph[i]) : make_float2(0.f, 0.f);
     }
-    if (LAST)
-      out[((size_t)(h - st) * nz + z) * n + tp] = a;
-    else
-      out[((size_t)z * Hs + h) * Ws + tp] = a;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * 256 < W) rs[i0 + u * 256] = v[u];
+  }
+  __syncthreads();
+  const int L2 = g.L >> 1;
+  const int nout = LAST ? min(Ws, n) : Ws;
+  float *orow = LAST ? out + ((size_t)(h - st) * nz + z) * n : out + ((size_t)z * Hs + h) * Ws;
+  for (int p = threadIdx.x; 2 * p < nout; p += 256) {
+    float a0 = 0.f, a1 = 0.f;
+    for (int m = 0; m < L2; ++m) {
+      const float2 x = rs[p + L2 - 1 - m];
+      a0 = fmaf(g.a[2 * m], x.x, a0);
+      a0 = fmaf(g.b[2 * m], x.y, a0);
+      a1 = fmaf(g.a[2 * m + 1], x.x, a1);
+      a1 = fmaf(g.b[2 * m + 1], x.y, a1);
+    }
+    orow[2 * p] = a0;
+    if (2 * p + 1 < nout) orow[2 * p + 1] = a1;
   }
 }
 
@@ -308,7 +378,7 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
   const int L = wv->len;
   const int64_t nz = P->nz;
   int H = (int)(P->nproj + P->nproj / 8), W = (int)P->n;
-  size_t rowmax = 0, synmax = 0, outmax = 0, partmax = 0;
+  size_t rowmax = 0, synmax = 0, outmax = 0, partmax = 0, xsum_total = 0;
   for (int k = 0; k < level; ++k) {
     FwLevel &l = P->lv[k];
     l.H = H;
@@ -351,7 +421,9 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
       TOMO_HIP(hipMemcpy(l.tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
       TOMO_HIP(hipMemcpy(l.gk.p, g.data(), sizeof(float) * l.nk, hipMemcpyHostToDevice));
     }
-    partmax = std::max(partmax, (size_t)ceil_div(l.Ho, kNotchRows) * kMaxNotch * l.Wo);
+    partmax = std::max(partmax, (size_t)ceil_div(l.Ho, 8) * kMaxNotch * l.Wo);
+    l.xoff = xsum_total;
+    xsum_total += (size_t)kMaxNotch * l.Wo * nz;
     const size_t Hs = 2 * (size_t)l.Ho - L + 2, Ws = 2 * (size_t)l.Wo - L + 2;
     synmax = std::max(synmax, Hs * l.Wo);
     outmax = std::max(outmax, Hs * Ws);
@@ -365,7 +437,8 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
       (rc = P->synhi.alloc(sizeof(float) * nz * synmax)) ||
       (rc = P->syn[0].alloc(sizeof(float) * nz * outmax)) ||
       (rc = P->syn[1].alloc(sizeof(float) * nz * outmax)) ||
-      (rc = P->notch.alloc(sizeof(float2) * nz * partmax)))
+      (rc = P->notch.alloc(sizeof(float2) * nz * partmax)) ||
+      (rc = P->notchX.alloc(sizeof(float2) * xsum_total)))
     return rc;
   P->fw_name = wv->name;
   P->fw_levels = level;
@@ -385,6 +458,9 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
   TOMO_REQUIRE(level >= 1 && level <= 16, "pre_remove_stripe_fw: level=%d out of range", level);
   const WaveletTaps *wv = find_wavelet(wname);
   TOMO_REQUIRE(wv != nullptr, "pre_remove_stripe_fw: unknown wavelet '%s'", wname);
+  TOMO_REQUIRE(wv->len % 2 == 0 && wv->len <= 32,
+               "pre_remove_stripe_fw: wavelet '%s' has %d taps (even, at most 32, supported)",
+               wname, wv->len);
   int rc = setup_fw(P, wv, sigma, level);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
@@ -414,39 +490,35 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
                          ana, l.H, l.W, l.Wo, nz, nproj, st0);
     }
     TOMO_LAUNCH_CHECK();
-    {
-      ProfScope ps_("k_fw_cols", st);
-      hipLaunchKernelGGL(k_fw_cols, dim3((unsigned)ceil_div(l.Wo, 64), (unsigned)ceil_div(l.Ho, kColOut), nz),
-                         dim3(256), 0, st,
-                         P->rowlo.as<float>(), P->rowhi.as<float>(), l.ll.as<float>(),
-                         l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(), ana, l.H, l.Wo,
-                         l.Ho);
-    }
-    TOMO_LAUNCH_CHECK();
-    if (l.nk > 0) {
-      const dim3 g((unsigned)ceil_div(l.Wo, 64), (unsigned)ceil_div(l.Ho, kNotchRows), nz);
-      float2 *part = P->notch.as<float2>();
-#define TOMO_NOTCH(KM)                                                                        \
-  do {                                                                                        \
-    {                                                                                         \
-      ProfScope ps_("k_fw_dft", st);                                                          \
-      hipLaunchKernelGGL(k_fw_dft<KM>, g, dim3(256), 0, st, l.hl.as<float>(),                 \
-                         l.tw.as<float2>(), part, l.Ho, l.Wo, l.nk);                          \
-    }                                                                                         \
-    TOMO_LAUNCH_CHECK();                                                                      \
-    ProfScope ps_("k_fw_notch", st);                                                          \
-    hipLaunchKernelGGL(k_fw_notch<KM>, g, dim3(256), 0, st, l.hl.as<float>(),                 \
-                       l.tw.as<float2>(), l.gk.as<float>(), part, l.Ho, l.Wo, l.nk);          \
+    const int otc = l.Ho >= 256 ? kColOut : (l.Ho >= 64 ? 16 : 8);
+    const int nrb = (int)ceil_div(l.Ho, otc);
+    const dim3 gc((unsigned)ceil_div(l.Wo, 64), nrb, nz);
+    float2 *part = P->notch.as<float2>();
+    float2 *Xk = P->notchX.as<float2>();
+#define TOMO_FW_ANA(KM)                                                                          \
+  do {                                                                                           \
+    {                                                                                            \
+      ProfScope ps_("k_fw_cols", st);                                                            \
+      hipLaunchKernelGGL(k_fw_cols<KM>, gc, dim3(256), 0, st, P->rowlo.as<float>(),              \
+                         P->rowhi.as<float>(), l.ll.as<float>(), l.lh.as<float>(),                \
+                         l.hl.as<float>(), l.hh.as<float>(), ana, l.H, l.Wo, l.Ho, otc,           \
+                         l.tw.as<float2>(), part, l.nk);                                         \
+    }                                                                                            \
+    TOMO_LAUNCH_CHECK();                                                                         \
+    if (l.nk > 0) {                                                                              \
+      ProfScope ps_("k_fw_xsum", st);                                                            \
+      hipLaunchKernelGGL(k_fw_xsum<KM>, dim3((unsigned)ceil_div(l.Wo, 256), l.nk, nz), dim3(256), \
+                         0, st, part, l.gk.as<float>(), Xk + l.xoff, l.Wo, nrb);                 \
+    }                                                                                            \
+    TOMO_LAUNCH_CHECK();                                                                         \
   } while (0)
-      if (l.nk <= 8)
-        TOMO_NOTCH(8);
-      else if (l.nk <= 16)
-        TOMO_NOTCH(16);
-      else
-        TOMO_NOTCH(32);
-#undef TOMO_NOTCH
-    }
-    TOMO_LAUNCH_CHECK();
+    if (l.nk <= 8)
+      TOMO_FW_ANA(8);
+    else if (l.nk <= 16)
+      TOMO_FW_ANA(16);
+    else
+      TOMO_FW_ANA(32);
+#undef TOMO_FW_ANA
   }
   // synthesis, deepest level first; `cur` is the current low band (pitch curH x curW)
   const float *cur = P->lv[level - 1].ll.as<float>();
@@ -456,13 +528,25 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
     const int Hb = l.Ho, Wb = l.Wo;
     TOMO_REQUIRE(curH >= Hb && curW >= Wb, "pre_remove_stripe_fw: internal band shape mismatch");
     const int Hs = 2 * Hb - L + 2, Ws = 2 * Wb - L + 2;
-    {
-      ProfScope ps_("k_fw_syn_cols", st);
-      hipLaunchKernelGGL(k_fw_syn_cols, dim3((unsigned)ceil_div(Wb, 64), (unsigned)ceil_div(Hs, kSynOut), nz),
-                         dim3(256), 0, st,
-                         cur, curH, curW, l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(),
-                         P->synlo.as<float>(), P->synhi.as<float>(), syn, Hb, Wb, Hs);
-    }
+    const int ots = Hs >= 512 ? kSynOut : (Hs >= 128 ? 32 : 16);
+    const dim3 gs((unsigned)ceil_div(Wb, 64), (unsigned)ceil_div(Hs, ots), nz);
+    const int KMk = l.nk <= 8 ? 8 : (l.nk <= 16 ? 16 : 32);
+    const float2 *Xk = P->notchX.as<float2>() + l.xoff;
+#define TOMO_FW_SYN(KM)                                                                          \
+  do {                                                                                           \
+    ProfScope ps_("k_fw_syn_cols", st);                                                          \
+    hipLaunchKernelGGL(k_fw_syn_cols<KM>, gs, dim3(256), 0, st, cur, curH, curW,                 \
+                       l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(),                     \
+                       P->synlo.as<float>(), P->synhi.as<float>(), syn, Hb, Wb, Hs, ots,         \
+                       l.tw.as<float2>(), Xk, l.nk);                                             \
+  } while (0)
+    if (KMk == 8)
+      TOMO_FW_SYN(8);
+    else if (KMk == 16)
+      TOMO_FW_SYN(16);
+    else
+      TOMO_FW_SYN(32);
+#undef TOMO_FW_SYN
     TOMO_LAUNCH_CHECK();
     float *out = P->syn[k & 1].as<float>();
     {

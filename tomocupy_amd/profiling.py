@@ -1,0 +1,131 @@
+"""Summaries of rocprofv3 output for this library's kernels.
+
+Usage (paths are rocprofv3 ``-f csv`` outputs)::
+
+  python -m tomocupy_amd.profiling stats  KERNEL_STATS_CSV  [--title T] [--cmd C]
+  python -m tomocupy_amd.profiling traffic FETCH_COUNTER_CSV WRITE_COUNTER_CSV
+         --copy-bytes B [--key-suffix S] [--json OUT.json] [--title T] [--cmd C]
+
+``stats`` prints a markdown table of the kernel-trace statistics.  ``traffic``
+averages ``FETCH_SIZE`` and ``WRITE_SIZE`` (KiB per dispatch) per kernel from
+two ``--pmc`` runs and scales each by the factor that makes the calibration
+copy kernel (the largest device-to-device copy in the trace, ``--copy-bytes``
+read and written) read and write exactly its size.
+"""
+import argparse
+import csv
+import json
+import re
+import sys
+from collections import defaultdict
+
+
+def short_name(name):
+    m = re.search(r'(k_[A-Za-z0-9_]+)', name)
+    if m:
+        base = m.group(1)
+        t = re.search(r'k_[A-Za-z0-9_]+<([^>]*)>', name)
+        return base + ('<%s>' % t.group(1) if t else '')
+    if 'copyBuffer' in name or 'direct_copy' in name or 'elementwise_kernel' in name:
+        return 'copy:' + name[:40]
+    return name[:60]
+
+
+def cmd_stats(a):
+    rows = list(csv.DictReader(open(a.csv)))
+    out = []
+    if a.title:
+        out.append('# %s\n' % a.title)
+    if a.cmd:
+        out.append('Command:\n\n```\n%s\n```\n' % a.cmd)
+    out.append('| kernel | calls | total ms | avg us | min us | max us | % |')
+    out.append('|---|---|---|---|---|---|---|')
+    for r in rows:
+        name = short_name(r['Name'])
+        if not name.startswith('k_'):
+            continue
+        out.append('| %s | %s | %.3f | %.1f | %.1f | %.1f | %.2f |' % (
+            name, r['Calls'], float(r['TotalDurationNs']) / 1e6, float(r['AverageNs']) / 1e3,
+            float(r['MinNs']) / 1e3, float(r['MaxNs']) / 1e3, float(r['Percentage'])))
+    print('\n'.join(out))
+
+
+def _per_kernel(path, counter):
+    acc = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r.get('Counter_Name', counter) != counter:
+            continue
+        acc[r['Kernel_Name']].append(float(r['Counter_Value']))
+    return acc
+
+
+def cmd_traffic(a):
+    fetch = _per_kernel(a.fetch, 'FETCH_SIZE')
+    write = _per_kernel(a.write, 'WRITE_SIZE')
+    # calibration: the dispatch with the largest fetch among copy kernels
+    copies = [(max(v), k) for k, v in fetch.items() if short_name(k).startswith('copy:')]
+    if not copies:
+        sys.exit('no copy kernel found for calibration')
+    cf, ck = max(copies)
+    cw = max(write.get(ck, [0.0]))
+    f_scale = a.copy_bytes / (cf * 1024.0)
+    w_scale = a.copy_bytes / (cw * 1024.0) if cw else 1.0
+    res = {}
+    lines = []
+    if a.title:
+        lines.append('# %s\n' % a.title)
+    if a.cmd:
+        lines.append('Commands:\n\n```\n%s\n```\n' % a.cmd)
+    lines.append('Calibration: %s copied %.1f MB; raw FETCH_SIZE %.1f MB, WRITE_SIZE %.1f MB '
+                 '-> scale %.3f (fetch), %.3f (write).\n'
+                 % (short_name(ck), a.copy_bytes / 1e6, cf * 1024 / 1e6, cw * 1024 / 1e6,
+                    f_scale, w_scale))
+    lines.append('| kernel | dispatches | fetch MB/dispatch | write MB/dispatch | total MB |')
+    lines.append('|---|---|---|---|---|')
+    for k in sorted(fetch):
+        sn = short_name(k)
+        if not sn.startswith('k_'):
+            continue
+        fv = fetch[k]
+        wv = write.get(k, [0.0])
+        fb = sum(fv) / len(fv) * 1024 * f_scale
+        wb = sum(wv) / len(wv) * 1024 * w_scale
+        base = sn.split('<')[0]
+        key = base + (a.key_suffix or '')
+        res[key] = round(fb + wb)
+        lines.append('| %s | %d | %.2f | %.2f | %.2f |' % (sn, len(fv), fb / 1e6, wb / 1e6,
+                                                          (fb + wb) / 1e6))
+    print('\n'.join(lines))
+    if a.json:
+        try:
+            old = json.load(open(a.json))
+        except (OSError, ValueError):
+            old = {}
+        old.update(res)
+        json.dump(old, open(a.json, 'w'), indent=1, sort_keys=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest='cmd_name', required=True)
+    s = sub.add_parser('stats')
+    s.add_argument('csv')
+    s.add_argument('--title', default='')
+    s.add_argument('--cmd', default='')
+    t = sub.add_parser('traffic')
+    t.add_argument('fetch')
+    t.add_argument('write')
+    t.add_argument('--copy-bytes', type=float, required=True)
+    t.add_argument('--key-suffix', default='')
+    t.add_argument('--json', default='')
+    t.add_argument('--title', default='')
+    t.add_argument('--cmd', default='')
+    a = ap.parse_args(argv)
+    if a.cmd_name == 'stats':
+        cmd_stats(a)
+    else:
+        cmd_traffic(a)
+
+
+if __name__ == '__main__':
+    main()
