@@ -39,7 +39,7 @@ def parse():
     ap.add_argument('--nz', type=int, default=2048)
     ap.add_argument('--chunk', type=int, default=8)
     ap.add_argument('--streams', type=int, default=3)
-    ap.add_argument('--dtype', default='float32')
+    ap.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
     ap.add_argument('--stripe', default='fw', choices=['fw', 'none'])
     ap.add_argument('--filter', default='ramp')
     ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
@@ -87,24 +87,29 @@ def fw_bytes(name, a):
 
 
 def algorithmic_bytes(name, a, s):
-    """External I/O of one launch of a kernel at storage precision (bytes)."""
+    """External I/O of one launch of a kernel at storage precision (bytes):
+    c = bytes of a stored complex value, r = of a stored real value."""
     if name.startswith('k_fw_') and a.stripe == 'fw':
         return fw_bytes(name, a)
     n, nproj, ch = a.n, a.nproj, a.chunk
+    r = 4 if a.dtype == 'float32' else 2
+    c = 2 * r
     nzc = (ch + 1) // 2
     grp = 4  # complex slices per gridding group
     if name == 'k_fr_grid':      # read the polar spectra, write the 2n x 2n grid
-        return grp * (nproj * n * 8 + (2 * n) ** 2 * 8)
+        return grp * (nproj * n * c + (2 * n) ** 2 * c)
     if name == 'k_fr_rowfft':    # read the grid rows, write the pruned half
-        return nzc * ((2 * n) ** 2 * 8 + 2 * n * n * 8)
+        return nzc * ((2 * n) ** 2 * c + 2 * n * n * c)
     if name == 'k_fr_colfft':    # read the pruned rows, write two real slices per complex slice
-        return nzc * (2 * n * n * 8) + ch * n * n * 4
+        return nzc * (2 * n * n * c) + ch * n * n * r
     if name in ('k_fr_prep', 'k_fr_filtprep'):  # read the real sinograms, write the spectra
-        return ch * nproj * n * 4 + grp * nproj * n * 8
+        return ch * nproj * n * r + grp * nproj * n * c
     if name == 'k_flt_rows':     # read + write the sinograms in place
-        return 2 * ch * nproj * n * 4
-    if name == 'k_pre_rows':     # read uint16 counts, write float32 sinograms
-        return ch * nproj * n * (2 + 4)
+        return 2 * ch * nproj * n * r
+    if name == 'k_pre_rows':     # read uint16 counts, write the sinograms
+        return ch * nproj * n * (2 + r)
+    if name == 'k_pre_log_swap':  # read float32 transmission, write the sinograms
+        return ch * nproj * n * (4 + r)
     return None
 
 
@@ -195,6 +200,7 @@ def main():
     nchunks = raw.shape[0]
     th = torch.from_numpy(theta).cuda()
     streams = [torch.cuda.Stream() for _ in range(a.streams)]
+    td = _lib.torch_dtype(a.dtype)
     pfs, bfs, sinos = [], [], []
     t0 = time.time()
     for s in streams:
@@ -203,8 +209,8 @@ def main():
                                      flat.shape[1], a.dtype, remove_stripe_method=a.stripe,
                                      fw_sigma=1.0, fw_filter='sym16', fw_level=7))
             bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, a.dtype, a.filter))
-            sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=torch.float32, device='cuda'))
-    rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=torch.float32, device='cuda')
+            sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=td, device='cuda'))
+    rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=td, device='cuda')
     torch.cuda.synchronize()
     print('# plans ready in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
 

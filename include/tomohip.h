@@ -15,7 +15,9 @@
  *     flight at a time (calls on one stream are ordered).  Distinct plans may
  *     be used from distinct threads.
  *   - dtype: TOMO_F32 (float32 storage and arithmetic), TOMO_F16 / TOMO_BF16
- *     (16-bit storage of the sinogram/spectra, FP32 arithmetic).
+ *     (IEEE half / bfloat16 storage of the sinograms, spectra, grids and
+ *     images; float32 arithmetic and accumulation; FP64-derived tables).
+ *     Arrays documented as "dtype" have the element type of the plan.
  */
 #ifndef TOMOHIP_H
 #define TOMOHIP_H
@@ -68,7 +70,10 @@ typedef struct tomo_flt_plan tomo_flt_plan;
  *            odd nz is accepted and paired with an implicit zero slice
  *   theta    device float32 (nproj,) angles in radians; copied at creation
  *            (the reference keeps a borrowed pointer)
- *   dtype    TOMO_F32 (this build); TOMO_F16/TOMO_BF16 return TOMO_EUNSUPPORTED
+ *   dtype    TOMO_F32, TOMO_F16 or TOMO_BF16 (the reference's float16 module,
+ *            cfunc_fourierrecfp16.i); 16-bit plans store their spectra 2^8
+ *            above the float32 scaling, so half storage stays in its normal
+ *            range for sinograms of the usual (post -log, filtered) magnitude
  *   stream   stream used for the creation-time uploads (synchronised)
  */
 int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
@@ -78,13 +83,13 @@ int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
  * Replaces FourierRec.backprojection(obj, data, stream)
  * (fourierrec.py:61-68) and cfunc_fourierrec::backprojection
  * (cfunc_fourierrec.cu:67-93).
- *   obj   output, device float32 (nz, n, n)
- *   data  input filtered sinograms, device float32 (nz, nproj, n); NOT modified
+ *   obj   output, device dtype (nz, n, n)
+ *   data  input filtered sinograms, device dtype (nz, nproj, n); NOT modified
  *         (the reference's Python layer protects it with a packed copy)
  * The real-pair packing data[c] + i*data[c+nz/2] and the unpacking are done
  * inside the kernels.  Deterministic: no floating-point atomics.
  */
-int fr_backprojection(tomo_fr_plan *plan, float *obj, const float *data, void *stream);
+int fr_backprojection(tomo_fr_plan *plan, void *obj, const void *data, void *stream);
 
 /*
  * fbp_filter_center followed by backprojection in one pass over the data
@@ -93,12 +98,12 @@ int fr_backprojection(tomo_fr_plan *plan, float *obj, const float *data, void *s
  * edge-replicated filter of length ne is applied exactly as a length-2n
  * circular convolution plus two edge terms; the filtered sinograms are never
  * written to memory.  Unlike flt_filter_center, `data` is NOT modified.
- *   obj      output, device float32 (nz, n, n)
- *   data     device float32 (nz, nproj, n) sinograms (after -log)
+ *   obj      output, device dtype (nz, n, n)
+ *   data     device dtype (nz, nproj, n) sinograms (after -log)
  *   wfilter, center, sht  as flt_filter_center
  */
-int fr_filter_backprojection(tomo_fr_plan *plan, tomo_flt_plan *flt, float *obj,
-                             const float *data, const float *wfilter, double center,
+int fr_filter_backprojection(tomo_fr_plan *plan, tomo_flt_plan *flt, void *obj,
+                             const void *data, const float *wfilter, double center,
                              const float *sht, void *stream);
 
 /*
@@ -108,8 +113,9 @@ int fr_filter_backprojection(tomo_fr_plan *plan, tomo_flt_plan *flt, float *obj,
  * sum(fr_backprojection(d) * o) == sum(d * fr_forward(o)) to rounding.
  *   data  output, device float32 (nz, nproj, n)
  *   obj   input, device float32 (nz, n, n); NOT modified
+ * float32 plans only (TOMO_EUNSUPPORTED for 16-bit plans).
  */
-int fr_forward(tomo_fr_plan *plan, float *data, const float *obj, void *stream);
+int fr_forward(tomo_fr_plan *plan, void *data, const void *obj, void *stream);
 
 /* Replaces cfunc_fourierrec::free (cfunc_fourierrec.cu:56-65). NULL is a no-op. */
 void fr_destroy(tomo_fr_plan *plan);
@@ -127,6 +133,8 @@ int fr_get_sizes(const tomo_fr_plan *plan, int64_t *n, int64_t *nproj, int64_t *
  *   n      unpadded detector width; even, >= 20
  *   ne     padded FFT length (4n in float32, backproj_functions.py:50-54); ne >= n, even
  *   nproj, nz  rows per call
+ *   dtype  storage of the filtered rows: TOMO_F32, TOMO_F16 (cfunc_filterfp16.i) or
+ *          TOMO_BF16; the arithmetic is float32
  */
 int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t nproj, int64_t nz,
                int dtype);
@@ -134,12 +142,12 @@ int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t nproj, int64
 /*
  * Replaces cfunc_filter::filter(g, w, stream) (cfunc_filter.cu:49-61, mulw
  * kernels_filter.cuh:4-18) as called by FBPFilter.filter (fbp_filter.py:54-58).
- *   g  device float32 (nz, nproj, ne), filtered in place: C2R(R2C(g) * w[z]),
+ *   g  device dtype (nz, nproj, ne), filtered in place: C2R(R2C(g) * w[z]),
  *      unnormalised (the 1/ne is folded into the filter by calc_filter)
  *   w  device complex64 (nz, ne/2+1) as interleaved float pairs
  * The imaginary part of w at the Nyquist bin is ignored, as a C2R does.
  */
-int flt_filter(tomo_flt_plan *plan, float *g, const float *w, void *stream);
+int flt_filter(tomo_flt_plan *plan, void *g, const float *w, void *stream);
 
 /*
  * Replaces BackprojFunctions.fbp_filter_center(data, sht)
@@ -147,12 +155,12 @@ int flt_filter(tomo_flt_plan *plan, float *g, const float *w, void *stream);
  * edge-replicate padding to ne, per-slice centre phase ramp
  * w[z,k] = wfilter[k] * exp(-2 pi i (-center + sht[z] + n/2) k/ne) computed in
  * FP64, filtering and cropping, without the (nz, nproj, ne) temporary.
- *   data     device float32 (nz, nproj, n), filtered in place
+ *   data     device dtype (nz, nproj, n), filtered in place
  *   wfilter  device float32 (ne/2+1,) from flt_calc_filter
  *   center   rotation centre (pixels)
  *   sht      device float32 (nz,) per-slice extra shift, or NULL for zeros
  */
-int flt_filter_center(tomo_flt_plan *plan, float *data, const float *wfilter,
+int flt_filter_center(tomo_flt_plan *plan, void *data, const float *wfilter,
                       double center, const float *sht, void *stream);
 
 void flt_destroy(tomo_flt_plan *plan);
@@ -186,11 +194,13 @@ int pre_create(tomo_pre_plan **plan, int64_t nproj, int64_t nz, int64_t n,
  * with dark0 = mean(dark), flat0 = mean(flat)/bright_ratio*(1+1e-5) - dark0,
  * and minus_log's clamp/NaN rules.  flat_linear != 0 interpolates the flat per
  * projection as proc_functions.py:62-66.
- *   out  device float32 (nz, nproj, n)
+ * Arithmetic is float32 (the reference computes in its dtype, float16 for
+ * its half-precision runs); out_dtype only sets the storage of `out`.
+ *   out        device (nz, nproj, n) of out_dtype (TOMO_F32, TOMO_F16, TOMO_BF16)
  */
 int pre_process(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark,
-                const uint16_t *flat, float *out, double bright_ratio, int flat_linear,
-                int minus_log, void *stream);
+                const uint16_t *flat, void *out, int out_dtype, double bright_ratio,
+                int flat_linear, int minus_log, void *stream);
 
 /*
  * Dark/flat correction only, in tomocupy's projection layout:
@@ -203,10 +213,11 @@ int pre_darkflat(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark
 /*
  * minus_log (proc_functions.py:76-83) fused with the swap to sinogram order:
  *   out[z,p,x] = -log(res[p,z,x]) with x<=0 -> 0, NaN -> 6, +-inf -> 0
- * (minus_log != 0), or a plain transpose (minus_log == 0).
+ * (minus_log != 0), or a plain transpose (minus_log == 0); `out` is stored
+ * as out_dtype (TOMO_F32, TOMO_F16, TOMO_BF16).
  */
-int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, float *out, int minus_log,
-                       void *stream);
+int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, void *out, int out_dtype,
+                       int minus_log, void *stream);
 
 /*
  * Fourier-wavelet stripe removal, remove_stripe_fw

@@ -52,7 +52,8 @@ def test_null_arguments():
     assert L.fr_forward(None, None, None, None) == _lib.TOMO_EINVAL
     assert L.flt_filter(None, None, None, None) == _lib.TOMO_EINVAL
     assert L.flt_filter_center(None, None, None, 0.0, None, None) == _lib.TOMO_EINVAL
-    assert L.pre_process(None, None, None, None, None, 1.0, 0, 1, None) == _lib.TOMO_EINVAL
+    assert L.pre_process(None, None, None, None, None, 0, 1.0, 0, 1, None) == _lib.TOMO_EINVAL
+    assert L.pre_minus_log_swap(None, None, None, 0, 1, None) == _lib.TOMO_EINVAL
     L.fr_destroy(None)
     L.flt_destroy(None)
     L.pre_destroy(None)

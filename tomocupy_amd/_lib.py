@@ -16,6 +16,13 @@ LIBPATH = os.environ.get('TOMOHIP_LIB', os.path.join(HERE, 'libtomohip.so'))
 TOMO_OK, TOMO_EINVAL, TOMO_EHIP, TOMO_ENOMEM, TOMO_EUNSUPPORTED = 0, -1, -2, -3, -4
 DTYPES = {'float32': 0, 'float16': 1, 'bfloat16': 2}
 
+
+def torch_dtype(name):
+    """torch element type of the storage precision ``name``."""
+    import torch
+    return {'float32': torch.float32, 'float16': torch.float16,
+            'bfloat16': torch.bfloat16}[name]
+
 _c_i64 = ctypes.c_int64
 _c_p = ctypes.c_void_p
 _c_int = ctypes.c_int
@@ -42,9 +49,9 @@ SIGNATURES = {
     'flt_destroy': (None, [_c_p]),
     'flt_calc_filter': (_c_int, [_c_i64, ctypes.c_char_p, _c_p]),
     'pre_create': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_i64, _c_i64]),
-    'pre_process': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_int, _c_p]),
+    'pre_process': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_int, _c_p]),
     'pre_darkflat': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_p]),
-    'pre_minus_log_swap': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_p]),
+    'pre_minus_log_swap': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_int, _c_p]),
     'pre_remove_stripe_fw': (_c_int, [_c_p, _c_p, _c_d, ctypes.c_char_p, _c_int, _c_p]),
     'pre_destroy': (None, [_c_p]),
 }

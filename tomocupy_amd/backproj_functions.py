@@ -28,6 +28,7 @@ class BackprojFunctions:
             raise NotImplementedError('only the fourierrec algorithm is implemented')
         self.n, self.nproj, self.ncz = int(n), int(nproj), int(ncz)
         self.center = float(center)
+        self.dtype = dtype
         self.ne = padded_length(self.n, dtype)
         self.cl_rec = FourierRec(self.n, self.nproj, self.ncz, theta, dtype)
         self.cl_filter = FBPFilter(self.ne, self.nproj, self.ncz, dtype, n_unpadded=self.n)
@@ -49,11 +50,13 @@ class BackprojFunctions:
     def fbp_filter_backprojection(self, obj, data, sht=0, stream=None):
         """``fbp_filter_center`` then ``backprojection`` in one pass
         (``rec.py:175-176``): ``obj (ncz, n, n)`` <- filtered backprojection of
-        ``data (ncz, nproj, n)``.  ``data`` is not modified."""
+        ``data (ncz, nproj, n)``, both of the plan's dtype.  ``data`` is not
+        modified."""
         import torch
         from . import _lib
-        po = _lib.dev_ptr(obj, (self.ncz, self.n, self.n), torch.float32, 'obj')
-        pd = _lib.dev_ptr(data, (self.ncz, self.nproj, self.n), torch.float32, 'data')
+        td = _lib.torch_dtype(self.dtype)
+        po = _lib.dev_ptr(obj, (self.ncz, self.n, self.n), td, 'obj')
+        pd = _lib.dev_ptr(data, (self.ncz, self.nproj, self.n), td, 'data')
         ps = None
         if isinstance(sht, torch.Tensor):
             shv = sht.to(torch.float32).contiguous()

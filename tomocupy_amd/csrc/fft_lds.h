@@ -258,6 +258,27 @@ __device__ __forceinline__ int xcd_remap(int id, int total) {
   return x * q + min(x, r) + i;
 }
 
+// Copies `count` items into LDS with U global loads of each thread in flight
+// at once (all loads of a round are issued before the first LDS store, so
+// the round costs one memory latency rather than U): item i is produced by
+// ld(i) and written by st(i, value).
+template <int U, typename T, typename Ld, typename St>
+__device__ __forceinline__ void stage_lds(int count, int tid, int nt, Ld ld, St st) {
+  for (int base = tid; base < count; base += U * nt) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * nt;
+      if (i < count) v[u] = ld(i);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * nt;
+      if (i < count) st(i, v[u]);
+    }
+  }
+}
+
 // Workgroup size for transforming `elems` complex values (all interleaved
 // sequences) with plan d: 16 values per thread for radices 2/4/8, 15 when a
 // radix-3 pass is present; at least 256 and a multiple of 64.  Returns -1 if

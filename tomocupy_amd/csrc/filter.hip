@@ -16,6 +16,7 @@
 #include "common.h"
 #include "fft_lds.h"
 #include "internal.h"
+#include "precision.h"
 
 using namespace tomo;
 
@@ -41,23 +42,25 @@ __global__ void k_flt_weights(const float *__restrict__ wfilter, const float *__
 
 // mode 0: rows of width ne already padded (cfunc_filter::filter);
 // mode 1: rows of width n, edge-replicate padded on load and cropped on store
-template <int MODE>
-__global__ void k_flt_rows(float *__restrict__ g, const float2 *__restrict__ w, FftDesc fd, int ne,
-                           int n, int nproj) {
+template <int MODE, typename T>
+__global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const float2 *__restrict__ w,
+                                                   FftDesc fd, int ne, int n, int nproj) {
   extern __shared__ float2 sm[];
   const int pp = blockIdx.x, z = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int p0 = 2 * pp, p1 = p0 + 1;
   const bool has1 = p1 < nproj;
   const int width = MODE == 0 ? ne : n;
-  float *r0 = g + ((size_t)z * nproj + p0) * width;
-  float *r1 = g + ((size_t)z * nproj + p1) * width;
+  T *r0 = g + ((size_t)z * nproj + p0) * width;
+  T *r1 = g + ((size_t)z * nproj + (has1 ? p1 : p0)) * width;
   const int pad = ne / 2 - n / 2;
-  for (int j = tid; j < ne; j += nt) {
-    int src = j;
-    if (MODE == 1) src = min(max(j - pad, 0), n - 1);
-    sm[lds_pad(j)] = make_float2(r0[src], has1 ? r1[src] : 0.f);
-  }
+  stage_lds<8, float2>(
+      ne, tid, nt,
+      [&](int j) {
+        const int src = MODE == 1 ? min(max(j - pad, 0), n - 1) : j;
+        return make_float2(ldr(r0 + src), has1 ? ldr(r1 + src) : 0.f);
+      },
+      [&](int j, float2 v) { sm[lds_pad(j)] = v; });
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
   const float2 *wz = w + (size_t)z * (ne / 2 + 1);
@@ -78,14 +81,14 @@ __global__ void k_flt_rows(float *__restrict__ g, const float2 *__restrict__ w, 
   if (MODE == 0) {
     for (int j = tid; j < ne; j += nt) {
       const float2 y = sm[lds_pad(j)];
-      r0[j] = y.x;
-      if (has1) r1[j] = y.y;
+      str(r0 + j, y.x);
+      if (has1) str(r1 + j, y.y);
     }
   } else {
     for (int j = tid; j < n; j += nt) {
       const float2 y = sm[lds_pad(pad + j)];
-      r0[j] = y.x;
-      if (has1) r1[j] = y.y;
+      str(r0 + j, y.x);
+      if (has1) str(r1 + j, y.y);
     }
   }
 }
@@ -345,10 +348,6 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   TOMO_REQUIRE(nproj >= 1 && nz >= 1, "flt_create: nproj and nz must be >= 1");
   TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
                "flt_create: unknown dtype %d", dtype);
-  if (dtype != TOMO_F32) {
-    set_error("flt_create: dtype %d is not implemented in this build (float32 only)", dtype);
-    return TOMO_EUNSUPPORTED;
-  }
   auto *P = new (std::nothrow) tomo_flt_plan();
   if (!P) {
     set_error("flt_create: host allocation failed");
@@ -358,6 +357,7 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   P->ne = ne;
   P->nproj = nproj;
   P->nz = nz;
+  P->dtype = dtype;
   int rc = P->fft.init((int)ne);
   if (!rc) rc = P->w.alloc(sizeof(float2) * nz * (ne / 2 + 1));
   if (rc) {
@@ -397,11 +397,13 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
     flt_destroy(P);
     return TOMO_EINVAL;
   }
-  hipError_t e = hipFuncSetAttribute((const void *)k_flt_rows<0>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void *)k_flt_rows<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)P->lds);
+  const void *rows_kernels[] = {
+      (const void *)k_flt_rows<0, float>, (const void *)k_flt_rows<1, float>,
+      (const void *)k_flt_rows<0, _Float16>, (const void *)k_flt_rows<1, _Float16>,
+      (const void *)k_flt_rows<0, __bf16>, (const void *)k_flt_rows<1, __bf16>};
+  hipError_t e = hipSuccess;
+  for (const void *k : rows_kernels)
+    if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds);
   if (e != hipSuccess) {
     set_error("flt_create: hipFuncSetAttribute failed: %s", hipGetErrorString(e));
     flt_destroy(P);
@@ -411,22 +413,36 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   return TOMO_OK;
 }
 
-extern "C" int flt_filter(tomo_flt_plan *P, float *g, const float *w, void *stream) {
-  clear_error();
-  TOMO_REQUIRE(P != nullptr, "flt_filter: plan is NULL");
-  TOMO_REQUIRE(g != nullptr && w != nullptr, "flt_filter: NULL array");
-  hipStream_t st = (hipStream_t)stream;
+template <int MODE, typename T>
+static void launch_rows(tomo_flt_plan *P, void *g, const float2 *w, hipStream_t st) {
+  hipLaunchKernelGGL((k_flt_rows<MODE, T>), dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
+                     dim3(P->threads), P->lds, st, (T *)g, w, P->fft.d, (int)P->ne, (int)P->n,
+                     (int)P->nproj);
+}
+
+template <int MODE>
+static int flt_rows(tomo_flt_plan *P, void *g, const float2 *w, hipStream_t st) {
   {
     ProfScope ps_("k_flt_rows", st);
-    hipLaunchKernelGGL(k_flt_rows<0>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
-                       dim3(P->threads), P->lds, st, g, reinterpret_cast<const float2 *>(w), P->fft.d,
-                       (int)P->ne, (int)P->n, (int)P->nproj);
+    if (P->dtype == TOMO_F16)
+      launch_rows<MODE, _Float16>(P, g, w, st);
+    else if (P->dtype == TOMO_BF16)
+      launch_rows<MODE, __bf16>(P, g, w, st);
+    else
+      launch_rows<MODE, float>(P, g, w, st);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
 
-extern "C" int flt_filter_center(tomo_flt_plan *P, float *data, const float *wfilter, double center,
+extern "C" int flt_filter(tomo_flt_plan *P, void *g, const float *w, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "flt_filter: plan is NULL");
+  TOMO_REQUIRE(g != nullptr && w != nullptr, "flt_filter: NULL array");
+  return flt_rows<0>(P, g, reinterpret_cast<const float2 *>(w), (hipStream_t)stream);
+}
+
+extern "C" int flt_filter_center(tomo_flt_plan *P, void *data, const float *wfilter, double center,
                                  const float *sht, void *stream) {
   clear_error();
   TOMO_REQUIRE(P != nullptr, "flt_filter_center: plan is NULL");
@@ -440,14 +456,7 @@ extern "C" int flt_filter_center(tomo_flt_plan *P, float *data, const float *wfi
                        st, wfilter, sht, P->w.as<float2>(), (int)P->ne, (int)P->n, (int)P->nz, center);
   }
   TOMO_LAUNCH_CHECK();
-  {
-    ProfScope ps_("k_flt_rows", st);
-    hipLaunchKernelGGL(k_flt_rows<1>, dim3((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz),
-                       dim3(P->threads), P->lds, st, data, P->w.as<float2>(), P->fft.d, (int)P->ne,
-                       (int)P->n, (int)P->nproj);
-  }
-  TOMO_LAUNCH_CHECK();
-  return TOMO_OK;
+  return flt_rows<1>(P, data, P->w.as<float2>(), st);
 }
 
 extern "C" void flt_destroy(tomo_flt_plan *P) {

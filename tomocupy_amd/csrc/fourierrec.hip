@@ -26,6 +26,7 @@
 #include "common.h"
 #include "fft_lds.h"
 #include "internal.h"
+#include "precision.h"
 
 using namespace tomo;
 
@@ -38,27 +39,29 @@ constexpr int kColEPT = 32;     // FFT values per thread in the column pass
 // Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
 
-__global__ __launch_bounds__(512) void k_fr_prep(const float *__restrict__ data, float2 *__restrict__ G, FftDesc fd,
-                          int n, int nproj, int nz, int nzc, float scale) {
+template <typename T>
+__global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx<T> *__restrict__ G,
+                                                 FftDesc fd, int n, int nproj, int nz, int nzc,
+                                                 float scale) {
   extern __shared__ float2 sm[];
   const int p = blockIdx.x, c = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
-  float2 *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
+  cpx<T> *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
   if (c >= nzc) {
-    for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = make_float2(0.f, 0.f);
+    for (int k = tid; k < n; k += nt) stc(out + (size_t)k * 4, make_float2(0.f, 0.f));
     return;
   }
-  const float *r1 = data + ((size_t)c * nproj + p) * n;
+  const T *r1 = data + ((size_t)c * nproj + p) * n;
   const int z2 = c + nzc;
-  const float *r2 = data + ((size_t)z2 * nproj + p) * n;
+  const T *r2 = data + ((size_t)(z2 < nz ? z2 : c) * nproj + p) * n;
   const bool has2 = z2 < nz;
-  for (int j = tid; j < n; j += nt) {
-    const float s = shift_sign(j);
-    sm[lds_pad(j)] = make_float2(s * r1[j], has2 ? s * r2[j] : 0.f);
-  }
+  stage_lds<8, float2>(
+      n, tid, nt, [&](int j) { return make_float2(ldr(r1 + j), has2 ? ldr(r2 + j) : 0.f); },
+      [&](int j, float2 v) { sm[lds_pad(j)] = cscale(v, shift_sign(j)); });
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
-  for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[lds_pad(k)], shift_sign(k) * scale);
+  for (int k = tid; k < n; k += nt)
+    stc(out + (size_t)k * 4, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
 }
 
 // Fused FBP filter + pair packing + shifted detector FFT (the fused form of
@@ -67,32 +70,37 @@ __global__ __launch_bounds__(512) void k_fr_prep(const float *__restrict__ data,
 // the two slices' wrapped filter spectra (Y = Z (H1+H2)/2 + conj(Z(-k))
 // (H1-H2)/2, exact for real rows), IFFT_2n, add the edge-replication terms
 // x[0] HL + x[n-1] HR per slice, then the shifted n-point DFT of k_fr_prep.
+template <typename T>
 __global__ __launch_bounds__(512) void k_fr_filtprep(
-    const float *__restrict__ data, float2 *__restrict__ G, const float2 *__restrict__ H2,
+    const T *__restrict__ data, cpx<T> *__restrict__ G, const float2 *__restrict__ H2,
     const float *__restrict__ HL, const float *__restrict__ HR, FftDesc fd2, FftDesc fd1, int n,
     int nproj, int nz, int nzc, float scale) {
   extern __shared__ float2 sm[];
   __shared__ float edge[4];
   const int p = blockIdx.x, c = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
-  float2 *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
+  cpx<T> *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
   if (c >= nzc) {
-    for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = make_float2(0.f, 0.f);
+    for (int k = tid; k < n; k += nt) stc(out + (size_t)k * 4, make_float2(0.f, 0.f));
     return;
   }
   const int n2 = 2 * n;
   const int z2 = c + nzc;
   const bool has2 = z2 < nz;
   const int zb = has2 ? z2 : c;
-  const float *r1 = data + ((size_t)c * nproj + p) * n;
-  const float *r2 = data + ((size_t)zb * nproj + p) * n;
-  for (int j = tid; j < n2; j += nt)
-    sm[lds_pad(j)] = j < n ? make_float2(r1[j], has2 ? r2[j] : 0.f) : make_float2(0.f, 0.f);
+  const T *r1 = data + ((size_t)c * nproj + p) * n;
+  const T *r2 = data + ((size_t)zb * nproj + p) * n;
+  stage_lds<8, float2>(
+      n2, tid, nt,
+      [&](int j) {
+        return j < n ? make_float2(ldr(r1 + j), has2 ? ldr(r2 + j) : 0.f) : make_float2(0.f, 0.f);
+      },
+      [&](int j, float2 v) { sm[lds_pad(j)] = v; });
   if (tid == 0) {
-    edge[0] = r1[0];
-    edge[1] = r1[n - 1];
-    edge[2] = has2 ? r2[0] : 0.f;
-    edge[3] = has2 ? r2[n - 1] : 0.f;
+    edge[0] = ldr(r1);
+    edge[1] = ldr(r1 + n - 1);
+    edge[2] = has2 ? ldr(r2) : 0.f;
+    edge[3] = has2 ? ldr(r2 + n - 1) : 0.f;
   }
   __syncthreads();
   lds_fft<-1>(sm, fd2, 0, tid, nt);
@@ -122,7 +130,8 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   }
   __syncthreads();
   lds_fft<-1>(sm, fd1, 0, tid, nt);
-  for (int k = tid; k < n; k += nt) out[(size_t)k * 4] = cscale(sm[lds_pad(k)], shift_sign(k) * scale);
+  for (int k = tid; k < n; k += nt)
+    stc(out + (size_t)k * 4, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -137,9 +146,10 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // chunk of it) in a fixed order; chunks of heavy tiles land in private
 // partial buffers that k_fr_grid_reduce sums in chunk order, so the result is
 // bitwise reproducible.
+template <typename T>
 __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
     const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
-    const float2 *__restrict__ G, float2 *__restrict__ F, float *__restrict__ part, int n,
+    const cpx<T> *__restrict__ G, cpx<T> *__restrict__ F, float *__restrict__ part, int n,
     int ntile, int nslots, int nsamp, int nzc, float K2) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int it = blockIdx.x * kGridWaves + wave;
@@ -148,31 +158,25 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
   const int4 item = items[it];
   const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
   const int k = lane >> 4, i = lane & 15;
-  const float4 *Gg = reinterpret_cast<const float4 *>(G + (size_t)grp * nsamp * 4);
+  const cpx4<T> *Gg = reinterpret_cast<const cpx4<T> *>(G + (size_t)grp * nsamp * 4);
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
   // entry {sample, footprint offsets (cols | rows << 8), du bits, dv bits};
   // loads run one step ahead of the matrix products
   const int eend = ebeg + cnt;
+  const cpx4<T> zero4 = {};
   int4 en = cnt > 0 ? ent[ebeg + k] : make_int4(-1, 0, 0, 0);
-  float4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
-  if (en.x >= 0) {
-    va = Gg[(size_t)en.x * 2];
-    vb = Gg[(size_t)en.x * 2 + 1];
-  }
+  cpx4<T> va = en.x >= 0 ? Gg[en.x] : zero4;
   for (int e = ebeg; e < eend; e += 4) {
     const int4 cur = en;
-    const float4 ca = va, cb = vb;
+    const cpx4<T> cv = va;
     if (e + 4 < eend) {
       en = ent[e + 4 + k];
-      if (en.x >= 0) {
-        va = Gg[(size_t)en.x * 2];
-        vb = Gg[(size_t)en.x * 2 + 1];
-      } else {
-        va = vb = float4{0.f, 0.f, 0.f, 0.f};
-      }
+      va = en.x >= 0 ? Gg[en.x] : zero4;
     }
+    const float gv[8] = {(float)cv.v[0].x, (float)cv.v[0].y, (float)cv.v[1].x, (float)cv.v[1].y,
+                         (float)cv.v[2].x, (float)cv.v[2].y, (float)cv.v[3].x, (float)cv.v[3].y};
     float e0v = 0.f, e1v = 0.f;
     if (cur.x >= 0) {
       const int rb0 = (int)(signed char)(cur.y & 0xff);
@@ -183,14 +187,9 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
       e0v = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
       e1v = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
     }
-    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.x, e0v, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.y, e0v, acc[1], 0, 0, 0);
-    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.z, e0v, acc[2], 0, 0, 0);
-    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * ca.w, e0v, acc[3], 0, 0, 0);
-    acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.x, e0v, acc[4], 0, 0, 0);
-    acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.y, e0v, acc[5], 0, 0, 0);
-    acc[6] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.z, e0v, acc[6], 0, 0, 0);
-    acc[7] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * cb.w, e0v, acc[7], 0, 0, 0);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * gv[m], e0v, acc[m], 0, 0, 0);
   }
   const int j = lane & 15, ib = 4 * (lane >> 4);
   if (slot < 0) {
@@ -201,11 +200,12 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (grp * 4 + c >= nzc) continue;
-      float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+      cpx<T> *Fc = F + (size_t)(grp * 4 + c) * plane;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int a = 16 * TA + ib + r;
-        if (a < n2 && b < n2) Fc[(size_t)a * n2 + b] = make_float2(acc[2 * c][r], acc[2 * c + 1][r]);
+        if (a < n2 && b < n2)
+          stc(Fc + (size_t)a * n2 + b, make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
       }
     }
   } else {
@@ -218,8 +218,9 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
 }
 
 // Sums the partial tiles of heavily loaded tiles in chunk order.
+template <typename T>
 __global__ void k_fr_grid_reduce(const int4 *__restrict__ red, const float *__restrict__ part,
-                                 float2 *__restrict__ F, int n, int ntile, int nslots, int nzc) {
+                                 cpx<T> *__restrict__ F, int n, int ntile, int nslots, int nzc) {
   const int4 r = red[blockIdx.x];
   const int grp = blockIdx.y;
   const int cell = threadIdx.x;  // 256 threads = 16 x 16 cells
@@ -237,32 +238,35 @@ __global__ void k_fr_grid_reduce(const int4 *__restrict__ red, const float *__re
 #pragma unroll
   for (int c = 0; c < 4; ++c)
     if (grp * 4 + c < nzc)
-      F[(size_t)(grp * 4 + c) * plane + (size_t)a * n2 + b] = make_float2(acc[2 * c], acc[2 * c + 1]);
+      stc(F + (size_t)(grp * 4 + c) * plane + (size_t)a * n2 + b,
+          make_float2(acc[2 * c], acc[2 * c + 1]));
 }
 
-__global__ __launch_bounds__(512) void k_fr_rowfft(const float2 *__restrict__ F, float2 *__restrict__ R, FftDesc fd,
-                            int n, int nzc) {
+template <typename T>
+__global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
+                                                   cpx<T> *__restrict__ R, FftDesc fd, int n,
+                                                   int nzc) {
   extern __shared__ float2 sm[];
   const int a = blockIdx.x, c = blockIdx.y;
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n2 = 2 * n;
-  const float2 *src = F + ((size_t)c * n2 + a) * n2;
-  for (int b = tid; b < n2; b += nt) {
-    const float2 x = src[b];
-    sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
-  }
+  const cpx<T> *src = F + ((size_t)c * n2 + a) * n2;
+  stage_lds<8, float2>(
+      n2, tid, nt, [&](int b) { return ldc(src + b); },
+      [&](int b, float2 x) { sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x; });
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
-  float2 *dst = R + ((size_t)c * n2 + a) * n;
+  cpx<T> *dst = R + ((size_t)c * n2 + a) * n;
   const int h = n / 2;
   for (int v = tid; v < n; v += nt) {
     const float2 x = sm[lds_pad(v + h)];
-    dst[v] = ((v + h) & 1) ? make_float2(-x.x, -x.y) : x;
+    stc(dst + v, ((v + h) & 1) ? make_float2(-x.x, -x.y) : x);
   }
 }
 
-__global__ __launch_bounds__(512) void k_fr_colfft(const float2 *__restrict__ R, float *__restrict__ obj,
+template <typename T>
+__global__ __launch_bounds__(512) void k_fr_colfft(const cpx<T> *__restrict__ R, T *__restrict__ obj,
                             const float *__restrict__ phix, const float *__restrict__ phiy,
                             const int2 *__restrict__ circ, FftDesc fd, int n, int nz, int nzc,
                             int lgCB) {
@@ -274,12 +278,17 @@ __global__ __launch_bounds__(512) void k_fr_colfft(const float2 *__restrict__ R,
   const int tid = threadIdx.x, nt = blockDim.x;
   const int CB = 1 << lgCB;
   const int n2 = 2 * n, v0 = cb * CB;
-  const float2 *src = R + (size_t)c * n2 * n;
-  for (int idx = tid; idx < CB * n2; idx += nt) {
-    const int a = idx >> lgCB, t = idx & (CB - 1);
-    float2 x = (v0 + t < n) ? src[(size_t)a * n + v0 + t] : make_float2(0.f, 0.f);
-    sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
-  }
+  const cpx<T> *src = R + (size_t)c * n2 * n;
+  stage_lds<16, float2>(
+      CB * n2, tid, nt,
+      [&](int idx) {
+        const int a = idx >> lgCB, t = idx & (CB - 1);
+        return (v0 + t < n) ? ldc(src + (size_t)a * n + v0 + t) : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 x) {
+        const int a = idx >> lgCB, t = idx & (CB - 1);
+        sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
+      });
   __syncthreads();
   lds_fft<1, kColEPT>(sm, fd, lgCB, tid, nt);
   const int z2 = c + nzc;
@@ -292,8 +301,8 @@ __global__ __launch_bounds__(512) void k_fr_colfft(const float2 *__restrict__ R,
     float2 x = sm[(lds_pad(u) << lgCB) + t];
     const int2 ci = circ[ty];
     const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
-    obj[((size_t)c * n + ty) * n + tx] = x.x * ph;
-    if (z2 < nz) obj[((size_t)z2 * n + ty) * n + tx] = x.y * ph;
+    str(obj + ((size_t)c * n + ty) * n + tx, x.x * ph);
+    if (z2 < nz) str(obj + ((size_t)z2 * n + ty) * n + tx, x.y * ph);
   }
 }
 
@@ -313,17 +322,24 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
   const int z2 = c + nzc;
   for (int idx = tid; idx < CB * lds_padded_len(n2); idx += nt) sm[idx] = make_float2(0.f, 0.f);
   __syncthreads();
-  for (int idx = tid; idx < CB * n; idx += nt) {
-    const int ty = idx >> lgCB, t = idx & (CB - 1);
-    const int tx = v0 + t;
-    if (tx >= n) continue;
-    const int u = ty + 1 + h;
-    const int2 ci = circ[ty];
-    const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
-    const float re = obj[((size_t)c * n + ty) * n + tx];
-    const float im = (z2 < nz) ? obj[((size_t)z2 * n + ty) * n + tx] : 0.f;
-    sm[(lds_pad(u) << lgCB) + t] = make_float2(re * ph, im * ph);
-  }
+  stage_lds<8, float2>(
+      CB * n, tid, nt,
+      [&](int idx) {
+        const int ty = idx >> lgCB, tx = v0 + (idx & (CB - 1));
+        if (tx >= n) return make_float2(0.f, 0.f);
+        return make_float2(obj[((size_t)c * n + ty) * n + tx],
+                           (z2 < nz) ? obj[((size_t)z2 * n + ty) * n + tx] : 0.f);
+      },
+      [&](int idx, float2 v) {
+        const int ty = idx >> lgCB, t = idx & (CB - 1);
+        const int tx = v0 + t;
+        if (tx >= n) return;
+        const int u = ty + 1 + h;
+        const int2 ci = circ[ty];
+        const float ph =
+            (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
+        sm[(lds_pad(u) << lgCB) + t] = make_float2(v.x * ph, v.y * ph);
+      });
   __syncthreads();
   lds_fft<-1, kColEPT>(sm, fd, lgCB, tid, nt);
   float2 *dst = R + (size_t)c * n2 * n;
@@ -343,14 +359,10 @@ __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict_
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n2 = 2 * n, h = n / 2;
   const float2 *src = R + ((size_t)c * n2 + a) * n;
-  for (int v = tid; v < n2; v += nt) {
-    float2 x = make_float2(0.f, 0.f);
-    if (v >= h && v < h + n) {
-      x = src[v - h];
-      if (v & 1) x = make_float2(-x.x, -x.y);
-    }
-    sm[lds_pad(v)] = x;
-  }
+  stage_lds<8, float2>(
+      n2, tid, nt,
+      [&](int v) { return (v >= h && v < h + n) ? src[v - h] : make_float2(0.f, 0.f); },
+      [&](int v, float2 x) { sm[lds_pad(v)] = (v & 1) ? make_float2(-x.x, -x.y) : x; });
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
   float2 *dst = F + ((size_t)c * n2 + a) * n2;
@@ -413,7 +425,9 @@ __global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ 
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const float2 *in = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
-  for (int k = tid; k < n; k += nt) sm[lds_pad(k)] = cscale(in[(size_t)k * 4], shift_sign(k) * scale);
+  stage_lds<8, float2>(
+      n, tid, nt, [&](int k) { return in[(size_t)k * 4]; },
+      [&](int k, float2 v) { sm[lds_pad(k)] = cscale(v, shift_sign(k) * scale); });
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
   const int z2 = c + nzc;
@@ -430,6 +444,8 @@ __global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ 
 
 struct tomo_fr_plan {
   int64_t n = 0, nproj = 0, nz = 0, nzc = 0, ngrp = 0;
+  int dtype = TOMO_F32;
+  size_t csz = sizeof(float2);  // bytes of one stored complex value
   int64_t nsamp = 0;
   FftPlan fft1, fft2;
   DevBuf geom, ent, items, red, part, phix, phiy, circ;
@@ -449,7 +465,11 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   const int m = 4;
   const double c0 = M_PI / (mu * 4.0 * (double)n * (double)n);
   const double C1n = -M_PI * M_PI / (mu * 4.0 * (double)n * (double)n);
-  P->scale1 = (float)(4.0 / (double)n * c0);
+  // Half-precision storage keeps the spectra and grids 2^8 above the
+  // reference's scaling (well inside the normal range of IEEE half); the
+  // factor is taken out again with the deapodisation.
+  const double sG = P->dtype == TOMO_F16 ? 256.0 : 1.0;
+  P->scale1 = (float)(4.0 / (double)n * c0 * sG);
   P->K2 = (float)(C1n / std::log(2.0));
 
   // polar sample geometry (kernels_fourierrec.cuh:23-36, :86-98), FP64
@@ -568,7 +588,7 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
     const double d = (double)t / (double)n - 0.5;
     px[t] = (float)std::exp(mu * (double)n * (double)n * d * d);
     py[t] = (float)(std::exp(mu * (double)n * (double)n * d * d) / (double)nproj *
-                    (double)(1 - n % 4));
+                    (double)(1 - n % 4) / sG);
   }
   // circle mask 4x^2 + 4y^2 < 1 (kernels_fourierrec.cuh:161-174), FP64, as a
   // per-row column interval
@@ -611,10 +631,6 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
                "fr_create: unknown dtype %d", dtype);
   TOMO_REQUIRE(nproj * n < (int64_t)1 << 31, "fr_create: nproj*n too large");
-  if (dtype != TOMO_F32) {
-    set_error("fr_create: dtype %d is not implemented in this build (float32 only)", dtype);
-    return TOMO_EUNSUPPORTED;
-  }
   auto *P = new (std::nothrow) tomo_fr_plan();
   if (!P) {
     set_error("fr_create: host allocation failed");
@@ -625,6 +641,8 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   P->nz = nz;
   P->nzc = (nz + 1) / 2;
   P->ngrp = ceil_div(P->nzc, 4);
+  P->dtype = dtype;
+  P->csz = dtype == TOMO_F32 ? sizeof(float2) : 2 * sizeof(uint16_t);
   hipStream_t st = (hipStream_t)stream;
   std::vector<float> th(nproj);
   int rc = TOMO_OK;
@@ -645,11 +663,13 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   if ((rc = P->fft2.init((int)(2 * n)))) return fail(rc);
   const int64_t n2 = 2 * n;
   const size_t plane = (size_t)n2 * n2;
+  // G holds the float32 spectra of the adjoint as well
   if ((rc = P->G.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * P->nsamp))) return fail(rc);
   if ((rc = P->part.alloc(sizeof(float) * 2048 * (size_t)P->ngrp * std::max(1, P->nslots))))
     return fail(rc);
-  if ((rc = P->F.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * plane))) return fail(rc);
-  if ((rc = P->R.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * n2 * n))) return fail(rc);
+  // F and R in the plan's storage precision (float32 when the adjoint may run)
+  if ((rc = P->F.alloc(P->csz * (size_t)P->ngrp * 4 * plane))) return fail(rc);
+  if ((rc = P->R.alloc(P->csz * (size_t)P->ngrp * 4 * n2 * n))) return fail(rc);
   // column pass: CB columns of length 2n in LDS
   const size_t maxlds = max_lds_per_block();
   const size_t col1 = (size_t)lds_padded_len((int)n2) * sizeof(float2);
@@ -670,14 +690,22 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
     set_error("fr_create: n=%lld needs more than 1024 threads per FFT workgroup", (long long)n);
     return fail(TOMO_EINVAL);
   }
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft,
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft<float>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft<_Float16>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft<__bf16>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft_adj,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
   const int rowLds = (int)(lds_padded_len((int)n2) * sizeof(float2));
   P->rowLds = rowLds;
   P->prepLds = (int)(lds_padded_len((int)n) * sizeof(float2));
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft,
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft<float>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft<_Float16>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
+  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft<__bf16>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft_adj,
                                hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
@@ -685,60 +713,89 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   return TOMO_OK;
 }
 
-// Gridding, the two pruned FFT passes and the output stage, from G.
-static int fr_tail(tomo_fr_plan *P, float *obj, hipStream_t st) {
+// Gridding, the two pruned FFT passes and the output stage, from G; T is the
+// storage type of G, F, R and obj.
+template <typename T>
+static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
   const int n = (int)P->n, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   const int n2 = 2 * n;
+  cpx<T> *G = P->G.as<cpx<T>>(), *F = P->F.as<cpx<T>>(), *R = P->R.as<cpx<T>>();
   {
     ProfScope ps_("k_fr_grid", st);
-    hipLaunchKernelGGL(k_fr_grid, dim3((unsigned)ceil_div(P->nitems, kGridWaves), ng),
+    hipLaunchKernelGGL(k_fr_grid<T>, dim3((unsigned)ceil_div(P->nitems, kGridWaves), ng),
                        dim3(64 * kGridWaves), 0, st, P->items.as<int4>(), P->nitems,
-                       P->ent.as<int4>(), P->G.as<float2>(),
-                       P->F.as<float2>(), P->part.as<float>(), n, P->ntile, P->nslots,
+                       P->ent.as<int4>(), G, F, P->part.as<float>(), n, P->ntile, P->nslots,
                        (int)P->nsamp, nzc, P->K2);
   }
   TOMO_LAUNCH_CHECK();
   if (P->nred > 0) {
     ProfScope ps_("k_fr_grid_reduce", st);
-    hipLaunchKernelGGL(k_fr_grid_reduce, dim3(P->nred, ng), dim3(256), 0, st, P->red.as<int4>(),
-                       P->part.as<float>(), P->F.as<float2>(), n, P->ntile, P->nslots, nzc);
+    hipLaunchKernelGGL(k_fr_grid_reduce<T>, dim3(P->nred, ng), dim3(256), 0, st,
+                       P->red.as<int4>(), P->part.as<float>(), F, n, P->ntile, P->nslots, nzc);
   }
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_rowfft", st);
-    hipLaunchKernelGGL(k_fr_rowfft, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds, st,
-                       P->F.as<float2>(), P->R.as<float2>(), P->fft2.d, n, nzc);
+    hipLaunchKernelGGL(k_fr_rowfft<T>, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds, st, F,
+                       R, P->fft2.d, n, nzc);
   }
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_colfft", st);
-    hipLaunchKernelGGL(k_fr_colfft, dim3((unsigned)ceil_div(n, P->colCB), ng * 4), dim3(P->colThreads),
-                       P->colLds, st, P->R.as<float2>(), obj, P->phix.as<float>(), P->phiy.as<float>(),
-                       P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+    hipLaunchKernelGGL(k_fr_colfft<T>, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
+                       dim3(P->colThreads), P->colLds, st, R, obj, P->phix.as<float>(),
+                       P->phiy.as<float>(), P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
 
-extern "C" int fr_backprojection(tomo_fr_plan *P, float *obj, const float *data, void *stream) {
-  clear_error();
-  TOMO_REQUIRE(P != nullptr, "fr_backprojection: plan is NULL");
-  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_backprojection: NULL array");
-  hipStream_t st = (hipStream_t)stream;
+template <typename T>
+static int fr_backprojection_t(tomo_fr_plan *P, T *obj, const T *data, hipStream_t st) {
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   {
     ProfScope ps_("k_fr_prep", st);
-    hipLaunchKernelGGL(k_fr_prep, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
-                       data, P->G.as<float2>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
+    hipLaunchKernelGGL(k_fr_prep<T>, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
+                       data, P->G.as<cpx<T>>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
   }
   TOMO_LAUNCH_CHECK();
-  return fr_tail(P, obj, st);
+  return fr_tail<T>(P, obj, st);
 }
 
-extern "C" int fr_filter_backprojection(tomo_fr_plan *P, tomo_flt_plan *F, float *obj,
-                                        const float *data, const float *wfilter, double center,
+template <typename T>
+static int fr_filter_backprojection_t(tomo_fr_plan *P, tomo_flt_plan *F, T *obj, const T *data,
+                                      hipStream_t st) {
+  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
+  const int ng = (int)P->ngrp;
+  {
+    ProfScope ps_("k_fr_filtprep", st);
+    hipLaunchKernelGGL(k_fr_filtprep<T>, dim3(nproj, ng * 4), dim3(P->rowThreads), P->rowLds,
+                       st, data, P->G.as<cpx<T>>(), F->H2.as<float2>(), F->HL.as<float>(),
+                       F->HR.as<float>(), F->fft2n.d, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  }
+  TOMO_LAUNCH_CHECK();
+  return fr_tail<T>(P, obj, st);
+}
+
+extern "C" int fr_backprojection(tomo_fr_plan *P, void *obj, const void *data, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "fr_backprojection: plan is NULL");
+  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_backprojection: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  switch (P->dtype) {
+    case TOMO_F16:
+      return fr_backprojection_t(P, (_Float16 *)obj, (const _Float16 *)data, st);
+    case TOMO_BF16:
+      return fr_backprojection_t(P, (__bf16 *)obj, (const __bf16 *)data, st);
+    default:
+      return fr_backprojection_t(P, (float *)obj, (const float *)data, st);
+  }
+}
+
+extern "C" int fr_filter_backprojection(tomo_fr_plan *P, tomo_flt_plan *F, void *obj,
+                                        const void *data, const float *wfilter, double center,
                                         const float *sht, void *stream) {
   clear_error();
   TOMO_REQUIRE(P != nullptr && F != nullptr, "fr_filter_backprojection: plan is NULL");
@@ -754,22 +811,26 @@ extern "C" int fr_filter_backprojection(tomo_fr_plan *P, tomo_flt_plan *F, float
   hipStream_t st = (hipStream_t)stream;
   int rc = flt_prepare_2n(F, wfilter, center, sht, st);
   if (rc) return rc;
-  const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
-  const int ng = (int)P->ngrp;
-  {
-    ProfScope ps_("k_fr_filtprep", st);
-    hipLaunchKernelGGL(k_fr_filtprep, dim3(nproj, ng * 4), dim3(P->rowThreads), P->rowLds, st,
-                       data, P->G.as<float2>(), F->H2.as<float2>(), F->HL.as<float>(),
-                       F->HR.as<float>(), F->fft2n.d, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+  switch (P->dtype) {
+    case TOMO_F16:
+      return fr_filter_backprojection_t(P, F, (_Float16 *)obj, (const _Float16 *)data, st);
+    case TOMO_BF16:
+      return fr_filter_backprojection_t(P, F, (__bf16 *)obj, (const __bf16 *)data, st);
+    default:
+      return fr_filter_backprojection_t(P, F, (float *)obj, (const float *)data, st);
   }
-  TOMO_LAUNCH_CHECK();
-  return fr_tail(P, obj, st);
 }
 
-extern "C" int fr_forward(tomo_fr_plan *P, float *data, const float *obj, void *stream) {
+extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *stream) {
   clear_error();
   TOMO_REQUIRE(P != nullptr, "fr_forward: plan is NULL");
-  TOMO_REQUIRE(obj != nullptr && data != nullptr, "fr_forward: NULL array");
+  TOMO_REQUIRE(obj_ != nullptr && data_ != nullptr, "fr_forward: NULL array");
+  if (P->dtype != TOMO_F32) {
+    set_error("fr_forward: the forward projection is implemented for float32 plans only");
+    return TOMO_EUNSUPPORTED;
+  }
+  float *data = (float *)data_;
+  const float *obj = (const float *)obj_;
   hipStream_t st = (hipStream_t)stream;
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;

@@ -7,6 +7,7 @@
 
 struct tomo_flt_plan {
   int64_t n = 0, ne = 0, nproj = 0, nz = 0;
+  int dtype = 0;           // storage of the filtered rows: TOMO_F32, TOMO_F16 or TOMO_BF16
   tomo::FftPlan fft;       // ne
   tomo::FftPlan fft2n;     // 2n (fused path)
   tomo::DevBuf w;          // (nz, ne/2+1) complex, filled by flt_filter_center

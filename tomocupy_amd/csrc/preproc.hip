@@ -5,6 +5,7 @@
 #include <string>
 
 #include "common.h"
+#include "precision.h"
 #include "preproc.h"
 
 using namespace tomo;
@@ -55,14 +56,15 @@ __device__ __forceinline__ float minus_log(float x) {
 }
 
 // One workgroup per (p, z) detector row: read data[p][z][:], write out[z][p][:].
-template <int MODE>  // 0: darkflat (+ optional -log) + swap; 1: darkflat only, projection order
+// TO is the storage type of the output.
+template <int MODE, typename TO>  // 0: darkflat (+ optional -log) + swap; 1: darkflat only, projection order
 __global__ void k_pre_rows(const uint16_t *__restrict__ data, const float *__restrict__ dark0,
                            const float *__restrict__ flatA, const float *__restrict__ flatB,
-                           float *__restrict__ out, int nproj, int nz, int n, int flat_linear,
+                           TO *__restrict__ out, int nproj, int nz, int n, int flat_linear,
                            int take_log) {
   const int p = blockIdx.x, z = blockIdx.y;
   const uint16_t *src = data + ((size_t)p * nz + z) * n;
-  float *dst = MODE == 0 ? out + ((size_t)z * nproj + p) * n : out + ((size_t)p * nz + z) * n;
+  TO *dst = MODE == 0 ? out + ((size_t)z * nproj + p) * n : out + ((size_t)p * nz + z) * n;
   const float v = nproj > 1 ? (float)((double)p / (double)(nproj - 1)) : 0.f;
   const float *d0 = dark0 + (size_t)z * n;
   const float *fa = flatA + (size_t)z * n;
@@ -70,18 +72,19 @@ __global__ void k_pre_rows(const uint16_t *__restrict__ data, const float *__res
   for (int x = threadIdx.x; x < n; x += blockDim.x) {
     float r = darkflat((float)src[x], d0[x], fa[x], flat_linear ? fb[x] : 0.f, v, flat_linear);
     if (MODE == 0 && take_log) r = minus_log(r);
-    dst[x] = r;
+    str(dst + x, r);
   }
 }
 
-__global__ void k_pre_log_swap(const float *__restrict__ res, float *__restrict__ out, int nproj,
+template <typename TO>
+__global__ void k_pre_log_swap(const float *__restrict__ res, TO *__restrict__ out, int nproj,
                                int nz, int n, int take_log) {
   const int p = blockIdx.x, z = blockIdx.y;
   const float *src = res + ((size_t)p * nz + z) * n;
-  float *dst = out + ((size_t)z * nproj + p) * n;
+  TO *dst = out + ((size_t)z * nproj + p) * n;
   for (int x = threadIdx.x; x < n; x += blockDim.x) {
     const float r = src[x];
-    dst[x] = take_log ? minus_log(r) : r;
+    str(dst + x, take_log ? minus_log(r) : r);
   }
 }
 
@@ -132,20 +135,40 @@ static int launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *
   return TOMO_OK;
 }
 
+template <typename TO>
+static void launch_rows0(tomo_pre_plan *P, const uint16_t *data, void *out, int flat_linear,
+                         int take_log, hipStream_t st) {
+  hipLaunchKernelGGL((k_pre_rows<0, TO>), dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0,
+                     st, data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(),
+                     (TO *)out, (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, take_log);
+}
+
+template <typename TO>
+static void launch_log_swap(tomo_pre_plan *P, const float *res, void *out, int take_log,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_pre_log_swap<TO>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0,
+                     st, res, (TO *)out, (int)P->nproj, (int)P->nz, (int)P->n, take_log);
+}
+
 extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
-                           const uint16_t *flat, float *out, double bright_ratio, int flat_linear,
-                           int minus_log, void *stream) {
+                           const uint16_t *flat, void *out, int out_dtype, double bright_ratio,
+                           int flat_linear, int minus_log, void *stream) {
   clear_error();
   TOMO_REQUIRE(P != nullptr, "pre_process: plan is NULL");
   TOMO_REQUIRE(data && dark && flat && out, "pre_process: NULL array");
+  TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
+               "pre_process: unknown out_dtype %d", out_dtype);
   hipStream_t st = (hipStream_t)stream;
   int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
   {
     ProfScope ps_("k_pre_rows", st);
-    hipLaunchKernelGGL(k_pre_rows<0>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
-                       data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), out,
-                       (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, minus_log);
+    if (out_dtype == TOMO_F16)
+      launch_rows0<_Float16>(P, data, out, flat_linear, minus_log, st);
+    else if (out_dtype == TOMO_BF16)
+      launch_rows0<__bf16>(P, data, out, flat_linear, minus_log, st);
+    else
+      launch_rows0<float>(P, data, out, flat_linear, minus_log, st);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
@@ -162,7 +185,7 @@ extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16
   if (rc) return rc;
   {
     ProfScope ps_("k_pre_rows", st);
-    hipLaunchKernelGGL(k_pre_rows<1>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_pre_rows<1, float>), dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
                        data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), res,
                        (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, 0);
   }
@@ -170,17 +193,23 @@ extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16
   return TOMO_OK;
 }
 
-extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, float *out, int minus_log,
-                                  void *stream) {
+extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, void *out, int out_dtype,
+                                  int minus_log, void *stream) {
   clear_error();
   TOMO_REQUIRE(P != nullptr, "pre_minus_log_swap: plan is NULL");
   TOMO_REQUIRE(res && out, "pre_minus_log_swap: NULL array");
+  TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
+               "pre_minus_log_swap: unknown out_dtype %d", out_dtype);
   TOMO_REQUIRE(res != out, "pre_minus_log_swap: in-place swap is not supported");
   hipStream_t st = (hipStream_t)stream;
   {
     ProfScope ps_("k_pre_log_swap", st);
-    hipLaunchKernelGGL(k_pre_log_swap, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
-                       res, out, (int)P->nproj, (int)P->nz, (int)P->n, minus_log);
+    if (out_dtype == TOMO_F16)
+      launch_log_swap<_Float16>(P, res, out, minus_log, st);
+    else if (out_dtype == TOMO_BF16)
+      launch_log_swap<__bf16>(P, res, out, minus_log, st);
+    else
+      launch_log_swap<float>(P, res, out, minus_log, st);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;

@@ -27,10 +27,11 @@ class FBPFilter:
         self._h = h
 
     def filter(self, data, w, stream=None):
-        """In place on padded ``data (nz, ntheta, ne)`` float32 with complex64
-        ``w (nz, ne/2+1)`` (``cfunc_filter::filter``)."""
+        """In place on padded ``data (nz, ntheta, ne)`` of the plan's dtype with
+        complex64 ``w (nz, ne/2+1)`` (``cfunc_filter::filter``)."""
         import torch
-        pd = _lib.dev_ptr(data, (self.nz, self.ntheta, self.n), torch.float32, 'data')
+        pd = _lib.dev_ptr(data, (self.nz, self.ntheta, self.n), _lib.torch_dtype(self.dtype),
+                          'data')
         if w.dtype == torch.complex64:
             w = torch.view_as_real(w.contiguous())
         pw = _lib.dev_ptr(w.contiguous(), (self.nz, self.n // 2 + 1, 2), torch.float32, 'w')
@@ -40,7 +41,8 @@ class FBPFilter:
     def filter_center(self, data, wfilter, center, sht=None, stream=None):
         """Fused pad + centre phase ramp + filter + crop on ``data (nz, ntheta, n_unpadded)``."""
         import torch
-        pd = _lib.dev_ptr(data, (self.nz, self.ntheta, self.n_unpadded), torch.float32, 'data')
+        pd = _lib.dev_ptr(data, (self.nz, self.ntheta, self.n_unpadded),
+                          _lib.torch_dtype(self.dtype), 'data')
         pw = _lib.dev_ptr(wfilter, (self.n // 2 + 1,), torch.float32, 'wfilter')
         ps = None
         if sht is not None:

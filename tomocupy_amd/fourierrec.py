@@ -10,7 +10,10 @@ Differences, all documented deviations:
 * odd ``nz`` is accepted (the last slice pairs with an implicit zero slice);
 * ``theta`` is copied when the plan is created (the reference keeps a borrowed
   device pointer alive, ``fourierrec.py:50``);
-* ``forward`` is the exact adjoint (USFFT forward projection), new.
+* ``forward`` is the exact adjoint (USFFT forward projection), new;
+* ``dtype`` ``'float16'`` / ``'bfloat16'`` store sinograms, spectra, grids and
+  images in 16 bits with float32 arithmetic (the reference's float16 module
+  computes in half precision throughout, ``cfunc_fourierrecfp16.i``).
 """
 import ctypes
 
@@ -42,16 +45,18 @@ class FourierRec:
         self._h = h
 
     def backprojection(self, obj, data, stream=None):
-        """``obj (nz, n, n)`` <- backprojection of filtered sinograms ``data (nz, nproj, n)``."""
-        import torch
-        po = _lib.dev_ptr(obj, (self.nz, self.n, self.n), torch.float32, 'obj')
-        pd = _lib.dev_ptr(data, (self.nz, self.nproj, self.n), torch.float32, 'data')
+        """``obj (nz, n, n)`` <- backprojection of filtered sinograms ``data (nz, nproj, n)``,
+        both of the plan's dtype."""
+        td = _lib.torch_dtype(self.dtype)
+        po = _lib.dev_ptr(obj, (self.nz, self.n, self.n), td, 'obj')
+        pd = _lib.dev_ptr(data, (self.nz, self.nproj, self.n), td, 'data')
         _lib.check(_lib.lib().fr_backprojection(self._handle(), po, pd, _lib.stream_ptr(stream)),
                    'fr_backprojection')
         return obj
 
     def forward(self, data, obj, stream=None):
-        """``data (nz, nproj, n)`` <- adjoint of :meth:`backprojection` applied to ``obj``."""
+        """``data (nz, nproj, n)`` <- adjoint of :meth:`backprojection` applied to ``obj``
+        (float32 plans)."""
         import torch
         pd = _lib.dev_ptr(data, (self.nz, self.nproj, self.n), torch.float32, 'data')
         po = _lib.dev_ptr(obj, (self.nz, self.n, self.n), torch.float32, 'obj')

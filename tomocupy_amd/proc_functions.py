@@ -6,6 +6,8 @@ for the steps on the fourierrec path: ``darkflat_correction``, ``minus_log``,
 Settings the reference reads from the global ``args`` are constructor
 arguments.  ``proc_chunk`` is the fused form used by the conveyor: raw uint16
 chunk -> dark/flat -> [stripe removal] -> minus log -> sinogram order.
+Arithmetic is float32; ``dtype`` sets the storage of the sinogram-order
+output (the reference computes in its dtype, ``proc_functions.py:67-74``).
 """
 import ctypes
 
@@ -21,6 +23,8 @@ class ProcFunctions:
                                       % remove_stripe_method)
         self.nproj, self.ncz, self.n = int(nproj), int(ncz), int(n)
         self.ndark, self.nflat = int(ndark), int(nflat)
+        if dtype not in _lib.DTYPES:
+            raise ValueError('dtype must be one of %s' % sorted(_lib.DTYPES))
         self.dtype = dtype
         self.take_minus_log = bool(minus_log)
         self.bright_ratio = float(bright_ratio)
@@ -68,29 +72,30 @@ class ProcFunctions:
         sinogram order (``rec.py:174``): ``(nproj, ncz, n) -> (ncz, nproj, n)``."""
         import torch
         sd = (self.nproj, self.ncz, self.n)
+        td = _lib.torch_dtype(self.dtype)
         if out is None:
-            out = torch.empty((self.ncz, self.nproj, self.n), dtype=torch.float32,
-                              device=data.device)
+            out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=data.device)
         _lib.check(_lib.lib().pre_minus_log_swap(
             self._h, _lib.dev_ptr(data, sd, torch.float32, 'data'),
-            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), torch.float32, 'out'),
-            int(self.take_minus_log), _lib.stream_ptr(None)), 'pre_minus_log_swap')
+            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), td, 'out'),
+            _lib.DTYPES[self.dtype], int(self.take_minus_log), _lib.stream_ptr(None)),
+            'pre_minus_log_swap')
         return out
 
     def proc_chunk(self, data, dark, flat, out=None, res=None):
         """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, n)``."""
         import torch
         sd, sk, sf, u16 = self._shapes()
+        td = _lib.torch_dtype(self.dtype)
         if out is None:
-            out = torch.empty((self.ncz, self.nproj, self.n), dtype=torch.float32,
-                              device=data.device)
+            out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=data.device)
         if self.remove_stripe_method == 'fw':
             res = self.proc_sino(data, dark, flat, res)
             return self.proc_proj_swap(res, out)
         _lib.check(_lib.lib().pre_process(
             self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
             _lib.dev_ptr(flat, sf, u16, 'flat'),
-            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), torch.float32, 'out'),
+            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), td, 'out'), _lib.DTYPES[self.dtype],
             self.bright_ratio, int(self.flat_linear), int(self.take_minus_log),
             _lib.stream_ptr(None)), 'pre_process')
         return out
