@@ -54,13 +54,13 @@ __global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const floa
   T *r0 = g + ((size_t)z * nproj + p0) * width;
   T *r1 = g + ((size_t)z * nproj + (has1 ? p1 : p0)) * width;
   const int pad = ne / 2 - n / 2;
-  stage_lds<8, float2>(
+  stage_lds<8, cpx<T>>(
       ne, tid, nt,
       [&](int j) {
         const int src = MODE == 1 ? min(max(j - pad, 0), n - 1) : j;
-        return make_float2(ldr(r0 + src), has1 ? ldr(r1 + src) : 0.f);
+        return cpx<T>{r0[src], has1 ? r1[src] : T(0)};
       },
-      [&](int j, float2 v) { sm[lds_pad(j)] = v; });
+      [&](int j, cpx<T> v) { sm[lds_pad(j)] = tof2(v); });
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
   const float2 *wz = w + (size_t)z * (ne / 2 + 1);

@@ -1,9 +1,9 @@
 // USFFT backprojector (tomocupy cfunc_fourierrec) and its adjoint for gfx950.
 //
 // Data flow of fr_backprojection for one call (nzc = ceil(nz/2) complex
-// slices, processed in groups of 4 so one sample carries 32 B of spectra):
-//   k_fr_prep    data (nz,nproj,n) f32 -> pair-pack, shifted n-point DFT,
-//                scale (4/n)*c0            -> G [grp][p*n+k][4] complex
+// slices; the gridding treats them in groups of 4, one MFMA column pair each):
+//   k_fr_prep    data (nz,nproj,n) -> pair-pack, shifted n-point DFT,
+//                scale (4/n)*c0            -> G [c][p*n+k] complex
 //   k_fr_grid    deterministic gather of the polar samples onto the
 //                periodic 2n x 2n grid as 16x16-tile matrix products on the
 //                matrix cores (tile sample lists built on the host in FP64),
@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -46,22 +47,19 @@ __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx
   extern __shared__ float2 sm[];
   const int p = blockIdx.x, c = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
-  cpx<T> *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
-  if (c >= nzc) {
-    for (int k = tid; k < n; k += nt) stc(out + (size_t)k * 4, make_float2(0.f, 0.f));
-    return;
-  }
+  if (c >= nzc) return;
+  cpx<T> *out = G + ((size_t)c * nproj + p) * n;
   const T *r1 = data + ((size_t)c * nproj + p) * n;
   const int z2 = c + nzc;
   const T *r2 = data + ((size_t)(z2 < nz ? z2 : c) * nproj + p) * n;
   const bool has2 = z2 < nz;
-  stage_lds<8, float2>(
-      n, tid, nt, [&](int j) { return make_float2(ldr(r1 + j), has2 ? ldr(r2 + j) : 0.f); },
-      [&](int j, float2 v) { sm[lds_pad(j)] = cscale(v, shift_sign(j)); });
+  stage_lds<8, cpx<T>>(
+      n, tid, nt, [&](int j) { return cpx<T>{r1[j], has2 ? r2[j] : T(0)}; },
+      [&](int j, cpx<T> v) { sm[lds_pad(j)] = cscale(tof2(v), shift_sign(j)); });
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
   for (int k = tid; k < n; k += nt)
-    stc(out + (size_t)k * 4, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
+    stc(out + k, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
 }
 
 // Fused FBP filter + pair packing + shifted detector FFT (the fused form of
@@ -79,23 +77,18 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   __shared__ float edge[4];
   const int p = blockIdx.x, c = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
-  cpx<T> *out = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
-  if (c >= nzc) {
-    for (int k = tid; k < n; k += nt) stc(out + (size_t)k * 4, make_float2(0.f, 0.f));
-    return;
-  }
+  if (c >= nzc) return;
+  cpx<T> *out = G + ((size_t)c * nproj + p) * n;
   const int n2 = 2 * n;
   const int z2 = c + nzc;
   const bool has2 = z2 < nz;
   const int zb = has2 ? z2 : c;
   const T *r1 = data + ((size_t)c * nproj + p) * n;
   const T *r2 = data + ((size_t)zb * nproj + p) * n;
-  stage_lds<8, float2>(
+  stage_lds<8, cpx<T>>(
       n2, tid, nt,
-      [&](int j) {
-        return j < n ? make_float2(ldr(r1 + j), has2 ? ldr(r2 + j) : 0.f) : make_float2(0.f, 0.f);
-      },
-      [&](int j, float2 v) { sm[lds_pad(j)] = v; });
+      [&](int j) { return j < n ? cpx<T>{r1[j], has2 ? r2[j] : T(0)} : cpx<T>{T(0), T(0)}; },
+      [&](int j, cpx<T> v) { sm[lds_pad(j)] = tof2(v); });
   if (tid == 0) {
     edge[0] = ldr(r1);
     edge[1] = ldr(r1 + n - 1);
@@ -131,7 +124,7 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   __syncthreads();
   lds_fft<-1>(sm, fd1, 0, tid, nt);
   for (int k = tid; k < n; k += nt)
-    stc(out + (size_t)k * 4, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
+    stc(out + k, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -158,25 +151,31 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
   const int4 item = items[it];
   const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
   const int k = lane >> 4, i = lane & 15;
-  const cpx4<T> *Gg = reinterpret_cast<const cpx4<T> *>(G + (size_t)grp * nsamp * 4);
+  // the four complex slices of the group (planes of G; slices past nzc read as 0)
+  const int nlive = min(4, nzc - 4 * grp);
+  const cpx<T> *Gg = G + (size_t)(4 * grp) * nsamp;
+  auto load4 = [&](int smp, cpx<T> *v) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      v[c] = (smp >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + smp] : cpx<T>{};
+  };
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
   // entry {sample, footprint offsets (cols | rows << 8), du bits, dv bits};
   // loads run one step ahead of the matrix products
   const int eend = ebeg + cnt;
-  const cpx4<T> zero4 = {};
   int4 en = cnt > 0 ? ent[ebeg + k] : make_int4(-1, 0, 0, 0);
-  cpx4<T> va = en.x >= 0 ? Gg[en.x] : zero4;
+  cpx<T> va[4];
+  load4(en.x, va);
   for (int e = ebeg; e < eend; e += 4) {
     const int4 cur = en;
-    const cpx4<T> cv = va;
+    const float gv[8] = {(float)va[0].x, (float)va[0].y, (float)va[1].x, (float)va[1].y,
+                         (float)va[2].x, (float)va[2].y, (float)va[3].x, (float)va[3].y};
     if (e + 4 < eend) {
       en = ent[e + 4 + k];
-      va = en.x >= 0 ? Gg[en.x] : zero4;
+      load4(en.x, va);
     }
-    const float gv[8] = {(float)cv.v[0].x, (float)cv.v[0].y, (float)cv.v[1].x, (float)cv.v[1].y,
-                         (float)cv.v[2].x, (float)cv.v[2].y, (float)cv.v[3].x, (float)cv.v[3].y};
     float e0v = 0.f, e1v = 0.f;
     if (cur.x >= 0) {
       const int rb0 = (int)(signed char)(cur.y & 0xff);
@@ -206,6 +205,132 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
         const int a = 16 * TA + ib + r;
         if (a < n2 && b < n2)
           stc(Fc + (size_t)a * n2 + b, make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
+      }
+    }
+  } else {
+    float *P = part + ((size_t)grp * nslots + slot) * 2048;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) P[m * 256 + (ib + r) * 16 + j] = acc[m][r];
+  }
+}
+
+// 16-bit plans: the same tile products with 16 samples per matrix
+// instruction (v_mfma_f32_16x16x16 f16 / bf16, FP32 accumulation).  Lane l
+// supplies A[l%16][4(l/16)+u] and B[4(l/16)+u][l%16] for u < 4, so lane group
+// l/16 evaluates the Gaussian factors of samples 4(l/16)..4(l/16)+3 of each
+// batch of 16; the factors and the weighted spectra are rounded to the
+// storage type, the sums stay in FP32.  Tile lists are padded to multiples of
+// 16 for these plans.
+template <typename T>
+struct Mma16;
+template <>
+struct Mma16<_Float16> {
+  typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 pack(float a, float b, float c, float d) {
+    return v4{(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+  }
+  static __device__ __forceinline__ floatx4 mma(v4 a, v4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mma16<__bf16> {
+  typedef short v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ short bits(float x) {
+    return __builtin_bit_cast(short, (__bf16)x);
+  }
+  static __device__ __forceinline__ v4 pack(float a, float b, float c, float d) {
+    return v4{bits(a), bits(b), bits(c), bits(d)};
+  }
+  static __device__ __forceinline__ floatx4 mma(v4 a, v4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
+    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
+    const cpx<T> *__restrict__ G, cpx<T> *__restrict__ F, float *__restrict__ part, int n,
+    int ntile, int nslots, int nsamp, int nzc, float K2) {
+  typedef Mma16<T> M;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int it = blockIdx.x * kGridWaves + wave;
+  const int grp = blockIdx.y;
+  if (it >= nitems) return;
+  const int4 item = items[it];
+  const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
+  const int g4 = lane >> 4, i = lane & 15;
+  const int nlive = min(4, nzc - 4 * grp);
+  const cpx<T> *Gg = G + (size_t)(4 * grp) * nsamp;
+  floatx4 acc[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int eend = ebeg + cnt;
+  int4 en[4];
+  cpx<T> gv[4][4];
+  auto load = [&](int e) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) en[u] = ent[e + 4 * g4 + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        gv[u][c] = (en[u].x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + en[u].x]
+                                                : cpx<T>{T(0), T(0)};
+  };
+  if (cnt > 0) load(ebeg);
+  for (int e = ebeg; e < eend; e += 16) {
+    int4 cur[4];
+    float g[4][8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cur[u] = en[u];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        g[u][2 * c] = (float)gv[u][c].x;
+        g[u][2 * c + 1] = (float)gv[u][c].y;
+      }
+    }
+    if (e + 16 < eend) load(e + 16);
+    float e0v[4], e1v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      e0v[u] = e1v[u] = 0.f;
+      if (cur[u].x >= 0) {
+        const int rb0 = (int)(signed char)(cur[u].y & 0xff);
+        const int rb1 = (int)(signed char)((cur[u].y >> 8) & 0xff);
+        const int d0 = i - rb0, d1 = i - rb1;
+        const float x0 = (float)(d0 - 4) - __int_as_float(cur[u].z);
+        const float x1 = (float)(d1 - 4) - __int_as_float(cur[u].w);
+        e0v[u] = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
+        e1v[u] = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
+      }
+    }
+    const typename M::v4 b = M::pack(e0v[0], e0v[1], e0v[2], e0v[3]);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const typename M::v4 a = M::pack(e1v[0] * g[0][m], e1v[1] * g[1][m], e1v[2] * g[2][m],
+                                        e1v[3] * g[3][m]);
+      acc[m] = M::mma(a, b, acc[m]);
+    }
+  }
+  const int j = lane & 15, ib = 4 * (lane >> 4);
+  if (slot < 0) {
+    const int n2 = 2 * n;
+    const int TA = tile / ntile, TB = tile - TA * ntile;
+    const int bcol = 16 * TB + j;
+    const size_t plane = (size_t)n2 * n2;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (grp * 4 + c >= nzc) continue;
+      cpx<T> *Fc = F + (size_t)(grp * 4 + c) * plane;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = 16 * TA + ib + r;
+        if (a < n2 && bcol < n2)
+          stc(Fc + (size_t)a * n2 + bcol, make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
       }
     }
   } else {
@@ -252,9 +377,12 @@ __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n2 = 2 * n;
   const cpx<T> *src = F + ((size_t)c * n2 + a) * n2;
-  stage_lds<8, float2>(
-      n2, tid, nt, [&](int b) { return ldc(src + b); },
-      [&](int b, float2 x) { sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x; });
+  stage_lds<8, cpx<T>>(
+      n2, tid, nt, [&](int b) { return src[b]; },
+      [&](int b, cpx<T> v) {
+        const float2 x = tof2(v);
+        sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
+      });
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
   cpx<T> *dst = R + ((size_t)c * n2 + a) * n;
@@ -279,14 +407,15 @@ __global__ __launch_bounds__(512) void k_fr_colfft(const cpx<T> *__restrict__ R,
   const int CB = 1 << lgCB;
   const int n2 = 2 * n, v0 = cb * CB;
   const cpx<T> *src = R + (size_t)c * n2 * n;
-  stage_lds<16, float2>(
+  stage_lds<16, cpx<T>>(
       CB * n2, tid, nt,
       [&](int idx) {
         const int a = idx >> lgCB, t = idx & (CB - 1);
-        return (v0 + t < n) ? ldc(src + (size_t)a * n + v0 + t) : make_float2(0.f, 0.f);
+        return (v0 + t < n) ? src[(size_t)a * n + v0 + t] : cpx<T>{T(0), T(0)};
       },
-      [&](int idx, float2 x) {
+      [&](int idx, cpx<T> v) {
         const int a = idx >> lgCB, t = idx & (CB - 1);
+        const float2 x = tof2(v);
         sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
       });
   __syncthreads();
@@ -413,9 +542,9 @@ __global__ void k_fr_interp(const float4 *__restrict__ geom, const float2 *__res
       }
     }
   }
-  float2 *out = G + ((size_t)grp * nsamp + s) * 4;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) out[c] = acc[c];
+  for (int c = 0; c < 4; ++c)
+    if (c0 + c < nzc) G[(size_t)(c0 + c) * nsamp + s] = acc[c];
 }
 
 __global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ data, FftDesc fd,
@@ -424,9 +553,9 @@ __global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ 
   const int p = blockIdx.x, c = blockIdx.y;
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
-  const float2 *in = G + ((size_t)(c >> 2) * nproj * n) * 4 + (c & 3) + (size_t)p * n * 4;
+  const float2 *in = G + ((size_t)c * nproj + p) * n;
   stage_lds<8, float2>(
-      n, tid, nt, [&](int k) { return in[(size_t)k * 4]; },
+      n, tid, nt, [&](int k) { return in[k]; },
       [&](int k, float2 v) { sm[lds_pad(k)] = cscale(v, shift_sign(k) * scale); });
   __syncthreads();
   lds_fft<1>(sm, fd, 0, tid, nt);
@@ -495,7 +624,8 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   }
   // 16x16 tile lists of the periodic grid: entries {sample, footprint start
   // relative to the tile (columns | rows << 8, each in [-8, 16))}, samples in
-  // increasing order, each list padded to a multiple of 4 with {-1, 0}
+  // increasing order, each list padded with {-1, 0} to a multiple of the
+  // samples per matrix instruction (4 for float32 plans, 16 for 16-bit plans)
   const int nt = (int)ceil_div(n2, 16);
   P->ntile = nt;
   auto tiles_of = [&](int base, int *tl, int *rel) {  // distinct tiles of rows base..base+8
@@ -525,7 +655,8 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
       for (int v = 0; v < nc; ++v) counts[(size_t)tr[u] * nt + tc[v]]++;
   }
   std::vector<int64_t> off(ntiles + 1, 0);
-  for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + 3) / 4 * 4;
+  const int64_t gpad = P->dtype == TOMO_F32 ? 4 : 16;
+  for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + gpad - 1) / gpad * gpad;
   std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, 0, 0, 0));
   std::vector<int64_t> fill(off.begin(), off.end() - 1);
   for (int64_t s = 0; s < S; ++s) {
@@ -723,10 +854,15 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
   cpx<T> *G = P->G.as<cpx<T>>(), *F = P->F.as<cpx<T>>(), *R = P->R.as<cpx<T>>();
   {
     ProfScope ps_("k_fr_grid", st);
-    hipLaunchKernelGGL(k_fr_grid<T>, dim3((unsigned)ceil_div(P->nitems, kGridWaves), ng),
-                       dim3(64 * kGridWaves), 0, st, P->items.as<int4>(), P->nitems,
-                       P->ent.as<int4>(), G, F, P->part.as<float>(), n, P->ntile, P->nslots,
-                       (int)P->nsamp, nzc, P->K2);
+    const dim3 gg((unsigned)ceil_div(P->nitems, kGridWaves), ng);
+    if constexpr (std::is_same<T, float>::value)
+      hipLaunchKernelGGL(k_fr_grid<T>, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),
+                         P->nitems, P->ent.as<int4>(), G, F, P->part.as<float>(), n, P->ntile,
+                         P->nslots, (int)P->nsamp, nzc, P->K2);
+    else
+      hipLaunchKernelGGL(k_fr_grid16<T>, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),
+                         P->nitems, P->ent.as<int4>(), G, F, P->part.as<float>(), n, P->ntile,
+                         P->nslots, (int)P->nsamp, nzc, P->K2);
   }
   TOMO_LAUNCH_CHECK();
   if (P->nred > 0) {

@@ -13,11 +13,12 @@ struct alignas(2 * sizeof(T)) cpx {
   T x, y;
 };
 
-// Four complex values of one polar sample (the four slices of a group).
+// Conversion of a loaded (raw) complex value; loading raw values first and
+// converting afterwards lets a thread keep several loads in flight.
 template <typename T>
-struct alignas(8 * sizeof(T) >= 16 ? 16 : 8 * sizeof(T)) cpx4 {
-  cpx<T> v[4];
-};
+__device__ __forceinline__ float2 tof2(cpx<T> v) {
+  return make_float2((float)v.x, (float)v.y);
+}
 
 template <typename T>
 __device__ __forceinline__ float ldr(const T *p) {

@@ -111,6 +111,7 @@ __global__ __launch_bounds__(256) void k_fw_cols(
     float *__restrict__ lh, float *__restrict__ hl, float *__restrict__ hh, Taps t, int H, int W,
     int Ho, int ot, const float2 *__restrict__ tw, float2 *__restrict__ part, int nk) {
   __shared__ float2 xs[kColIn][64];
+  __shared__ float2 twl[KM * kColOut];  // twiddles of this block's rows
   static_assert(2 * KM <= kColIn, "partial-sum reduction must fit the staging LDS");
   const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
   const int w = blockIdx.x * 64 + lane;
@@ -124,6 +125,13 @@ __global__ __launch_bounds__(256) void k_fw_cols(
   const float *plo = lo + (size_t)z * H * W + (live ? w : 0);
   const float *phi = hi + (size_t)z * H * W + (live ? w : 0);
   constexpr int kIt = (kColIn + 3) / 4;
+  constexpr int kTw = (KM * kColOut + 255) / 256;
+  float2 twv[kTw];
+#pragma unroll
+  for (int u = 0; u < kTw; ++u) {
+    const int idx = threadIdx.x + 256 * u, k = idx / kColOut, q = idx % kColOut;
+    twv[u] = (k < nk && q < ot && oh0 + q < Ho) ? tw[k * Ho + oh0 + q] : make_float2(0.f, 0.f);
+  }
   float2 v[kIt];
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
@@ -137,6 +145,9 @@ __global__ __launch_bounds__(256) void k_fw_cols(
 #pragma unroll
   for (int it = 0; it < kIt; ++it)
     if (sub + 4 * it < nin) xs[sub + 4 * it][lane] = v[it];
+#pragma unroll
+  for (int u = 0; u < kTw; ++u)
+    if (threadIdx.x + 256 * u < KM * kColOut) twl[threadIdx.x + 256 * u] = twv[u];
   __syncthreads();
   float xr[KM], xi[KM];
 #pragma unroll
@@ -152,12 +163,10 @@ __global__ __launch_bounds__(256) void k_fw_cols(
       b0 = fmaf(t.a[j], x.y, b0);
       b1 = fmaf(t.b[j], x.y, b1);
     }
-    // wave-uniform row: the twiddles come through scalar loads
-    const int ohu = __builtin_amdgcn_readfirstlane(oh);
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
       if (k < nk) {
-        const float2 c = tw[k * Ho + ohu];
+        const float2 c = twl[k * kColOut + q];
         xr[k] = fmaf(b0, c.x, xr[k]);
         xi[k] = fmaf(-b0, c.y, xi[k]);
       }
@@ -244,6 +253,7 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
     float *__restrict__ hi, Taps g, int H, int W, int Hs, int ot, const float2 *__restrict__ tw,
     const float2 *__restrict__ X, int nk) {
   __shared__ float4 xs[kSynIn][64];
+  __shared__ float2 twl[KM * kSynIn];  // twiddles of the staged band rows
   const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
   const int w = blockIdx.x * 64 + lane;
   const int tp0 = blockIdx.y * ot, z = blockIdx.z;
@@ -259,6 +269,13 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
     xi[k] = x.y;
   }
   constexpr int kIt = (kSynIn + 3) / 4;
+  constexpr int kTw = (KM * kSynIn + 255) / 256;
+  float2 twv[kTw];
+#pragma unroll
+  for (int u = 0; u < kTw; ++u) {
+    const int idx = threadIdx.x + 256 * u, k = idx / kSynIn, q = idx % kSynIn;
+    twv[u] = (k < nk && q < nin && i0 + q < H) ? tw[k * H + i0 + q] : make_float2(0.f, 0.f);
+  }
   float4 v[kIt];
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
@@ -270,16 +287,19 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
     }
   }
 #pragma unroll
+  for (int u = 0; u < kTw; ++u)
+    if (threadIdx.x + 256 * u < KM * kSynIn) twl[threadIdx.x + 256 * u] = twv[u];
+  __syncthreads();
+#pragma unroll
   for (int it = 0; it < kIt; ++it) {
     const int q = sub + 4 * it;
     if (q < nin) {
-      const int iu = __builtin_amdgcn_readfirstlane(i0 + q);
-      if (iu < H) {
+      if (i0 + q < H) {
         float c = v[it].z;
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
           if (k < nk) {
-            const float2 e = tw[k * H + iu];
+            const float2 e = twl[k * kSynIn + q];
             c -= xr[k] * e.x - xi[k] * e.y;
           }
         }
