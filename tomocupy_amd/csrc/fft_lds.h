@@ -14,6 +14,8 @@
 // both directions.
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace tomo {
@@ -235,10 +237,11 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
   int Ns = 1;
   for (int s = 0; s < d.nr; ++s) {
     const int R = d.r[s];
-    if (R == 16)
-      fft_pass<16, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-    else if (R == 8)
+    if (R == 16) {
+      if constexpr (EPT >= 16) fft_pass<16, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    } else if (R == 8) {
       fft_pass<8, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    }
     else if (R == 4)
       fft_pass<4, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 2)
@@ -246,6 +249,92 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
     else
       fft_pass<3, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     Ns *= R;
+  }
+}
+
+// ---- transforms of a compile-time power-of-two size -------------------------
+// The same Stockham passes with N, the radix sequence (radix 16 passes, then
+// one radix 8/4/2 pass for the remaining factor, as FftPlan::init picks), the
+// sub-transform lengths, the twiddle steps and the interleave 2^LGH all
+// constants.  With one pad slot per 16 elements, the R reads of a butterfly
+// (stride N/R, a multiple of 16) and its R writes (stride Ns, a multiple of 16,
+// or consecutive when Ns = 1) sit at constant offsets from one base address,
+// so a pass costs one address computation per butterfly instead of one per
+// element.
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+template <int N>
+struct Pow2Plan {
+  static_assert(N >= 256 && (N & (N - 1)) == 0, "power of two >= 256");
+  static constexpr int n16 = ilog2c(N) / 4;
+  static constexpr int rest = 1 << (ilog2c(N) % 4);
+  static constexpr int passes = n16 + (rest > 1 ? 1 : 0);
+  static constexpr int radix(int s) { return s < n16 ? 16 : rest; }
+};
+
+template <int R, int Ns, int N, int DIR, int EPT, int LGH>
+__device__ __forceinline__ void fft_pass_c(float2 *buf, const float2 *__restrict__ tw, int tid,
+                                           int nt) {
+  constexpr int MAXB = EPT / R;
+  constexpr int H = 1 << LGH;
+  constexpr int nbPer = N / R;
+  constexpr int nb = nbPer << LGH;
+  constexpr int twstep = N / (Ns * R);
+  static_assert(MAXB >= 1, "EPT must cover the radix");
+  static_assert(nbPer % 16 == 0 && (Ns == 1 || Ns % 16 == 0), "constant-offset layout");
+  constexpr int rstride = (nbPer + nbPer / 16) << LGH;
+  constexpr int wstride = Ns == 1 ? H : (Ns + Ns / 16) << LGH;
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> LGH;
+      const int base = (lds_pad(j) << LGH) + t;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[base + r * rstride];
+      if constexpr (Ns > 1) twiddle<R, DIR>(v[q], tw, (unsigned)((j & (Ns - 1)) * twstep));
+      butterfly<R, DIR>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> LGH;
+      const int k = j & (Ns - 1);
+      const int base = (lds_pad((j - k) * R + k) << LGH) + t;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * wstride] = v[q][out_slot<R>(r)];
+    }
+  }
+  __syncthreads();
+}
+
+// lds_fft for a compile-time size N (a power of two >= 256); `tw` is the
+// twiddle table of size N.  Same layout, requirements and barriers as lds_fft.
+template <int N, int DIR, int EPT, int LGH, int S = 0, int Ns = 1>
+__device__ __forceinline__ void lds_fft_c(float2 *buf, const float2 *__restrict__ tw, int tid,
+                                          int nt) {
+  if constexpr (S < Pow2Plan<N>::passes) {
+    constexpr int R = Pow2Plan<N>::radix(S);
+    fft_pass_c<R, Ns, N, DIR, EPT, LGH>(buf, tw, tid, nt);
+    lds_fft_c<N, DIR, EPT, LGH, S + 1, Ns * R>(buf, tw, tid, nt);
+  }
+}
+
+// Calls f(std::integral_constant<int, N>) for the power-of-two sizes with a
+// compiled specialisation, f(std::integral_constant<int, 0>) otherwise.
+template <typename F>
+inline void with_static_size(int N, F &&f) {
+  switch (N) {
+    case 512: f(std::integral_constant<int, 512>()); break;
+    case 1024: f(std::integral_constant<int, 1024>()); break;
+    case 2048: f(std::integral_constant<int, 2048>()); break;
+    case 4096: f(std::integral_constant<int, 4096>()); break;
+    case 8192: f(std::integral_constant<int, 8192>()); break;
+    default: f(std::integral_constant<int, 0>()); break;
   }
 }
 

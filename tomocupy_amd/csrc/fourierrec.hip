@@ -36,11 +36,16 @@ namespace {
 constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
 constexpr int kGridChunk = 512; // max list entries per work item (multiple of 4)
 constexpr int kColEPT = 32;     // FFT values per thread in the column pass
+// log2 of the columns a column-pass workgroup transforms for 2n = N: as many
+// as fit 512 threads at kColEPT values each (and the LDS), at most 16
+constexpr int col_lg(int N) { return ilog2c(16384 / N) > 4 ? 4 : ilog2c(16384 / N); }
 
 // Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
 
-template <typename T>
+// NS: n as a compile-time constant (0 = sizes known at run time only), as in
+// every FFT kernel of this file.
+template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx<T> *__restrict__ G,
                                                  FftDesc fd, int n, int nproj, int nz, int nzc,
                                                  float scale) {
@@ -57,7 +62,10 @@ __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx
       n, tid, nt, [&](int j) { return cpx<T>{r1[j], has2 ? r2[j] : T(0)}; },
       [&](int j, cpx<T> v) { sm[lds_pad(j)] = cscale(tof2(v), shift_sign(j)); });
   __syncthreads();
-  lds_fft<-1>(sm, fd, 0, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd.tw, tid, nt);
+  else
+    lds_fft<-1>(sm, fd, 0, tid, nt);
   for (int k = tid; k < n; k += nt)
     stc(out + k, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
 }
@@ -68,7 +76,7 @@ __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx
 // the two slices' wrapped filter spectra (Y = Z (H1+H2)/2 + conj(Z(-k))
 // (H1-H2)/2, exact for real rows), IFFT_2n, add the edge-replication terms
 // x[0] HL + x[n-1] HR per slice, then the shifted n-point DFT of k_fr_prep.
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_filtprep(
     const T *__restrict__ data, cpx<T> *__restrict__ G, const float2 *__restrict__ H2,
     const float *__restrict__ HL, const float *__restrict__ HR, FftDesc fd2, FftDesc fd1, int n,
@@ -96,7 +104,10 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
     edge[3] = has2 ? ldr(r2 + n - 1) : 0.f;
   }
   __syncthreads();
-  lds_fft<-1>(sm, fd2, 0, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<2 * NS, -1, kFftEPT, 0>(sm, fd2.tw, tid, nt);
+  else
+    lds_fft<-1>(sm, fd2, 0, tid, nt);
   const float2 *Ha = H2 + (size_t)c * n2;
   const float2 *Hb = H2 + (size_t)zb * n2;
   for (int k = tid; k <= n; k += nt) {
@@ -111,7 +122,10 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
     if (km != k) sm[lds_pad(km)] = ym;
   }
   __syncthreads();
-  lds_fft<1>(sm, fd2, 0, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<2 * NS, 1, kFftEPT, 0>(sm, fd2.tw, tid, nt);
+  else
+    lds_fft<1>(sm, fd2, 0, tid, nt);
   const float *hl1 = HL + (size_t)c * n, *hr1 = HR + (size_t)c * n;
   const float *hl2 = HL + (size_t)zb * n, *hr2 = HR + (size_t)zb * n;
   const float e0 = edge[0], e1 = edge[1], e2 = edge[2], e3 = edge[3];
@@ -122,7 +136,10 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
     sm[lds_pad(j)] = cscale(y, shift_sign(j));
   }
   __syncthreads();
-  lds_fft<-1>(sm, fd1, 0, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd1.tw, tid, nt);
+  else
+    lds_fft<-1>(sm, fd1, 0, tid, nt);
   for (int k = tid; k < n; k += nt)
     stc(out + k, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
 }
@@ -367,7 +384,7 @@ __global__ void k_fr_grid_reduce(const int4 *__restrict__ red, const float *__re
           make_float2(acc[2 * c], acc[2 * c + 1]));
 }
 
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
                                                    cpx<T> *__restrict__ R, FftDesc fd, int n,
                                                    int nzc) {
@@ -384,7 +401,10 @@ __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
         sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
       });
   __syncthreads();
-  lds_fft<1>(sm, fd, 0, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<NS, 1, kFftEPT, 0>(sm, fd.tw, tid, nt);
+  else
+    lds_fft<1>(sm, fd, 0, tid, nt);
   cpx<T> *dst = R + ((size_t)c * n2 + a) * n;
   const int h = n / 2;
   for (int v = tid; v < n; v += nt) {
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
   }
 }
 
-template <typename T>
+template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_colfft(const cpx<T> *__restrict__ R, T *__restrict__ obj,
                             const float *__restrict__ phix, const float *__restrict__ phiy,
                             const int2 *__restrict__ circ, FftDesc fd, int n, int nz, int nzc,
@@ -419,7 +439,10 @@ __global__ __launch_bounds__(512) void k_fr_colfft(const cpx<T> *__restrict__ R,
         sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
       });
   __syncthreads();
-  lds_fft<1, kColEPT>(sm, fd, lgCB, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<NS, 1, kColEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
+  else
+    lds_fft<1, kColEPT>(sm, fd, lgCB, tid, nt);
   const int z2 = c + nzc;
   const int h = n / 2;
   for (int idx = tid; idx < CB * n; idx += nt) {
@@ -581,6 +604,7 @@ struct tomo_fr_plan {
   int ntile = 0, nitems = 0, nred = 0, nslots = 0;
   DevBuf G, F, R;
   int colCB = 1, lgCB = 0, colThreads = 256, prepThreads = 256, rowThreads = 256;
+  int nsRow = 0, nsCol = 0, nsPrep = 0;  // compiled FFT sizes in use (0 = run-time sizes)
   int rowLds = 0, prepLds = 0;
   size_t colLds = 0;
   float scale1 = 0.f, K2 = 0.f;
@@ -821,23 +845,45 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
     set_error("fr_create: n=%lld needs more than 1024 threads per FFT workgroup", (long long)n);
     return fail(TOMO_EINVAL);
   }
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft<float>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft<_Float16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft<__bf16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
+  // size-specialised kernels where 2n is a compiled power of two (and the
+  // column grouping matches the specialisation)
+  P->nsRow = (int)n2;
+  P->nsCol = (lg == col_lg((int)n2)) ? (int)n2 : 0;
+  with_static_size(P->nsRow, [&](auto ns) { P->nsRow = decltype(ns)::value; });
+  with_static_size(P->nsCol, [&](auto ns) { P->nsCol = decltype(ns)::value; });
+  P->nsPrep = (int)n;
+  with_static_size(P->nsPrep, [&](auto ns) { P->nsPrep = decltype(ns)::value; });
+  if (P->nsRow == 0 || 2 * P->nsPrep != P->nsRow) P->nsPrep = 0;
+  hipError_t ea = hipSuccess;
+  auto lds_attr = [&](const void *k, size_t bytes) {
+    if (ea == hipSuccess)
+      ea = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  };
+  with_static_size(P->nsCol, [&](auto ns) {
+    constexpr int NS = decltype(ns)::value;
+    lds_attr((const void *)k_fr_colfft<float, NS>, P->colLds);
+    lds_attr((const void *)k_fr_colfft<_Float16, NS>, P->colLds);
+    lds_attr((const void *)k_fr_colfft<__bf16, NS>, P->colLds);
+  });
+  if (ea != hipSuccess) {
+    set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
+    return fail(TOMO_EHIP);
+  }
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft_adj,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
   const int rowLds = (int)(lds_padded_len((int)n2) * sizeof(float2));
   P->rowLds = rowLds;
   P->prepLds = (int)(lds_padded_len((int)n) * sizeof(float2));
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft<float>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft<_Float16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft<__bf16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
+  with_static_size(P->nsRow, [&](auto ns) {
+    constexpr int NS = decltype(ns)::value;
+    lds_attr((const void *)k_fr_rowfft<float, NS>, rowLds);
+    lds_attr((const void *)k_fr_rowfft<_Float16, NS>, rowLds);
+    lds_attr((const void *)k_fr_rowfft<__bf16, NS>, rowLds);
+  });
+  if (ea != hipSuccess) {
+    set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
+    return fail(TOMO_EHIP);
+  }
   TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft_adj,
                                hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
   *plan = P;
@@ -873,15 +919,20 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_rowfft", st);
-    hipLaunchKernelGGL(k_fr_rowfft<T>, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds, st, F,
-                       R, P->fft2.d, n, nzc);
+    with_static_size(P->nsRow, [&](auto ns) {
+      hipLaunchKernelGGL((k_fr_rowfft<T, decltype(ns)::value>), dim3(n2, ng * 4),
+                         dim3(P->rowThreads), P->rowLds, st, F, R, P->fft2.d, n, nzc);
+    });
   }
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_colfft", st);
-    hipLaunchKernelGGL(k_fr_colfft<T>, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
-                       dim3(P->colThreads), P->colLds, st, R, obj, P->phix.as<float>(),
-                       P->phiy.as<float>(), P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+    with_static_size(P->nsCol, [&](auto ns) {
+      hipLaunchKernelGGL((k_fr_colfft<T, decltype(ns)::value>),
+                         dim3((unsigned)ceil_div(n, P->colCB), ng * 4), dim3(P->colThreads),
+                         P->colLds, st, R, obj, P->phix.as<float>(), P->phiy.as<float>(),
+                         P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+    });
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
@@ -893,8 +944,11 @@ static int fr_backprojection_t(tomo_fr_plan *P, T *obj, const T *data, hipStream
   const int ng = (int)P->ngrp;
   {
     ProfScope ps_("k_fr_prep", st);
-    hipLaunchKernelGGL(k_fr_prep<T>, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
-                       data, P->G.as<cpx<T>>(), P->fft1.d, n, nproj, nz, nzc, P->scale1);
+    with_static_size(P->nsPrep, [&](auto ns) {
+      hipLaunchKernelGGL((k_fr_prep<T, decltype(ns)::value>), dim3(nproj, ng * 4),
+                         dim3(P->prepThreads), P->prepLds, st, data, P->G.as<cpx<T>>(), P->fft1.d,
+                         n, nproj, nz, nzc, P->scale1);
+    });
   }
   TOMO_LAUNCH_CHECK();
   return fr_tail<T>(P, obj, st);
@@ -907,9 +961,12 @@ static int fr_filter_backprojection_t(tomo_fr_plan *P, tomo_flt_plan *F, T *obj,
   const int ng = (int)P->ngrp;
   {
     ProfScope ps_("k_fr_filtprep", st);
-    hipLaunchKernelGGL(k_fr_filtprep<T>, dim3(nproj, ng * 4), dim3(P->rowThreads), P->rowLds,
-                       st, data, P->G.as<cpx<T>>(), F->H2.as<float2>(), F->HL.as<float>(),
-                       F->HR.as<float>(), F->fft2n.d, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+    with_static_size(P->nsPrep, [&](auto ns) {
+      hipLaunchKernelGGL((k_fr_filtprep<T, decltype(ns)::value>), dim3(nproj, ng * 4),
+                         dim3(P->rowThreads), P->rowLds, st, data, P->G.as<cpx<T>>(),
+                         F->H2.as<float2>(), F->HL.as<float>(), F->HR.as<float>(), F->fft2n.d,
+                         P->fft1.d, n, nproj, nz, nzc, P->scale1);
+    });
   }
   TOMO_LAUNCH_CHECK();
   return fr_tail<T>(P, obj, st);
