@@ -34,7 +34,8 @@ using namespace tomo;
 namespace {
 
 constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
-constexpr int kGridChunk = 512; // max list entries per work item (multiple of 4)
+constexpr int kGridChunk = 512; // max list entries per work item (multiple of 16)
+constexpr int kGridBatch = 64;  // list entries a gridding wave stages at a time
 constexpr int kColEPT = 32;     // FFT values per thread in the column pass
 // log2 of the columns a column-pass workgroup transforms for 2n = N: as many
 // as fit 512 threads at kColEPT values each (and the LDS), at most 16
@@ -171,41 +172,58 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
   // the four complex slices of the group (planes of G; slices past nzc read as 0)
   const int nlive = min(4, nzc - 4 * grp);
   const cpx<T> *Gg = G + (size_t)(4 * grp) * nsamp;
-  auto load4 = [&](int smp, cpx<T> *v) {
+  // Entries {sample, footprint offsets (cols | rows << 8), du bits, dv bits}
+  // and their spectra arrive kGridBatch at a time, one per lane, and are
+  // staged in this wave's LDS; the loads of the next batch are in flight while
+  // the matrix products of the current one run.
+  __shared__ int4 se[kGridWaves][2][kGridBatch];
+  __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
+  const int eend = ebeg + cnt;
+  int4 re = make_int4(-1, 0, 0, 0);
+  cpx<T> rg[4];
+  auto fetch = [&](int base) {
+    const int idx = base + lane;
+    re = idx < eend ? ent[idx] : make_int4(-1, 0, 0, 0);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      v[c] = (smp >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + smp] : cpx<T>{};
+      rg[c] = (re.x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + re.x] : cpx<T>{T(0), T(0)};
   };
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-  // entry {sample, footprint offsets (cols | rows << 8), du bits, dv bits};
-  // loads run one step ahead of the matrix products
-  const int eend = ebeg + cnt;
-  int4 en = cnt > 0 ? ent[ebeg + k] : make_int4(-1, 0, 0, 0);
-  cpx<T> va[4];
-  load4(en.x, va);
-  for (int e = ebeg; e < eend; e += 4) {
-    const int4 cur = en;
-    const float gv[8] = {(float)va[0].x, (float)va[0].y, (float)va[1].x, (float)va[1].y,
-                         (float)va[2].x, (float)va[2].y, (float)va[3].x, (float)va[3].y};
-    if (e + 4 < eend) {
-      en = ent[e + 4 + k];
-      load4(en.x, va);
-    }
-    float e0v = 0.f, e1v = 0.f;
-    if (cur.x >= 0) {
-      const int rb0 = (int)(signed char)(cur.y & 0xff);
-      const int rb1 = (int)(signed char)((cur.y >> 8) & 0xff);
-      const int d0 = i - rb0, d1 = i - rb1;
-      const float x0 = (float)(d0 - 4) - __int_as_float(cur.z);
-      const float x1 = (float)(d1 - 4) - __int_as_float(cur.w);
-      e0v = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
-      e1v = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
-    }
+  if (cnt > 0) fetch(ebeg);
+  int buf = 0;
+  for (int b = ebeg; b < eend; b += kGridBatch, buf ^= 1) {
+    se[wave][buf][lane] = re;
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
-      acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * gv[m], e0v, acc[m], 0, 0, 0);
+    for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = tof2(rg[c]);
+    __builtin_amdgcn_wave_barrier();
+    if (b + kGridBatch < eend) fetch(b + kGridBatch);
+    const int ngr = min(kGridBatch, eend - b) >> 2;
+    for (int q = 0; q < ngr; ++q) {
+      const int4 cur = se[wave][buf][4 * q + k];
+      float gv[8];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float2 g = sg[wave][buf][c][4 * q + k];
+        gv[2 * c] = g.x;
+        gv[2 * c + 1] = g.y;
+      }
+      float e0v = 0.f, e1v = 0.f;
+      if (cur.x >= 0) {
+        const int rb0 = (int)(signed char)(cur.y & 0xff);
+        const int rb1 = (int)(signed char)((cur.y >> 8) & 0xff);
+        const int d0 = i - rb0, d1 = i - rb1;
+        const float x0 = (float)(d0 - 4) - __int_as_float(cur.z);
+        const float x1 = (float)(d1 - 4) - __int_as_float(cur.w);
+        e0v = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
+        e1v = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * gv[m], e0v, acc[m], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
   }
   const int j = lane & 15, ib = 4 * (lane >> 4);
   if (slot < 0) {
@@ -284,54 +302,59 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  __shared__ int4 se[kGridWaves][2][kGridBatch];
+  __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
   const int eend = ebeg + cnt;
-  int4 en[4];
-  cpx<T> gv[4][4];
-  auto load = [&](int e) {
+  int4 re = make_int4(-1, 0, 0, 0);
+  cpx<T> rg[4];
+  auto fetch = [&](int base) {
+    const int idx = base + lane;
+    re = idx < eend ? ent[idx] : make_int4(-1, 0, 0, 0);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) en[u] = ent[e + 4 * g4 + u];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        gv[u][c] = (en[u].x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + en[u].x]
-                                                : cpx<T>{T(0), T(0)};
+    for (int c = 0; c < 4; ++c)
+      rg[c] = (re.x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + re.x] : cpx<T>{T(0), T(0)};
   };
-  if (cnt > 0) load(ebeg);
-  for (int e = ebeg; e < eend; e += 16) {
-    int4 cur[4];
-    float g[4][8];
+  if (cnt > 0) fetch(ebeg);
+  int buf = 0;
+  for (int bb = ebeg; bb < eend; bb += kGridBatch, buf ^= 1) {
+    se[wave][buf][lane] = re;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      cur[u] = en[u];
+    for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = tof2(rg[c]);
+    __builtin_amdgcn_wave_barrier();
+    if (bb + kGridBatch < eend) fetch(bb + kGridBatch);
+    const int ngr = min(kGridBatch, eend - bb) >> 4;
+    for (int q = 0; q < ngr; ++q) {
+      float e0v[4], e1v[4], g[4][8];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        g[u][2 * c] = (float)gv[u][c].x;
-        g[u][2 * c + 1] = (float)gv[u][c].y;
+      for (int u = 0; u < 4; ++u) {
+        const int slot_e = 16 * q + 4 * g4 + u;
+        const int4 cur = se[wave][buf][slot_e];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float2 v = sg[wave][buf][c][slot_e];
+          g[u][2 * c] = v.x;
+          g[u][2 * c + 1] = v.y;
+        }
+        e0v[u] = e1v[u] = 0.f;
+        if (cur.x >= 0) {
+          const int rb0 = (int)(signed char)(cur.y & 0xff);
+          const int rb1 = (int)(signed char)((cur.y >> 8) & 0xff);
+          const int d0 = i - rb0, d1 = i - rb1;
+          const float x0 = (float)(d0 - 4) - __int_as_float(cur.z);
+          const float x1 = (float)(d1 - 4) - __int_as_float(cur.w);
+          e0v[u] = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
+          e1v[u] = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
+        }
+      }
+      const typename M::v4 b = M::pack(e0v[0], e0v[1], e0v[2], e0v[3]);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const typename M::v4 a = M::pack(e1v[0] * g[0][m], e1v[1] * g[1][m], e1v[2] * g[2][m],
+                                          e1v[3] * g[3][m]);
+        acc[m] = M::mma(a, b, acc[m]);
       }
     }
-    if (e + 16 < eend) load(e + 16);
-    float e0v[4], e1v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      e0v[u] = e1v[u] = 0.f;
-      if (cur[u].x >= 0) {
-        const int rb0 = (int)(signed char)(cur[u].y & 0xff);
-        const int rb1 = (int)(signed char)((cur[u].y >> 8) & 0xff);
-        const int d0 = i - rb0, d1 = i - rb1;
-        const float x0 = (float)(d0 - 4) - __int_as_float(cur[u].z);
-        const float x1 = (float)(d1 - 4) - __int_as_float(cur[u].w);
-        e0v[u] = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
-        e1v[u] = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
-      }
-    }
-    const typename M::v4 b = M::pack(e0v[0], e0v[1], e0v[2], e0v[3]);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const typename M::v4 a = M::pack(e1v[0] * g[0][m], e1v[1] * g[1][m], e1v[2] * g[2][m],
-                                        e1v[3] * g[3][m]);
-      acc[m] = M::mma(a, b, acc[m]);
-    }
+    __builtin_amdgcn_wave_barrier();
   }
   const int j = lane & 15, ib = 4 * (lane >> 4);
   if (slot < 0) {
