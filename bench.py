@@ -48,6 +48,13 @@ def parse():
     ap.add_argument('--cpu-baseline', default='on', choices=['on', 'off'])
     ap.add_argument('--cpu-workers', type=int, default=0, help='0 = min(16, cpu count)')
     ap.add_argument('--no-prof', action='store_true', help='do not time kernels with events')
+    ap.add_argument('--form', default='pipeline', choices=['pipeline', 'batch', 'cg'],
+                    help='pipeline: the chunked reconstruction (default); batch: 96 independent '
+                         'reconstructions of mixed sizes as one HIP graph vs eager launches; '
+                         'cg: CG least squares with the autograd projector')
+    ap.add_argument('--requests', type=int, default=96, help='batch form: number of requests')
+    ap.add_argument('--iters', type=int, default=50, help='cg form: CG iterations')
+    ap.add_argument('--tune', action='store_true', help='batch form: autotune each request size')
     ap.add_argument('--copy-calibration-mb', type=float, default=0.0,
                     help='run one device-to-device copy of this size before the timed steps '
                          '(a known-size kernel for HBM counter calibration)')
@@ -181,12 +188,115 @@ def _cpu_pair(arg):
     return 0
 
 
+def bench_batch(a):
+    """Batching form: independent reconstructions with n in {256, 512, 768,
+    1024, 1536, 2048}, nproj = round(1.5 n) from a random start angle, nz in
+    [1, 32], filtered-ready float32 sinograms (one synthetic ellipse slice per
+    request, repeated over its nz), Parzen filter; one HIP graph over 4
+    streams against eager launches of the same work on one stream."""
+    import torch
+    from tomocupy_amd import synth
+    from tomocupy_amd.batch import BatchRec
+    rng = np.random.default_rng(2024)
+    sizes = [256, 512, 768, 1024, 1536, 2048]
+    reqs = []
+    t0 = time.time()
+    cache = {}
+    for k in range(a.requests):
+        n = int(rng.choice(sizes))
+        nproj = int(round(1.5 * n))
+        th0 = rng.uniform(0, np.pi)
+        theta = (th0 + np.arange(nproj) * np.pi / nproj).astype(np.float32)
+        nz = int(rng.integers(1, 33))
+        if n not in cache:
+            cache[n] = synth.sinograms(4, n, synth.angles(nproj), 1)[0].astype(np.float32)
+        reqs.append({'n': n, 'nz': nz, 'theta': theta, 'center': n / 2, 'sino': cache[n]})
+    print('# %d requests generated in %.1fs' % (len(reqs), time.time() - t0), file=sys.stderr,
+          flush=True)
+    opts = {}
+    if a.tune:
+        from tomocupy_amd.autotune import tune
+        for n in sorted({r['n'] for r in reqs}):
+            opts[n] = tune(n, int(round(1.5 * n)), 8, verbose=True)
+    for r in reqs:
+        r['rec_options'] = opts.get(r['n'])
+    b = BatchRec(reqs, dtype=a.dtype, fbp_filter='parzen', streams=4)
+    td = torch.float32 if a.dtype == 'float32' else torch.float16
+    for r, d in zip(reqs, b.inputs()):
+        d.copy_(torch.from_numpy(r['sino'])[None].expand(r['nz'], -1, -1).to(td))
+
+    def timed(fn):
+        for _ in range(a.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / a.steps
+
+    t_seq = timed(lambda: b.run_eager(sequential=True))
+    b.capture()
+    t_graph = timed(b.run)
+    slices = b.slices
+    line = {'metric': 'slices/s, %d independent mixed-size reconstructions (filter + '
+                      'backprojection), one HIP graph' % len(reqs),
+            'value': round(slices / t_graph, 2), 'unit': 'slices/s', 'n_gpus': 1,
+            'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(t_graph * 1e3, 3),
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': {'float32': 'f32', 'float16': 'f16', 'bfloat16': 'bf16'}[a.dtype],
+            'data': 'synthetic ellipse sinograms, one slice per request size repeated over nz',
+            'config': {'workload': 'batching form: %d requests, n in %s, nproj=1.5n, nz in [1,32], '
+                                   'parzen, graph over 4 streams vs eager on 1 stream'
+                                   % (len(reqs), sizes),
+                       'requests': len(reqs), 'slices': slices, 'tuned': bool(a.tune)},
+            'sequential_eager': {'value': round(slices / t_seq, 2),
+                                 'ms_per_step': round(t_seq * 1e3, 3)},
+            'graph_speedup': round(t_seq / t_graph, 3)}
+    print(json.dumps(line))
+
+
+def bench_cg(a):
+    """Differentiation form: n=1024, nproj=180 angles in [0, pi), nz=32, FP32;
+    CG least squares with a Tikhonov term through the USFFT projector and its
+    adjoint; reports iterations/s."""
+    import torch
+    from tomocupy_amd import FourierRec, synth
+    from tomocupy_amd.autograd import cgls
+    n, nproj, nz = 1024, 180, 32
+    theta = synth.angles(nproj)
+    t0 = time.time()
+    sino = synth.sinograms(5, n, theta, 4)
+    y = torch.from_numpy(np.concatenate([sino] * (nz // 4)).astype(np.float32)).cuda()
+    print('# data in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
+    rec = FourierRec(n, nproj, nz, torch.from_numpy(theta).cuda(), 'float32')
+    cgls(rec, y, lam=1e-3, niter=2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    x = cgls(rec, y, lam=1e-3, niter=a.iters)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    line = {'metric': 'CG iterations/s (n=1024, nproj=180, nz=32, FP32)',
+            'value': round(a.iters / dt, 2), 'unit': 'iterations/s', 'n_gpus': 1,
+            'steps': a.iters, 'warmup': 2, 'ms_per_step': round(dt / a.iters * 1e3, 3),
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+            'data': 'synthetic ellipse sinograms (4 distinct slices repeated)',
+            'config': {'workload': 'differentiation form: %d CG iterations on (A^T A + 1e-3 I) x '
+                                   '= A^T y' % a.iters, 'n': n, 'nproj': nproj, 'nz': nz},
+            'x_norm': float(torch.linalg.norm(x))}
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU only (this build has no multi-GPU path); got --gpus %d'
               % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.form == 'batch':
+        return bench_batch(a)
+    if a.form == 'cg':
+        return bench_cg(a)
     center = a.center if a.center is not None else a.n / 2 + 0.5
     theta, h_data, h_dark, h_flat = host_data(a)
     cpu = None

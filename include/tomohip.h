@@ -80,6 +80,19 @@ int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
               const float *theta, int dtype, void *stream);
 
 /*
+ * fr_create with tuning options (no counterpart in the reference; used by the
+ * plan autotuner, tomocupy_amd/autotune.py).  Both options change only the
+ * rounding of the results; each configuration is deterministic.
+ *   grid_chunk   max sample-list entries per gridding work item: 0 = default
+ *                (512), else a multiple of 64 in [64, 8192]
+ *   fft_generic  1 = run-time radix plans for every FFT, 0 = compiled
+ *                size-specialised transforms where 2n is 1024..8192
+ */
+int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
+                   const float *theta, int dtype, int grid_chunk, int fft_generic,
+                   void *stream);
+
+/*
  * Replaces FourierRec.backprojection(obj, data, stream)
  * (fourierrec.py:61-68) and cfunc_fourierrec::backprojection
  * (cfunc_fourierrec.cu:67-93).

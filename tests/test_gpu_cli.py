@@ -1,0 +1,86 @@
+"""``python -m tomocupy_amd recon`` end to end on a small synthetic data set.
+
+* full: every slice's TIFF against the FP64 oracle chain (per-slice rel-L2
+  <= 1e-5; the chunking pairs slices within a chunk of nsino-per-chunk, as the
+  conveyor does);
+* try: the TIFF of each trial centre against the oracle reconstructed at that
+  centre (<= 1e-5).
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import filter_fp64 as ff
+from oracle import fourierrec_fp64 as fr
+from oracle import preproc_fp64 as pp
+from tomocupy_amd import io, synth
+from tomocupy_amd.cli import main
+
+pytestmark = pytest.mark.gpu
+
+
+def _dataset(tmp_path, n=128, nproj=90, nz=6):
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(7, n, theta, nz, ndark=3, nflat=4)
+    p = tmp_path / 'scan.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    return str(p), data, dark, flat, theta
+
+
+def _rel(got, ref):
+    return np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())
+
+
+def test_cli_full(tmp_path):
+    path, data, dark, flat, theta = _dataset(tmp_path)
+    out = str(tmp_path / 'rec')
+    main(['recon', '--file-name', path, '--reconstruction-type', 'full', '--nsino-per-chunk', '4',
+          '--out-path-name', out, '--fbp-filter', 'shepp', '--rotation-axis', '63.5'])
+    n, nz = data.shape[2], data.shape[1]
+    ne = 4 * n
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    for c0 in range(0, nz, 4):
+        s = slice(c0, min(nz, c0 + 4))
+        sino = pp.proc_chain(data[:, s], dark[:, s], flat[:, s])
+        lz = sino.shape[0]
+        sino = np.concatenate([sino, np.zeros((4 - lz,) + sino.shape[1:])])
+        ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'shepp'), ne, 63.5),
+                                th)[:lz]
+        for k in range(lz):
+            got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % (c0 + k)))
+            err = _rel(got.astype(np.float64), ref[k])
+            print('slice', c0 + k, err)
+            assert err <= 1e-5
+
+
+def test_cli_try(tmp_path):
+    path, data, dark, flat, theta = _dataset(tmp_path, nz=4)
+    out = str(tmp_path / 'try')
+    main(['recon', '--file-name', path, '--reconstruction-type', 'try', '--out-path-name', out,
+          '--center-search-width', '3', '--center-search-step', '1.5', '--nsino', '0.5'])
+    n = data.shape[2]
+    ne = 4 * n
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    idz = int(0.5 * (data.shape[1] - 1))
+    sino = pp.proc_chain(data[:, idz:idz + 1], dark[:, idz:idz + 1], flat[:, idz:idz + 1])
+    files = sorted(f for f in os.listdir(out) if f.endswith('.tiff'))
+    shifts = np.arange(-3, 3, 1.5)
+    assert len(files) == len(shifts)
+    # the GPU batch: nsino-per-chunk (8) copies of the sinogram, shifts on the
+    # first len(shifts) and zero on the rest, paired c with c + 4
+    ncz = 8
+    shv = np.zeros(ncz)
+    shv[:len(shifts)] = shifts
+    batch = np.concatenate([sino] * ncz)
+    ref = fr.backprojection(ff.fbp_filter_center(batch, ff.calc_filter(ne, 'parzen'), ne, n / 2,
+                                                 shv), th)
+    for k, sh in enumerate(shifts):
+        centre = n / 2 - sh
+        got = io.read_tiff(os.path.join(out, 'recon_slice%04d_center%05.2f.tiff' % (idz, centre)))
+        err = _rel(got.astype(np.float64), ref[k])
+        print('centre', centre, err)
+        assert err <= 1e-5

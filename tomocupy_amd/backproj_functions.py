@@ -23,14 +23,16 @@ def padded_length(n, dtype='float32'):
 
 class BackprojFunctions:
     def __init__(self, n, nproj, ncz, theta, center, dtype='float32', fbp_filter='parzen',
-                 reconstruction_algorithm='fourierrec'):
+                 reconstruction_algorithm='fourierrec', rec_options=None):
+        """``rec_options``: FourierRec tuning options (``grid_chunk``, ``fft``),
+        e.g. from :func:`tomocupy_amd.autotune.tune`."""
         if reconstruction_algorithm != 'fourierrec':
             raise NotImplementedError('only the fourierrec algorithm is implemented')
         self.n, self.nproj, self.ncz = int(n), int(nproj), int(ncz)
         self.center = float(center)
         self.dtype = dtype
         self.ne = padded_length(self.n, dtype)
-        self.cl_rec = FourierRec(self.n, self.nproj, self.ncz, theta, dtype)
+        self.cl_rec = FourierRec(self.n, self.nproj, self.ncz, theta, dtype, **(rec_options or {}))
         self.cl_filter = FBPFilter(self.ne, self.nproj, self.ncz, dtype, n_unpadded=self.n)
         self.wfilter = self.cl_filter.calc_filter(fbp_filter)
         self.pad = self.ne // 2 - self.n // 2

@@ -1,0 +1,184 @@
+"""``tomocupy recon`` for the fourierrec path.
+
+``python -m tomocupy_amd recon --file-name F [options]`` mirrors the
+reference's command (``src/tomocupy/__main__.py``, flags and defaults of
+``config.py``) for ``--reconstruction-algorithm fourierrec``:
+
+* ``--reconstruction-type full``: every row chunk through the conveyor
+  (:class:`tomocupy_amd.rec.GPURec`, ``rec.py:113-208``), one TIFF per slice
+  ``<out>/recon_%05d.tiff`` (``writer.py:258-264``);
+* ``--reconstruction-type try``: the slices at ``--nsino`` (fractions of the
+  height, ``reader.py:192-197``) reconstructed at the centres
+  ``centre - arange(-w, w, step)`` (``reader.py:224-246``), each a per-slice
+  shift of the same sinogram (``rec.py:210-275``), one TIFF per centre
+  ``<out>/recon_slice%04d_center%05.2f.tiff`` (``writer.py:309-313``).
+
+Settings outside this path (binning, double-FOV, other stripe filters,
+phase retrieval, ...) are not offered.  Input formats: see
+:mod:`tomocupy_amd.io`.
+"""
+import argparse
+import ast
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def _bool(s):
+    return str(s).lower() in ('true', '1', 'yes')
+
+
+def parser():
+    ap = argparse.ArgumentParser(prog='tomocupy_amd', description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest='cmd', required=True)
+    r = sub.add_parser('recon', help='reconstruct a DXchange-like data set')
+    r.add_argument('--file-name', required=True)
+    r.add_argument('--out-path-name', default=None)
+    r.add_argument('--reconstruction-type', default='try', choices=['full', 'try'])
+    r.add_argument('--reconstruction-algorithm', default='fourierrec', choices=['fourierrec'])
+    r.add_argument('--rotation-axis', type=float, default=-1.0)
+    r.add_argument('--center-search-width', type=float, default=50.0)
+    r.add_argument('--center-search-step', type=float, default=0.5)
+    r.add_argument('--nsino', default='0.5')
+    r.add_argument('--nsino-per-chunk', type=int, default=8)
+    r.add_argument('--start-row', type=int, default=0)
+    r.add_argument('--end-row', type=int, default=-1)
+    r.add_argument('--fbp-filter', default='parzen',
+                   choices=['none', 'ramp', 'shepp', 'hann', 'hamming', 'parzen', 'cosine',
+                            'cosine2'])
+    r.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
+    r.add_argument('--remove-stripe-method', default='None', choices=['None', 'none', 'fw'])
+    r.add_argument('--fw-sigma', type=float, default=1.0)
+    r.add_argument('--fw-filter', default='sym16', choices=['haar', 'db5', 'sym5', 'sym16'])
+    r.add_argument('--fw-level', type=int, default=7)
+    r.add_argument('--minus-log', default='True')
+    r.add_argument('--flat-linear', default='False')
+    r.add_argument('--bright-ratio', type=float, default=1.0)
+    r.add_argument('--max-write-threads', type=int, default=8)
+    r.add_argument('--save-format', default='tiff', choices=['tiff', 'npy'])
+    return ap
+
+
+def _out_dir(a, kind):
+    if a.out_path_name:
+        return a.out_path_name
+    base = os.path.splitext(os.path.basename(a.file_name))[0]
+    root = os.path.dirname(os.path.abspath(a.file_name)) + '_rec'
+    return os.path.join(root, 'try_center', base) if kind == 'try' else \
+        os.path.join(root, base + '_rec')
+
+
+def _common(a, ds):
+    z0 = a.start_row
+    z1 = ds.nz if a.end_row == -1 else a.end_row
+    if not (0 <= z0 < z1 <= ds.nz):
+        raise SystemExit('invalid --start-row/--end-row for %d rows' % ds.nz)
+    centre = ds.ni / 2 if a.rotation_axis == -1 else a.rotation_axis
+    stripe = 'fw' if a.remove_stripe_method == 'fw' else 'none'
+    return z0, z1, centre, stripe
+
+
+def recon_full(a, ds):
+    from .io import write_tiff
+    from .rec import GPURec
+    z0, z1, centre, stripe = _common(a, ds)
+    nz = z1 - z0
+    out = _out_dir(a, 'full')
+    os.makedirs(out, exist_ok=True)
+    rec = GPURec(ds.ni, ds.nproj, nz, a.nsino_per_chunk, ds.theta, centre,
+                 ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype, fbp_filter=a.fbp_filter,
+                 remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
+                 fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
+                 flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads)
+    ncz = rec.ncz
+    stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
+                                      dtype=np.float16 if a.dtype == 'float16' else np.float32,
+                                      shape=(nz, ds.ni, ds.ni)) if a.save_format == 'npy' else None
+
+    def read(k):
+        s0 = z0 + k * ncz
+        return ds.rows(s0, min(z1, s0 + ncz))
+
+    def write(k, r):
+        for kk in range(len(r)):
+            fid = z0 + k * ncz + kk
+            if stack is not None:
+                stack[fid - z0] = r[kk]
+            else:
+                write_tiff(os.path.join(out, 'recon_%05d.tiff' % fid), r[kk])
+
+    t = time.perf_counter()
+    rec.recon_all(read, write)
+    rec.free()
+    if stack is not None:
+        stack.flush()
+    dt = time.perf_counter() - t
+    print('reconstructed %d slices of %dx%d in %.2f s (%.1f slices/s) -> %s'
+          % (nz, ds.ni, ds.ni, dt, nz / dt, out))
+    return out
+
+
+def recon_try(a, ds):
+    import torch
+    from . import BackprojFunctions, ProcFunctions, _lib
+    from .io import write_tiff
+    z0, z1, centre, stripe = _common(a, ds)
+    nz = z1 - z0
+    out = _out_dir(a, 'try')
+    os.makedirs(out, exist_ok=True)
+    fractions = ast.literal_eval(a.nsino)
+    if not isinstance(fractions, (list, tuple)):
+        fractions = [fractions]
+    ids = [int(f * (nz - 1)) for f in fractions]
+    shifts = np.arange(-a.center_search_width, a.center_search_width,
+                       a.center_search_step).astype(np.float32)
+    centres = centre - shifts
+    ncz = max(2, a.nsino_per_chunk)
+    td = _lib.torch_dtype(a.dtype)
+    theta = torch.from_numpy(ds.theta).cuda()
+    pf = ProcFunctions(ds.nproj, 1, ds.ni, ds.dark.shape[0], ds.flat.shape[0], a.dtype,
+                       minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
+                       flat_linear=_bool(a.flat_linear), remove_stripe_method=stripe,
+                       fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level)
+    bf = BackprojFunctions(ds.ni, ds.nproj, ncz, theta, centre, a.dtype, a.fbp_filter)
+    written = []
+    for idz in ids:
+        data, dark, flat = (torch.from_numpy(x).cuda() for x in ds.rows(z0 + idz, z0 + idz + 1))
+        sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, ni)
+        batch = sino.expand(ncz, -1, -1).contiguous()
+        obj = torch.empty((ncz, ds.ni, ds.ni), dtype=td, device='cuda')
+        for c0 in range(0, len(shifts), ncz):
+            lz = min(ncz, len(shifts) - c0)
+            sht = torch.zeros(ncz, dtype=torch.float32, device='cuda')
+            sht[:lz] = torch.from_numpy(shifts[c0:c0 + lz]).cuda()
+            bf.fbp_filter_backprojection(obj, batch, sht)
+            host = obj[:lz].float().cpu().numpy()
+            for kk in range(lz):
+                img = host[kk].astype(np.float16) if a.dtype == 'float16' else host[kk]
+                path = os.path.join(out, 'recon_slice%04d_center%05.2f.tiff'
+                                    % (z0 + idz, centres[c0 + kk]))
+                write_tiff(path, img)
+                written.append(path)
+    pf.free()
+    bf.free()
+    print('wrote %d try images (%d slices x %d centres) -> %s'
+          % (len(written), len(ids), len(shifts), out))
+    return written
+
+
+def main(argv=None):
+    a = parser().parse_args(argv)
+    from .io import Dataset
+    ds = Dataset(a.file_name)
+    if a.reconstruction_type == 'full':
+        recon_full(a, ds)
+    else:
+        recon_try(a, ds)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

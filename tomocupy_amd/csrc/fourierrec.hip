@@ -34,7 +34,7 @@ using namespace tomo;
 namespace {
 
 constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
-constexpr int kGridChunk = 512; // max list entries per work item (multiple of 16)
+constexpr int kGridChunk = 512; // default max list entries per work item (multiple of 16)
 constexpr int kGridBatch = 64;  // list entries a gridding wave stages at a time
 constexpr int kColEPT = 32;     // FFT values per thread in the column pass
 // log2 of the columns a column-pass workgroup transforms for 2n = N: as many
@@ -628,6 +628,7 @@ struct tomo_fr_plan {
   DevBuf G, F, R;
   int colCB = 1, lgCB = 0, colThreads = 256, prepThreads = 256, rowThreads = 256;
   int nsRow = 0, nsCol = 0, nsPrep = 0;  // compiled FFT sizes in use (0 = run-time sizes)
+  int gridChunk = kGridChunk;            // max list entries per gridding work item
   int rowLds = 0, prepLds = 0;
   size_t colLds = 0;
   float scale1 = 0.f, K2 = 0.f;
@@ -724,19 +725,20 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
       }
   }
   TOMO_REQUIRE(off[ntiles] < ((int64_t)1 << 31), "fr_create: tile lists too large");
-  // work items: whole tiles, or chunks of kGridChunk entries for heavy tiles
+  // work items: whole tiles, or chunks of gridChunk entries for heavy tiles
+  const int64_t kChunk = P->gridChunk;
   std::vector<int4> items, red;
   int nslots = 0;
   for (size_t t = 0; t < ntiles; ++t) {
     const int64_t L = off[t + 1] - off[t];
-    if (L <= kGridChunk) {
+    if (L <= kChunk) {
       items.push_back(make_int4((int)t, (int)off[t], (int)L, -1));
     } else {
-      const int nch = (int)ceil_div(L, kGridChunk);
+      const int nch = (int)ceil_div(L, kChunk);
       red.push_back(make_int4((int)t, nslots, nch, 0));
       for (int q = 0; q < nch; ++q)
-        items.push_back(make_int4((int)t, (int)(off[t] + (int64_t)q * kGridChunk),
-                                  (int)std::min<int64_t>(kGridChunk, L - (int64_t)q * kGridChunk),
+        items.push_back(make_int4((int)t, (int)(off[t] + (int64_t)q * kChunk),
+                                  (int)std::min<int64_t>(kChunk, L - (int64_t)q * kChunk),
                                   nslots + q));
       nslots += nch;
     }
@@ -797,6 +799,12 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
 
 extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
                          const float *theta, int dtype, void *stream) {
+  return fr_create_opts(plan, n, nproj, nz, theta, dtype, 0, 0, stream);
+}
+
+extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
+                              const float *theta, int dtype, int grid_chunk, int fft_generic,
+                              void *stream) {
   clear_error();
   TOMO_REQUIRE(plan != nullptr, "fr_create: plan pointer is NULL");
   *plan = nullptr;
@@ -809,6 +817,10 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
                "fr_create: unknown dtype %d", dtype);
   TOMO_REQUIRE(nproj * n < (int64_t)1 << 31, "fr_create: nproj*n too large");
+  TOMO_REQUIRE(grid_chunk == 0 || (grid_chunk >= 64 && grid_chunk <= 8192 && grid_chunk % 64 == 0),
+               "fr_create: grid_chunk=%d must be 0 (default) or a multiple of 64 in [64, 8192]",
+               grid_chunk);
+  TOMO_REQUIRE(fft_generic == 0 || fft_generic == 1, "fr_create: fft_generic must be 0 or 1");
   auto *P = new (std::nothrow) tomo_fr_plan();
   if (!P) {
     set_error("fr_create: host allocation failed");
@@ -820,6 +832,7 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   P->nzc = (nz + 1) / 2;
   P->ngrp = ceil_div(P->nzc, 4);
   P->dtype = dtype;
+  if (grid_chunk > 0) P->gridChunk = grid_chunk;
   P->csz = dtype == TOMO_F32 ? sizeof(float2) : 2 * sizeof(uint16_t);
   hipStream_t st = (hipStream_t)stream;
   std::vector<float> th(nproj);
@@ -870,11 +883,11 @@ extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t 
   }
   // size-specialised kernels where 2n is a compiled power of two (and the
   // column grouping matches the specialisation)
-  P->nsRow = (int)n2;
-  P->nsCol = (lg == col_lg((int)n2)) ? (int)n2 : 0;
+  P->nsRow = fft_generic ? 0 : (int)n2;
+  P->nsCol = (!fft_generic && lg == col_lg((int)n2)) ? (int)n2 : 0;
   with_static_size(P->nsRow, [&](auto ns) { P->nsRow = decltype(ns)::value; });
   with_static_size(P->nsCol, [&](auto ns) { P->nsCol = decltype(ns)::value; });
-  P->nsPrep = (int)n;
+  P->nsPrep = fft_generic ? 0 : (int)n;
   with_static_size(P->nsPrep, [&](auto ns) { P->nsPrep = decltype(ns)::value; });
   if (P->nsRow == 0 || 2 * P->nsPrep != P->nsRow) P->nsPrep = 0;
   hipError_t ea = hipSuccess;

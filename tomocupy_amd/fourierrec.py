@@ -25,7 +25,11 @@ from . import _lib
 class FourierRec:
     """Backprojection by the Fourier-based method."""
 
-    def __init__(self, n, nproj, nz, theta, dtype='float32'):
+    def __init__(self, n, nproj, nz, theta, dtype='float32', grid_chunk=0, fft='static'):
+        """``grid_chunk`` and ``fft`` are tuning options (see
+        :mod:`tomocupy_amd.autotune`): the maximum sample-list entries per
+        gridding work item (0 = default) and ``'static'`` (size-specialised
+        transforms where available) or ``'generic'`` radix plans."""
         import torch
         if dtype not in _lib.DTYPES:
             raise ValueError('dtype must be one of %s' % sorted(_lib.DTYPES))
@@ -40,8 +44,12 @@ class FourierRec:
         self.theta = theta
         L = _lib.lib()
         h = ctypes.c_void_p()
-        _lib.check(L.fr_create(ctypes.byref(h), self.n, self.nproj, self.nz, theta.data_ptr(),
-                               _lib.DTYPES[dtype], _lib.stream_ptr(None)), 'fr_create')
+        if fft not in ('static', 'generic'):
+            raise ValueError("fft must be 'static' or 'generic'")
+        self.grid_chunk, self.fft = int(grid_chunk), fft
+        _lib.check(L.fr_create_opts(ctypes.byref(h), self.n, self.nproj, self.nz, theta.data_ptr(),
+                                    _lib.DTYPES[dtype], self.grid_chunk, int(fft == 'generic'),
+                                    _lib.stream_ptr(None)), 'fr_create')
         self._h = h
 
     def backprojection(self, obj, data, stream=None):

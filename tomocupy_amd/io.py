@@ -1,0 +1,111 @@
+"""File formats on either side of the reconstruction path.
+
+Input (reference: ``dataio/reader.py:274-359``, DXchange layout
+``/exchange/data``, ``data_white``, ``data_dark``, ``theta`` in degrees):
+* HDF5 through ``h5py`` when it is importable;
+* ``.npz`` archives with the same four keys (``exchange/`` prefix optional).
+When ``theta`` is absent the angles are ``linspace(0, pi, nproj)`` including
+the endpoint, as ``reader.py:317``.
+
+Output (reference: ``dataio/writer.py``): one uncompressed single-strip
+little-endian TIFF per slice (float32, or float16 for half plans, sample
+format IEEE float), written by :func:`write_tiff`, or one ``.npy`` stack.
+"""
+import os
+import struct
+
+import numpy as np
+
+_KEYS = ('data', 'data_white', 'data_dark', 'theta')
+
+
+class Dataset:
+    """Lazy access to the projections ``(nproj, nz, ni)`` uint16 and the
+    flat/dark fields of a DXchange-like file."""
+
+    def __init__(self, path):
+        self.path = path
+        ext = os.path.splitext(path)[1].lower()
+        if ext in ('.h5', '.hdf5', '.hdf', '.nxs'):
+            try:
+                import h5py
+            except ImportError as e:  # pragma: no cover - depends on the environment
+                raise RuntimeError('reading HDF5 needs h5py, which is not installed; convert '
+                                   'the file to .npz with keys %s' % (_KEYS,)) from e
+            self._f = h5py.File(path, 'r')
+            self._get = lambda k: self._f['/exchange/' + k] if ('/exchange/' + k) in self._f else None
+        elif ext == '.npz':
+            self._f = np.load(path, allow_pickle=False, mmap_mode='r')
+            files = set(self._f.files)
+
+            def get(k):
+                for name in (k, 'exchange/' + k, 'exchange_' + k):
+                    if name in files:
+                        return self._f[name]
+                return None
+            self._get = get
+        else:
+            raise ValueError('unsupported input %s (expected .h5 or .npz)' % path)
+        self.data = self._get('data')
+        self.flat = self._get('data_white')
+        self.dark = self._get('data_dark')
+        if self.data is None or self.flat is None or self.dark is None:
+            raise ValueError('%s lacks data, data_white or data_dark' % path)
+        self.nproj, self.nz, self.ni = (int(v) for v in self.data.shape)
+        th = self._get('theta')
+        if th is not None and len(th) == self.nproj:
+            self.theta = (np.asarray(th, dtype=np.float64) / 180.0 * np.pi).astype(np.float32)
+        else:
+            self.theta = np.linspace(0, np.pi, self.nproj).astype(np.float32)
+
+    def rows(self, z0, z1):
+        """``(data, dark, flat)`` uint16 for detector rows [z0, z1)."""
+        return (np.ascontiguousarray(self.data[:, z0:z1], dtype=np.uint16),
+                np.ascontiguousarray(self.dark[:, z0:z1], dtype=np.uint16),
+                np.ascontiguousarray(self.flat[:, z0:z1], dtype=np.uint16))
+
+
+def write_tiff(path, img):
+    """Write a 2-D float32 or float16 array as an uncompressed TIFF."""
+    img = np.ascontiguousarray(img)
+    if img.dtype not in (np.float32, np.float16) or img.ndim != 2:
+        raise ValueError('write_tiff takes a 2-D float32/float16 array')
+    h, w = img.shape
+    bits = img.dtype.itemsize * 8
+    data = img.astype(img.dtype.newbyteorder('<'), copy=False).tobytes()
+    entries = [(256, 4, 1, w), (257, 4, 1, h), (258, 3, 1, bits), (259, 3, 1, 1),
+               (262, 3, 1, 1), (273, 4, 1, 0), (277, 3, 1, 1), (278, 4, 1, h),
+               (279, 4, 1, len(data)), (339, 3, 1, 3)]
+    ifd_off = 8
+    data_off = ifd_off + 2 + 12 * len(entries) + 4
+    out = bytearray(b'II*\x00' + struct.pack('<I', ifd_off))
+    out += struct.pack('<H', len(entries))
+    for tag, typ, cnt, val in entries:
+        if tag == 273:
+            val = data_off
+        if typ == 3:
+            out += struct.pack('<HHIHH', tag, typ, cnt, val, 0)
+        else:
+            out += struct.pack('<HHII', tag, typ, cnt, val)
+    out += struct.pack('<I', 0)
+    out += data
+    with open(path, 'wb') as f:
+        f.write(out)
+
+
+def read_tiff(path):
+    """Read a TIFF written by :func:`write_tiff`."""
+    b = open(path, 'rb').read()
+    if b[:4] != b'II*\x00':
+        raise ValueError('not a little-endian TIFF')
+    off = struct.unpack_from('<I', b, 4)[0]
+    n = struct.unpack_from('<H', b, off)[0]
+    tags = {}
+    for i in range(n):
+        tag, typ, cnt = struct.unpack_from('<HHI', b, off + 2 + 12 * i)
+        fmt = '<H' if typ == 3 else '<I'
+        tags[tag] = struct.unpack_from(fmt, b, off + 2 + 12 * i + 8)[0]
+    w, h, bits = tags[256], tags[257], tags[258]
+    dt = np.float32 if bits == 32 else np.float16
+    return np.frombuffer(b, dtype=np.dtype(dt).newbyteorder('<'), count=w * h,
+                         offset=tags[273]).reshape(h, w).astype(dt)

@@ -86,7 +86,10 @@ class GPURec:
         self.stream2 = torch.cuda.Stream()
         self.stream3 = torch.cuda.Stream()
         self.max_write_threads = int(max_write_threads)
-        self._sino = torch.empty((self.ncz, self.nproj, self.n), dtype=torch.float32,
+        from . import _lib
+        self.dtype = dtype
+        self.tdtype = _lib.torch_dtype(dtype)
+        self._sino = torch.empty((self.ncz, self.nproj, self.n), dtype=self.tdtype,
                                  device='cuda')
 
     def _process(self, data, dark, flat, rec):
@@ -96,7 +99,7 @@ class GPURec:
     def recon_resident(self, raw, dark, flat, rec):
         """Reconstruct a GPU-resident volume: ``raw (nzchunk, nproj, ncz, n)`` uint16,
         ``dark``/``flat`` ``(nzchunk, ndark|nflat, ncz, n)``, into
-        ``rec (nzchunk*ncz, n, n)`` float32, on stream 2."""
+        ``rec (nzchunk*ncz, n, n)`` of the plan's dtype, on stream 2."""
         import torch
         with torch.cuda.stream(self.stream2):
             for k in range(self.nzchunk):
@@ -109,7 +112,8 @@ class GPURec:
         ``read_chunk(k)`` returns ``(data, dark, flat)`` numpy uint16 arrays of
         shapes ``(nproj, lz, n)``, ``(ndark, lz, n)``, ``(nflat, lz, n)`` with
         ``lz = lzchunk[k]``; ``write_chunk(k, rec)`` receives a ``(lz, n, n)``
-        float32 numpy view that is valid until it returns.
+        numpy view (float32, or float16 for 16-bit plans) that is valid until it
+        returns.
         """
         import torch
         ncz, nproj, n = self.ncz, self.nproj, self.n
@@ -121,8 +125,10 @@ class GPURec:
         }
         item_gpu = {k: torch.empty_like(v, device='cuda') for k, v in item_pinned.items()}
         item_gpu['flat'].copy_(item_pinned['flat'])
-        rec_pinned = torch.zeros((self.max_write_threads, ncz, n, n), dtype=torch.float32, **pin)
-        rec_gpu = torch.zeros((2, ncz, n, n), dtype=torch.float32, device='cuda')
+        # bfloat16 has no numpy type: it is widened on the GPU before the copy
+        host_dt = torch.float32 if self.tdtype == torch.bfloat16 else self.tdtype
+        rec_pinned = torch.zeros((self.max_write_threads, ncz, n, n), dtype=host_dt, **pin)
+        rec_gpu = torch.zeros((2, ncz, n, n), dtype=self.tdtype, device='cuda')
         writers = [_WRThread() for _ in range(self.max_write_threads)]
         nzchunk = self.nzchunk
         for k in range(nzchunk + 2):
