@@ -120,6 +120,19 @@ int fr_filter_backprojection(tomo_fr_plan *plan, tomo_flt_plan *flt, void *obj,
                              const float *sht, void *stream);
 
 /*
+ * fr_filter_backprojection in two steps, so that a caller filtering many
+ * chunks with one filter, centre and shift computes the filter kernels once
+ * (no counterpart in the reference): flt_prepare builds them in `flt`
+ * (same arguments as flt_filter_center), fr_filter_backprojection_prepared
+ * uses the last prepared ones.  Stream-ordered: prepare on the stream (or
+ * before the streams) the backprojections run on.
+ */
+int flt_prepare(tomo_flt_plan *flt, const float *wfilter, double center, const float *sht,
+                void *stream);
+int fr_filter_backprojection_prepared(tomo_fr_plan *plan, tomo_flt_plan *flt, void *obj,
+                                      const void *data, void *stream);
+
+/*
  * Adjoint of fr_backprojection (USFFT forward projection); no counterpart in
  * the reference's tomography path, structured like cfunc_usfft2d::fwd
  * (src/cuda/cfunc_usfft2d.cu:53-70).  For real arrays

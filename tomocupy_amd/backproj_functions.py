@@ -59,16 +59,31 @@ class BackprojFunctions:
         td = _lib.torch_dtype(self.dtype)
         po = _lib.dev_ptr(obj, (self.ncz, self.n, self.n), td, 'obj')
         pd = _lib.dev_ptr(data, (self.ncz, self.nproj, self.n), td, 'data')
-        ps = None
+        L = _lib.lib()
+        sp = _lib.stream_ptr(stream)
         if isinstance(sht, torch.Tensor):
+            # per-slice shifts: kernels for this call
             shv = sht.to(torch.float32).contiguous()
             ps = _lib.dev_ptr(shv, (self.ncz,), torch.float32, 'sht')
-        elif sht != 0:
-            shv = torch.full((self.ncz,), float(sht), dtype=torch.float32, device=data.device)
-            ps = shv.data_ptr()
-        _lib.check(_lib.lib().fr_filter_backprojection(
-            self.cl_rec._handle(), self.cl_filter._h, po, pd, self.wfilter.data_ptr(),
-            self.center, ps, _lib.stream_ptr(stream)), 'fr_filter_backprojection')
+            self._prepared = None
+            _lib.check(L.fr_filter_backprojection(
+                self.cl_rec._handle(), self.cl_filter._h, po, pd, self.wfilter.data_ptr(),
+                self.center, ps, sp), 'fr_filter_backprojection')
+            return obj
+        # a scalar shift: the filter kernels are built once per (centre, shift)
+        key = (self.center, float(sht))
+        if getattr(self, '_prepared', None) != key:
+            ps = None
+            if sht != 0:
+                self._shv = torch.full((self.ncz,), float(sht), dtype=torch.float32,
+                                       device=data.device)
+                ps = self._shv.data_ptr()
+            _lib.check(L.flt_prepare(self.cl_filter._h, self.wfilter.data_ptr(), self.center, ps,
+                                     sp), 'flt_prepare')
+            self._prepared = key
+        _lib.check(L.fr_filter_backprojection_prepared(
+            self.cl_rec._handle(), self.cl_filter._h, po, pd, sp),
+            'fr_filter_backprojection_prepared')
         return obj
 
     def free(self):

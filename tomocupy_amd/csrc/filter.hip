@@ -192,7 +192,17 @@ int tomo::flt_prepare_2n(tomo_flt_plan *P, const float *wfilter, double center, 
                      sht, P->H2.as<float2>(), P->HL.as<float>(), P->HR.as<float>(),
                      P->P.as<double>(), P->fft.d, P->fft2n.d, (int)P->ne, (int)P->n, center);
   TOMO_LAUNCH_CHECK();
+  P->prepared = 1;
   return TOMO_OK;
+}
+
+extern "C" int flt_prepare(tomo_flt_plan *P, const float *wfilter, double center,
+                           const float *sht, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr, "flt_prepare: plan is NULL");
+  TOMO_REQUIRE(wfilter != nullptr, "flt_prepare: wfilter is NULL");
+  TOMO_REQUIRE(std::isfinite(center), "flt_prepare: center is not finite");
+  return flt_prepare_2n(P, wfilter, center, sht, (hipStream_t)stream);
 }
 
 namespace {

@@ -18,6 +18,7 @@ struct tomo_flt_plan {
   tomo::DevBuf H2, HL, HR, P;
   int kthreads = 256;
   size_t klds = 0;
+  int prepared = 0;  // H2/HL/HR hold the kernels of the last flt_prepare
 };
 
 namespace tomo {

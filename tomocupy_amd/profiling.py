@@ -71,6 +71,7 @@ def cmd_traffic(a):
     f_scale = a.copy_bytes / (cf * 1024.0)
     w_scale = a.copy_bytes / (cw * 1024.0) if cw else 1.0
     res = {}
+    base_acc = defaultdict(lambda: [0.0, 0])  # base name -> [bytes summed over dispatches, count]
     lines = []
     if a.title:
         lines.append('# %s\n' % a.title)
@@ -91,10 +92,15 @@ def cmd_traffic(a):
         fb = sum(fv) / len(fv) * 1024 * f_scale
         wb = sum(wv) / len(wv) * 1024 * w_scale
         base = sn.split('<')[0]
-        key = base + (a.key_suffix or '')
-        res[key] = round(fb + wb)
+        base_acc[base][0] += (fb + wb) * len(fv)
+        base_acc[base][1] += len(fv)
         lines.append('| %s | %d | %.2f | %.2f | %.2f |' % (sn, len(fv), fb / 1e6, wb / 1e6,
                                                           (fb + wb) / 1e6))
+    # per base kernel name (all template instances and launch shapes): mean
+    # HBM bytes per dispatch, the figure the bench's roofline pairs with the
+    # mean algorithmic bytes per launch
+    for base, (tot, cnt) in base_acc.items():
+        res[base + (a.key_suffix or '')] = round(tot / cnt)
     print('\n'.join(lines))
     if a.json:
         try:
