@@ -254,6 +254,21 @@ int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, void *out, int out
 int pre_remove_stripe_fw(tomo_pre_plan *plan, float *res, double sigma, const char *wname,
                          int level, void *stream);
 
+/*
+ * The whole chunk preprocessing with Fourier-wavelet stripe removal, fused:
+ * proc_sino (dark/flat, remove_stripe_fw) + proc_proj (minus_log) + the swap
+ * to sinogram order (proc_functions.py:140-198, rec.py:172-174).  The dark/flat
+ * correction happens in the first wavelet pass and minus_log + swap in the
+ * last, so the transmission chunk is never written.
+ *   data, dark, flat   uint16 chunk as pre_process
+ *   out                device (nz, nproj, n) of out_dtype
+ *   sigma, wname, level  as pre_remove_stripe_fw
+ */
+int pre_process_fw(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark,
+                   const uint16_t *flat, void *out, int out_dtype, double bright_ratio,
+                   int flat_linear, int minus_log, double sigma, const char *wname, int level,
+                   void *stream);
+
 void pre_destroy(tomo_pre_plan *plan);
 
 #ifdef __cplusplus

@@ -329,3 +329,29 @@ def test_fused_filter_backprojection_matches_oracle(n, nproj, nz, name, center, 
     print('fused rel-L2', err)
     assert np.all(err <= RTOL32), err
     assert torch.equal(d, d0)
+
+
+@pytest.mark.parametrize("flat_linear,dtype", [(False, 'float32'), (True, 'float32'),
+                                                (False, 'float16')])
+def test_fused_fw_preprocessing_matches_unfused(flat_linear, dtype):
+    """pre_process_fw (dark/flat in the first wavelet pass, -log + swap in the
+    last) against darkflat -> remove_stripe_fw -> minus_log_swap."""
+    from tomocupy_amd import ProcFunctions
+    nproj, nz, n = 180, 4, 256
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(3, n, theta, nz, ndark=3, nflat=6, stripes=True,
+                                           atten=0.25)
+    g = [_cuda(a, torch.uint16) for a in (data, dark, flat)]
+    pf = ProcFunctions(nproj, nz, n, dark.shape[0], flat.shape[0], dtype, flat_linear=flat_linear,
+                       remove_stripe_method='fw', fw_level=5)
+    fused = pf.proc_chunk(*g).float()
+    res = pf.proc_sino(*g)
+    unfused = pf.proc_proj_swap(res).float()
+    torch.cuda.synchronize()
+    rel = (torch.linalg.norm(fused - unfused) / torch.linalg.norm(unfused)).item()
+    print('fused vs unfused rel-L2 %.3g' % rel)
+    assert rel <= 1e-6
+    ref = pp.proc_chain(data, dark, flat, stripe='fw', fw_level=5, flat_linear=flat_linear)
+    err = _rel_l2_per_slice(fused.cpu().numpy(), ref)
+    print('fused vs oracle', err)
+    assert np.all(err <= (1e-5 if dtype == 'float32' else 1e-3)), err

@@ -57,6 +57,8 @@ SIGNATURES = {
     'pre_darkflat': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_p]),
     'pre_minus_log_swap': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_int, _c_p]),
     'pre_remove_stripe_fw': (_c_int, [_c_p, _c_p, _c_d, ctypes.c_char_p, _c_int, _c_p]),
+    'pre_process_fw': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_int, _c_d,
+                                ctypes.c_char_p, _c_int, _c_p]),
     'pre_destroy': (None, [_c_p]),
 }
 

@@ -6,6 +6,8 @@
 
 #include "common.h"
 
+struct tomo_pre_plan;
+
 namespace tomo {
 
 // One level of the Fourier-wavelet stripe removal: analysis input H x W,
@@ -18,6 +20,30 @@ struct FwLevel {
   DevBuf gk;   // (nk,) float: c_k g_k / Ho
   DevBuf ll, lh, hl, hh;  // (nz, Ho, Wo) each
 };
+
+// dark/flat correction of one count (proc_functions.py:67-74): flat0 =
+// flat mean (or the linear interpolation v of the two halves' means,
+// :62-66) * float32(1 + 1e-5) - dark0; result (x - dark0) / flat0
+__device__ __forceinline__ float darkflat(float x, float d0, float fa, float fb, float v,
+                                          int flat_linear) {
+  const float eps = 1.0f + 1e-5f;  // float32(1 + 1e-5), proc_functions.py:71
+  float f0 = flat_linear ? (1.0f - v) * fa + v * fb : fa;
+  f0 = f0 * eps - d0;
+  return (x - d0) / f0;
+}
+
+__device__ __forceinline__ float minus_log(float x) {
+  // x <= 0 -> 1; -log; NaN -> 6; +-inf -> 0 (proc_functions.py:76-83)
+  if (x <= 0.f) x = 1.f;
+  float r = -logf(x);
+  if (isnan(r)) r = 6.0f;
+  if (isinf(r)) r = 0.0f;
+  return r;
+}
+
+// dark0 = mean(dark), flat means, in the plan's scratch (k_pre_means)
+int pre_launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *flat,
+                     double bright_ratio, int flat_linear, hipStream_t st);
 
 }  // namespace tomo
 

@@ -38,23 +38,6 @@ __global__ void k_pre_means(const uint16_t *__restrict__ dark, const uint16_t *_
   }
 }
 
-__device__ __forceinline__ float darkflat(float x, float d0, float fa, float fb, float v,
-                                          int flat_linear) {
-  const float eps = 1.0f + 1e-5f;  // float32(1 + 1e-5), proc_functions.py:71
-  float f0 = flat_linear ? (1.0f - v) * fa + v * fb : fa;
-  f0 = f0 * eps - d0;
-  return (x - d0) / f0;
-}
-
-__device__ __forceinline__ float minus_log(float x) {
-  // x <= 0 -> 1; -log; NaN -> 6; +-inf -> 0 (proc_functions.py:76-83)
-  if (x <= 0.f) x = 1.f;
-  float r = -logf(x);
-  if (isnan(r)) r = 6.0f;
-  if (isinf(r)) r = 0.0f;
-  return r;
-}
-
 // One workgroup per (p, z) detector row: read data[p][z][:], write out[z][p][:].
 // TO is the storage type of the output.
 template <int MODE, typename TO>  // 0: darkflat (+ optional -log) + swap; 1: darkflat only, projection order
@@ -120,8 +103,8 @@ extern "C" int pre_create(tomo_pre_plan **plan, int64_t nproj, int64_t nz, int64
   return TOMO_OK;
 }
 
-static int launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *flat,
-                        double bright_ratio, int flat_linear, hipStream_t st) {
+int tomo::pre_launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *flat,
+                           double bright_ratio, int flat_linear, hipStream_t st) {
   TOMO_REQUIRE(bright_ratio > 0 && std::isfinite(bright_ratio), "bright_ratio must be > 0");
   TOMO_REQUIRE(!flat_linear || P->nflat >= 3, "flat_linear needs at least 3 flat frames");
   const int zn = (int)(P->nz * P->n);
@@ -159,7 +142,7 @@ extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_
   TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
                "pre_process: unknown out_dtype %d", out_dtype);
   hipStream_t st = (hipStream_t)stream;
-  int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  int rc = pre_launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
   {
     ProfScope ps_("k_pre_rows", st);
@@ -181,7 +164,7 @@ extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16
   TOMO_REQUIRE(P != nullptr, "pre_darkflat: plan is NULL");
   TOMO_REQUIRE(data && dark && flat && res, "pre_darkflat: NULL array");
   hipStream_t st = (hipStream_t)stream;
-  int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  int rc = pre_launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
   {
     ProfScope ps_("k_pre_rows", st);

@@ -43,37 +43,62 @@ __device__ __forceinline__ int refl_fast(int m, int N) {
   return (unsigned)m < (unsigned)N ? m : refl(m, N);
 }
 
+// Dark/flat means for the fused first level (k_pre_means output).
+struct DarkFlat {
+  const float *d0, *fa, *fb;
+  int flat_linear;
+};
+
 // Analysis along the detector axis (rows), one workgroup per (row, slice).
 // MODE 0 reads the chunk layout res[p][z][x] with the zero angle padding;
-// MODE 1 reads a band (z, H, W).  The reflected row x is staged in LDS and
+// MODE 1 reads a band (z, H, W); MODE 2 reads the raw uint16 chunk
+// data[p][z][x] and applies the dark/flat correction on the fly (the fused
+// form of darkflat_correction followed by the first level).  The reflected row x is staged in LDS and
 // read as pairs (x[2k], x[2k+1]): lo[o] = sum_m a[2m] x[2(o+m)] + a[2m+1] x[2(o+m)+1].
 template <int MODE>
-__global__ __launch_bounds__(256) void k_fw_rows(const float *__restrict__ src,
+__global__ __launch_bounds__(256) void k_fw_rows(const void *__restrict__ src_,
                                                  float *__restrict__ lo, float *__restrict__ hi,
                                                  Taps t, int H, int W, int Wo, int nz, int nproj,
-                                                 int st) {
+                                                 int st, DarkFlat df) {
   extern __shared__ float2 xs2[];
   float *xs = reinterpret_cast<float *>(xs2);
   const int h = blockIdx.x, z = blockIdx.y;
   const int L = t.L;
   const int pad = (2 * (Wo - 1) - W + L) / 2;
   const int np = 2 * Wo + L - 2;
-  const float *row;
+  const float *row = nullptr;
+  const uint16_t *rawrow = nullptr;
   bool zero = false;
-  if (MODE == 0) {
-    const int p = h - st;
+  const int p = h - st;
+  if (MODE == 0 || MODE == 2) {
     zero = (p < 0 || p >= nproj);
-    row = src + ((size_t)(zero ? 0 : p) * nz + z) * W;
+    const size_t off = ((size_t)(zero ? 0 : p) * nz + z) * W;
+    if (MODE == 0)
+      row = static_cast<const float *>(src_) + off;
+    else
+      rawrow = static_cast<const uint16_t *>(src_) + off;
   } else {
-    row = src + ((size_t)z * H + h) * W;
+    row = static_cast<const float *>(src_) + ((size_t)z * H + h) * W;
   }
+  // linear flat interpolation weight of projection p (proc_functions.py:62-66)
+  const float vlin = (MODE == 2 && nproj > 1) ? (float)((double)p / (double)(nproj - 1)) : 0.f;
   constexpr int U = 8;
   for (int i0 = threadIdx.x; i0 < np; i0 += U * 256) {
     float v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = i0 + u * 256;
-      v[u] = (!zero && i < np) ? row[refl_fast(i - pad, W)] : 0.f;
+      if (MODE == 2) {
+        v[u] = 0.f;
+        if (!zero && i < np) {
+          const int x = refl_fast(i - pad, W);
+          const size_t zx = (size_t)z * W + x;
+          v[u] = darkflat((float)rawrow[x], df.d0[zx], df.fa[zx],
+                          df.flat_linear ? df.fb[zx] : 0.f, vlin, df.flat_linear);
+        }
+      } else {
+        v[u] = (!zero && i < np) ? row[refl_fast(i - pad, W)] : 0.f;
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -335,14 +360,17 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
 }
 
 // Synthesis along the detector axis, one workgroup per output row with both
-// input rows staged in LDS; outputs in pairs as in k_fw_syn_cols.  LAST writes
-// rows [st, st+nproj) and columns [0, n) straight into the chunk layout
-// res[p][z][x].
-template <int LAST>
+// input rows staged in LDS; outputs in pairs as in k_fw_syn_cols.  LAST = 1
+// writes rows [st, st+nproj) and columns [0, n) straight into the chunk
+// layout res[p][z][x]; LAST = 2 writes them in sinogram order out[z][p][x]
+// as TO, after minus_log when take_log (the fused form of proc_proj and the
+// swap of rec.py:174).
+template <int LAST, typename TO>
 __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ lo,
                                                      const float *__restrict__ hi,
-                                                     float *__restrict__ out, Taps g, int Hs, int W,
-                                                     int Ws, int nz, int nproj, int st, int n) {
+                                                     TO *__restrict__ out, Taps g, int Hs, int W,
+                                                     int Ws, int nz, int nproj, int st, int n,
+                                                     int take_log) {
   extern __shared__ float2 rs[];
   const int h = blockIdx.x, z = blockIdx.y;
   if (LAST && (h < st || h >= st + nproj)) return;
@@ -363,7 +391,9 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
   __syncthreads();
   const int L2 = g.L >> 1;
   const int nout = LAST ? min(Ws, n) : Ws;
-  float *orow = LAST ? out + ((size_t)(h - st) * nz + z) * n : out + ((size_t)z * Hs + h) * Ws;
+  TO *orow = LAST == 2   ? out + ((size_t)z * nproj + (h - st)) * n
+             : LAST == 1 ? out + ((size_t)(h - st) * nz + z) * n
+                         : out + ((size_t)z * Hs + h) * Ws;
   for (int p = threadIdx.x; 2 * p < nout; p += 256) {
     float a0 = 0.f, a1 = 0.f;
     for (int m = 0; m < L2; ++m) {
@@ -373,8 +403,12 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
       a1 = fmaf(g.a[2 * m + 1], x.x, a1);
       a1 = fmaf(g.b[2 * m + 1], x.y, a1);
     }
-    orow[2 * p] = a0;
-    if (2 * p + 1 < nout) orow[2 * p + 1] = a1;
+    if (LAST == 2 && take_log) {
+      a0 = minus_log(a0);
+      a1 = minus_log(a1);
+    }
+    orow[2 * p] = (TO)a0;
+    if (2 * p + 1 < nout) orow[2 * p + 1] = (TO)a1;
   }
 }
 
@@ -469,21 +503,18 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
 
 }  // namespace
 
-extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, const char *wname,
-                                    int level, void *stream) {
-  clear_error();
-  TOMO_REQUIRE(P != nullptr && res != nullptr && wname != nullptr,
-               "pre_remove_stripe_fw: NULL argument");
-  TOMO_REQUIRE(sigma > 0 && std::isfinite(sigma), "pre_remove_stripe_fw: sigma must be > 0");
-  TOMO_REQUIRE(level >= 1 && level <= 16, "pre_remove_stripe_fw: level=%d out of range", level);
-  const WaveletTaps *wv = find_wavelet(wname);
-  TOMO_REQUIRE(wv != nullptr, "pre_remove_stripe_fw: unknown wavelet '%s'", wname);
-  TOMO_REQUIRE(wv->len % 2 == 0 && wv->len <= 32,
-               "pre_remove_stripe_fw: wavelet '%s' has %d taps (even, at most 32, supported)",
-               wname, wv->len);
+namespace {
+
+// The level loop of the Fourier-wavelet stripe removal.  src_mode 0: src is
+// the float32 transmission chunk res[p][z][x]; 2: src is the raw uint16 chunk
+// (dark/flat applied in the first pass, means in the plan).  dst_mode 1: dst
+// is the float32 chunk res[p][z][x] (in place allowed); 2: dst is the
+// sinogram-order output of dtype out_dtype, after minus_log when take_log.
+int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, const void *src,
+           int src_mode, DarkFlat df, void *dst, int dst_mode, int out_dtype, int take_log,
+           hipStream_t st) {
   int rc = setup_fw(P, wv, sigma, level);
   if (rc) return rc;
-  hipStream_t st = (hipStream_t)stream;
   const int L = wv->len;
   Taps ana{}, syn{};
   ana.L = syn.L = L;
@@ -499,15 +530,20 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
   for (int k = 0; k < level; ++k) {
     FwLevel &l = P->lv[k];
     const int lds = (2 * l.Wo + L - 2) * (int)sizeof(float);
-    if (k == 0) {
+    {
       ProfScope ps_("k_fw_rows", st);
-      hipLaunchKernelGGL(k_fw_rows<0>, dim3(l.H, nz), dim3(256), lds, st, res, P->rowlo.as<float>(),
-                         P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz, nproj, st0);
-    } else {
-      ProfScope ps_("k_fw_rows", st);
-      hipLaunchKernelGGL(k_fw_rows<1>, dim3(l.H, nz), dim3(256), lds, st,
-                         P->lv[k - 1].ll.as<float>(), P->rowlo.as<float>(), P->rowhi.as<float>(),
-                         ana, l.H, l.W, l.Wo, nz, nproj, st0);
+      if (k > 0)
+        hipLaunchKernelGGL(k_fw_rows<1>, dim3(l.H, nz), dim3(256), lds, st,
+                           (const void *)P->lv[k - 1].ll.as<float>(), P->rowlo.as<float>(),
+                           P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz, nproj, st0, df);
+      else if (src_mode == 2)
+        hipLaunchKernelGGL(k_fw_rows<2>, dim3(l.H, nz), dim3(256), lds, st, src,
+                           P->rowlo.as<float>(), P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz,
+                           nproj, st0, df);
+      else
+        hipLaunchKernelGGL(k_fw_rows<0>, dim3(l.H, nz), dim3(256), lds, st, src,
+                           P->rowlo.as<float>(), P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz,
+                           nproj, st0, df);
     }
     TOMO_LAUNCH_CHECK();
     const int otc = l.Ho >= 256 ? kColOut : (l.Ho >= 64 ? 16 : 8);
@@ -572,12 +608,23 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
     {
       ProfScope ps_("k_fw_syn_rows", st);
       const size_t rl = sizeof(float2) * Wb;
-      if (k == 0)
-        hipLaunchKernelGGL(k_fw_syn_rows<1>, dim3(Hs, nz), dim3(256), rl, st, P->synlo.as<float>(),
-                           P->synhi.as<float>(), res, syn, Hs, Wb, Ws, nz, nproj, st0, n);
+      const dim3 gr(Hs, nz);
+      const float *slo = P->synlo.as<float>(), *shi = P->synhi.as<float>();
+      if (k > 0)
+        hipLaunchKernelGGL((k_fw_syn_rows<0, float>), gr, dim3(256), rl, st, slo, shi, out, syn,
+                           Hs, Wb, Ws, nz, nproj, st0, n, 0);
+      else if (dst_mode == 1)
+        hipLaunchKernelGGL((k_fw_syn_rows<1, float>), gr, dim3(256), rl, st, slo, shi,
+                           (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, 0);
+      else if (out_dtype == TOMO_F16)
+        hipLaunchKernelGGL((k_fw_syn_rows<2, _Float16>), gr, dim3(256), rl, st, slo, shi,
+                           (_Float16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+      else if (out_dtype == TOMO_BF16)
+        hipLaunchKernelGGL((k_fw_syn_rows<2, __bf16>), gr, dim3(256), rl, st, slo, shi,
+                           (__bf16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
       else
-        hipLaunchKernelGGL(k_fw_syn_rows<0>, dim3(Hs, nz), dim3(256), rl, st, P->synlo.as<float>(),
-                           P->synhi.as<float>(), out, syn, Hs, Wb, Ws, nz, nproj, st0, n);
+        hipLaunchKernelGGL((k_fw_syn_rows<2, float>), gr, dim3(256), rl, st, slo, shi,
+                           (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
     }
     TOMO_LAUNCH_CHECK();
     cur = out;
@@ -585,4 +632,57 @@ extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, 
     curW = Ws;
   }
   return TOMO_OK;
+}
+
+const WaveletTaps *check_wavelet(const char *fn, const char *wname, double sigma, int level) {
+  if (!(sigma > 0 && std::isfinite(sigma))) {
+    set_error("%s: sigma must be > 0", fn);
+    return nullptr;
+  }
+  if (level < 1 || level > 16) {
+    set_error("%s: level=%d out of range", fn, level);
+    return nullptr;
+  }
+  const WaveletTaps *wv = find_wavelet(wname);
+  if (!wv) {
+    set_error("%s: unknown wavelet '%s'", fn, wname);
+    return nullptr;
+  }
+  if (wv->len % 2 != 0 || wv->len > 32) {
+    set_error("%s: wavelet '%s' has %d taps (even, at most 32, supported)", fn, wname, wv->len);
+    return nullptr;
+  }
+  return wv;
+}
+
+}  // namespace
+
+extern "C" int pre_remove_stripe_fw(tomo_pre_plan *P, float *res, double sigma, const char *wname,
+                                    int level, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr && res != nullptr && wname != nullptr,
+               "pre_remove_stripe_fw: NULL argument");
+  const WaveletTaps *wv = check_wavelet("pre_remove_stripe_fw", wname, sigma, level);
+  if (!wv) return TOMO_EINVAL;
+  return fw_run(P, wv, sigma, level, res, 0, DarkFlat{nullptr, nullptr, nullptr, 0}, res, 1,
+                TOMO_F32, 0, (hipStream_t)stream);
+}
+
+extern "C" int pre_process_fw(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
+                              const uint16_t *flat, void *out, int out_dtype, double bright_ratio,
+                              int flat_linear, int minus_log, double sigma, const char *wname,
+                              int level, void *stream) {
+  clear_error();
+  TOMO_REQUIRE(P != nullptr && data && dark && flat && out && wname,
+               "pre_process_fw: NULL argument");
+  TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
+               "pre_process_fw: unknown out_dtype %d", out_dtype);
+  const WaveletTaps *wv = check_wavelet("pre_process_fw", wname, sigma, level);
+  if (!wv) return TOMO_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = pre_launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  if (rc) return rc;
+  const DarkFlat df{P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(),
+                    flat_linear};
+  return fw_run(P, wv, sigma, level, data, 2, df, out, 2, out_dtype, minus_log, st);
 }

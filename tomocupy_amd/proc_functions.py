@@ -90,8 +90,16 @@ class ProcFunctions:
         if out is None:
             out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=data.device)
         if self.remove_stripe_method == 'fw':
-            res = self.proc_sino(data, dark, flat, res)
-            return self.proc_proj_swap(res, out)
+            # dark/flat fused into the first wavelet pass, minus_log + swap
+            # into the last (the transmission chunk is never written)
+            _lib.check(_lib.lib().pre_process_fw(
+                self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
+                _lib.dev_ptr(flat, sf, u16, 'flat'),
+                _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), td, 'out'),
+                _lib.DTYPES[self.dtype], self.bright_ratio, int(self.flat_linear),
+                int(self.take_minus_log), self.fw_sigma, self.fw_filter.encode(), self.fw_level,
+                _lib.stream_ptr(None)), 'pre_process_fw')
+            return out
         _lib.check(_lib.lib().pre_process(
             self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
             _lib.dev_ptr(flat, sf, u16, 'flat'),
