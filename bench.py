@@ -371,7 +371,9 @@ def main():
             tf = os.path.join(ROOT, 'profiles', 'traffic.json')
             if os.path.exists(tf):
                 tj = json.load(open(tf))
-                key = '%s@n%d_nproj%d_chunk%d' % (dom, a.n, a.nproj, a.chunk)
+                key = '%s@n%d_nproj%d_chunk%d_%s' % (dom, a.n, a.nproj, a.chunk,
+                                                   {'float32': 'f32', 'float16': 'f16',
+                                                    'bfloat16': 'bf16'}[a.dtype])
                 traffic = tj.get(key)
             roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1),
                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),

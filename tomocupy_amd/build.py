@@ -22,6 +22,10 @@ FLAGS = [
     '-munsafe-fp-atomics',
     # no packed FP32 (v_pk_*_f32) code generation
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    # matrix-core accumulators in ordinary VGPRs: with the accumulation
+    # registers (AGPRs) the allocator rotated the gridding accumulators through
+    # ~50 copy instructions per iteration
+    '-mllvm', '-amdgpu-mfma-vgpr-form',
     '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
 ]
 

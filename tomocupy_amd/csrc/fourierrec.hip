@@ -147,6 +147,22 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Gaussian factors of one list entry {sample, footprint offsets (cols |
+// rows << 8, signed bytes), du bits, dv bits} for tile column / row i:
+// e0 = 2^(K2 (i - rb0 - 4 - du)^2) inside the 9-cell footprint, else 0, and
+// e1 likewise for the rows.  Branch-free (padding entries, sample -1, give
+// 0); K2 x^2 >= -26 for |x| <= 5, far from the range where the hardware
+// exponential would flush, so v_exp_f32 is used directly.
+__device__ __forceinline__ void gauss_pair(const int4 &e, int i, float K2, float &e0, float &e1) {
+  const int d0 = i - (int)(signed char)(e.y & 0xff);
+  const int d1 = i - (int)(signed char)((e.y >> 8) & 0xff);
+  const float x0 = (float)(d0 - 4) - __int_as_float(e.z);
+  const float x1 = (float)(d1 - 4) - __int_as_float(e.w);
+  const bool ok = e.x >= 0;
+  e0 = (ok && (unsigned)d0 <= 8u) ? __builtin_amdgcn_exp2f(K2 * x0 * x0) : 0.f;
+  e1 = (ok && (unsigned)d1 <= 8u) ? __builtin_amdgcn_exp2f(K2 * x1 * x1) : 0.f;
+}
+
 // Gridding as a matrix product on the matrix cores.  For one 16x16 cell tile
 // (rows a, columns b of the periodic 2n x 2n grid) and four polar samples s:
 //   F[a][b] += sum_s (E1_s(a) * G_s) * E0_s(b)
@@ -209,16 +225,8 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
         gv[2 * c] = g.x;
         gv[2 * c + 1] = g.y;
       }
-      float e0v = 0.f, e1v = 0.f;
-      if (cur.x >= 0) {
-        const int rb0 = (int)(signed char)(cur.y & 0xff);
-        const int rb1 = (int)(signed char)((cur.y >> 8) & 0xff);
-        const int d0 = i - rb0, d1 = i - rb1;
-        const float x0 = (float)(d0 - 4) - __int_as_float(cur.z);
-        const float x1 = (float)(d1 - 4) - __int_as_float(cur.w);
-        e0v = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
-        e1v = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
-      }
+      float e0v, e1v;
+      gauss_pair(cur, i, K2, e0v, e1v);
 #pragma unroll
       for (int m = 0; m < 8; ++m)
         acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * gv[m], e0v, acc[m], 0, 0, 0);
@@ -335,16 +343,7 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
           g[u][2 * c] = v.x;
           g[u][2 * c + 1] = v.y;
         }
-        e0v[u] = e1v[u] = 0.f;
-        if (cur.x >= 0) {
-          const int rb0 = (int)(signed char)(cur.y & 0xff);
-          const int rb1 = (int)(signed char)((cur.y >> 8) & 0xff);
-          const int d0 = i - rb0, d1 = i - rb1;
-          const float x0 = (float)(d0 - 4) - __int_as_float(cur.z);
-          const float x1 = (float)(d1 - 4) - __int_as_float(cur.w);
-          e0v[u] = ((unsigned)d0 <= 8u) ? exp2f(K2 * x0 * x0) : 0.f;
-          e1v[u] = ((unsigned)d1 <= 8u) ? exp2f(K2 * x1 * x1) : 0.f;
-        }
+        gauss_pair(cur, i, K2, e0v[u], e1v[u]);
       }
       const typename M::v4 b = M::pack(e0v[0], e0v[1], e0v[2], e0v[3]);
 #pragma unroll
