@@ -104,37 +104,58 @@ __global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const floa
 // circular convolution of length 2n of the zero-padded row with h2 (h wrapped
 // to lags (-n, n), h2[n] = 0): FFT_2n, multiply by H2 = FFT_2n(h2)/(2n), and
 // the unnormalised IFFT_2n.
-// One workgroup per slice builds H2, HL, HR; the window sums use FP64 prefix
-// sums of h.
-__global__ __launch_bounds__(1024) void k_flt_kernel2n(
-    const float *__restrict__ wfilter, const float *__restrict__ sht, float2 *__restrict__ H2,
-    float *__restrict__ HL, float *__restrict__ HR, double *__restrict__ P, FftDesc fde,
-    FftDesc fd2, int ne, int n, double center) {
-  extern __shared__ float2 sm[];
-  const int z = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+// H2, HL and HR are built in FP64 and rounded once: the low-frequency values
+// of H2 are tiny next to max|h| (the ramp vanishes at DC), so a float32
+// round trip h -> H2 loses them (measured 4e-5 rel-L2 in the image at
+// n = 4096).  With shift = -center + sht[z] + n/2 and phi_d = 2 pi (d - shift)/ne,
+// the C2R of w[k] = wfilter[k] exp(-2 pi i k shift/ne) (imaginary parts of the
+// DC and Nyquist bins dropped) is
+//   h[d] = wf[0] + (-1)^d wf[ne/2] cos(pi shift) + 2 sum_{0<k<ne/2} wf[k] cos(k phi_d),
+// summed with a rotating phasor (error ~ ne/2 ulp of FP64).
+__device__ __forceinline__ double flt_shift(const float *sht, int z, double center, int n) {
+  return -center + (sht ? (double)sht[z] : 0.0) + (double)(n / 2);
+}
+
+__global__ __launch_bounds__(256) void k_flt_h64(const float *__restrict__ wfilter,
+                                                 const float *__restrict__ sht,
+                                                 double *__restrict__ h, int ne, int n,
+                                                 double center) {
+  const int d = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
+  if (d >= ne) return;
   const int nh = ne / 2;
-  const double shift = -center + (sht ? (double)sht[z] : 0.0) + (double)(n / 2);
-  for (int k = tid; k < ne; k += nt) {
-    const int kk = k <= nh ? k : ne - k;
-    double f = (double)kk * shift / (double)ne;
-    f -= std::floor(f);
-    double sn, cs;
-    sincospi(-2.0 * f, &sn, &cs);
-    const double a = (double)wfilter[kk];
-    float2 W = make_float2((float)(a * cs), (float)(a * sn));
-    if (k == 0 || k == nh) W.y = 0.f;
-    if (k > nh) W.y = -W.y;
-    sm[lds_pad(k)] = W;
+  const double shift = flt_shift(sht, z, center, n);
+  double f = ((double)d - shift) / (double)ne;  // phi_d / (2 pi), reduced mod 1
+  f -= std::floor(f);
+  double sn, cs;
+  sincospi(2.0 * f, &sn, &cs);
+  double rr = cs, ri = sn;  // exp(i k phi_d) for k = 1
+  double acc = 0.0;
+  for (int k = 1; k < nh; ++k) {
+    acc += (double)wfilter[k] * rr;
+    const double t = rr * cs - ri * sn;
+    ri = rr * sn + ri * cs;
+    rr = t;
   }
-  __syncthreads();
-  lds_fft<1>(sm, fde, 0, tid, nt);  // h[d] = Re sm[d] (unnormalised C2R)
-  // FP64 prefix sums P[d] = sum_{e<d} h[e], d = 0..ne: per-thread chunk sums,
-  // then a serial scan of the chunk totals
+  double g = shift;  // cos(pi shift) with the argument reduced mod 2
+  g -= 2.0 * std::floor(g / 2.0);
+  const double cnyq = cospi(g);
+  h[(size_t)z * ne + d] = (double)wfilter[0] + ((d & 1) ? -1.0 : 1.0) * (double)wfilter[nh] * cnyq +
+                          2.0 * acc;
+}
+
+// FP64 prefix sums P[d] = sum_{e<d} h[e] (one workgroup per slice), then the
+// edge responses HL[j] = P[pad+j+1] - P[j+1], HR[j] = P[j-n+ne+1] - P[pad+j+1].
+__global__ __launch_bounds__(1024) void k_flt_edges64(const double *__restrict__ h,
+                                                      double *__restrict__ P,
+                                                      float *__restrict__ HL,
+                                                      float *__restrict__ HR, int ne, int n) {
   __shared__ double csum[1024];
+  const int z = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const double *hz = h + (size_t)z * ne;
   const int chunk = (ne + nt - 1) / nt;
   const int d0 = tid * chunk, d1 = min(ne, d0 + chunk);
   double acc = 0.0;
-  for (int d = d0; d < d1; ++d) acc += (double)sm[lds_pad(d)].x;
+  for (int d = d0; d < d1; ++d) acc += hz[d];
   csum[tid] = acc;
   __syncthreads();
   if (tid == 0) {
@@ -150,34 +171,37 @@ __global__ __launch_bounds__(1024) void k_flt_kernel2n(
   acc = csum[tid];
   for (int d = d0; d < d1; ++d) {
     Pz[d] = acc;
-    acc += (double)sm[lds_pad(d)].x;
+    acc += hz[d];
   }
   if (d1 == ne && d0 < d1) Pz[ne] = acc;
   __syncthreads();  // the global prefix sums are visible to the workgroup
-  const int pad = nh - n / 2;
+  const int pad = ne / 2 - n / 2;
   for (int j = tid; j < n; j += nt) {
     HL[(size_t)z * n + j] = (float)(Pz[pad + j + 1] - Pz[j + 1]);
     HR[(size_t)z * n + j] = (float)(Pz[j - n + ne + 1] - Pz[pad + j + 1]);
   }
-  // h2 (lags -(n-1)..(n-1) wrapped to length 2n) -> H2 = FFT_2n(h2)
-  float hv[8];
+}
+
+// H2[k] = FFT_2n(h2)[k] / (2n), h2 = h on lags (-n, n) wrapped to length 2n
+// (h2[n] = 0); FP64 sums with a rotating phasor, rounded once.
+__global__ __launch_bounds__(256) void k_flt_H2_64(const double *__restrict__ h,
+                                                   float2 *__restrict__ H2, int ne, int n) {
+  const int k = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
   const int n2 = 2 * n;
-  // read h for this thread's h2 entries before the LDS buffer is reused
-  int cnt = 0;
-  for (int d = tid; d < n2 && cnt < 8; d += nt, ++cnt) {
-    float v = 0.f;
-    if (d < n) v = sm[lds_pad(d)].x;
-    else if (d > n) v = sm[lds_pad(ne - n2 + d)].x;
-    hv[cnt] = v;
+  if (k >= n2) return;
+  const double *hz = h + (size_t)z * ne;
+  double sn, cs;
+  sincospi(-2.0 * (double)k / (double)n2, &sn, &cs);
+  double rr = 1.0, ri = 0.0, ar = 0.0, ai = 0.0;
+  for (int d = 0; d < n2; ++d) {
+    const double v = d < n ? hz[d] : (d > n ? hz[ne - n2 + d] : 0.0);
+    ar += v * rr;
+    ai += v * ri;
+    const double t = rr * cs - ri * sn;
+    ri = rr * sn + ri * cs;
+    rr = t;
   }
-  __syncthreads();
-  cnt = 0;
-  for (int d = tid; d < n2 && cnt < 8; d += nt, ++cnt) sm[lds_pad(d)] = make_float2(hv[cnt], 0.f);
-  __syncthreads();
-  lds_fft<-1>(sm, fd2, 0, tid, nt);
-  // 1/(2n) of the unnormalised inverse transform that follows rides on H2
-  const float inv2n = 1.0f / (float)n2;
-  for (int k = tid; k < n2; k += nt) H2[(size_t)z * n2 + k] = cscale(sm[lds_pad(k)], inv2n);
+  H2[(size_t)z * n2 + k] = make_float2((float)(ar / n2), (float)(ai / n2));
 }
 
 }  // namespace
@@ -187,10 +211,24 @@ int tomo::flt_prepare_2n(tomo_flt_plan *P, const float *wfilter, double center, 
   TOMO_REQUIRE(P->fft2n.d.N == 2 * P->n && P->H2.p != nullptr,
                "fused filter path not available for n=%lld, ne=%lld", (long long)P->n,
                (long long)P->ne);
-  ProfScope ps_("k_flt_kernel2n", st);
-  hipLaunchKernelGGL(k_flt_kernel2n, dim3((unsigned)P->nz), dim3(P->kthreads), P->klds, st, wfilter,
-                     sht, P->H2.as<float2>(), P->HL.as<float>(), P->HR.as<float>(),
-                     P->P.as<double>(), P->fft.d, P->fft2n.d, (int)P->ne, (int)P->n, center);
+  const int ne = (int)P->ne, n = (int)P->n, nz = (int)P->nz;
+  {
+    ProfScope ps_("k_flt_kernel2n", st);
+    hipLaunchKernelGGL(k_flt_h64, dim3((unsigned)ceil_div(ne, 256), nz), dim3(256), 0, st,
+                       wfilter, sht, P->Hd.as<double>(), ne, n, center);
+  }
+  TOMO_LAUNCH_CHECK();
+  {
+    ProfScope ps_("k_flt_kernel2n", st);
+    hipLaunchKernelGGL(k_flt_edges64, dim3(nz), dim3(1024), 0, st, P->Hd.as<double>(),
+                       P->P.as<double>(), P->HL.as<float>(), P->HR.as<float>(), ne, n);
+  }
+  TOMO_LAUNCH_CHECK();
+  {
+    ProfScope ps_("k_flt_kernel2n", st);
+    hipLaunchKernelGGL(k_flt_H2_64, dim3((unsigned)ceil_div(2 * n, 256), nz), dim3(256), 0, st,
+                       P->Hd.as<double>(), P->H2.as<float2>(), ne, n);
+  }
   TOMO_LAUNCH_CHECK();
   P->prepared = 1;
   return TOMO_OK;
@@ -381,23 +419,10 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
     if (!rc) rc = P->HL.alloc(sizeof(float) * nz * n);
     if (!rc) rc = P->HR.alloc(sizeof(float) * nz * n);
     if (!rc) rc = P->P.alloc(sizeof(double) * nz * (ne + 1));
+    if (!rc) rc = P->Hd.alloc(sizeof(double) * nz * ne);
     if (rc) {
       flt_destroy(P);
       return rc;
-    }
-    P->kthreads = fft_threads(P->fft.d, ne);
-    P->klds = sizeof(float2) * lds_padded_len((int)ne);
-    if (!(P->kthreads > 0 && P->kthreads * 8 >= 2 * n)) {
-      set_error("flt_create: ne=%lld too large for the fused filter setup", (long long)ne);
-      flt_destroy(P);
-      return TOMO_EINVAL;
-    }
-    hipError_t e2 = hipFuncSetAttribute((const void *)k_flt_kernel2n,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->klds);
-    if (e2 != hipSuccess) {
-      set_error("flt_create: hipFuncSetAttribute failed: %s", hipGetErrorString(e2));
-      flt_destroy(P);
-      return TOMO_EHIP;
     }
   }
   P->threads = fft_threads(P->fft.d, ne);
@@ -478,5 +503,6 @@ extern "C" void flt_destroy(tomo_flt_plan *P) {
   P->HL.release();
   P->HR.release();
   P->P.release();
+  P->Hd.release();
   delete P;
 }

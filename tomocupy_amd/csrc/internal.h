@@ -15,9 +15,7 @@ struct tomo_flt_plan {
   size_t lds = 0;
   // fused path: per slice, the spectrum of the impulse response wrapped to
   // length 2n, and the edge-replication responses HL/HR (see filter.hip)
-  tomo::DevBuf H2, HL, HR, P;
-  int kthreads = 256;
-  size_t klds = 0;
+  tomo::DevBuf H2, HL, HR, P, Hd;  // Hd: FP64 impulse response (nz, ne)
   int prepared = 0;  // H2/HL/HR hold the kernels of the last flt_prepare
 };
 
