@@ -8,6 +8,13 @@ with 10 dark and 20 flat frames and column-gain stripes -> dark/flat ->
 USFFT backprojection.  All inputs are resident in HBM before the timed region.
 One step = the whole nz-slice volume.  Prints one JSON line.
 
+Kernel timing: an untimed pass over the first 16 chunks on ONE stream brackets
+every kernel launch with HIP events (isolated kernel durations:
+``kernels_untimed_breakdown_step``, ``roofline``) and names the dominant
+kernel; the timed steps bracket only that kernel's launches
+(``roofline_overlapped_events``: start-to-end times on its stream while the
+streams overlap, which include waits behind the other streams' kernels).
+
 ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
 import argparse
@@ -23,10 +30,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "slices/s at 2048^2 x1500 proj, FP32 vs FP16 at rel-L2 vs FP64; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+ISO_CHUNKS = 16  # chunks of the untimed one-stream breakdown pass
 KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_xsum', 'k_fw_syn_cols',
-           'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_flt_kernel2n',
+           'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_flt_prepare',
            'k_fr_prep', 'k_fr_filtprep',
-           'k_fr_grid', 'k_fr_grid_reduce', 'k_fr_rowfft', 'k_fr_colfft']
+           'k_fr_grid', 'k_fr_grid16', 'k_fr_grid_reduce', 'k_fr_rowfft', 'k_fr_colfft']
 
 
 def parse():
@@ -103,7 +111,7 @@ def algorithmic_bytes(name, a, s):
     c = 2 * r
     nzc = (ch + 1) // 2
     grp = 4  # complex slices per gridding group
-    if name == 'k_fr_grid':      # read the polar spectra, write the 2n x 2n grid
+    if name in ('k_fr_grid', 'k_fr_grid16'):  # read the polar spectra, write the 2n x 2n grid
         return grp * (nproj * n * c + (2 * n) ** 2 * c)
     if name == 'k_fr_rowfft':    # read the grid rows, write the pruned half
         return nzc * ((2 * n) ** 2 * c + 2 * n * n * c)
@@ -324,9 +332,9 @@ def main():
     torch.cuda.synchronize()
     print('# plans ready in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
 
-    def step():
-        for k in range(nchunks):
-            i = k % len(streams)
+    def step(nstreams=len(streams), count=nchunks):
+        for k in range(count):
+            i = k % nstreams
             with torch.cuda.stream(streams[i]):
                 out = pfs[i].proc_chunk(raw[k], dark[k], flat[k], out=sinos[i])
                 bfs[i].fbp_filter_backprojection(rec[k * a.chunk:(k + 1) * a.chunk], out)
@@ -341,8 +349,53 @@ def main():
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    _lib.prof_reset()
+    # per-kernel breakdown (untimed): the first ISO_CHUNKS chunks on ONE
+    # stream with every kernel bracketed by events, i.e. isolated kernel
+    # durations; it names the dominant kernel (largest total) and gives the
+    # roofline.  (With several streams an event pair around a launch also
+    # spans the wait behind the other streams' kernels, so event durations
+    # under overlap are not kernel durations.)
+    kern = {}
+    dom = None
+    roof = None
+    sfx = {'float32': 'f32', 'float16': 'f16', 'bfloat16': 'bf16'}[a.dtype]
     if not a.no_prof:
+        _lib.prof_only(None)
+        _lib.prof_reset()
+        _lib.prof_enable(True)
+        step(1, min(nchunks, ISO_CHUNKS))
+        torch.cuda.synchronize()
+        _lib.prof_enable(False)
+        for k in KERNELS:
+            ms, cnt = _lib.prof_read(k)
+            if cnt:
+                kern[k] = {'avg_ms': ms / cnt, 'launches': cnt, 'total_ms': ms}
+                ab = algorithmic_bytes(k, a, None)
+                if ab is not None:
+                    kern[k]['hbm_frac'] = round(ab / (ms / cnt * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        if kern:
+            dom = max(kern, key=lambda k: kern[k]['total_ms'])
+            ab = algorithmic_bytes(dom, a, None)
+            if ab is not None:
+                ach = ab / (kern[dom]['avg_ms'] * 1e-3) / 1e9
+                traffic = None
+                tf = os.path.join(ROOT, 'profiles', 'traffic.json')
+                if os.path.exists(tf):
+                    traffic = json.load(open(tf)).get(
+                        '%s@n%d_nproj%d_chunk%d_%s' % (dom, a.n, a.nproj, a.chunk, sfx))
+                roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1),
+                        'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                        'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                        'algorithmic_bytes_per_launch': ab,
+                        'avg_launch_ms': round(kern[dom]['avg_ms'], 5),
+                        'launches': kern[dom]['launches'],
+                        'timing': 'HIP events around each launch, first %d chunks of the '
+                                  'workload on one stream (untimed pass before the timed steps)'
+                                  % min(nchunks, ISO_CHUNKS)}
+    # timed region: events around the dominant kernel's launches only
+    _lib.prof_reset()
+    if dom is not None:
+        _lib.prof_only(dom)
         _lib.prof_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -352,37 +405,27 @@ def main():
         print('# step %d done' % i, file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     _lib.prof_enable(False)
+    _lib.prof_only(None)
     dt = (t1 - t0) / a.steps
     slices = a.nz
     value = slices / dt
 
-    kern = {}
-    for k in KERNELS:
-        ms, cnt = _lib.prof_read(k)
-        if cnt:
-            kern[k] = {'avg_ms': ms / cnt, 'launches': cnt, 'total_ms_per_step': ms / a.steps}
-    roof = None
-    if kern:
-        dom = max(kern, key=lambda k: kern[k]['total_ms_per_step'])
+    roof_ov = None
+    if dom is not None:
+        ms, cnt = _lib.prof_read(dom)
         ab = algorithmic_bytes(dom, a, None)
-        if ab is not None:
-            ach = ab / (kern[dom]['avg_ms'] * 1e-3) / 1e9
-            traffic = None
-            tf = os.path.join(ROOT, 'profiles', 'traffic.json')
-            if os.path.exists(tf):
-                tj = json.load(open(tf))
-                key = '%s@n%d_nproj%d_chunk%d_%s' % (dom, a.n, a.nproj, a.chunk,
-                                                   {'float32': 'f32', 'float16': 'f16',
-                                                    'bfloat16': 'bf16'}[a.dtype])
-                traffic = tj.get(key)
-            roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1),
-                    'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
-                    'traffic': traffic, 'algorithmic_bytes_per_launch': ab}
+        if ab is not None and cnt:
+            avg_ms = ms / cnt
+            roof_ov = {'kernel': dom, 'event_avg_ms': round(avg_ms, 5), 'launches': cnt,
+                       'achieved': round(ab / (avg_ms * 1e-3) / 1e9, 1), 'unit': 'GB/s',
+                       'note': 'events over the timed %d-stream steps: start-to-end on the '
+                               'launch stream, including waits behind other streams'
+                               % a.streams}
     line = {
         'metric': METRIC, 'value': round(value, 2), 'unit': 'slices/s', 'n_gpus': 1,
         'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(dt * 1e3, 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': {'float32': 'f32', 'float16': 'f16', 'bfloat16': 'bf16'}[a.dtype],
+        'dtype': sfx,
         'data': 'synthetic: %d distinct seeded ellipse-phantom slices (uint16 counts, 10 dark, '
                 '20 flat, column-gain stripes) tiled over nz' % min(a.distinct, a.chunk),
         'config': {'workload': 'fourierrec full chunk pipeline: n=%d nproj=%d nz=%d chunk=%d '
@@ -391,8 +434,9 @@ def main():
                                   center),
                    'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk,
                    'streams': a.streams, 'stripe': a.stripe, 'filter': a.filter,
-                   'timed_with_kernel_events': not a.no_prof},
-        'roofline': roof, 'cpu_baseline': cpu, 'kernels': kern,
+                   'timed_with_kernel_events': 'dominant kernel only' if dom else False},
+        'roofline': roof, 'cpu_baseline': cpu, 'roofline_overlapped_events': roof_ov,
+        'kernels_untimed_breakdown_step': kern,
     }
     print(json.dumps(line))
 

@@ -48,9 +48,13 @@ const char *tomo_version(void);
  * profiling hooks).  While enabled, each kernel launch is bracketed by two HIP
  * events on its own stream; tomo_prof_read() waits for the recorded events and
  * returns the accumulated milliseconds and launch count of one kernel by name
- * (e.g. "k_fr_grid").  tomo_prof_reset() clears the accumulators.
+ * (e.g. "k_fr_grid").  tomo_prof_only(name) restricts the timing to one
+ * kernel name (NULL or "" = every kernel), so that a measured run records two
+ * events per launch of that kernel only.  tomo_prof_reset() clears the
+ * accumulators.
  */
 int tomo_prof_enable(int on);
+int tomo_prof_only(const char *kernel);
 int tomo_prof_reset(void);
 int tomo_prof_read(const char *kernel, double *total_ms, int64_t *launches);
 

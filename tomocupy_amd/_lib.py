@@ -33,6 +33,7 @@ SIGNATURES = {
     'tomo_last_error': (ctypes.c_char_p, []),
     'tomo_version': (ctypes.c_char_p, []),
     'tomo_prof_enable': (_c_int, [_c_int]),
+    'tomo_prof_only': (_c_int, [ctypes.c_char_p]),
     'tomo_prof_reset': (_c_int, []),
     'tomo_prof_read': (_c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(_c_i64)]),
@@ -148,6 +149,11 @@ def calc_filter(ne, name):
 
 def prof_enable(on=True):
     lib().tomo_prof_enable(1 if on else 0)
+
+
+def prof_only(kernel=None):
+    """Restrict kernel timing to one kernel name (None: every kernel)."""
+    lib().tomo_prof_only(kernel.encode() if kernel else None)
 
 
 def prof_reset():

@@ -26,7 +26,7 @@ FLAGS = [
     # registers (AGPRs) the allocator rotated the gridding accumulators through
     # ~50 copy instructions per iteration
     '-mllvm', '-amdgpu-mfma-vgpr-form',
-    '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
+    '-Wall', '-Wno-unused-function', '-Wno-unused-variable', '-Werror=return-type',
 ]
 
 

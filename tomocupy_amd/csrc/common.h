@@ -80,10 +80,12 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 size_t max_lds_per_block();
 
 // ---- kernel timing (tomo_prof_*) -----------------------------------------
-// When enabled, every launch wrapped in a ProfScope is bracketed by two HIP
-// events recorded on the launch stream; durations are collected lazily by
-// tomo_prof_read().
-bool prof_enabled();
+// When enabled, every launch wrapped in a ProfScope whose kernel name is
+// selected (all names, or the one given to tomo_prof_only) is bracketed by
+// two HIP events recorded on the launch stream; durations are collected
+// lazily by tomo_prof_read().  Events come from a pool and are reused.
+bool prof_wants(const char *name);
+bool prof_take(hipEvent_t *a, hipEvent_t *b);
 void prof_push(const char *name, hipEvent_t a, hipEvent_t b);
 
 struct ProfScope {
@@ -91,7 +93,7 @@ struct ProfScope {
   hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(const char *nm, hipStream_t s) : name(nm), st(s) {
-    if (prof_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+    if (prof_wants(name) && prof_take(&a, &b))
       (void)hipEventRecord(a, st);
     else
       a = b = nullptr;

@@ -1,4 +1,5 @@
 // Status handling, version string and FFT plan construction for libtomohip.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -55,11 +56,34 @@ int FftPlan::init(int N) {
 }
 
 static std::mutex g_prof_mu;
-static bool g_prof_on = false;
+static std::atomic<bool> g_prof_on{false};
+static std::string g_prof_only;  // empty: every kernel
 static std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> g_prof_pending;
+static std::vector<hipEvent_t> g_prof_pool;
 static std::map<std::string, std::pair<double, int64_t>> g_prof_acc;
 
-bool prof_enabled() { return g_prof_on; }
+bool prof_wants(const char *name) {
+  if (!g_prof_on.load(std::memory_order_relaxed)) return false;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  return g_prof_only.empty() || g_prof_only == name;
+}
+
+bool prof_take(hipEvent_t *a, hipEvent_t *b) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  hipEvent_t e[2];
+  for (auto &x : e) {
+    if (!g_prof_pool.empty()) {
+      x = g_prof_pool.back();
+      g_prof_pool.pop_back();
+    } else if (hipEventCreate(&x) != hipSuccess) {
+      if (&x != e) g_prof_pool.push_back(e[0]);
+      return false;
+    }
+  }
+  *a = e[0];
+  *b = e[1];
+  return true;
+}
 
 void prof_push(const char *name, hipEvent_t a, hipEvent_t b) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -77,8 +101,8 @@ static int prof_drain() {
       acc.first += ms;
       acc.second += 1;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    g_prof_pool.push_back(a);
+    g_prof_pool.push_back(b);
   }
   g_prof_pending.clear();
   return TOMO_OK;
@@ -87,8 +111,13 @@ static int prof_drain() {
 }  // namespace tomo
 
 extern "C" int tomo_prof_enable(int on) {
+  tomo::g_prof_on.store(on != 0);
+  return TOMO_OK;
+}
+
+extern "C" int tomo_prof_only(const char *kernel) {
   std::lock_guard<std::mutex> lk(tomo::g_prof_mu);
-  tomo::g_prof_on = on != 0;
+  tomo::g_prof_only = kernel ? kernel : "";
   return TOMO_OK;
 }
 

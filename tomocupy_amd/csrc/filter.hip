@@ -111,36 +111,58 @@ __global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const floa
 // the C2R of w[k] = wfilter[k] exp(-2 pi i k shift/ne) (imaginary parts of the
 // DC and Nyquist bins dropped) is
 //   h[d] = wf[0] + (-1)^d wf[ne/2] cos(pi shift) + 2 sum_{0<k<ne/2} wf[k] cos(k phi_d),
-// summed with a rotating phasor (error ~ ne/2 ulp of FP64).
+// summed with a rotating phasor.
+// Each sum is split into kSeg contiguous segments, one thread each, whose
+// rotating phasor is seeded from an exactly reduced phase; the segment sums
+// are added in order.
+constexpr int kSeg = 16;
+
 __device__ __forceinline__ double flt_shift(const float *sht, int z, double center, int n) {
   return -center + (sht ? (double)sht[z] : 0.0) + (double)(n / 2);
 }
 
-__global__ __launch_bounds__(256) void k_flt_h64(const float *__restrict__ wfilter,
-                                                 const float *__restrict__ sht,
-                                                 double *__restrict__ h, int ne, int n,
-                                                 double center) {
-  const int d = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
-  if (d >= ne) return;
+__global__ __launch_bounds__(1024) void k_flt_h64(const float *__restrict__ wfilter,
+                                                  const float *__restrict__ sht,
+                                                  double *__restrict__ h, int ne, int n,
+                                                  double center) {
+  __shared__ double part[kSeg][64];
+  const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+  const int d = blockIdx.x * 64 + lane, z = blockIdx.y;
   const int nh = ne / 2;
   const double shift = flt_shift(sht, z, center, n);
-  double f = ((double)d - shift) / (double)ne;  // phi_d / (2 pi), reduced mod 1
-  f -= std::floor(f);
-  double sn, cs;
-  sincospi(2.0 * f, &sn, &cs);
-  double rr = cs, ri = sn;  // exp(i k phi_d) for k = 1
+  const int len = (nh - 1 + kSeg - 1) / kSeg;
+  const int k0 = 1 + s * len, k1 = min(nh, k0 + len);
   double acc = 0.0;
-  for (int k = 1; k < nh; ++k) {
-    acc += (double)wfilter[k] * rr;
-    const double t = rr * cs - ri * sn;
-    ri = rr * sn + ri * cs;
-    rr = t;
+  if (d < ne && k0 < k1) {
+    // phase k0 (d - shift)/ne mod 1, with k0 d reduced in integers
+    double fs = (double)k0 * shift / (double)ne;
+    fs -= std::floor(fs);
+    double f = (double)(((long long)k0 * d) % ne) / (double)ne - fs;
+    f -= std::floor(f);
+    double rs, rc;
+    sincospi(2.0 * f, &rs, &rc);
+    double g = ((double)d - shift) / (double)ne;  // the step phi_d / (2 pi)
+    g -= std::floor(g);
+    double sn, cs;
+    sincospi(2.0 * g, &sn, &cs);
+    for (int k = k0; k < k1; ++k) {
+      acc += (double)wfilter[k] * rc;
+      const double t = rc * cs - rs * sn;
+      rs = rc * sn + rs * cs;
+      rc = t;
+    }
   }
+  part[s][lane] = acc;
+  __syncthreads();
+  if (s != 0 || d >= ne) return;
+  double sum = 0.0;
+#pragma unroll
+  for (int q = 0; q < kSeg; ++q) sum += part[q][lane];
   double g = shift;  // cos(pi shift) with the argument reduced mod 2
   g -= 2.0 * std::floor(g / 2.0);
   const double cnyq = cospi(g);
-  h[(size_t)z * ne + d] = (double)wfilter[0] + ((d & 1) ? -1.0 : 1.0) * (double)wfilter[nh] * cnyq +
-                          2.0 * acc;
+  h[(size_t)z * ne + d] =
+      (double)wfilter[0] + ((d & 1) ? -1.0 : 1.0) * (double)wfilter[nh] * cnyq + 2.0 * sum;
 }
 
 // FP64 prefix sums P[d] = sum_{e<d} h[e] (one workgroup per slice), then the
@@ -183,25 +205,42 @@ __global__ __launch_bounds__(1024) void k_flt_edges64(const double *__restrict__
 }
 
 // H2[k] = FFT_2n(h2)[k] / (2n), h2 = h on lags (-n, n) wrapped to length 2n
-// (h2[n] = 0); FP64 sums with a rotating phasor, rounded once.
-__global__ __launch_bounds__(256) void k_flt_H2_64(const double *__restrict__ h,
-                                                   float2 *__restrict__ H2, int ne, int n) {
-  const int k = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
+// (h2[n] = 0); FP64 segment sums as above, rounded once.
+__global__ __launch_bounds__(1024) void k_flt_H2_64(const double *__restrict__ h,
+                                                    float2 *__restrict__ H2, int ne, int n) {
+  __shared__ double pre[kSeg][64], pim[kSeg][64];
+  const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + lane, z = blockIdx.y;
   const int n2 = 2 * n;
-  if (k >= n2) return;
   const double *hz = h + (size_t)z * ne;
-  double sn, cs;
-  sincospi(-2.0 * (double)k / (double)n2, &sn, &cs);
-  double rr = 1.0, ri = 0.0, ar = 0.0, ai = 0.0;
-  for (int d = 0; d < n2; ++d) {
-    const double v = d < n ? hz[d] : (d > n ? hz[ne - n2 + d] : 0.0);
-    ar += v * rr;
-    ai += v * ri;
-    const double t = rr * cs - ri * sn;
-    ri = rr * sn + ri * cs;
-    rr = t;
+  const int len = (n2 + kSeg - 1) / kSeg;
+  const int d0 = s * len, d1 = min(n2, d0 + len);
+  double ar = 0.0, ai = 0.0;
+  if (k < n2 && d0 < d1) {
+    double ri, rr;
+    sincospi(-2.0 * (double)(((long long)k * d0) % n2) / (double)n2, &ri, &rr);
+    double sn, cs;
+    sincospi(-2.0 * (double)k / (double)n2, &sn, &cs);
+    for (int d = d0; d < d1; ++d) {
+      const double v = d < n ? hz[d] : (d > n ? hz[ne - n2 + d] : 0.0);
+      ar += v * rr;
+      ai += v * ri;
+      const double t = rr * cs - ri * sn;
+      ri = rr * sn + ri * cs;
+      rr = t;
+    }
   }
-  H2[(size_t)z * n2 + k] = make_float2((float)(ar / n2), (float)(ai / n2));
+  pre[s][lane] = ar;
+  pim[s][lane] = ai;
+  __syncthreads();
+  if (s != 0 || k >= n2) return;
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int q = 0; q < kSeg; ++q) {
+    sr += pre[q][lane];
+    si += pim[q][lane];
+  }
+  H2[(size_t)z * n2 + k] = make_float2((float)(sr / n2), (float)(si / n2));
 }
 
 }  // namespace
@@ -213,20 +252,20 @@ int tomo::flt_prepare_2n(tomo_flt_plan *P, const float *wfilter, double center, 
                (long long)P->ne);
   const int ne = (int)P->ne, n = (int)P->n, nz = (int)P->nz;
   {
-    ProfScope ps_("k_flt_kernel2n", st);
-    hipLaunchKernelGGL(k_flt_h64, dim3((unsigned)ceil_div(ne, 256), nz), dim3(256), 0, st,
+    ProfScope ps_("k_flt_prepare", st);
+    hipLaunchKernelGGL(k_flt_h64, dim3((unsigned)ceil_div(ne, 64), nz), dim3(1024), 0, st,
                        wfilter, sht, P->Hd.as<double>(), ne, n, center);
   }
   TOMO_LAUNCH_CHECK();
   {
-    ProfScope ps_("k_flt_kernel2n", st);
+    ProfScope ps_("k_flt_prepare", st);
     hipLaunchKernelGGL(k_flt_edges64, dim3(nz), dim3(1024), 0, st, P->Hd.as<double>(),
                        P->P.as<double>(), P->HL.as<float>(), P->HR.as<float>(), ne, n);
   }
   TOMO_LAUNCH_CHECK();
   {
-    ProfScope ps_("k_flt_kernel2n", st);
-    hipLaunchKernelGGL(k_flt_H2_64, dim3((unsigned)ceil_div(2 * n, 256), nz), dim3(256), 0, st,
+    ProfScope ps_("k_flt_prepare", st);
+    hipLaunchKernelGGL(k_flt_H2_64, dim3((unsigned)ceil_div(2 * n, 64), nz), dim3(1024), 0, st,
                        P->Hd.as<double>(), P->H2.as<float2>(), ne, n);
   }
   TOMO_LAUNCH_CHECK();

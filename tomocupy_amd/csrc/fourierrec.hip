@@ -963,7 +963,7 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
   const int n2 = 2 * n;
   cpx<T> *G = P->G.as<cpx<T>>(), *F = P->F.as<cpx<T>>(), *R = P->R.as<cpx<T>>();
   {
-    ProfScope ps_("k_fr_grid", st);
+    ProfScope ps_(std::is_same<T, float>::value ? "k_fr_grid" : "k_fr_grid16", st);
     const dim3 gg((unsigned)ceil_div(P->nitems, kGridWaves), ng);
     if constexpr (std::is_same<T, float>::value)
       hipLaunchKernelGGL(k_fr_grid<T>, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),

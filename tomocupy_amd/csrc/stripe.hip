@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "common.h"
 #include "preproc.h"
@@ -49,21 +50,65 @@ struct DarkFlat {
   int flat_linear;
 };
 
+// Compile-time filter lengths of the supported wavelets (haar 2, db5 and sym5
+// 10, sym16 32): the tap loops unroll, the taps are kernel-argument constants
+// held in scalar registers, and each thread reuses one set of staged inputs
+// for several outputs.
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename F>
+void with_len(int L, F &&f) {
+  if (L == 2)
+    f(IC<2>());
+  else if (L == 10)
+    f(IC<10>());
+  else
+    f(IC<32>());
+}
+
+template <typename F>
+void with_km(int nk, F &&f) {
+  if (nk <= 8)
+    f(IC<8>());
+  else if (nk <= 16)
+    f(IC<16>());
+  else
+    f(IC<32>());
+}
+
+template <typename F>
+void with_r(int R, F &&f) {
+  if (R >= 8)
+    f(IC<8>());
+  else if (R >= 4)
+    f(IC<4>());
+  else
+    f(IC<2>());
+}
+
+// LDS bytes of a row kernel's staged row: `pairs` float2 entries plus the
+// pair that the last thread's float4 reads may touch beyond them.
+inline size_t row_lds_bytes(int pairs) { return sizeof(float4) * (size_t)((pairs + 3) / 2 + 1); }
+
 // Analysis along the detector axis (rows), one workgroup per (row, slice).
 // MODE 0 reads the chunk layout res[p][z][x] with the zero angle padding;
 // MODE 1 reads a band (z, H, W); MODE 2 reads the raw uint16 chunk
 // data[p][z][x] and applies the dark/flat correction on the fly (the fused
-// form of darkflat_correction followed by the first level).  The reflected row x is staged in LDS and
-// read as pairs (x[2k], x[2k+1]): lo[o] = sum_m a[2m] x[2(o+m)] + a[2m+1] x[2(o+m)+1].
-template <int MODE>
+// form of darkflat_correction followed by the first level).  The reflected
+// row x is staged in LDS and read as pairs (x[2k], x[2k+1]):
+//   lo[o] = sum_m a[2m] x[2(o+m)] + a[2m+1] x[2(o+m)+1].
+// A thread produces outputs o, o+1 (o even) from the pairs o .. o+L/2, read as
+// float4 (two pairs, consecutive lanes on consecutive 16-byte words).
+template <int MODE, int L>
 __global__ __launch_bounds__(256) void k_fw_rows(const void *__restrict__ src_,
                                                  float *__restrict__ lo, float *__restrict__ hi,
                                                  Taps t, int H, int W, int Wo, int nz, int nproj,
                                                  int st, DarkFlat df) {
-  extern __shared__ float2 xs2[];
-  float *xs = reinterpret_cast<float *>(xs2);
+  extern __shared__ float4 xs4[];
+  float *xs = reinterpret_cast<float *>(xs4);
+  constexpr int L2 = L / 2, NQ = (L2 + 2) / 2;
   const int h = blockIdx.x, z = blockIdx.y;
-  const int L = t.L;
   const int pad = (2 * (Wo - 1) - W + L) / 2;
   const int np = 2 * Wo + L - 2;
   const float *row = nullptr;
@@ -107,106 +152,122 @@ __global__ __launch_bounds__(256) void k_fw_rows(const void *__restrict__ src_,
   __syncthreads();
   float *plo = lo + ((size_t)z * H + h) * Wo;
   float *phi = hi + ((size_t)z * H + h) * Wo;
-  const int L2 = L >> 1;
-  for (int o = threadIdx.x; o < Wo; o += 256) {
-    float a = 0.f, b = 0.f;
-    for (int m = 0; m < L2; ++m) {
-      const float2 x = xs2[o + m];
-      a = fmaf(t.a[2 * m], x.x, a);
-      a = fmaf(t.a[2 * m + 1], x.y, a);
-      b = fmaf(t.b[2 * m], x.x, b);
-      b = fmaf(t.b[2 * m + 1], x.y, b);
+  for (int q = threadIdx.x; 2 * q < Wo; q += 256) {
+    float2 x[2 * NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const float4 v = xs4[q + u];
+      x[2 * u] = make_float2(v.x, v.y);
+      x[2 * u + 1] = make_float2(v.z, v.w);
     }
-    plo[o] = a;
-    phi[o] = b;
+    float a0 = 0.f, b0 = 0.f, a1 = 0.f, b1 = 0.f;
+#pragma unroll
+    for (int m = 0; m < L2; ++m) {
+      a0 = fmaf(t.a[2 * m], x[m].x, a0);
+      a0 = fmaf(t.a[2 * m + 1], x[m].y, a0);
+      b0 = fmaf(t.b[2 * m], x[m].x, b0);
+      b0 = fmaf(t.b[2 * m + 1], x[m].y, b0);
+      a1 = fmaf(t.a[2 * m], x[m + 1].x, a1);
+      a1 = fmaf(t.a[2 * m + 1], x[m + 1].y, a1);
+      b1 = fmaf(t.b[2 * m], x[m + 1].x, b1);
+      b1 = fmaf(t.b[2 * m + 1], x[m + 1].y, b1);
+    }
+    const int o = 2 * q;
+    plo[o] = a0;
+    phi[o] = b0;
+    if (o + 1 < Wo) {
+      plo[o + 1] = a1;
+      phi[o + 1] = b1;
+    }
   }
 }
 
 // Analysis along the angle axis (columns) of both row bands.  A workgroup
-// produces `ot` (<= kColOut) output rows of 64 columns; the 2*ot+L-2 input
-// rows it needs (reflected at the borders) are staged once in LDS, with every
-// load of a thread issued before the first LDS store.  It also accumulates the
-// partial DFT X_k (k < nk) of its rows of the stripe band hl along angles,
-// written per row block for the notch (k_fw_xsum).
-constexpr int kColOut = 32;
-constexpr int kColIn = 2 * kColOut + 30;  // taps L <= 32
-template <int KM>
+// produces 4R output rows of 64 columns; the 8R+L-2 input rows it needs
+// (reflected at the borders) are staged once in LDS, with every load of a
+// thread issued before the first LDS store.  Each wave produces R consecutive
+// output rows, reading each of its 2R+L-2 staged rows once.  The workgroup
+// also accumulates the partial DFT X_k (k < nk) of its rows of the stripe
+// band hl along angles, written per row block for the notch (k_fw_xsum); the
+// twiddles of a row are the same for the whole wave (scalar loads).
+template <int KM, int L, int R>
 __global__ __launch_bounds__(256) void k_fw_cols(
     const float *__restrict__ lo, const float *__restrict__ hi, float *__restrict__ ll,
     float *__restrict__ lh, float *__restrict__ hl, float *__restrict__ hh, Taps t, int H, int W,
-    int Ho, int ot, const float2 *__restrict__ tw, float2 *__restrict__ part, int nk) {
-  __shared__ float2 xs[kColIn][64];
-  __shared__ float2 twl[KM * kColOut];  // twiddles of this block's rows
-  static_assert(2 * KM <= kColIn, "partial-sum reduction must fit the staging LDS");
-  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    int Ho, const float2 *__restrict__ tw, float2 *__restrict__ part, int nk) {
+  constexpr int OT = 4 * R, NIN = 2 * OT + L - 2;
+  constexpr int NS = NIN > 2 * KM ? NIN : 2 * KM;  // staging, then the reduction
+  __shared__ float2 xs[NS][64];
+  const int lane = threadIdx.x & 63;
+  const int sub = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = blockIdx.x * 64 + lane;
-  const int rb = blockIdx.y, oh0 = rb * ot, z = blockIdx.z;
+  const int rb = blockIdx.y, oh0 = rb * OT, z = blockIdx.z;
   const int nrb = gridDim.y;
-  const int L = t.L;
   const int pad = (2 * (Ho - 1) - H + L) / 2;
-  const int nin = 2 * ot + L - 2;
   const int base = 2 * oh0 - pad;
   const bool live = w < W;
   const float *plo = lo + (size_t)z * H * W + (live ? w : 0);
   const float *phi = hi + (size_t)z * H * W + (live ? w : 0);
-  constexpr int kIt = (kColIn + 3) / 4;
-  constexpr int kTw = (KM * kColOut + 255) / 256;
-  float2 twv[kTw];
-#pragma unroll
-  for (int u = 0; u < kTw; ++u) {
-    const int idx = threadIdx.x + 256 * u, k = idx / kColOut, q = idx % kColOut;
-    twv[u] = (k < nk && q < ot && oh0 + q < Ho) ? tw[k * Ho + oh0 + q] : make_float2(0.f, 0.f);
-  }
+  constexpr int kIt = (NIN + 3) / 4;
   float2 v[kIt];
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
     const int i = sub + 4 * it;
     v[it] = make_float2(0.f, 0.f);
-    if (live && i < nin) {
+    if (live && i < NIN) {
       const size_t r = (size_t)refl_fast(base + i, H) * W;
       v[it] = make_float2(plo[r], phi[r]);
     }
   }
 #pragma unroll
   for (int it = 0; it < kIt; ++it)
-    if (sub + 4 * it < nin) xs[sub + 4 * it][lane] = v[it];
-#pragma unroll
-  for (int u = 0; u < kTw; ++u)
-    if (threadIdx.x + 256 * u < KM * kColOut) twl[threadIdx.x + 256 * u] = twv[u];
+    if (sub + 4 * it < NIN) xs[sub + 4 * it][lane] = v[it];
   __syncthreads();
+  const int q0 = sub * R;
+  float a0[R], a1[R], b0[R], b1[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) a0[r] = a1[r] = b0[r] = b1[r] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 2 * R + L - 2; ++i) {
+    const float2 x = xs[2 * q0 + i][lane];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = i - 2 * r;
+      if (j >= 0 && j < L) {
+        a0[r] = fmaf(t.a[j], x.x, a0[r]);
+        a1[r] = fmaf(t.b[j], x.x, a1[r]);
+        b0[r] = fmaf(t.a[j], x.y, b0[r]);
+        b1[r] = fmaf(t.b[j], x.y, b1[r]);
+      }
+    }
+  }
   float xr[KM], xi[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) xr[k] = xi[k] = 0.f;
-  for (int q = sub; q < ot; q += 4) {
-    const int oh = oh0 + q;
-    if (oh >= Ho) break;
-    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-    for (int j = 0; j < L; ++j) {
-      const float2 x = xs[2 * q + j][lane];
-      a0 = fmaf(t.a[j], x.x, a0);
-      a1 = fmaf(t.b[j], x.x, a1);
-      b0 = fmaf(t.a[j], x.y, b0);
-      b1 = fmaf(t.b[j], x.y, b1);
-    }
 #pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < nk) {
-        const float2 c = twl[k * kColOut + q];
-        xr[k] = fmaf(b0, c.x, xr[k]);
-        xi[k] = fmaf(-b0, c.y, xi[k]);
+  for (int r = 0; r < R; ++r) {
+    const int oh = oh0 + q0 + r;
+    if (oh < Ho) {
+      if (live) {
+        const size_t o = ((size_t)z * Ho + oh) * W + w;
+        ll[o] = a0[r];
+        lh[o] = a1[r];
+        hl[o] = b0[r];
+        hh[o] = b1[r];
       }
-    }
-    if (live) {
-      const size_t o = ((size_t)z * Ho + oh) * W + w;
-      ll[o] = a0;
-      lh[o] = a1;
-      hl[o] = b0;
-      hh[o] = b1;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        if (k < nk) {
+          const float2 c = tw[k * Ho + oh];
+          xr[k] = fmaf(b0[r], c.x, xr[k]);
+          xi[k] = fmaf(-b0[r], c.y, xi[k]);
+        }
+      }
     }
   }
   if (nk == 0) return;
-  // reduce the four row groups' partial sums through the (now free) staging
-  // LDS: groups 2,3 hand to 0,1, then 1 hands to 0
+  // reduce the four waves' partial sums through the (now free) staging LDS:
+  // waves 2,3 hand to 0,1, then 1 hands to 0
   float2 *red = &xs[0][0];
   __syncthreads();
   if (sub >= 2) {
@@ -266,25 +327,22 @@ __global__ __launch_bounds__(256) void k_fw_xsum(const float2 *__restrict__ part
 // notched stripe band, corrected while it is staged:
 //   hl'[i] = hl[i] - sum_k Re(X_k e^{2 pi i k i/H}).
 // With t = t' + L - 2 even (L even), outputs t and t+1 read the same band
-// rows i = t/2 - m, m < L/2, through the even and the odd taps, so a thread
-// produces output rows in pairs.  A workgroup produces `ot` (<= kSynOut) rows
-// of 64 columns from ot/2 + L/2 - 1 band rows staged in LDS.
-constexpr int kSynOut = 64;
-constexpr int kSynIn = kSynOut / 2 + 15;  // taps L <= 32
-template <int KM>
+// rows i = t/2 - m, m < L/2, through the even and the odd taps, so outputs
+// come in pairs.  A workgroup produces 4R pairs (8R rows) of 64 columns from
+// 4R + L/2 - 1 band rows staged in LDS; each wave produces R consecutive
+// pairs, reading each of its R + L/2 - 1 staged rows once.
+template <int KM, int L, int R>
 __global__ __launch_bounds__(256) void k_fw_syn_cols(
     const float *__restrict__ ll, int llH, int llW, const float *__restrict__ lh,
     const float *__restrict__ hl, const float *__restrict__ hh, float *__restrict__ lo,
-    float *__restrict__ hi, Taps g, int H, int W, int Hs, int ot, const float2 *__restrict__ tw,
+    float *__restrict__ hi, Taps g, int H, int W, int Hs, const float2 *__restrict__ tw,
     const float2 *__restrict__ X, int nk) {
-  __shared__ float4 xs[kSynIn][64];
-  __shared__ float2 twl[KM * kSynIn];  // twiddles of the staged band rows
-  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  constexpr int L2 = L / 2, PT = 4 * R, NIN = PT + L2 - 1;
+  __shared__ float4 xs[NIN][64];
+  const int lane = threadIdx.x & 63;
+  const int sub = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = blockIdx.x * 64 + lane;
-  const int tp0 = blockIdx.y * ot, z = blockIdx.z;
-  const int L2 = g.L >> 1;
-  const int i0 = tp0 >> 1;
-  const int nin = ot / 2 + L2 - 1;
+  const int i0 = blockIdx.y * PT, tp0 = 2 * i0, z = blockIdx.z;
   const bool live = w < W;
   float xr[KM], xi[KM];
 #pragma unroll
@@ -293,38 +351,27 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
     xr[k] = x.x;
     xi[k] = x.y;
   }
-  constexpr int kIt = (kSynIn + 3) / 4;
-  constexpr int kTw = (KM * kSynIn + 255) / 256;
-  float2 twv[kTw];
-#pragma unroll
-  for (int u = 0; u < kTw; ++u) {
-    const int idx = threadIdx.x + 256 * u, k = idx / kSynIn, q = idx % kSynIn;
-    twv[u] = (k < nk && q < nin && i0 + q < H) ? tw[k * H + i0 + q] : make_float2(0.f, 0.f);
-  }
+  constexpr int kIt = (NIN + 3) / 4;
   float4 v[kIt];
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
     const int q = sub + 4 * it, i = i0 + q;
     v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (live && q < nin && i < H) {
+    if (live && q < NIN && i < H) {
       const size_t ob = ((size_t)z * H + i) * W + w;
       v[it] = make_float4(ll[((size_t)z * llH + i) * llW + w], lh[ob], hl[ob], hh[ob]);
     }
   }
 #pragma unroll
-  for (int u = 0; u < kTw; ++u)
-    if (threadIdx.x + 256 * u < KM * kSynIn) twl[threadIdx.x + 256 * u] = twv[u];
-  __syncthreads();
-#pragma unroll
   for (int it = 0; it < kIt; ++it) {
-    const int q = sub + 4 * it;
-    if (q < nin) {
-      if (i0 + q < H) {
+    const int q = sub + 4 * it, i = i0 + q;
+    if (q < NIN) {
+      if (i < H) {
         float c = v[it].z;
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
           if (k < nk) {
-            const float2 e = twl[k * kSynIn + q];
+            const float2 e = tw[k * H + i];
             c -= xr[k] * e.x - xi[k] * e.y;
           }
         }
@@ -335,43 +382,60 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
   }
   __syncthreads();
   if (!live) return;
-  for (int pq = sub; pq < ot / 2; pq += 4) {
-    const int tp = tp0 + 2 * pq;
-    if (tp >= Hs) break;
-    float a0 = 0.f, b0 = 0.f, a1 = 0.f, b1 = 0.f;
-    for (int m = 0; m < L2; ++m) {
-      const float4 x = xs[pq + L2 - 1 - m][lane];
-      const float ge = g.a[2 * m], he = g.b[2 * m], go = g.a[2 * m + 1], ho = g.b[2 * m + 1];
-      a0 = fmaf(ge, x.x, a0);
-      a0 = fmaf(he, x.y, a0);
-      b0 = fmaf(ge, x.z, b0);
-      b0 = fmaf(he, x.w, b0);
-      a1 = fmaf(go, x.x, a1);
-      a1 = fmaf(ho, x.y, a1);
-      b1 = fmaf(go, x.z, b1);
-      b1 = fmaf(ho, x.w, b1);
+  const int pq0 = sub * R;
+  float a0[R], b0[R], a1[R], b1[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) a0[r] = b0[r] = a1[r] = b1[r] = 0.f;
+  // pair pq0+r takes rows pq0+r+L2-1-m for m = 0, 1, ... (in that order)
+#pragma unroll
+  for (int i = R + L2 - 2; i >= 0; --i) {
+    const float4 x = xs[pq0 + i][lane];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = r + L2 - 1 - i;
+      if (m >= 0 && m < L2) {
+        const float ge = g.a[2 * m], he = g.b[2 * m], go = g.a[2 * m + 1], ho = g.b[2 * m + 1];
+        a0[r] = fmaf(ge, x.x, a0[r]);
+        a0[r] = fmaf(he, x.y, a0[r]);
+        b0[r] = fmaf(ge, x.z, b0[r]);
+        b0[r] = fmaf(he, x.w, b0[r]);
+        a1[r] = fmaf(go, x.x, a1[r]);
+        a1[r] = fmaf(ho, x.y, a1[r]);
+        b1[r] = fmaf(go, x.z, b1[r]);
+        b1[r] = fmaf(ho, x.w, b1[r]);
+      }
     }
-    const size_t o = ((size_t)z * Hs + tp) * W + w;
-    lo[o] = a0;
-    lo[o + W] = a1;
-    hi[o] = b0;
-    hi[o + W] = b1;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int tp = tp0 + 2 * (pq0 + r);
+    if (tp < Hs) {  // Hs is even, so row tp + 1 exists too
+      const size_t o = ((size_t)z * Hs + tp) * W + w;
+      lo[o] = a0[r];
+      lo[o + W] = a1[r];
+      hi[o] = b0[r];
+      hi[o + W] = b1[r];
+    }
   }
 }
 
 // Synthesis along the detector axis, one workgroup per output row with both
-// input rows staged in LDS; outputs in pairs as in k_fw_syn_cols.  LAST = 1
+// input rows staged in LDS as (lo, hi) pairs; outputs 2p, 2p+1 read the pairs
+// p + L/2 - 1 - m (m < L/2) through the even and odd taps.  A thread produces
+// outputs 4u .. 4u+3 from the pairs 2u .. 2u+L/2, read as float4.  LAST = 1
 // writes rows [st, st+nproj) and columns [0, n) straight into the chunk
 // layout res[p][z][x]; LAST = 2 writes them in sinogram order out[z][p][x]
 // as TO, after minus_log when take_log (the fused form of proc_proj and the
 // swap of rec.py:174).
-template <int LAST, typename TO>
+template <int LAST, typename TO, int L>
 __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ lo,
                                                      const float *__restrict__ hi,
                                                      TO *__restrict__ out, Taps g, int Hs, int W,
                                                      int Ws, int nz, int nproj, int st, int n,
                                                      int take_log) {
-  extern __shared__ float2 rs[];
+  extern __shared__ float4 rs4[];
+  float2 *rs = reinterpret_cast<float2 *>(rs4);
+  constexpr int L2 = L / 2, NQ = (L2 + 2) / 2;
   const int h = blockIdx.x, z = blockIdx.y;
   if (LAST && (h < st || h >= st + nproj)) return;
   const float *pl = lo + ((size_t)z * Hs + h) * W;
@@ -389,26 +453,36 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
       if (i0 + u * 256 < W) rs[i0 + u * 256] = v[u];
   }
   __syncthreads();
-  const int L2 = g.L >> 1;
   const int nout = LAST ? min(Ws, n) : Ws;
   TO *orow = LAST == 2   ? out + ((size_t)z * nproj + (h - st)) * n
              : LAST == 1 ? out + ((size_t)(h - st) * nz + z) * n
                          : out + ((size_t)z * Hs + h) * Ws;
-  for (int p = threadIdx.x; 2 * p < nout; p += 256) {
-    float a0 = 0.f, a1 = 0.f;
+  for (int u = threadIdx.x; 4 * u < nout; u += 256) {
+    float2 x[2 * NQ];
+#pragma unroll
+    for (int e = 0; e < NQ; ++e) {
+      const float4 v = rs4[u + e];
+      x[2 * e] = make_float2(v.x, v.y);
+      x[2 * e + 1] = make_float2(v.z, v.w);
+    }
+    float y[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int m = 0; m < L2; ++m) {
-      const float2 x = rs[p + L2 - 1 - m];
-      a0 = fmaf(g.a[2 * m], x.x, a0);
-      a0 = fmaf(g.b[2 * m], x.y, a0);
-      a1 = fmaf(g.a[2 * m + 1], x.x, a1);
-      a1 = fmaf(g.b[2 * m + 1], x.y, a1);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {  // pair 2u + s
+        const float2 xv = x[s + L2 - 1 - m];
+        y[2 * s] = fmaf(g.a[2 * m], xv.x, y[2 * s]);
+        y[2 * s] = fmaf(g.b[2 * m], xv.y, y[2 * s]);
+        y[2 * s + 1] = fmaf(g.a[2 * m + 1], xv.x, y[2 * s + 1]);
+        y[2 * s + 1] = fmaf(g.b[2 * m + 1], xv.y, y[2 * s + 1]);
+      }
     }
-    if (LAST == 2 && take_log) {
-      a0 = minus_log(a0);
-      a1 = minus_log(a1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float yc = y[c];
+      if (LAST == 2 && take_log) yc = minus_log(yc);
+      if (4 * u + c < nout) orow[4 * u + c] = (TO)yc;
     }
-    orow[2 * p] = (TO)a0;
-    if (2 * p + 1 < nout) orow[2 * p + 1] = (TO)a1;
   }
 }
 
@@ -529,52 +603,50 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
   const int st0 = (npad - nproj) / 2;
   for (int k = 0; k < level; ++k) {
     FwLevel &l = P->lv[k];
-    const int lds = (2 * l.Wo + L - 2) * (int)sizeof(float);
-    {
+    const size_t lds = row_lds_bytes(l.Wo + L / 2 - 1);
+    with_len(L, [&](auto lc) {
+      constexpr int LC = decltype(lc)::value;
       ProfScope ps_("k_fw_rows", st);
       if (k > 0)
-        hipLaunchKernelGGL(k_fw_rows<1>, dim3(l.H, nz), dim3(256), lds, st,
+        hipLaunchKernelGGL((k_fw_rows<1, LC>), dim3(l.H, nz), dim3(256), lds, st,
                            (const void *)P->lv[k - 1].ll.as<float>(), P->rowlo.as<float>(),
                            P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz, nproj, st0, df);
       else if (src_mode == 2)
-        hipLaunchKernelGGL(k_fw_rows<2>, dim3(l.H, nz), dim3(256), lds, st, src,
+        hipLaunchKernelGGL((k_fw_rows<2, LC>), dim3(l.H, nz), dim3(256), lds, st, src,
                            P->rowlo.as<float>(), P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz,
                            nproj, st0, df);
       else
-        hipLaunchKernelGGL(k_fw_rows<0>, dim3(l.H, nz), dim3(256), lds, st, src,
+        hipLaunchKernelGGL((k_fw_rows<0, LC>), dim3(l.H, nz), dim3(256), lds, st, src,
                            P->rowlo.as<float>(), P->rowhi.as<float>(), ana, l.H, l.W, l.Wo, nz,
                            nproj, st0, df);
-    }
+    });
     TOMO_LAUNCH_CHECK();
-    const int otc = l.Ho >= 256 ? kColOut : (l.Ho >= 64 ? 16 : 8);
-    const int nrb = (int)ceil_div(l.Ho, otc);
+    // 4R output rows per workgroup: R = 8 for tall bands, fewer (more
+    // workgroups) for the short bands of the deep levels
+    const int Rc = l.Ho >= 256 ? 8 : (l.Ho >= 64 ? 4 : 2);
+    const int nrb = (int)ceil_div(l.Ho, 4 * Rc);
     const dim3 gc((unsigned)ceil_div(l.Wo, 64), nrb, nz);
     float2 *part = P->notch.as<float2>();
     float2 *Xk = P->notchX.as<float2>();
-#define TOMO_FW_ANA(KM)                                                                          \
-  do {                                                                                           \
-    {                                                                                            \
-      ProfScope ps_("k_fw_cols", st);                                                            \
-      hipLaunchKernelGGL(k_fw_cols<KM>, gc, dim3(256), 0, st, P->rowlo.as<float>(),              \
-                         P->rowhi.as<float>(), l.ll.as<float>(), l.lh.as<float>(),                \
-                         l.hl.as<float>(), l.hh.as<float>(), ana, l.H, l.Wo, l.Ho, otc,           \
-                         l.tw.as<float2>(), part, l.nk);                                         \
-    }                                                                                            \
-    TOMO_LAUNCH_CHECK();                                                                         \
-    if (l.nk > 0) {                                                                              \
-      ProfScope ps_("k_fw_xsum", st);                                                            \
-      hipLaunchKernelGGL(k_fw_xsum<KM>, dim3((unsigned)ceil_div(l.Wo, 256), l.nk, nz), dim3(256), \
-                         0, st, part, l.gk.as<float>(), Xk + l.xoff, l.Wo, nrb);                 \
-    }                                                                                            \
-    TOMO_LAUNCH_CHECK();                                                                         \
-  } while (0)
-    if (l.nk <= 8)
-      TOMO_FW_ANA(8);
-    else if (l.nk <= 16)
-      TOMO_FW_ANA(16);
-    else
-      TOMO_FW_ANA(32);
-#undef TOMO_FW_ANA
+    with_km(l.nk, [&](auto kmc) {
+      constexpr int KM = decltype(kmc)::value;
+      with_len(L, [&](auto lc) {
+        with_r(Rc, [&](auto rc) {
+          constexpr int LC = decltype(lc)::value, RC = decltype(rc)::value;
+          ProfScope ps_("k_fw_cols", st);
+          hipLaunchKernelGGL((k_fw_cols<KM, LC, RC>), gc, dim3(256), 0, st,
+                             P->rowlo.as<float>(), P->rowhi.as<float>(), l.ll.as<float>(),
+                             l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(), ana, l.H,
+                             l.Wo, l.Ho, l.tw.as<float2>(), part, l.nk);
+        });
+      });
+      if (l.nk > 0) {
+        ProfScope ps_("k_fw_xsum", st);
+        hipLaunchKernelGGL(k_fw_xsum<KM>, dim3((unsigned)ceil_div(l.Wo, 256), l.nk, nz),
+                           dim3(256), 0, st, part, l.gk.as<float>(), Xk + l.xoff, l.Wo, nrb);
+      }
+    });
+    TOMO_LAUNCH_CHECK();
   }
   // synthesis, deepest level first; `cur` is the current low band (pitch curH x curW)
   const float *cur = P->lv[level - 1].ll.as<float>();
@@ -584,47 +656,48 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
     const int Hb = l.Ho, Wb = l.Wo;
     TOMO_REQUIRE(curH >= Hb && curW >= Wb, "pre_remove_stripe_fw: internal band shape mismatch");
     const int Hs = 2 * Hb - L + 2, Ws = 2 * Wb - L + 2;
-    const int ots = Hs >= 512 ? kSynOut : (Hs >= 128 ? 32 : 16);
-    const dim3 gs((unsigned)ceil_div(Wb, 64), (unsigned)ceil_div(Hs, ots), nz);
-    const int KMk = l.nk <= 8 ? 8 : (l.nk <= 16 ? 16 : 32);
+    // 8R output rows per workgroup
+    const int Rs = Hs >= 512 ? 8 : (Hs >= 128 ? 4 : 2);
+    const dim3 gs((unsigned)ceil_div(Wb, 64), (unsigned)ceil_div(Hs, 8 * Rs), nz);
     const float2 *Xk = P->notchX.as<float2>() + l.xoff;
-#define TOMO_FW_SYN(KM)                                                                          \
-  do {                                                                                           \
-    ProfScope ps_("k_fw_syn_cols", st);                                                          \
-    hipLaunchKernelGGL(k_fw_syn_cols<KM>, gs, dim3(256), 0, st, cur, curH, curW,                 \
-                       l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(),                     \
-                       P->synlo.as<float>(), P->synhi.as<float>(), syn, Hb, Wb, Hs, ots,         \
-                       l.tw.as<float2>(), Xk, l.nk);                                             \
-  } while (0)
-    if (KMk == 8)
-      TOMO_FW_SYN(8);
-    else if (KMk == 16)
-      TOMO_FW_SYN(16);
-    else
-      TOMO_FW_SYN(32);
-#undef TOMO_FW_SYN
+    with_km(l.nk, [&](auto kmc) {
+      with_len(L, [&](auto lc) {
+        with_r(Rs, [&](auto rc) {
+          constexpr int KM = decltype(kmc)::value, LC = decltype(lc)::value,
+                        RC = decltype(rc)::value;
+          ProfScope ps_("k_fw_syn_cols", st);
+          hipLaunchKernelGGL((k_fw_syn_cols<KM, LC, RC>), gs, dim3(256), 0, st, cur, curH, curW,
+                             l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(),
+                             P->synlo.as<float>(), P->synhi.as<float>(), syn, Hb, Wb, Hs,
+                             l.tw.as<float2>(), Xk, l.nk);
+        });
+      });
+    });
     TOMO_LAUNCH_CHECK();
     float *out = P->syn[k & 1].as<float>();
     {
-      ProfScope ps_("k_fw_syn_rows", st);
-      const size_t rl = sizeof(float2) * Wb;
+      const size_t rl = row_lds_bytes(Wb);
       const dim3 gr(Hs, nz);
       const float *slo = P->synlo.as<float>(), *shi = P->synhi.as<float>();
-      if (k > 0)
-        hipLaunchKernelGGL((k_fw_syn_rows<0, float>), gr, dim3(256), rl, st, slo, shi, out, syn,
-                           Hs, Wb, Ws, nz, nproj, st0, n, 0);
-      else if (dst_mode == 1)
-        hipLaunchKernelGGL((k_fw_syn_rows<1, float>), gr, dim3(256), rl, st, slo, shi,
-                           (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, 0);
-      else if (out_dtype == TOMO_F16)
-        hipLaunchKernelGGL((k_fw_syn_rows<2, _Float16>), gr, dim3(256), rl, st, slo, shi,
-                           (_Float16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
-      else if (out_dtype == TOMO_BF16)
-        hipLaunchKernelGGL((k_fw_syn_rows<2, __bf16>), gr, dim3(256), rl, st, slo, shi,
-                           (__bf16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
-      else
-        hipLaunchKernelGGL((k_fw_syn_rows<2, float>), gr, dim3(256), rl, st, slo, shi,
-                           (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+      with_len(L, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        ProfScope ps_("k_fw_syn_rows", st);
+        if (k > 0)
+          hipLaunchKernelGGL((k_fw_syn_rows<0, float, LC>), gr, dim3(256), rl, st, slo, shi, out,
+                             syn, Hs, Wb, Ws, nz, nproj, st0, n, 0);
+        else if (dst_mode == 1)
+          hipLaunchKernelGGL((k_fw_syn_rows<1, float, LC>), gr, dim3(256), rl, st, slo, shi,
+                             (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, 0);
+        else if (out_dtype == TOMO_F16)
+          hipLaunchKernelGGL((k_fw_syn_rows<2, _Float16, LC>), gr, dim3(256), rl, st, slo, shi,
+                             (_Float16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+        else if (out_dtype == TOMO_BF16)
+          hipLaunchKernelGGL((k_fw_syn_rows<2, __bf16, LC>), gr, dim3(256), rl, st, slo, shi,
+                             (__bf16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+        else
+          hipLaunchKernelGGL((k_fw_syn_rows<2, float, LC>), gr, dim3(256), rl, st, slo, shi,
+                             (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+      });
     }
     TOMO_LAUNCH_CHECK();
     cur = out;
@@ -648,8 +721,8 @@ const WaveletTaps *check_wavelet(const char *fn, const char *wname, double sigma
     set_error("%s: unknown wavelet '%s'", fn, wname);
     return nullptr;
   }
-  if (wv->len % 2 != 0 || wv->len > 32) {
-    set_error("%s: wavelet '%s' has %d taps (even, at most 32, supported)", fn, wname, wv->len);
+  if (wv->len != 2 && wv->len != 10 && wv->len != 32) {
+    set_error("%s: wavelet '%s' has %d taps (2, 10 or 32 supported)", fn, wname, wv->len);
     return nullptr;
   }
   return wv;

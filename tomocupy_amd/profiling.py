@@ -5,8 +5,11 @@ Usage (paths are rocprofv3 ``-f csv`` outputs)::
   python -m tomocupy_amd.profiling stats  KERNEL_STATS_CSV  [--title T] [--cmd C]
   python -m tomocupy_amd.profiling traffic FETCH_COUNTER_CSV WRITE_COUNTER_CSV
          --copy-bytes B [--key-suffix S] [--json OUT.json] [--title T] [--cmd C]
+  python -m tomocupy_amd.profiling window KERNEL_TRACE_CSV --kernel K --skip N --count M
 
-``stats`` prints a markdown table of the kernel-trace statistics.  ``traffic``
+``stats`` prints a markdown table of the kernel-trace statistics.  ``window``
+prints the mean duration of dispatches N .. N+M-1 (in start order) of the
+kernels whose base name is K, e.g. the launches of one pass of a run.  ``traffic``
 averages ``FETCH_SIZE`` and ``WRITE_SIZE`` (KiB per dispatch) per kernel from
 two ``--pmc`` runs and scales each by the factor that makes the calibration
 copy kernel (the largest device-to-device copy in the trace, ``--copy-bytes``
@@ -20,7 +23,51 @@ import sys
 from collections import defaultdict
 
 
+_MANGLED_ARGS = (('DF16_', '_Float16'), ('DF16b', '__bf16'), ('f', 'float'), ('d', 'double'))
+
+
+def _demangle_kernel(name):
+    """``k_name<args>`` from an Itanium-mangled kernel symbol that the trace
+    left mangled (its demangler does not know _Float16 / __bf16)."""
+    pos, ident = 3 if name.startswith('_ZN') else 2, None
+    while pos < len(name):  # <length><identifier> components
+        m = re.match(r'\d+', name[pos:])
+        if not m:
+            return None
+        ln = int(m.group(0))
+        pos += m.end()
+        comp = name[pos:pos + ln]
+        pos += ln
+        if comp.startswith('k_'):
+            ident = comp
+            break
+    if ident is None:
+        return None
+    rest = name[pos:]
+    if not rest.startswith('I'):
+        return ident
+    rest, args = rest[1:], []
+    while rest and not rest.startswith('E'):
+        lit = re.match(r'Li(-?\d+)E', rest)
+        if lit:
+            args.append(lit.group(1))
+            rest = rest[lit.end():]
+            continue
+        for code, text in _MANGLED_ARGS:
+            if rest.startswith(code):
+                args.append(text)
+                rest = rest[len(code):]
+                break
+        else:
+            return ident
+    return '%s<%s>' % (ident, ', '.join(args))
+
+
 def short_name(name):
+    if name.startswith('_Z'):
+        d = _demangle_kernel(name)
+        if d:
+            return d
     m = re.search(r'(k_[A-Za-z0-9_]+)', name)
     if m:
         base = m.group(1)
@@ -48,6 +95,18 @@ def cmd_stats(a):
             name, r['Calls'], float(r['TotalDurationNs']) / 1e6, float(r['AverageNs']) / 1e3,
             float(r['MinNs']) / 1e3, float(r['MaxNs']) / 1e3, float(r['Percentage'])))
     print('\n'.join(out))
+
+
+def cmd_window(a):
+    rows = [r for r in csv.DictReader(open(a.csv))
+            if short_name(r['Kernel_Name']).split('<')[0] == a.kernel]
+    rows.sort(key=lambda r: int(r['Start_Timestamp']))
+    sel = rows[a.skip:a.skip + a.count]
+    if not sel:
+        sys.exit('no dispatches of %s in [%d, %d)' % (a.kernel, a.skip, a.skip + a.count))
+    d = [(int(r['End_Timestamp']) - int(r['Start_Timestamp'])) / 1e3 for r in sel]
+    print('%s: dispatches %d..%d of %d, mean %.1f us, min %.1f us, max %.1f us'
+          % (a.kernel, a.skip, a.skip + len(sel) - 1, len(rows), sum(d) / len(d), min(d), max(d)))
 
 
 def _per_kernel(path, counter):
@@ -126,9 +185,16 @@ def main(argv=None):
     t.add_argument('--json', default='')
     t.add_argument('--title', default='')
     t.add_argument('--cmd', default='')
+    w = sub.add_parser('window')
+    w.add_argument('csv')
+    w.add_argument('--kernel', required=True)
+    w.add_argument('--skip', type=int, default=0)
+    w.add_argument('--count', type=int, required=True)
     a = ap.parse_args(argv)
     if a.cmd_name == 'stats':
         cmd_stats(a)
+    elif a.cmd_name == 'window':
+        cmd_window(a)
     else:
         cmd_traffic(a)
 
