@@ -15,10 +15,14 @@ namespace tomo {
 struct FwLevel {
   int H = 0, W = 0, Ho = 0, Wo = 0;
   int nk = 0;
-  size_t xoff = 0;  // offset of this level's notch coefficients in notchX (float2)
-  DevBuf tw;   // (nk, Ho) float2: cos/sin(2 pi k t / Ho)
-  DevBuf gk;   // (nk,) float: c_k g_k / Ho
-  DevBuf ll, lh, hl, hh;  // (nz, Ho, Wo) each
+  int KM = 8;   // notch frequencies a kernel instance holds (8, 16 or 32 >= nk)
+  int R = 8;    // output rows per wave of the column analysis
+  int nrb = 0;  // row blocks of the column analysis
+  DevBuf tw;    // (nk, Ho) float2: cos/sin(2 pi k t / Ho)
+  DevBuf gk;    // (nk,) float: c_k g_k / Ho
+  DevBuf ll;    // (nz, Ho, Wo): low band = input of the next level
+  DevBuf part;  // (nz, nrb, KM, Wo) float2: partial notch DFTs of the stripe band
+  DevBuf X;     // (nz, KM, Wo) float2: notch coefficients, kept until the synthesis
 };
 
 // dark/flat correction of one count (proc_functions.py:67-74): flat0 =
@@ -56,5 +60,5 @@ struct tomo_pre_plan {
   double fw_sigma = 0.0;
   int fw_L = 0;
   std::vector<tomo::FwLevel> lv;
-  tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2], notch, notchX;
+  tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2];
 };

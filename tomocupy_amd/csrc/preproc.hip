@@ -207,9 +207,8 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
     l.tw.release();
     l.gk.release();
     l.ll.release();
-    l.lh.release();
-    l.hl.release();
-    l.hh.release();
+    l.part.release();
+    l.X.release();
   }
   P->rowlo.release();
   P->rowhi.release();
@@ -217,7 +216,5 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->synhi.release();
   P->syn[0].release();
   P->syn[1].release();
-  P->notch.release();
-  P->notchX.release();
   delete P;
 }

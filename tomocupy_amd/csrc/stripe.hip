@@ -182,19 +182,30 @@ __global__ __launch_bounds__(256) void k_fw_rows(const void *__restrict__ src_,
   }
 }
 
-// Analysis along the angle axis (columns) of both row bands.  A workgroup
-// produces 4R output rows of 64 columns; the 8R+L-2 input rows it needs
+// The stripe removal in correction form.  Only the detector-high/angle-low
+// band hl of each level is changed (hl' = hl + D_k, D_k the rank-nk notch
+// correction), and the synthesis filter bank is linear and reconstructs the
+// analysis input exactly (half-sample symmetric extension, L-2 crop), so with
+// x_k the input of level k and r_k its reconstruction,
+//   r_k = IDWT(r_{k+1}, lh_k, hl_k + D_k, hh_k) = x_k + IDWT(d_{k+1}, 0, D_k, 0),
+// i.e. d_k = r_k - x_k = IDWT(d_{k+1}, 0, D_k, 0) with d_level = 0, and the
+// output is x_0 + d_0.  The analysis therefore keeps only ll (the next
+// level's input) and the notch DFT of hl; the synthesis filters only the
+// correction, and the last pass adds it to x_0.
+
+// Analysis along the angle axis (columns).  A workgroup produces 4R output
+// rows of 64 columns; the 8R+L-2 input rows of both row bands it needs
 // (reflected at the borders) are staged once in LDS, with every load of a
 // thread issued before the first LDS store.  Each wave produces R consecutive
-// output rows, reading each of its 2R+L-2 staged rows once.  The workgroup
-// also accumulates the partial DFT X_k (k < nk) of its rows of the stripe
-// band hl along angles, written per row block for the notch (k_fw_xsum); the
-// twiddles of a row are the same for the whole wave (scalar loads).
+// output rows, reading each of its 2R+L-2 staged rows once: ll (low-pass of
+// the low row band), stored, and hl (low-pass of the high row band), of which
+// only the partial DFT X_k (k < nk) along angles over the workgroup's rows is
+// kept, written per row block for the synthesis; the twiddles of a row are
+// the same for the whole wave (scalar loads).
 template <int KM, int L, int R>
 __global__ __launch_bounds__(256) void k_fw_cols(
-    const float *__restrict__ lo, const float *__restrict__ hi, float *__restrict__ ll,
-    float *__restrict__ lh, float *__restrict__ hl, float *__restrict__ hh, Taps t, int H, int W,
-    int Ho, const float2 *__restrict__ tw, float2 *__restrict__ part, int nk) {
+    const float *__restrict__ lo, const float *__restrict__ hi, float *__restrict__ ll, Taps t,
+    int H, int W, int Ho, const float2 *__restrict__ tw, float2 *__restrict__ part, int nk) {
   constexpr int OT = 4 * R, NIN = 2 * OT + L - 2;
   constexpr int NS = NIN > 2 * KM ? NIN : 2 * KM;  // staging, then the reduction
   __shared__ float2 xs[NS][64];
@@ -224,9 +235,9 @@ __global__ __launch_bounds__(256) void k_fw_cols(
     if (sub + 4 * it < NIN) xs[sub + 4 * it][lane] = v[it];
   __syncthreads();
   const int q0 = sub * R;
-  float a0[R], a1[R], b0[R], b1[R];
+  float a0[R], b0[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) a0[r] = a1[r] = b0[r] = b1[r] = 0.f;
+  for (int r = 0; r < R; ++r) a0[r] = b0[r] = 0.f;
 #pragma unroll
   for (int i = 0; i < 2 * R + L - 2; ++i) {
     const float2 x = xs[2 * q0 + i][lane];
@@ -235,9 +246,7 @@ __global__ __launch_bounds__(256) void k_fw_cols(
       const int j = i - 2 * r;
       if (j >= 0 && j < L) {
         a0[r] = fmaf(t.a[j], x.x, a0[r]);
-        a1[r] = fmaf(t.b[j], x.x, a1[r]);
         b0[r] = fmaf(t.a[j], x.y, b0[r]);
-        b1[r] = fmaf(t.b[j], x.y, b1[r]);
       }
     }
   }
@@ -248,13 +257,7 @@ __global__ __launch_bounds__(256) void k_fw_cols(
   for (int r = 0; r < R; ++r) {
     const int oh = oh0 + q0 + r;
     if (oh < Ho) {
-      if (live) {
-        const size_t o = ((size_t)z * Ho + oh) * W + w;
-        ll[o] = a0[r];
-        lh[o] = a1[r];
-        hl[o] = b0[r];
-        hh[o] = b1[r];
-      }
+      if (live) ll[((size_t)z * Ho + oh) * W + w] = a0[r];
 #pragma unroll
       for (int k = 0; k < KM; ++k) {
         if (k < nk) {
@@ -299,9 +302,9 @@ __global__ __launch_bounds__(256) void k_fw_cols(
   }
 }
 
-// Notch coefficients of the stripe band: X_k = (sum over row blocks, in
-// order, of the partial DFTs) * gk_k, gk_k = c_k g_k / my.  One thread per
-// (column, k, slice).
+// Notch coefficients of the stripe band: X_k = (sum over the analysis row
+// blocks, in order, of the partial DFTs) * gk_k, gk_k = c_k g_k / my.  One
+// thread per (column, k, slice).
 template <int KM>
 __global__ __launch_bounds__(256) void k_fw_xsum(const float2 *__restrict__ part,
                                                  const float *__restrict__ gk,
@@ -321,24 +324,23 @@ __global__ __launch_bounds__(256) void k_fw_xsum(const float2 *__restrict__ part
   X[((size_t)z * KM + k) * W + w] = make_float2(sr * s, si * s);
 }
 
-// Synthesis along angles: lo = sfb(ll, lh), hi = sfb(hl', hh); output rows
-// t' in [0, 2H-L+2) (the L-2 crop of sfb1d).  ll has its own pitch (llH, llW)
-// because it is the cropped reconstruction of the deeper level.  hl' is the
-// notched stripe band, corrected while it is staged:
-//   hl'[i] = hl[i] - sum_k Re(X_k e^{2 pi i k i/H}).
+// Synthesis along angles of the correction: lo = sfb(d, 0), hi = sfb(D, 0)
+// for output rows t' in [0, 2H-L+2) (the L-2 crop of sfb1d).  d is the
+// correction of the deeper level (pitch dH x dW, NULL at the deepest level:
+// zero).  D is generated while staging from the notch coefficients X_k
+// (k_fw_xsum): D[i] = -sum_k Re(X_k e^{2 pi i k i/H}).
 // With t = t' + L - 2 even (L even), outputs t and t+1 read the same band
-// rows i = t/2 - m, m < L/2, through the even and the odd taps, so outputs
-// come in pairs.  A workgroup produces 4R pairs (8R rows) of 64 columns from
-// 4R + L/2 - 1 band rows staged in LDS; each wave produces R consecutive
+// rows i = t/2 - m, m < L/2, through the even and the odd low-pass taps, so
+// outputs come in pairs.  A workgroup produces 4R pairs (8R rows) of 64
+// columns from 4R + L/2 - 1 staged rows; each wave produces R consecutive
 // pairs, reading each of its R + L/2 - 1 staged rows once.
 template <int KM, int L, int R>
 __global__ __launch_bounds__(256) void k_fw_syn_cols(
-    const float *__restrict__ ll, int llH, int llW, const float *__restrict__ lh,
-    const float *__restrict__ hl, const float *__restrict__ hh, float *__restrict__ lo,
-    float *__restrict__ hi, Taps g, int H, int W, int Hs, const float2 *__restrict__ tw,
-    const float2 *__restrict__ X, int nk) {
+    const float *__restrict__ dl, int dH, int dW, const float2 *__restrict__ X,
+    const float2 *__restrict__ tw, int nk, float *__restrict__ lo,
+    float *__restrict__ hi, Taps g, int H, int W, int Hs) {
   constexpr int L2 = L / 2, PT = 4 * R, NIN = PT + L2 - 1;
-  __shared__ float4 xs[NIN][64];
+  __shared__ float2 xs[NIN][64];
   const int lane = threadIdx.x & 63;
   const int sub = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = blockIdx.x * 64 + lane;
@@ -352,22 +354,18 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
     xi[k] = x.y;
   }
   constexpr int kIt = (NIN + 3) / 4;
-  float4 v[kIt];
+  float dv[kIt];
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
     const int q = sub + 4 * it, i = i0 + q;
-    v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (live && q < NIN && i < H) {
-      const size_t ob = ((size_t)z * H + i) * W + w;
-      v[it] = make_float4(ll[((size_t)z * llH + i) * llW + w], lh[ob], hl[ob], hh[ob]);
-    }
+    dv[it] = (dl && live && q < NIN && i < H) ? dl[((size_t)z * dH + i) * dW + w] : 0.f;
   }
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
     const int q = sub + 4 * it, i = i0 + q;
     if (q < NIN) {
+      float c = 0.f;
       if (i < H) {
-        float c = v[it].z;
 #pragma unroll
         for (int k = 0; k < KM; ++k) {
           if (k < nk) {
@@ -375,9 +373,8 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
             c -= xr[k] * e.x - xi[k] * e.y;
           }
         }
-        v[it].z = c;
       }
-      xs[q][lane] = v[it];
+      xs[q][lane] = make_float2(dv[it], c);
     }
   }
   __syncthreads();
@@ -389,20 +386,16 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
   // pair pq0+r takes rows pq0+r+L2-1-m for m = 0, 1, ... (in that order)
 #pragma unroll
   for (int i = R + L2 - 2; i >= 0; --i) {
-    const float4 x = xs[pq0 + i][lane];
+    const float2 x = xs[pq0 + i][lane];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int m = r + L2 - 1 - i;
       if (m >= 0 && m < L2) {
-        const float ge = g.a[2 * m], he = g.b[2 * m], go = g.a[2 * m + 1], ho = g.b[2 * m + 1];
+        const float ge = g.a[2 * m], go = g.a[2 * m + 1];
         a0[r] = fmaf(ge, x.x, a0[r]);
-        a0[r] = fmaf(he, x.y, a0[r]);
-        b0[r] = fmaf(ge, x.z, b0[r]);
-        b0[r] = fmaf(he, x.w, b0[r]);
+        b0[r] = fmaf(ge, x.y, b0[r]);
         a1[r] = fmaf(go, x.x, a1[r]);
-        a1[r] = fmaf(ho, x.y, a1[r]);
-        b1[r] = fmaf(go, x.z, b1[r]);
-        b1[r] = fmaf(ho, x.w, b1[r]);
+        b1[r] = fmaf(go, x.y, b1[r]);
       }
     }
   }
@@ -422,17 +415,19 @@ __global__ __launch_bounds__(256) void k_fw_syn_cols(
 // Synthesis along the detector axis, one workgroup per output row with both
 // input rows staged in LDS as (lo, hi) pairs; outputs 2p, 2p+1 read the pairs
 // p + L/2 - 1 - m (m < L/2) through the even and odd taps.  A thread produces
-// outputs 4u .. 4u+3 from the pairs 2u .. 2u+L/2, read as float4.  LAST = 1
-// writes rows [st, st+nproj) and columns [0, n) straight into the chunk
-// layout res[p][z][x]; LAST = 2 writes them in sinogram order out[z][p][x]
-// as TO, after minus_log when take_log (the fused form of proc_proj and the
-// swap of rec.py:174).
+// outputs 4u .. 4u+3 from the pairs 2u .. 2u+L/2, read as float4.  LAST = 0
+// writes the correction d_k (pitch Hs x Ws).  LAST = 1 adds d_0 to the
+// float32 chunk res[p][z][x] in place (rows [st, st+nproj), columns [0, n)).
+// LAST = 2 adds d_0 to x_0 = darkflat(raw[p][z][x]) and writes the sinograms
+// out[z][p][x] as TO, after minus_log when take_log (the fused form of
+// proc_proj and the swap of rec.py:174).
 template <int LAST, typename TO, int L>
 __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ lo,
                                                      const float *__restrict__ hi,
+                                                     const void *__restrict__ src_,
                                                      TO *__restrict__ out, Taps g, int Hs, int W,
                                                      int Ws, int nz, int nproj, int st, int n,
-                                                     int take_log) {
+                                                     DarkFlat df, int take_log) {
   extern __shared__ float4 rs4[];
   float2 *rs = reinterpret_cast<float2 *>(rs4);
   constexpr int L2 = L / 2, NQ = (L2 + 2) / 2;
@@ -453,10 +448,16 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
       if (i0 + u * 256 < W) rs[i0 + u * 256] = v[u];
   }
   __syncthreads();
+  const int p = h - st;
   const int nout = LAST ? min(Ws, n) : Ws;
-  TO *orow = LAST == 2   ? out + ((size_t)z * nproj + (h - st)) * n
-             : LAST == 1 ? out + ((size_t)(h - st) * nz + z) * n
+  TO *orow = LAST == 2   ? out + ((size_t)z * nproj + p) * n
+             : LAST == 1 ? out + ((size_t)p * nz + z) * n
                          : out + ((size_t)z * Hs + h) * Ws;
+  const uint16_t *rawrow =
+      LAST == 2 ? static_cast<const uint16_t *>(src_) + ((size_t)p * nz + z) * n : nullptr;
+  // linear flat interpolation weight of projection p (proc_functions.py:62-66)
+  const float vlin = (LAST == 2 && nproj > 1) ? (float)((double)p / (double)(nproj - 1)) : 0.f;
+  const bool vec4 = (n & 3) == 0 && ((uintptr_t)rawrow & 7) == 0;
   for (int u = threadIdx.x; 4 * u < nout; u += 256) {
     float2 x[2 * NQ];
 #pragma unroll
@@ -477,11 +478,41 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
         y[2 * s + 1] = fmaf(g.b[2 * m + 1], xv.y, y[2 * s + 1]);
       }
     }
+    float xin[4] = {0.f, 0.f, 0.f, 0.f};  // x_0 of the four outputs (LAST 1, 2)
+    if (LAST == 2 && vec4 && 4 * u + 3 < nout) {
+      // whole aligned quadruples: one 8-byte load of counts, 16-byte loads of
+      // the dark/flat means
+      const size_t zx = (size_t)z * n + 4 * u;
+      const ushort4 r4 = *reinterpret_cast<const ushort4 *>(rawrow + 4 * u);
+      const float4 d4 = *reinterpret_cast<const float4 *>(df.d0 + zx);
+      const float4 a4 = *reinterpret_cast<const float4 *>(df.fa + zx);
+      const float4 b4 = df.flat_linear ? *reinterpret_cast<const float4 *>(df.fb + zx)
+                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+      xin[0] = darkflat((float)r4.x, d4.x, a4.x, b4.x, vlin, df.flat_linear);
+      xin[1] = darkflat((float)r4.y, d4.y, a4.y, b4.y, vlin, df.flat_linear);
+      xin[2] = darkflat((float)r4.z, d4.z, a4.z, b4.z, vlin, df.flat_linear);
+      xin[3] = darkflat((float)r4.w, d4.w, a4.w, b4.w, vlin, df.flat_linear);
+    } else if (LAST) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int x0 = 4 * u + c;
+        if (x0 >= nout) continue;
+        if (LAST == 1) {
+          xin[c] = (float)orow[x0];
+        } else {
+          const size_t zx = (size_t)z * n + x0;
+          xin[c] = darkflat((float)rawrow[x0], df.d0[zx], df.fa[zx],
+                            df.flat_linear ? df.fb[zx] : 0.f, vlin, df.flat_linear);
+        }
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      float yc = y[c];
+      const int x0 = 4 * u + c;
+      if (x0 >= nout) continue;
+      float yc = y[c] + xin[c];
       if (LAST == 2 && take_log) yc = minus_log(yc);
-      if (4 * u + c < nout) orow[4 * u + c] = (TO)yc;
+      orow[x0] = (TO)yc;
     }
   }
 }
@@ -498,15 +529,14 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
     l.tw.release();
     l.gk.release();
     l.ll.release();
-    l.lh.release();
-    l.hl.release();
-    l.hh.release();
+    l.part.release();
+    l.X.release();
   }
   P->lv.assign(level, FwLevel());
   const int L = wv->len;
   const int64_t nz = P->nz;
   int H = (int)(P->nproj + P->nproj / 8), W = (int)P->n;
-  size_t rowmax = 0, synmax = 0, outmax = 0, partmax = 0, xsum_total = 0;
+  size_t rowmax = 0, synmax = 0, outmax = 0;
   for (int k = 0; k < level; ++k) {
     FwLevel &l = P->lv[k];
     l.H = H;
@@ -514,11 +544,8 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
     l.Ho = (H + L - 1) / 2;
     l.Wo = (W + L - 1) / 2;
     rowmax = std::max(rowmax, (size_t)H * l.Wo);
-    const size_t band = sizeof(float) * nz * l.Ho * l.Wo;
     int rc;
-    if ((rc = l.ll.alloc(band)) || (rc = l.lh.alloc(band)) || (rc = l.hl.alloc(band)) ||
-        (rc = l.hh.alloc(band)))
-      return rc;
+    if ((rc = l.ll.alloc(sizeof(float) * nz * l.Ho * l.Wo))) return rc;
     // notch: kept frequencies k <= my/2 with g_k >= 1e-9 (yhat of :276-279)
     const int my = l.Ho;
     std::vector<int> ks;
@@ -549,9 +576,15 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
       TOMO_HIP(hipMemcpy(l.tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
       TOMO_HIP(hipMemcpy(l.gk.p, g.data(), sizeof(float) * l.nk, hipMemcpyHostToDevice));
     }
-    partmax = std::max(partmax, (size_t)ceil_div(l.Ho, 8) * kMaxNotch * l.Wo);
-    l.xoff = xsum_total;
-    xsum_total += (size_t)kMaxNotch * l.Wo * nz;
+    // column analysis: 4R output rows per workgroup (R = 8 for tall bands,
+    // fewer, for more workgroups, in the short bands of the deep levels);
+    // partial notch DFTs per row block, kept until the synthesis
+    l.KM = l.nk <= 8 ? 8 : (l.nk <= 16 ? 16 : 32);
+    l.R = l.Ho >= 256 ? 8 : (l.Ho >= 64 ? 4 : 2);
+    l.nrb = (int)ceil_div(l.Ho, 4 * l.R);
+    if (l.nk > 0 && ((rc = l.part.alloc(sizeof(float2) * nz * l.nrb * l.KM * l.Wo)) ||
+                     (rc = l.X.alloc(sizeof(float2) * nz * l.KM * l.Wo))))
+      return rc;
     const size_t Hs = 2 * (size_t)l.Ho - L + 2, Ws = 2 * (size_t)l.Wo - L + 2;
     synmax = std::max(synmax, Hs * l.Wo);
     outmax = std::max(outmax, Hs * Ws);
@@ -564,9 +597,7 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
       (rc = P->synlo.alloc(sizeof(float) * nz * synmax)) ||
       (rc = P->synhi.alloc(sizeof(float) * nz * synmax)) ||
       (rc = P->syn[0].alloc(sizeof(float) * nz * outmax)) ||
-      (rc = P->syn[1].alloc(sizeof(float) * nz * outmax)) ||
-      (rc = P->notch.alloc(sizeof(float2) * nz * partmax)) ||
-      (rc = P->notchX.alloc(sizeof(float2) * xsum_total)))
+      (rc = P->syn[1].alloc(sizeof(float) * nz * outmax)))
     return rc;
   P->fw_name = wv->name;
   P->fw_levels = level;
@@ -601,6 +632,8 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
   const int nz = (int)P->nz, nproj = (int)P->nproj, n = (int)P->n;
   const int npad = nproj + nproj / 8;
   const int st0 = (npad - nproj) / 2;
+  TOMO_REQUIRE(dst_mode != 2 || src_mode == 2,
+               "fw_run: the sinogram output needs the raw chunk as input");
   for (int k = 0; k < level; ++k) {
     FwLevel &l = P->lv[k];
     const size_t lds = row_lds_bytes(l.Wo + L / 2 - 1);
@@ -621,45 +654,41 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
                            nproj, st0, df);
     });
     TOMO_LAUNCH_CHECK();
-    // 4R output rows per workgroup: R = 8 for tall bands, fewer (more
-    // workgroups) for the short bands of the deep levels
-    const int Rc = l.Ho >= 256 ? 8 : (l.Ho >= 64 ? 4 : 2);
-    const int nrb = (int)ceil_div(l.Ho, 4 * Rc);
-    const dim3 gc((unsigned)ceil_div(l.Wo, 64), nrb, nz);
-    float2 *part = P->notch.as<float2>();
-    float2 *Xk = P->notchX.as<float2>();
+    const dim3 gc((unsigned)ceil_div(l.Wo, 64), l.nrb, nz);
     with_km(l.nk, [&](auto kmc) {
-      constexpr int KM = decltype(kmc)::value;
       with_len(L, [&](auto lc) {
-        with_r(Rc, [&](auto rc) {
-          constexpr int LC = decltype(lc)::value, RC = decltype(rc)::value;
+        with_r(l.R, [&](auto rc) {
+          constexpr int KM = decltype(kmc)::value, LC = decltype(lc)::value,
+                        RC = decltype(rc)::value;
           ProfScope ps_("k_fw_cols", st);
           hipLaunchKernelGGL((k_fw_cols<KM, LC, RC>), gc, dim3(256), 0, st,
-                             P->rowlo.as<float>(), P->rowhi.as<float>(), l.ll.as<float>(),
-                             l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(), ana, l.H,
-                             l.Wo, l.Ho, l.tw.as<float2>(), part, l.nk);
+                             P->rowlo.as<float>(), P->rowhi.as<float>(), l.ll.as<float>(), ana,
+                             l.H, l.Wo, l.Ho, l.tw.as<float2>(), l.part.as<float2>(), l.nk);
         });
       });
       if (l.nk > 0) {
+        constexpr int KM = decltype(kmc)::value;
         ProfScope ps_("k_fw_xsum", st);
         hipLaunchKernelGGL(k_fw_xsum<KM>, dim3((unsigned)ceil_div(l.Wo, 256), l.nk, nz),
-                           dim3(256), 0, st, part, l.gk.as<float>(), Xk + l.xoff, l.Wo, nrb);
+                           dim3(256), 0, st, l.part.as<float2>(), l.gk.as<float>(),
+                           l.X.as<float2>(), l.Wo, l.nrb);
       }
     });
     TOMO_LAUNCH_CHECK();
   }
-  // synthesis, deepest level first; `cur` is the current low band (pitch curH x curW)
-  const float *cur = P->lv[level - 1].ll.as<float>();
-  int curH = P->lv[level - 1].Ho, curW = P->lv[level - 1].Wo;
+  // synthesis of the correction, deepest level first; `cur` is the correction
+  // of the level below (pitch curH x curW), NULL (zero) at the deepest level
+  const float *cur = nullptr;
+  int curH = 0, curW = 0;
   for (int k = level - 1; k >= 0; --k) {
     FwLevel &l = P->lv[k];
     const int Hb = l.Ho, Wb = l.Wo;
-    TOMO_REQUIRE(curH >= Hb && curW >= Wb, "pre_remove_stripe_fw: internal band shape mismatch");
+    TOMO_REQUIRE(!cur || (curH >= Hb && curW >= Wb),
+                 "pre_remove_stripe_fw: internal band shape mismatch");
     const int Hs = 2 * Hb - L + 2, Ws = 2 * Wb - L + 2;
     // 8R output rows per workgroup
     const int Rs = Hs >= 512 ? 8 : (Hs >= 128 ? 4 : 2);
     const dim3 gs((unsigned)ceil_div(Wb, 64), (unsigned)ceil_div(Hs, 8 * Rs), nz);
-    const float2 *Xk = P->notchX.as<float2>() + l.xoff;
     with_km(l.nk, [&](auto kmc) {
       with_len(L, [&](auto lc) {
         with_r(Rs, [&](auto rc) {
@@ -667,9 +696,8 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
                         RC = decltype(rc)::value;
           ProfScope ps_("k_fw_syn_cols", st);
           hipLaunchKernelGGL((k_fw_syn_cols<KM, LC, RC>), gs, dim3(256), 0, st, cur, curH, curW,
-                             l.lh.as<float>(), l.hl.as<float>(), l.hh.as<float>(),
-                             P->synlo.as<float>(), P->synhi.as<float>(), syn, Hb, Wb, Hs,
-                             l.tw.as<float2>(), Xk, l.nk);
+                             l.X.as<float2>(), l.tw.as<float2>(), l.nk, P->synlo.as<float>(),
+                             P->synhi.as<float>(), syn, Hb, Wb, Hs);
         });
       });
     });
@@ -683,20 +711,25 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
         constexpr int LC = decltype(lc)::value;
         ProfScope ps_("k_fw_syn_rows", st);
         if (k > 0)
-          hipLaunchKernelGGL((k_fw_syn_rows<0, float, LC>), gr, dim3(256), rl, st, slo, shi, out,
-                             syn, Hs, Wb, Ws, nz, nproj, st0, n, 0);
+          hipLaunchKernelGGL((k_fw_syn_rows<0, float, LC>), gr, dim3(256), rl, st, slo, shi,
+                             (const void *)nullptr, out, syn, Hs, Wb, Ws, nz, nproj, st0, n, df,
+                             0);
         else if (dst_mode == 1)
           hipLaunchKernelGGL((k_fw_syn_rows<1, float, LC>), gr, dim3(256), rl, st, slo, shi,
-                             (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, 0);
+                             (const void *)nullptr, (float *)dst, syn, Hs, Wb, Ws, nz, nproj,
+                             st0, n, df, 0);
         else if (out_dtype == TOMO_F16)
           hipLaunchKernelGGL((k_fw_syn_rows<2, _Float16, LC>), gr, dim3(256), rl, st, slo, shi,
-                             (_Float16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+                             src, (_Float16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, df,
+                             take_log);
         else if (out_dtype == TOMO_BF16)
           hipLaunchKernelGGL((k_fw_syn_rows<2, __bf16, LC>), gr, dim3(256), rl, st, slo, shi,
-                             (__bf16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+                             src, (__bf16 *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, df,
+                             take_log);
         else
           hipLaunchKernelGGL((k_fw_syn_rows<2, float, LC>), gr, dim3(256), rl, st, slo, shi,
-                             (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, take_log);
+                             src, (float *)dst, syn, Hs, Wb, Ws, nz, nproj, st0, n, df,
+                             take_log);
       });
     }
     TOMO_LAUNCH_CHECK();
