@@ -81,21 +81,28 @@ def fw_levels(a, L=32, level=7):
 
 
 def fw_bytes(name, a):
-    """Mean external float32 I/O per launch of a Fourier-wavelet kernel (one
-    launch per level and chunk, levels launched equally often)."""
+    """Mean external I/O per launch of a Fourier-wavelet kernel (one launch per
+    level and chunk, levels launched equally often; correction form: the
+    column analysis writes ll only, the column synthesis reads the
+    correction of the level below, the last pass reads the raw counts and
+    writes the sinograms at storage precision)."""
     ch = a.chunk
+    r = 4 if a.dtype == 'float32' else 2
     per = []
     for k, (H, W, Ho, Wo, Hs, Ws) in enumerate(fw_levels(a)):
-        if name == 'k_fw_rows':      # read the rows (level 0: the unpadded chunk), write lo+hi
-            rd = a.nproj * a.n if k == 0 else H * W
-            per.append(4 * ch * (rd + 2 * H * Wo))
-        elif name == 'k_fw_cols':    # read lo+hi, write the four bands
-            per.append(4 * ch * (2 * H * Wo + 4 * Ho * Wo))
-        elif name == 'k_fw_syn_cols':  # read the four bands, write lo+hi
-            per.append(4 * ch * (4 * Ho * Wo + 2 * Hs * Wo))
-        elif name == 'k_fw_syn_rows':  # read lo+hi, write the level (last: the chunk rows)
-            wr = a.nproj * a.n if k == 0 else Hs * Ws
-            per.append(4 * ch * (2 * Hs * Wo + wr))
+        if name == 'k_fw_rows':      # read the rows (level 0: uint16 counts), write lo+hi
+            rd = 2 * a.nproj * a.n if k == 0 else 4 * H * W
+            per.append(ch * (rd + 4 * 2 * H * Wo))
+        elif name == 'k_fw_cols':    # read lo+hi, write ll
+            per.append(4 * ch * (2 * H * Wo + Ho * Wo))
+        elif name == 'k_fw_syn_cols':  # read the correction of the level below, write lo+hi
+            rd = 0 if k == len(fw_levels(a)) - 1 else Ho * Wo
+            per.append(4 * ch * (rd + 2 * Hs * Wo))
+        elif name == 'k_fw_syn_rows':  # read lo+hi (last: + counts), write the correction / sinograms
+            if k == 0:
+                per.append(ch * (4 * 2 * Hs * Wo + 2 * a.nproj * a.n + r * a.nproj * a.n))
+            else:
+                per.append(4 * ch * (2 * Hs * Wo + Hs * Ws))
         else:
             return None
     return sum(per) // len(per)
@@ -174,8 +181,16 @@ def cpu_baseline(a, data, dark, flat, theta, center):
     with ctx.Pool(workers) as pool:
         pool.map(_cpu_pair, args)
     dt = time.time() - t0
+    model = ''
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {'value': round(2 * pairs / dt, 3), 'unit': 'slices/s', 'cores': workers,
-            'kind': 'port',
+            'kind': 'port', 'host': {'cpu_count': os.cpu_count(), 'model': model},
             'sample': '%d complex slice pairs (%d slices) of the same workload through the FP64 '
                       'oracle (dark/flat, %s-log, %s filter, USFFT backprojection), one pair '
                       'per process, %.1fs wall' % (pairs, 2 * pairs,
@@ -383,12 +398,22 @@ def main():
                 if os.path.exists(tf):
                     traffic = json.load(open(tf)).get(
                         '%s@n%d_nproj%d_chunk%d_%s' % (dom, a.n, a.nproj, a.chunk, sfx))
+                # end to end: every kernel's algorithmic bytes over the total
+                # kernel time of the pass (kernels without a byte model, the
+                # small reductions and table builders, count as time only)
+                tot_b = sum((algorithmic_bytes(k, a, None) or 0) * v['launches']
+                            for k, v in kern.items())
+                tot_ms = sum(v['total_ms'] for v in kern.values())
+                e2e = {'achieved': round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
+                       'frac': round(tot_b / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       'bytes_per_chunk': round(tot_b / min(nchunks, ISO_CHUNKS)),
+                       'kernel_ms_per_chunk': round(tot_ms / min(nchunks, ISO_CHUNKS), 4)}
                 roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1),
                         'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                         'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': traffic,
                         'algorithmic_bytes_per_launch': ab,
                         'avg_launch_ms': round(kern[dom]['avg_ms'], 5),
-                        'launches': kern[dom]['launches'],
+                        'launches': kern[dom]['launches'], 'end_to_end': e2e,
                         'timing': 'HIP events around each launch, first %d chunks of the '
                                   'workload on one stream (untimed pass before the timed steps)'
                                   % min(nchunks, ISO_CHUNKS)}
