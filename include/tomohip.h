@@ -250,6 +250,19 @@ int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, void *out, int out
                        int minus_log, void *stream);
 
 /*
+ * ProcFunctions.remove_outliers (src/tomocupy/processing/proc_functions.py:96-107),
+ * applied by proc_sino to the projections, darks and flats (:146-148):
+ *   f = median over the width x 1 x width window (angle, slice, detector)
+ *   data = (data > f && data - f > threshold) ? f : data
+ * in place on `data` (nrows, nz, n) uint16 of the plan's nz and n, bit-exact
+ * with ndimage.median_filter(data, [w, 1, w]) ('reflect' borders, rank
+ * w*w/2).  width 0 or 1 does nothing; widths up to 9 are supported.  The
+ * median scratch is allocated on the plan's first call.
+ */
+int pre_remove_outliers(tomo_pre_plan *plan, uint16_t *data, int64_t nrows, int width,
+                        int threshold, void *stream);
+
+/*
  * Fourier-wavelet stripe removal, remove_stripe_fw
  * (src/tomocupy/processing/remove_stripe.py:253-293), in place on
  *   res (nproj, nz, n) float32.

@@ -13,7 +13,10 @@ Parity status: ``darkflat_correction`` and ``minus_log`` are closed forms
 checked on hand-made cases (clamp, NaN and inf rules); the DWT analysis and
 synthesis (``afb1d``/``sfb1d``) are pinned against PyWavelets ``dwt2``/``idwt2``
 golden vectors and perfect reconstruction; the damping is a closed form
-(``tests/test_oracle_preproc.py``).
+(``tests/test_oracle_preproc.py``).  ``remove_outliers`` is
+``scipy.ndimage.median_filter`` (the CPU twin of the ``cupyx`` routine the
+reference calls) plus the replacement rule, pinned by a brute-force window
+median on small arrays.
 """
 
 import json
@@ -33,6 +36,21 @@ def wavelet_taps(name):
             _TAPS = json.load(f)['taps']
     t = _TAPS[name]
     return tuple(np.array(t[k], dtype=np.float64) for k in ('dec_lo', 'dec_hi', 'rec_lo', 'rec_hi'))
+
+
+def remove_outliers(data, width, threshold):
+    """``ProcFunctions.remove_outliers`` (P:96-107) on a uint16 ``(rows, nz, n)``
+    array: median over ``[width, 1, width]`` ('reflect' borders), then
+    ``where(data > f and data - f > threshold, f, data)``.  Returns a copy."""
+    from scipy import ndimage
+    data = np.asarray(data)
+    if int(width) <= 0:
+        return data.copy()
+    w = int(width)
+    f = ndimage.median_filter(data, [w, 1, w])
+    d = data.astype(np.int64)
+    fi = f.astype(np.int64)
+    return np.where((d > fi) & (d - fi > threshold), f, data).astype(data.dtype)
 
 
 def darkflat_correction(data, dark, flat, bright_ratio=1.0, flat_linear=False):
@@ -175,9 +193,15 @@ def remove_stripe_fw(data, sigma, wname, level):
 
 
 def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym16',
-               fw_level=7, take_minus_log=True, bright_ratio=1.0, flat_linear=False):
+               fw_level=7, take_minus_log=True, bright_ratio=1.0, flat_linear=False,
+               dezinger=0, dezinger_threshold=5000):
     """``proc_sino`` then ``proc_proj`` then the swap to sinogram order
-    (P:140-198, ``rec.py:172-174``): darkflat -> [fw] -> [-log] -> ``(nz, nproj, n)``."""
+    (P:140-198, ``rec.py:172-174``): [outliers] -> darkflat -> [fw] -> [-log] ->
+    ``(nz, nproj, n)``."""
+    if dezinger > 0:
+        data = remove_outliers(data, dezinger, dezinger_threshold)
+        dark = remove_outliers(dark, dezinger, dezinger_threshold)
+        flat = remove_outliers(flat, dezinger, dezinger_threshold)
     res = darkflat_correction(data, dark, flat, bright_ratio, flat_linear)
     if stripe == 'fw':
         res = remove_stripe_fw(res, fw_sigma, fw_filter, fw_level)

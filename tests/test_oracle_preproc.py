@@ -80,3 +80,52 @@ def test_fw_removes_constant_stripe():
     data = (smooth + stripe[None, :])[:, None, :]
     out = pp.remove_stripe_fw(data, 1.0, 'sym16', 3)[:, 0]
     assert np.abs(out - smooth)[:, [10, 31, 50]].mean() < 0.3 * 0.05
+
+
+def _brute_outliers(data, w, thr):
+    """Pure-Python window median with explicit half-sample reflection, the
+    window offsets -(w//2) .. w-1-w//2 and rank (w*w)//2 (tiny arrays only)."""
+    rows, nz, n = data.shape
+
+    def refl(m, N):
+        P = 2 * N
+        m %= P
+        return P - 1 - m if m >= N else m
+    out = data.copy()
+    for p in range(rows):
+        for z in range(nz):
+            for x in range(n):
+                win = sorted(int(data[refl(p + a - w // 2, rows), z, refl(x + b - w // 2, n)])
+                             for a in range(w) for b in range(w))
+                f = win[(w * w) // 2]
+                v = int(data[p, z, x])
+                if v > f and v - f > thr:
+                    out[p, z, x] = f
+    return out
+
+
+@pytest.mark.parametrize("w", [2, 3, 4, 5])
+def test_remove_outliers_matches_brute_force(w):
+    rng = np.random.default_rng(40 + w)
+    data = rng.integers(900, 1100, size=(7, 2, 9)).astype(np.uint16)
+    data[rng.random(data.shape) < 0.1] = 60000  # zingers
+    got = pp.remove_outliers(data, w, 500)
+    ref = _brute_outliers(data, w, 500)
+    assert np.array_equal(got, ref)
+    assert (got == 60000).sum() < (data == 60000).sum()
+
+
+def test_remove_outliers_rank_and_origin_matter():
+    # a planted off-by-one in the rank or the window origin changes the result
+    rng = np.random.default_rng(7)
+    data = rng.integers(0, 65535, size=(6, 1, 8)).astype(np.uint16)
+    ref = _brute_outliers(data, 4, 0)
+    got = pp.remove_outliers(data, 4, 0)
+    assert np.array_equal(got, ref)
+    shifted = np.roll(_brute_outliers(np.roll(data, 1, axis=2), 4, 0), -1, axis=2)
+    assert not np.array_equal(shifted, ref)
+
+
+def test_remove_outliers_width_zero_is_identity():
+    data = np.arange(60, dtype=np.uint16).reshape(3, 2, 10)
+    assert np.array_equal(pp.remove_outliers(data, 0, 5000), data)

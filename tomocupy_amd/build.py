@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libtomohip.so')
 BUILD = os.path.join(HERE, '_build')
-SOURCES = ['common.hip', 'fourierrec.hip', 'filter.hip', 'preproc.hip', 'stripe.hip']
+SOURCES = ['common.hip', 'fourierrec.hip', 'filter.hip', 'preproc.hip', 'stripe.hip',
+           'outliers.hip']
 ARCH = os.environ.get('TOMOHIP_ARCH', 'gfx950')
 
 FLAGS = [

@@ -57,6 +57,8 @@ def parser():
     r.add_argument('--minus-log', default='True')
     r.add_argument('--flat-linear', default='False')
     r.add_argument('--bright-ratio', type=float, default=1.0)
+    r.add_argument('--dezinger', type=int, default=0)
+    r.add_argument('--dezinger-threshold', type=int, default=5000)
     r.add_argument('--max-write-threads', type=int, default=8)
     r.add_argument('--save-format', default='tiff', choices=['tiff', 'npy'])
     return ap
@@ -92,7 +94,8 @@ def recon_full(a, ds):
                  ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype, fbp_filter=a.fbp_filter,
                  remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
                  fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
-                 flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads)
+                 flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads,
+                 dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold)
     ncz = rec.ncz
     stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
                                       dtype=np.float16 if a.dtype == 'float16' else np.float32,
@@ -142,7 +145,8 @@ def recon_try(a, ds):
     pf = ProcFunctions(ds.nproj, 1, ds.ni, ds.dark.shape[0], ds.flat.shape[0], a.dtype,
                        minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                        flat_linear=_bool(a.flat_linear), remove_stripe_method=stripe,
-                       fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level)
+                       fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level,
+                       dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold)
     bf = BackprojFunctions(ds.ni, ds.nproj, ncz, theta, centre, a.dtype, a.fbp_filter)
     written = []
     for idz in ids:

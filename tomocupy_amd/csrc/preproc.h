@@ -10,6 +10,8 @@ struct tomo_pre_plan;
 
 namespace tomo {
 
+constexpr int kMaxDezinger = 9;  // largest median window of pre_remove_outliers
+
 // One level of the Fourier-wavelet stripe removal: analysis input H x W,
 // bands Ho x Wo; the notch keeps nk angular frequencies of the stripe band.
 struct FwLevel {
@@ -61,4 +63,5 @@ struct tomo_pre_plan {
   int fw_L = 0;
   std::vector<tomo::FwLevel> lv;
   tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2];
+  tomo::DevBuf outl;  // medians of pre_remove_outliers, allocated on first use
 };

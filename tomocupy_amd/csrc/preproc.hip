@@ -216,5 +216,6 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->synhi.release();
   P->syn[0].release();
   P->syn[1].release();
+  P->outl.release();
   delete P;
 }

@@ -1,8 +1,9 @@
 """Chunk preprocessing with the reference's interface.
 
 Mirrors ``ProcFunctions`` (``src/tomocupy/processing/proc_functions.py:46-198``)
-for the steps on the fourierrec path: ``darkflat_correction``, ``minus_log``,
-``proc_sino`` (dark/flat, stripe removal) and ``proc_proj`` (minus log).
+for the steps on the fourierrec path: ``remove_outliers`` (dezinger),
+``darkflat_correction``, ``minus_log``, ``proc_sino`` (outliers, dark/flat,
+stripe removal) and ``proc_proj`` (minus log).
 Settings the reference reads from the global ``args`` are constructor
 arguments.  ``proc_chunk`` is the fused form used by the conveyor: raw uint16
 chunk -> dark/flat -> [stripe removal] -> minus log -> sinogram order.
@@ -17,7 +18,8 @@ from . import _lib
 class ProcFunctions:
     def __init__(self, nproj, ncz, n, ndark, nflat, dtype='float32', minus_log=True,
                  bright_ratio=1.0, flat_linear=False, remove_stripe_method='none',
-                 fw_sigma=1.0, fw_filter='sym16', fw_level=7):
+                 fw_sigma=1.0, fw_filter='sym16', fw_level=7, dezinger=0,
+                 dezinger_threshold=5000):
         if remove_stripe_method not in ('none', 'fw'):
             raise NotImplementedError('remove_stripe_method %r is not implemented'
                                       % remove_stripe_method)
@@ -31,6 +33,7 @@ class ProcFunctions:
         self.flat_linear = bool(flat_linear)
         self.remove_stripe_method = remove_stripe_method
         self.fw_sigma, self.fw_filter, self.fw_level = float(fw_sigma), fw_filter, int(fw_level)
+        self.dezinger, self.dezinger_threshold = int(dezinger), int(dezinger_threshold)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().pre_create(ctypes.byref(h), self.nproj, self.ncz, self.n,
                                          self.ndark, self.nflat), 'pre_create')
@@ -40,6 +43,29 @@ class ProcFunctions:
         import torch
         return ((self.nproj, self.ncz, self.n), (self.ndark, self.ncz, self.n),
                 (self.nflat, self.ncz, self.n), torch.uint16)
+
+    def remove_outliers(self, data):
+        """Zinger removal in place on a uint16 ``(rows, ncz, n)`` array
+        (``proc_functions.py:96-107``): pixels more than ``dezinger_threshold``
+        above the median of their ``dezinger`` x 1 x ``dezinger`` window
+        (angle, slice, detector) take the median value."""
+        import torch
+        if self.dezinger > 0:
+            if data.dim() != 3 or tuple(data.shape[1:]) != (self.ncz, self.n):
+                raise ValueError('remove_outliers: expected (rows, %d, %d), got %s'
+                                 % (self.ncz, self.n, tuple(data.shape)))
+            _lib.check(_lib.lib().pre_remove_outliers(
+                self._h, _lib.dev_ptr(data, tuple(data.shape), torch.uint16, 'data'),
+                int(data.shape[0]), self.dezinger, self.dezinger_threshold,
+                _lib.stream_ptr(None)), 'pre_remove_outliers')
+        return data
+
+    def _outliers(self, data, dark, flat):
+        # proc_sino order (proc_functions.py:146-148): data, dark, flat
+        if self.dezinger > 0:
+            self.remove_outliers(data)
+            self.remove_outliers(dark)
+            self.remove_outliers(flat)
 
     def darkflat_correction(self, data, dark, flat, res=None):
         import torch
@@ -61,7 +87,9 @@ class ProcFunctions:
         return res
 
     def proc_sino(self, data, dark, flat, res=None):
-        """dark/flat correction, then stripe removal (``proc_functions.py:140-161``)."""
+        """outliers, dark/flat correction, then stripe removal
+        (``proc_functions.py:140-161``)."""
+        self._outliers(data, dark, flat)
         res = self.darkflat_correction(data, dark, flat, res)
         if self.remove_stripe_method == 'fw':
             self.remove_stripe_fw(res)
@@ -83,9 +111,12 @@ class ProcFunctions:
         return out
 
     def proc_chunk(self, data, dark, flat, out=None, res=None):
-        """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, n)``."""
+        """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, n)``.
+        With ``dezinger > 0`` the outliers of ``data``, ``dark`` and ``flat``
+        are removed in place first, as ``proc_sino`` does."""
         import torch
         sd, sk, sf, u16 = self._shapes()
+        self._outliers(data, dark, flat)
         td = _lib.torch_dtype(self.dtype)
         if out is None:
             out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=data.device)
