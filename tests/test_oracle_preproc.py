@@ -129,3 +129,39 @@ def test_remove_outliers_rank_and_origin_matter():
 def test_remove_outliers_width_zero_is_identity():
     data = np.arange(60, dtype=np.uint16).reshape(3, 2, 10)
     assert np.array_equal(pp.remove_outliers(data, 0, 5000), data)
+
+
+def _fw_correction_form(data, sigma, wname, level):
+    """The order of operations of the GPU kernels: only the notch correction
+    D_k of the stripe band goes through the synthesis bank,
+    d_k = IDWT(d_{k+1}, 0, D_k, 0), and the output is x_0 + d_0."""
+    nproj, nz, ni = data.shape
+    npad = nproj + nproj // 8
+    st = (npad - nproj) // 2
+    x0 = np.zeros((nz, npad, ni))
+    x0[:, st:st + nproj] = np.asarray(data, np.float64).swapaxes(0, 1)
+    sli, D, shapes = x0, [], []
+    for _ in range(level):
+        sli, (lh, hl, hh) = pp.dwt2(sli, wname)
+        f = np.fft.rfft(hl, axis=-2) * pp.fw_damping(hl.shape[-2], sigma)[:, None]
+        D.append(np.fft.irfft(f, hl.shape[-2], axis=-2) - hl)
+        shapes.append(hl.shape[-2:])
+    d = None
+    for k in range(level)[::-1]:
+        h, w = shapes[k]
+        dd = np.zeros((nz, h, w)) if d is None else d[..., :h, :w]
+        zero = np.zeros_like(D[k])
+        d = pp.idwt2(dd, (zero, D[k], zero), wname)
+    out = x0 + d[..., :npad, :ni]
+    return out[:, st:st + nproj].swapaxes(0, 1)
+
+
+@pytest.mark.parametrize("wname,level,sigma,shape", [
+    ('sym16', 3, 1.0, (90, 2, 64)), ('db5', 4, 2.0, (181, 1, 129)),
+    ('haar', 5, 1.5, (60, 2, 47)), ('sym5', 2, 1.0, (100, 1, 100))])
+def test_fw_correction_form_equals_reference_order(wname, level, sigma, shape):
+    rng = np.random.default_rng(3)
+    data = 1 + 0.3 * rng.random(shape) + 0.2 * (np.arange(shape[2]) % 7 == 0)
+    a = pp.remove_stripe_fw(data, sigma, wname, level)
+    b = _fw_correction_form(data, sigma, wname, level)
+    assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a))
