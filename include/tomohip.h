@@ -263,6 +263,18 @@ int pre_remove_outliers(tomo_pre_plan *plan, uint16_t *data, int64_t nrows, int 
                         int threshold, void *stream);
 
 /*
+ * remove_stripe_ti (src/tomocupy/processing/remove_stripe.py:296-309), the
+ * Titarenko ring filter, in place on the float32 chunk res (nproj, nz, n):
+ * v = mean over angles minus its first column, c = v circularly convolved with
+ * gamma_k = beta ((1-beta)/(1+beta))^|k| (minus 1 at k = 0), res += c on the
+ * central floor(mask*n) columns.  FP64 means and convolution; even n only
+ * (as the reference); beta in (0, 1), mask in [0, 1].  Scratch allocated on
+ * the plan's first call.
+ */
+int pre_remove_stripe_ti(tomo_pre_plan *plan, float *res, double beta, double mask,
+                         void *stream);
+
+/*
  * Fourier-wavelet stripe removal, remove_stripe_fw
  * (src/tomocupy/processing/remove_stripe.py:253-293), in place on
  *   res (nproj, nz, n) float32.

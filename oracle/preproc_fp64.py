@@ -38,6 +38,26 @@ def wavelet_taps(name):
     return tuple(np.array(t[k], dtype=np.float64) for k in ('dec_lo', 'dec_hi', 'rec_lo', 'rec_hi'))
 
 
+def remove_stripe_ti(data, beta, mask_size):
+    """``remove_stripe_ti`` (R:296-309) on ``data (nproj, nz, ni)``, FP64, even
+    ``ni`` (the reference's irfft returns ``ni - 1`` samples for odd ``ni``).
+    The mask covers the central ``floor(mask_size * ni)`` columns (the
+    reference slices with ``mask_size * ni``, a float for its float option)."""
+    data = np.asarray(data, dtype=np.float64)
+    ni = data.shape[-1]
+    if ni % 2:
+        raise ValueError('remove_stripe_ti needs an even detector width')
+    gamma = beta * ((1 - beta) / (1 + beta)) ** np.abs(np.fft.fftfreq(ni) * ni)
+    gamma[0] -= 1
+    v = data.mean(axis=0)
+    v = v - v[:, 0:1]
+    v = np.fft.irfft(np.fft.rfft(v) * np.fft.rfft(gamma), ni)
+    mask = np.zeros(v.shape)
+    m = int(np.floor(mask_size * ni))
+    mask[:, ni // 2 - m // 2:ni // 2 + m // 2] = 1
+    return data + v * mask
+
+
 def remove_outliers(data, width, threshold):
     """``ProcFunctions.remove_outliers`` (P:96-107) on a uint16 ``(rows, nz, n)``
     array: median over ``[width, 1, width]`` ('reflect' borders), then
@@ -194,7 +214,7 @@ def remove_stripe_fw(data, sigma, wname, level):
 
 def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym16',
                fw_level=7, take_minus_log=True, bright_ratio=1.0, flat_linear=False,
-               dezinger=0, dezinger_threshold=5000):
+               dezinger=0, dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0):
     """``proc_sino`` then ``proc_proj`` then the swap to sinogram order
     (P:140-198, ``rec.py:172-174``): [outliers] -> darkflat -> [fw] -> [-log] ->
     ``(nz, nproj, n)``."""
@@ -205,6 +225,8 @@ def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym1
     res = darkflat_correction(data, dark, flat, bright_ratio, flat_linear)
     if stripe == 'fw':
         res = remove_stripe_fw(res, fw_sigma, fw_filter, fw_level)
+    elif stripe == 'ti':
+        res = remove_stripe_ti(res, ti_beta, ti_mask)
     elif stripe != 'none':
         raise ValueError(stripe)
     if take_minus_log:

@@ -165,3 +165,31 @@ def test_fw_correction_form_equals_reference_order(wname, level, sigma, shape):
     a = pp.remove_stripe_fw(data, sigma, wname, level)
     b = _fw_correction_form(data, sigma, wname, level)
     assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a))
+
+
+def test_ti_equals_direct_circular_convolution():
+    rng = np.random.default_rng(11)
+    nproj, nz, ni, beta = 9, 2, 16, 0.05
+    data = 1 + 0.1 * rng.random((nproj, nz, ni))
+    got = pp.remove_stripe_ti(data, beta, 1.0)
+    r = (1 - beta) / (1 + beta)
+    gam = np.array([beta * r ** min(k, ni - k) for k in range(ni)])
+    gam[0] -= 1
+    v = data.mean(axis=0)
+    v = v - v[:, :1]
+    c = np.array([[sum(v[z, j] * gam[(x - j) % ni] for j in range(ni)) for x in range(ni)]
+                  for z in range(nz)])
+    assert np.allclose(got, data + c[None], rtol=0, atol=1e-13)
+
+
+def test_ti_mask_and_constant_rows():
+    rng = np.random.default_rng(12)
+    data = 1 + 0.1 * rng.random((5, 1, 20))
+    # a zero-width mask changes nothing; half the width changes only the centre
+    assert np.array_equal(pp.remove_stripe_ti(data, 0.1, 0.0), data)
+    half = pp.remove_stripe_ti(data, 0.1, 0.5)
+    changed = np.any(half != data, axis=(0, 1))
+    assert changed[5:15].all() and not changed[:5].any() and not changed[15:].any()
+    # rows constant along the detector carry no stripe: unchanged
+    flat = np.broadcast_to(rng.random((5, 1, 1)), (5, 1, 20)).copy()
+    assert np.allclose(pp.remove_stripe_ti(flat, 0.1, 1.0), flat, atol=1e-15)

@@ -13,8 +13,9 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
   shift of the same sinogram (``rec.py:210-275``), one TIFF per centre
   ``<out>/recon_slice%04d_center%05.2f.tiff`` (``writer.py:309-313``).
 
-Settings outside this path (binning, double-FOV, other stripe filters,
-phase retrieval, ...) are not offered.  Input formats: see
+Stripe removal: ``fw`` and ``ti``; zinger removal: ``--dezinger``.  Settings
+outside this path (binning, double-FOV, vo-all stripe removal, phase
+retrieval, ...) are not offered.  Input formats: see
 :mod:`tomocupy_amd.io`.
 """
 import argparse
@@ -50,10 +51,12 @@ def parser():
                    choices=['none', 'ramp', 'shepp', 'hann', 'hamming', 'parzen', 'cosine',
                             'cosine2'])
     r.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
-    r.add_argument('--remove-stripe-method', default='None', choices=['None', 'none', 'fw'])
+    r.add_argument('--remove-stripe-method', default='None', choices=['None', 'none', 'fw', 'ti'])
     r.add_argument('--fw-sigma', type=float, default=1.0)
     r.add_argument('--fw-filter', default='sym16', choices=['haar', 'db5', 'sym5', 'sym16'])
     r.add_argument('--fw-level', type=int, default=7)
+    r.add_argument('--ti-beta', type=float, default=0.022)
+    r.add_argument('--ti-mask', type=float, default=1.0)
     r.add_argument('--minus-log', default='True')
     r.add_argument('--flat-linear', default='False')
     r.add_argument('--bright-ratio', type=float, default=1.0)
@@ -79,7 +82,7 @@ def _common(a, ds):
     if not (0 <= z0 < z1 <= ds.nz):
         raise SystemExit('invalid --start-row/--end-row for %d rows' % ds.nz)
     centre = ds.ni / 2 if a.rotation_axis == -1 else a.rotation_axis
-    stripe = 'fw' if a.remove_stripe_method == 'fw' else 'none'
+    stripe = a.remove_stripe_method if a.remove_stripe_method in ('fw', 'ti') else 'none'
     return z0, z1, centre, stripe
 
 
@@ -95,7 +98,8 @@ def recon_full(a, ds):
                  remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
                  fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                  flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads,
-                 dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold)
+                 dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
+                 ti_beta=a.ti_beta, ti_mask=a.ti_mask)
     ncz = rec.ncz
     stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
                                       dtype=np.float16 if a.dtype == 'float16' else np.float32,
@@ -146,7 +150,8 @@ def recon_try(a, ds):
                        minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                        flat_linear=_bool(a.flat_linear), remove_stripe_method=stripe,
                        fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level,
-                       dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold)
+                       dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
+                 ti_beta=a.ti_beta, ti_mask=a.ti_mask)
     bf = BackprojFunctions(ds.ni, ds.nproj, ncz, theta, centre, a.dtype, a.fbp_filter)
     written = []
     for idz in ids:

@@ -64,4 +64,5 @@ struct tomo_pre_plan {
   std::vector<tomo::FwLevel> lv;
   tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2];
   tomo::DevBuf outl;  // medians of pre_remove_outliers, allocated on first use
+  tomo::DevBuf ti;    // column means and corrections of pre_remove_stripe_ti (first use)
 };

@@ -217,5 +217,6 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->syn[0].release();
   P->syn[1].release();
   P->outl.release();
+  P->ti.release();
   delete P;
 }

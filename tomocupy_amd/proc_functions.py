@@ -19,8 +19,8 @@ class ProcFunctions:
     def __init__(self, nproj, ncz, n, ndark, nflat, dtype='float32', minus_log=True,
                  bright_ratio=1.0, flat_linear=False, remove_stripe_method='none',
                  fw_sigma=1.0, fw_filter='sym16', fw_level=7, dezinger=0,
-                 dezinger_threshold=5000):
-        if remove_stripe_method not in ('none', 'fw'):
+                 dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0):
+        if remove_stripe_method not in ('none', 'fw', 'ti'):
             raise NotImplementedError('remove_stripe_method %r is not implemented'
                                       % remove_stripe_method)
         self.nproj, self.ncz, self.n = int(nproj), int(ncz), int(n)
@@ -34,6 +34,8 @@ class ProcFunctions:
         self.remove_stripe_method = remove_stripe_method
         self.fw_sigma, self.fw_filter, self.fw_level = float(fw_sigma), fw_filter, int(fw_level)
         self.dezinger, self.dezinger_threshold = int(dezinger), int(dezinger_threshold)
+        self.ti_beta, self.ti_mask = float(ti_beta), float(ti_mask)
+        self._res = None  # float32 transmission chunk of the unfused ti path
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().pre_create(ctypes.byref(h), self.nproj, self.ncz, self.n,
                                          self.ndark, self.nflat), 'pre_create')
@@ -86,6 +88,16 @@ class ProcFunctions:
             self.fw_filter.encode(), self.fw_level, _lib.stream_ptr(None)), 'pre_remove_stripe_fw')
         return res
 
+    def remove_stripe_ti(self, res):
+        """Titarenko ring removal in place on the float32 chunk
+        (``remove_stripe.py:296-309``)."""
+        import torch
+        sd = (self.nproj, self.ncz, self.n)
+        _lib.check(_lib.lib().pre_remove_stripe_ti(
+            self._h, _lib.dev_ptr(res, sd, torch.float32, 'res'), self.ti_beta, self.ti_mask,
+            _lib.stream_ptr(None)), 'pre_remove_stripe_ti')
+        return res
+
     def proc_sino(self, data, dark, flat, res=None):
         """outliers, dark/flat correction, then stripe removal
         (``proc_functions.py:140-161``)."""
@@ -93,6 +105,8 @@ class ProcFunctions:
         res = self.darkflat_correction(data, dark, flat, res)
         if self.remove_stripe_method == 'fw':
             self.remove_stripe_fw(res)
+        elif self.remove_stripe_method == 'ti':
+            self.remove_stripe_ti(res)
         return res
 
     def proc_proj_swap(self, data, out=None):
@@ -131,6 +145,14 @@ class ProcFunctions:
                 int(self.take_minus_log), self.fw_sigma, self.fw_filter.encode(), self.fw_level,
                 _lib.stream_ptr(None)), 'pre_process_fw')
             return out
+        if self.remove_stripe_method == 'ti':
+            # needs the whole transmission chunk (means over angles): dark/flat
+            # into a float32 chunk, ring removal in place, then -log + swap
+            if self._res is None or self._res.device != data.device:
+                self._res = torch.empty(sd, dtype=torch.float32, device=data.device)
+            self.darkflat_correction(data, dark, flat, self._res)
+            self.remove_stripe_ti(self._res)
+            return self.proc_proj_swap(self._res, out)
         _lib.check(_lib.lib().pre_process(
             self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
             _lib.dev_ptr(flat, sf, u16, 'flat'),

@@ -69,7 +69,8 @@ class GPURec:
     def __init__(self, n, nproj, nz, ncz, theta, center, ndark, nflat, dtype='float32',
                  fbp_filter='parzen', remove_stripe_method='none', fw_sigma=1.0,
                  fw_filter='sym16', fw_level=7, minus_log=True, bright_ratio=1.0,
-                 flat_linear=False, max_write_threads=8, dezinger=0, dezinger_threshold=5000):
+                 flat_linear=False, max_write_threads=8, dezinger=0, dezinger_threshold=5000,
+                 ti_beta=0.022, ti_mask=1.0):
         import torch
         if ncz == 1:
             ncz = 2  # fourierrec pairs two slices (reader.py:113-115)
@@ -81,7 +82,8 @@ class GPURec:
                                 minus_log=minus_log, bright_ratio=bright_ratio,
                                 flat_linear=flat_linear, remove_stripe_method=remove_stripe_method,
                                 fw_sigma=fw_sigma, fw_filter=fw_filter, fw_level=fw_level,
-                                dezinger=dezinger, dezinger_threshold=dezinger_threshold)
+                                dezinger=dezinger, dezinger_threshold=dezinger_threshold,
+                                ti_beta=ti_beta, ti_mask=ti_mask)
         self.bf = BackprojFunctions(self.n, self.nproj, self.ncz, theta, center, dtype, fbp_filter)
         self.stream1 = torch.cuda.Stream()
         self.stream2 = torch.cuda.Stream()
