@@ -46,10 +46,44 @@ __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f
 
 // NS: n as a compile-time constant (0 = sizes known at run time only), as in
 // every FFT kernel of this file.
+// max over the workgroup of the threads' m; thread 0 gets the result
+// (LDS for the per-wave maxima; every thread must call it)
+__device__ __forceinline__ float block_max(float m) {
+  __shared__ float wmax[16];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wmax[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 1; q < nw; ++q) m = fmaxf(m, wmax[q]);
+  return m;
+}
+
+// Writes the shifted spectrum row out[k] = sm[k] * (-1)^k * scale and, for
+// float32 plans with a gmax buffer, the row's largest |re|, |im| to gmax[row]
+// (the gridding scales its split half-precision factors with the maximum).
+template <typename T>
+__device__ __forceinline__ void store_spectrum(cpx<T> *out, const float2 *sm, int n, float scale,
+                                               int tid, int nt, float *gmax, size_t row) {
+  float m = 0.f;
+  for (int k = tid; k < n; k += nt) {
+    const float2 v = cscale(sm[lds_pad(k)], shift_sign(k) * scale);
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    stc(out + k, v);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (gmax) {
+      m = block_max(m);
+      if (tid == 0) gmax[row] = m;
+    }
+  }
+}
+
 template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx<T> *__restrict__ G,
                                                  FftDesc fd, int n, int nproj, int nz, int nzc,
-                                                 float scale) {
+                                                 float scale, float *__restrict__ gmax) {
   extern __shared__ float2 sm[];
   const int p = blockIdx.x, c = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -67,8 +101,7 @@ __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx
     lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd.tw, tid, nt);
   else
     lds_fft<-1>(sm, fd, 0, tid, nt);
-  for (int k = tid; k < n; k += nt)
-    stc(out + k, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
+  store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
 }
 
 // Fused FBP filter + pair packing + shifted detector FFT (the fused form of
@@ -81,7 +114,7 @@ template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_filtprep(
     const T *__restrict__ data, cpx<T> *__restrict__ G, const float2 *__restrict__ H2,
     const float *__restrict__ HL, const float *__restrict__ HR, FftDesc fd2, FftDesc fd1, int n,
-    int nproj, int nz, int nzc, float scale) {
+    int nproj, int nz, int nzc, float scale, float *__restrict__ gmax) {
   extern __shared__ float2 sm[];
   __shared__ float edge[4];
   const int p = blockIdx.x, c = blockIdx.y;
@@ -141,8 +174,7 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
     lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd1.tw, tid, nt);
   else
     lds_fft<-1>(sm, fd1, 0, tid, nt);
-  for (int k = tid; k < n; k += nt)
-    stc(out + k, cscale(sm[lds_pad(k)], shift_sign(k) * scale));
+  store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -164,101 +196,15 @@ __device__ __forceinline__ void gauss_pair(const int4 &e, int i, float K2, float
 }
 
 // Gridding as a matrix product on the matrix cores.  For one 16x16 cell tile
-// (rows a, columns b of the periodic 2n x 2n grid) and four polar samples s:
+// (rows a, columns b of the periodic 2n x 2n grid) and a batch of polar
+// samples s:
 //   F[a][b] += sum_s (E1_s(a) * G_s) * E0_s(b)
-// with the separable Gaussian factors E1 (rows) and E0 (columns): A = E1*G is
-// 16 x 4, B = E0 is 4 x 16, one v_mfma_f32_16x16x4f32 per slice and re/im
-// part.  Lane l holds A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+r][l%16] in
-// register r.  A wave walks one work item (a tile's sample list or a bounded
-// chunk of it) in a fixed order; chunks of heavy tiles land in private
-// partial buffers that k_fr_grid_reduce sums in chunk order, so the result is
-// bitwise reproducible.
-template <typename T>
-__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid(
-    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
-    const cpx<T> *__restrict__ G, cpx<T> *__restrict__ F, float *__restrict__ part, int n,
-    int ntile, int nslots, int nsamp, int nzc, float K2) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int it = blockIdx.x * kGridWaves + wave;
-  const int grp = blockIdx.y;
-  if (it >= nitems) return;
-  const int4 item = items[it];
-  const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
-  const int k = lane >> 4, i = lane & 15;
-  // the four complex slices of the group (planes of G; slices past nzc read as 0)
-  const int nlive = min(4, nzc - 4 * grp);
-  const cpx<T> *Gg = G + (size_t)(4 * grp) * nsamp;
-  // Entries {sample, footprint offsets (cols | rows << 8), du bits, dv bits}
-  // and their spectra arrive kGridBatch at a time, one per lane, and are
-  // staged in this wave's LDS; the loads of the next batch are in flight while
-  // the matrix products of the current one run.
-  __shared__ int4 se[kGridWaves][2][kGridBatch];
-  __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
-  const int eend = ebeg + cnt;
-  int4 re = make_int4(-1, 0, 0, 0);
-  cpx<T> rg[4];
-  auto fetch = [&](int base) {
-    const int idx = base + lane;
-    re = idx < eend ? ent[idx] : make_int4(-1, 0, 0, 0);
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      rg[c] = (re.x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + re.x] : cpx<T>{T(0), T(0)};
-  };
-  floatx4 acc[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-  if (cnt > 0) fetch(ebeg);
-  int buf = 0;
-  for (int b = ebeg; b < eend; b += kGridBatch, buf ^= 1) {
-    se[wave][buf][lane] = re;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = tof2(rg[c]);
-    __builtin_amdgcn_wave_barrier();
-    if (b + kGridBatch < eend) fetch(b + kGridBatch);
-    const int ngr = min(kGridBatch, eend - b) >> 2;
-    for (int q = 0; q < ngr; ++q) {
-      const int4 cur = se[wave][buf][4 * q + k];
-      float gv[8];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const float2 g = sg[wave][buf][c][4 * q + k];
-        gv[2 * c] = g.x;
-        gv[2 * c + 1] = g.y;
-      }
-      float e0v, e1v;
-      gauss_pair(cur, i, K2, e0v, e1v);
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1v * gv[m], e0v, acc[m], 0, 0, 0);
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  const int j = lane & 15, ib = 4 * (lane >> 4);
-  if (slot < 0) {
-    const int n2 = 2 * n;
-    const int TA = tile / ntile, TB = tile - TA * ntile;
-    const int b = 16 * TB + j;
-    const size_t plane = (size_t)n2 * n2;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (grp * 4 + c >= nzc) continue;
-      cpx<T> *Fc = F + (size_t)(grp * 4 + c) * plane;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int a = 16 * TA + ib + r;
-        if (a < n2 && b < n2)
-          stc(Fc + (size_t)a * n2 + b, make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
-      }
-    }
-  } else {
-    float *P = part + ((size_t)grp * nslots + slot) * 2048;
-#pragma unroll
-    for (int m = 0; m < 8; ++m)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) P[m * 256 + (ib + r) * 16 + j] = acc[m][r];
-  }
-}
-
+// with the separable Gaussian factors E1 (rows) and E0 (columns): A = E1*G
+// (16 x K), B = E0 (K x 16), one matrix instruction per slice and re/im part.
+// A wave walks one work item (a tile's sample list or a bounded chunk of it)
+// in a fixed order; chunks of heavy tiles land in private partial buffers
+// that k_fr_grid_reduce sums in chunk order, so the result is bitwise
+// reproducible.
 // 16-bit plans: the same tile products with 16 samples per matrix
 // instruction (v_mfma_f32_16x16x16 f16 / bf16, FP32 accumulation).  Lane l
 // supplies A[l%16][4(l/16)+u] and B[4(l/16)+u][l%16] for u < 4, so lane group
@@ -378,6 +324,163 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
     for (int m = 0; m < 8; ++m)
 #pragma unroll
       for (int r = 0; r < 4; ++r) P[m * 256 + (ib + r) * 16 + j] = acc[m][r];
+  }
+}
+
+// Scale exponent of the float32 gridding: kA = 13 - floor(log2(max |G|))
+// over the per-row maxima of the spectra, so every weighted spectrum value
+// times 2^kA stays below 2^14 (0 for an all-zero chunk).
+__global__ __launch_bounds__(256) void k_fr_gscale(const float *__restrict__ gmax, int count,
+                                                   int *__restrict__ ksc) {
+  float m = 0.f;
+  for (int i = threadIdx.x; i < count; i += 256) m = fmaxf(m, gmax[i]);
+  m = block_max(m);
+  if (threadIdx.x == 0) {
+    int k = 0;
+    if (m > 0.f && m <= 3.4e38f) k = 13 - ilogbf(m);
+    ksc[0] = max(-120, min(120, k));
+  }
+}
+
+// The float32 gridding on the half-precision matrix cores.  Same tile
+// products and lane layout as k_fr_grid16 (16 samples per instruction), with
+// each float32 factor split into half-precision high and low parts,
+// x = hi + lo (hi: x with its 13 low mantissa bits cleared, exact in half
+// precision; lo = x - hi rounded toward zero), and three products
+// hi*hi + hi*lo + lo*hi per tile product (the dropped lo*lo term and the
+// rounding of lo are 2^-20 relative).  To keep both parts in half precision's
+// normal range the row factors carry 2^kA (k_fr_gscale: the weighted spectra
+// stay below 2^14) and the column factors 2^10, folded into the exponents of
+// the Gaussians; the accumulators are scaled back by 2^-(kA+10) (exact) on
+// store.  On gfx950 the float32 matrix instruction costs as much as 2.4
+// half-precision ones and takes a quarter of the samples, and matrix and
+// vector work do not overlap.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef __fp16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float hi_part(float x) {
+  return __int_as_float(__float_as_int(x) & (int)0xffffe000);
+}
+
+__device__ __forceinline__ half4_t pack_rtz(float a, float b, float c, float d) {
+  const half2_t p = __builtin_amdgcn_cvt_pkrtz(a, b), q = __builtin_amdgcn_cvt_pkrtz(c, d);
+  uint2 u;
+  u.x = __builtin_bit_cast(unsigned, p);
+  u.y = __builtin_bit_cast(unsigned, q);
+  return __builtin_bit_cast(half4_t, u);
+}
+
+__device__ __forceinline__ void split4(const float *x, half4_t &hi, half4_t &lo) {
+  const float h0 = hi_part(x[0]), h1 = hi_part(x[1]), h2 = hi_part(x[2]), h3 = hi_part(x[3]);
+  hi = pack_rtz(h0, h1, h2, h3);
+  lo = pack_rtz(x[0] - h0, x[1] - h1, x[2] - h2, x[3] - h3);
+}
+
+// Gaussian factors as gauss_pair, times 2^s0 (columns) and 2^s1 (rows).
+__device__ __forceinline__ void gauss_pair_s(const int4 &e, int i, float K2, float s0, float s1,
+                                             float &e0, float &e1) {
+  const int d0 = i - (int)(signed char)(e.y & 0xff);
+  const int d1 = i - (int)(signed char)((e.y >> 8) & 0xff);
+  const float x0 = (float)(d0 - 4) - __int_as_float(e.z);
+  const float x1 = (float)(d1 - 4) - __int_as_float(e.w);
+  const bool ok = e.x >= 0;
+  e0 = (ok && (unsigned)d0 <= 8u) ? __builtin_amdgcn_exp2f(fmaf(K2 * x0, x0, s0)) : 0.f;
+  e1 = (ok && (unsigned)d1 <= 8u) ? __builtin_amdgcn_exp2f(fmaf(K2 * x1, x1, s1)) : 0.f;
+}
+
+__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
+    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
+    const float2 *__restrict__ G, float2 *__restrict__ F, float *__restrict__ part,
+    const int *__restrict__ ksc, int n, int ntile, int nslots, int nsamp, int nzc, float K2) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int it = blockIdx.x * kGridWaves + wave;
+  const int grp = blockIdx.y;
+  if (it >= nitems) return;
+  const int4 item = items[it];
+  const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
+  const int g4 = lane >> 4, i = lane & 15;
+  const int nlive = min(4, nzc - 4 * grp);
+  const float2 *Gg = G + (size_t)(4 * grp) * nsamp;
+  const int kA = ksc[0];
+  const float sA = (float)kA, sB = 10.f;
+  floatx4 acc[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  __shared__ int4 se[kGridWaves][2][kGridBatch];
+  __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
+  const int eend = ebeg + cnt;
+  int4 re = make_int4(-1, 0, 0, 0);
+  float2 rg[4];
+  auto fetch = [&](int base) {
+    const int idx = base + lane;
+    re = idx < eend ? ent[idx] : make_int4(-1, 0, 0, 0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      rg[c] = (re.x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + re.x] : make_float2(0.f, 0.f);
+  };
+  if (cnt > 0) fetch(ebeg);
+  int buf = 0;
+  for (int bb = ebeg; bb < eend; bb += kGridBatch, buf ^= 1) {
+    se[wave][buf][lane] = re;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = rg[c];
+    __builtin_amdgcn_wave_barrier();
+    if (bb + kGridBatch < eend) fetch(bb + kGridBatch);
+    const int ngr = min(kGridBatch, eend - bb) >> 4;
+    for (int q = 0; q < ngr; ++q) {
+      float e0v[4], e1v[4], g[4][8];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int slot_e = 16 * q + 4 * g4 + u;
+        const int4 cur = se[wave][buf][slot_e];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float2 v = sg[wave][buf][c][slot_e];
+          g[u][2 * c] = v.x;
+          g[u][2 * c + 1] = v.y;
+        }
+        gauss_pair_s(cur, i, K2, sB, sA, e0v[u], e1v[u]);
+      }
+      half4_t bh, bl;
+      split4(e0v, bh, bl);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const float a[4] = {e1v[0] * g[0][m], e1v[1] * g[1][m], e1v[2] * g[2][m],
+                            e1v[3] * g[3][m]};
+        half4_t ah, al;
+        split4(a, ah, al);
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[m], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const float unscale = ldexpf(1.f, -(kA + 10));
+  const int j = lane & 15, ib = 4 * (lane >> 4);
+  if (slot < 0) {
+    const int n2 = 2 * n;
+    const int TA = tile / ntile, TB = tile - TA * ntile;
+    const int bcol = 16 * TB + j;
+    const size_t plane = (size_t)n2 * n2;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (grp * 4 + c >= nzc) continue;
+      float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = 16 * TA + ib + r;
+        if (a < n2 && bcol < n2)
+          Fc[(size_t)a * n2 + bcol] =
+              make_float2(acc[2 * c][r] * unscale, acc[2 * c + 1][r] * unscale);
+      }
+    }
+  } else {
+    float *P = part + ((size_t)grp * nslots + slot) * 2048;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) P[m * 256 + (ib + r) * 16 + j] = acc[m][r] * unscale;
   }
 }
 
@@ -643,6 +746,7 @@ struct tomo_fr_plan {
   int64_t nsamp = 0;
   FftPlan fft1, fft2;
   DevBuf geom, ent, items, red, part, phix, phiy, circ;
+  DevBuf gmax, gsc;  // float32 plans: per-row spectrum maxima, gridding scale exponent
   int ntile = 0, nitems = 0, nred = 0, nslots = 0;
   DevBuf G, F, R;
   int colCB = 1, lgCB = 0, colThreads = 256, prepThreads = 256, rowThreads = 256;
@@ -724,7 +828,7 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
       for (int v = 0; v < nc; ++v) counts[(size_t)tr[u] * nt + tc[v]]++;
   }
   std::vector<int64_t> off(ntiles + 1, 0);
-  const int64_t gpad = P->dtype == TOMO_F32 ? 4 : 16;
+  const int64_t gpad = 16;  // samples per half-precision matrix instruction
   for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + gpad - 1) / gpad * gpad;
   std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, 0, 0, 0));
   std::vector<int64_t> fill(off.begin(), off.end() - 1);
@@ -879,6 +983,10 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   const size_t plane = (size_t)n2 * n2;
   // G holds the float32 spectra of the adjoint as well
   if ((rc = P->G.alloc(sizeof(float2) * (size_t)P->ngrp * 4 * P->nsamp))) return fail(rc);
+  if (P->dtype == TOMO_F32 &&
+      ((rc = P->gmax.alloc(sizeof(float) * (size_t)P->ngrp * 4 * P->nproj)) ||
+       (rc = P->gsc.alloc(256))))
+    return fail(rc);
   if ((rc = P->part.alloc(sizeof(float) * 2048 * (size_t)P->ngrp * std::max(1, P->nslots))))
     return fail(rc);
   // F and R in the plan's storage precision (float32 when the adjoint may run)
@@ -963,16 +1071,24 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
   const int n2 = 2 * n;
   cpx<T> *G = P->G.as<cpx<T>>(), *F = P->F.as<cpx<T>>(), *R = P->R.as<cpx<T>>();
   {
-    ProfScope ps_(std::is_same<T, float>::value ? "k_fr_grid" : "k_fr_grid16", st);
     const dim3 gg((unsigned)ceil_div(P->nitems, kGridWaves), ng);
-    if constexpr (std::is_same<T, float>::value)
-      hipLaunchKernelGGL(k_fr_grid<T>, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),
-                         P->nitems, P->ent.as<int4>(), G, F, P->part.as<float>(), n, P->ntile,
-                         P->nslots, (int)P->nsamp, nzc, P->K2);
-    else
+    if constexpr (std::is_same<T, float>::value) {
+      {
+        ProfScope ps1_("k_fr_gscale", st);
+        hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, st, P->gmax.as<float>(),
+                           nzc * (int)P->nproj, P->gsc.as<int>());
+      }
+      ProfScope ps_("k_fr_grid", st);
+      hipLaunchKernelGGL(k_fr_grid_x3, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),
+                         P->nitems, P->ent.as<int4>(), (const float2 *)G, (float2 *)F,
+                         P->part.as<float>(), P->gsc.as<int>(), n, P->ntile, P->nslots,
+                         (int)P->nsamp, nzc, P->K2);
+    } else {
+      ProfScope ps_("k_fr_grid16", st);
       hipLaunchKernelGGL(k_fr_grid16<T>, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),
                          P->nitems, P->ent.as<int4>(), G, F, P->part.as<float>(), n, P->ntile,
                          P->nslots, (int)P->nsamp, nzc, P->K2);
+    }
   }
   TOMO_LAUNCH_CHECK();
   if (P->nred > 0) {
@@ -1012,7 +1128,7 @@ static int fr_backprojection_t(tomo_fr_plan *P, T *obj, const T *data, hipStream
     with_static_size(P->nsPrep, [&](auto ns) {
       hipLaunchKernelGGL((k_fr_prep<T, decltype(ns)::value>), dim3(nproj, ng * 4),
                          dim3(P->prepThreads), P->prepLds, st, data, P->G.as<cpx<T>>(), P->fft1.d,
-                         n, nproj, nz, nzc, P->scale1);
+                         n, nproj, nz, nzc, P->scale1, P->gmax.as<float>());
     });
   }
   TOMO_LAUNCH_CHECK();
@@ -1030,7 +1146,7 @@ static int fr_filter_backprojection_t(tomo_fr_plan *P, tomo_flt_plan *F, T *obj,
       hipLaunchKernelGGL((k_fr_filtprep<T, decltype(ns)::value>), dim3(nproj, ng * 4),
                          dim3(P->rowThreads), P->rowLds, st, data, P->G.as<cpx<T>>(),
                          F->H2.as<float2>(), F->HL.as<float>(), F->HR.as<float>(), F->fft2n.d,
-                         P->fft1.d, n, nproj, nz, nzc, P->scale1);
+                         P->fft1.d, n, nproj, nz, nzc, P->scale1, P->gmax.as<float>());
     });
   }
   TOMO_LAUNCH_CHECK();
@@ -1143,6 +1259,8 @@ extern "C" void fr_destroy(tomo_fr_plan *P) {
   P->items.release();
   P->red.release();
   P->part.release();
+  P->gmax.release();
+  P->gsc.release();
   P->phix.release();
   P->phiy.release();
   P->circ.release();
