@@ -205,13 +205,13 @@ __device__ __forceinline__ void gauss_pair(const int4 &e, int i, float K2, float
 // in a fixed order; chunks of heavy tiles land in private partial buffers
 // that k_fr_grid_reduce sums in chunk order, so the result is bitwise
 // reproducible.
-// 16-bit plans: the same tile products with 16 samples per matrix
-// instruction (v_mfma_f32_16x16x16 f16 / bf16, FP32 accumulation).  Lane l
-// supplies A[l%16][4(l/16)+u] and B[4(l/16)+u][l%16] for u < 4, so lane group
-// l/16 evaluates the Gaussian factors of samples 4(l/16)..4(l/16)+3 of each
-// batch of 16; the factors and the weighted spectra are rounded to the
-// storage type, the sums stay in FP32.  Tile lists are padded to multiples of
-// 16 for these plans.
+// 16-bit plans: the same tile products with 32 samples per matrix
+// instruction (v_mfma_f32_16x16x32 f16 / bf16, FP32 accumulation).  Lane l
+// supplies the 8 K entries of row l%16 of A and of column l%16 of B that
+// belong to samples 8(l/16)..8(l/16)+7 of each batch of 32 (the same samples
+// for both operands, so the order of K inside the instruction does not
+// matter); the factors and the weighted spectra are rounded to the storage
+// type, the sums stay in FP32.  Tile lists are padded to multiples of 32.
 template <typename T>
 struct Mma16;
 template <>
@@ -222,6 +222,16 @@ struct Mma16<_Float16> {
   }
   static __device__ __forceinline__ floatx4 mma(v4 a, v4 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+  }
+  typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ v8 pack8(const float *x) {
+    v8 r;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = (_Float16)x[u];
+    return r;
+  }
+  static __device__ __forceinline__ floatx4 mma8(v8 a, v8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
 };
 template <>
@@ -235,6 +245,16 @@ struct Mma16<__bf16> {
   }
   static __device__ __forceinline__ floatx4 mma(v4 a, v4 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ v8 pack8(const float *x) {
+    v8 r;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = (__bf16)x[u];
+    return r;
+  }
+  static __device__ __forceinline__ floatx4 mma8(v8 a, v8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
 };
 
@@ -276,27 +296,26 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
     for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = tof2(rg[c]);
     __builtin_amdgcn_wave_barrier();
     if (bb + kGridBatch < eend) fetch(bb + kGridBatch);
-    const int ngr = min(kGridBatch, eend - bb) >> 4;
+    const int ngr = min(kGridBatch, eend - bb) >> 5;
     for (int q = 0; q < ngr; ++q) {
-      float e0v[4], e1v[4], g[4][8];
+      // 32 samples per instruction: lane group g4 supplies samples
+      // 32q + 8 g4 + u (u < 8) to both operands
+      float e0v[8], e1v[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int slot_e = 16 * q + 4 * g4 + u;
-        const int4 cur = se[wave][buf][slot_e];
+      for (int u = 0; u < 8; ++u)
+        gauss_pair(se[wave][buf][32 * q + 8 * g4 + u], i, K2, e0v[u], e1v[u]);
+      const typename M::v8 b = M::pack8(e0v);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const float2 v = sg[wave][buf][c][slot_e];
-          g[u][2 * c] = v.x;
-          g[u][2 * c + 1] = v.y;
+      for (int c = 0; c < 4; ++c) {
+        float ar[8], ai[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float2 v = sg[wave][buf][c][32 * q + 8 * g4 + u];
+          ar[u] = e1v[u] * v.x;
+          ai[u] = e1v[u] * v.y;
         }
-        gauss_pair(cur, i, K2, e0v[u], e1v[u]);
-      }
-      const typename M::v4 b = M::pack(e0v[0], e0v[1], e0v[2], e0v[3]);
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const typename M::v4 a = M::pack(e1v[0] * g[0][m], e1v[1] * g[1][m], e1v[2] * g[2][m],
-                                          e1v[3] * g[3][m]);
-        acc[m] = M::mma(a, b, acc[m]);
+        acc[2 * c] = M::mma8(M::pack8(ar), b, acc[2 * c]);
+        acc[2 * c + 1] = M::mma8(M::pack8(ai), b, acc[2 * c + 1]);
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -343,7 +362,7 @@ __global__ __launch_bounds__(256) void k_fr_gscale(const float *__restrict__ gma
 }
 
 // The float32 gridding on the half-precision matrix cores.  Same tile
-// products and lane layout as k_fr_grid16 (16 samples per instruction), with
+// products and lane layout as k_fr_grid16 (32 samples per instruction), with
 // each float32 factor split into half-precision high and low parts,
 // x = hi + lo (hi: x with its 13 low mantissa bits cleared, exact in half
 // precision; lo = x - hi rounded toward zero), and three products
@@ -370,10 +389,26 @@ __device__ __forceinline__ half4_t pack_rtz(float a, float b, float c, float d) 
   return __builtin_bit_cast(half4_t, u);
 }
 
-__device__ __forceinline__ void split4(const float *x, half4_t &hi, half4_t &lo) {
-  const float h0 = hi_part(x[0]), h1 = hi_part(x[1]), h2 = hi_part(x[2]), h3 = hi_part(x[3]);
-  hi = pack_rtz(h0, h1, h2, h3);
-  lo = pack_rtz(x[0] - h0, x[1] - h1, x[2] - h2, x[3] - h3);
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ half8_t pack8_rtz(const float *x) {
+  uint4 u;
+  u.x = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[0], x[1]));
+  u.y = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[2], x[3]));
+  u.z = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[4], x[5]));
+  u.w = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[6], x[7]));
+  return __builtin_bit_cast(half8_t, u);
+}
+
+__device__ __forceinline__ void split8(const float *x, half8_t &hi, half8_t &lo) {
+  float h[8], l[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    h[u] = hi_part(x[u]);
+    l[u] = x[u] - h[u];
+  }
+  hi = pack8_rtz(h);
+  lo = pack8_rtz(l);
 }
 
 // Gaussian factors as gauss_pair, times 2^s0 (columns) and 2^s1 (rows).
@@ -426,32 +461,34 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
     for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = rg[c];
     __builtin_amdgcn_wave_barrier();
     if (bb + kGridBatch < eend) fetch(bb + kGridBatch);
-    const int ngr = min(kGridBatch, eend - bb) >> 4;
+    const int ngr = min(kGridBatch, eend - bb) >> 5;
     for (int q = 0; q < ngr; ++q) {
-      float e0v[4], e1v[4], g[4][8];
+      // lane group g4 supplies samples 32q + 8 g4 + u (u < 8) to both
+      // operands, so the K order inside the instruction does not matter
+      float e0v[8], e1v[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int slot_e = 16 * q + 4 * g4 + u;
-        const int4 cur = se[wave][buf][slot_e];
+      for (int u = 0; u < 8; ++u)
+        gauss_pair_s(se[wave][buf][32 * q + 8 * g4 + u], i, K2, sB, sA, e0v[u], e1v[u]);
+      half8_t bh, bl;
+      split8(e0v, bh, bl);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const float2 v = sg[wave][buf][c][slot_e];
-          g[u][2 * c] = v.x;
-          g[u][2 * c + 1] = v.y;
+      for (int c = 0; c < 4; ++c) {
+        float ar[8], ai[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float2 v = sg[wave][buf][c][32 * q + 8 * g4 + u];
+          ar[u] = e1v[u] * v.x;
+          ai[u] = e1v[u] * v.y;
         }
-        gauss_pair_s(cur, i, K2, sB, sA, e0v[u], e1v[u]);
-      }
-      half4_t bh, bl;
-      split4(e0v, bh, bl);
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const float a[4] = {e1v[0] * g[0][m], e1v[1] * g[1][m], e1v[2] * g[2][m],
-                            e1v[3] * g[3][m]};
-        half4_t ah, al;
-        split4(a, ah, al);
-        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[m], 0, 0, 0);
-        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[m], 0, 0, 0);
-        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[m], 0, 0, 0);
+        half8_t ah, al;
+        split8(ar, ah, al);
+        acc[2 * c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[2 * c], 0, 0, 0);
+        acc[2 * c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[2 * c], 0, 0, 0);
+        acc[2 * c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[2 * c], 0, 0, 0);
+        split8(ai, ah, al);
+        acc[2 * c + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[2 * c + 1], 0, 0, 0);
+        acc[2 * c + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[2 * c + 1], 0, 0, 0);
+        acc[2 * c + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[2 * c + 1], 0, 0, 0);
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -828,7 +865,7 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
       for (int v = 0; v < nc; ++v) counts[(size_t)tr[u] * nt + tc[v]]++;
   }
   std::vector<int64_t> off(ntiles + 1, 0);
-  const int64_t gpad = 16;  // samples per half-precision matrix instruction
+  const int64_t gpad = 32;  // samples per half-precision matrix instruction
   for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + gpad - 1) / gpad * gpad;
   std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, 0, 0, 0));
   std::vector<int64_t> fill(off.begin(), off.end() - 1);
