@@ -275,6 +275,19 @@ int pre_remove_stripe_ti(tomo_pre_plan *plan, float *res, double beta, double ma
                          void *stream);
 
 /*
+ * ProcFunctions.pad360 (src/tomocupy/processing/proc_functions.py:109-124),
+ * the double field of view of 360-degree scans (file_type double_fov,
+ * :193-194), on sinograms in the plan's layout: sino (nz, nproj, n) ->
+ * out (nz, nproj, 2n), both of dtype.  The detector axis is flipped when
+ * flip != 0 (rotation axis in the left half, reader.py:143-148), the last
+ * w = max(1, int(2 (n - center))) columns are tapered by
+ * t^5 (126 - 420 t + 540 t^2 - 315 t^3 + 70 t^4), t = 1 - i/w (FP64), and the
+ * right half is zero.  `center` is the axis after the flip rule (>= n/2).
+ */
+int pre_pad360(tomo_pre_plan *plan, const void *sino, void *out, int dtype, double center,
+               int flip, void *stream);
+
+/*
  * Fourier-wavelet stripe removal, remove_stripe_fw
  * (src/tomocupy/processing/remove_stripe.py:253-293), in place on
  *   res (nproj, nz, n) float32.

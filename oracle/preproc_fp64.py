@@ -58,6 +58,22 @@ def remove_stripe_ti(data, beta, mask_size):
     return data + v * mask
 
 
+def pad360(sino, centeri):
+    """``ProcFunctions.pad360`` (P:109-124) on sinograms ``(nz, nproj, ni)``
+    with the centre rule of ``reader.py:143-148``: returns ``(padded
+    (nz, nproj, 2 ni) FP64, centre of the padded sinogram, flipped)``."""
+    sino = np.asarray(sino, dtype=np.float64)
+    ni = sino.shape[-1]
+    flip = centeri < ni // 2
+    center = ni - centeri if flip else centeri
+    data = (sino[..., ::-1] if flip else sino).copy()
+    w = max(1, int(2 * (ni - center)))
+    v = np.linspace(1, 0, w, endpoint=False)
+    v = v ** 5 * (126 - 420 * v + 540 * v ** 2 - 315 * v ** 3 + 70 * v ** 4)
+    data[..., -w:] *= v
+    return np.pad(data, ((0, 0), (0, 0), (0, ni)), 'constant'), center, flip
+
+
 def remove_outliers(data, width, threshold):
     """``ProcFunctions.remove_outliers`` (P:96-107) on a uint16 ``(rows, nz, n)``
     array: median over ``[width, 1, width]`` ('reflect' borders), then

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     src = open(os.path.join(ROOT, 'include', 'tomohip.h')).read()
     src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
-    return sorted(set(re.findall(r'\b([a-z_]+)\s*\(', src)) - {'if', 'sizeof'})
+    return sorted(set(re.findall(r'\b([a-z_][a-z0-9_]*)\s*\(', src)) - {'if', 'sizeof'})
 
 
 def test_exports_every_declared_symbol():

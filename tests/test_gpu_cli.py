@@ -84,3 +84,53 @@ def test_cli_try(tmp_path):
         err = _rel(got.astype(np.float64), ref[k])
         print('centre', centre, err)
         assert err <= 1e-5
+
+
+def test_cli_double_fov_full_and_try(tmp_path):
+    # 360-degree scan with the axis near the left border: the CLI flips,
+    # tapers and pads to 2 ni (reader.py:143-148, proc_functions.py:109-124)
+    ni, nproj, nz = 64, 90, 2
+    theta = (np.arange(nproj) * (2 * np.pi / nproj)).astype(np.float32)
+    data, dark, flat, _ = synth.raw_counts(7, ni, theta, nz, ndark=3, nflat=4)
+    p = tmp_path / 'scan360.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    centeri = 10.0
+    n = 2 * ni
+    ne = 4 * n
+    sino = pp.proc_chain(data, dark, flat)
+    padded, center, flip = pp.pad360(sino, centeri)
+    assert flip
+    out = str(tmp_path / 'rec360')
+    main(['recon', '--file-name', str(p), '--reconstruction-type', 'full', '--nsino-per-chunk',
+          '2', '--out-path-name', out, '--file-type', 'double_fov', '--rotation-axis',
+          str(centeri)])
+    ref = fr.backprojection(ff.fbp_filter_center(padded, ff.calc_filter(ne, 'parzen'), ne,
+                                                 center), th)
+    for k in range(nz):
+        got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k))
+        assert got.shape == (n, n)
+        err = _rel(got.astype(np.float64), ref[k])
+        print('double-fov full slice', k, err)
+        assert err <= 1e-5
+    # try mode: the shifts are mirrored with the data (reader.py:233-234);
+    # file names carry the centres of the original detector frame
+    outt = str(tmp_path / 'try360')
+    main(['recon', '--file-name', str(p), '--reconstruction-type', 'try', '--out-path-name',
+          outt, '--file-type', 'double_fov', '--rotation-axis', str(centeri),
+          '--center-search-width', '2', '--center-search-step', '1', '--nsino', '0.0',
+          '--nsino-per-chunk', '4'])
+    shifts = np.arange(-2, 2, 1.0)
+    ncz = 4
+    shv = np.zeros(ncz)
+    shv[:len(shifts)] = -shifts
+    batch = np.concatenate([padded[:1]] * ncz)
+    reft = fr.backprojection(ff.fbp_filter_center(batch, ff.calc_filter(ne, 'parzen'), ne,
+                                                  center, shv), th)
+    for k, sh in enumerate(shifts):
+        got = io.read_tiff(os.path.join(outt, 'recon_slice%04d_center%05.2f.tiff'
+                                        % (0, centeri - sh)))
+        err = _rel(got.astype(np.float64), reft[k])
+        print('double-fov try centre', centeri - sh, err)
+        assert err <= 1e-5

@@ -193,3 +193,25 @@ def test_ti_mask_and_constant_rows():
     # rows constant along the detector carry no stripe: unchanged
     flat = np.broadcast_to(rng.random((5, 1, 1)), (5, 1, 20)).copy()
     assert np.allclose(pp.remove_stripe_ti(flat, 0.1, 1.0), flat, atol=1e-15)
+
+
+def test_pad360_taper_closed_forms():
+    # the taper t^5 (126 - 420 t + 540 t^2 - 315 t^3 + 70 t^4) is the order-9
+    # smoothstep: 1 at t = 1, 1/2 at t = 1/2, 0 at t = 0, monotone in between
+    ni = 40
+    ones = np.ones((1, 1, ni))
+    out, center, flip = pp.pad360(ones, 30.0)       # w = 2 (40 - 30) = 20
+    assert not flip and center == 30.0 and out.shape == (1, 1, 2 * ni)
+    taper = out[0, 0, ni - 20:ni]
+    assert taper[0] == 1.0 and abs(taper[10] - 0.5) < 1e-15
+    assert np.all(np.diff(taper) < 0) and np.all(out[0, 0, :ni - 20] == 1.0)
+    assert np.all(out[0, 0, ni:] == 0.0)
+
+
+def test_pad360_flip_rule():
+    ni = 32
+    x = np.arange(ni, dtype=np.float64)[None, None, :]
+    out, center, flip = pp.pad360(x, 5.0)            # axis in the left half
+    assert flip and center == ni - 5.0
+    w = int(2 * (ni - center))
+    assert np.array_equal(out[0, 0, :ni - w], x[0, 0, ::-1][:ni - w])

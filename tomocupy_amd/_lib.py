@@ -62,6 +62,7 @@ SIGNATURES = {
                                 ctypes.c_char_p, _c_int, _c_p]),
     'pre_remove_outliers': (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_int, _c_p]),
     'pre_remove_stripe_ti': (_c_int, [_c_p, _c_p, _c_d, _c_d, _c_p]),
+    'pre_pad360': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_p]),
     'pre_destroy': (None, [_c_p]),
 }
 

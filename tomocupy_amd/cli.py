@@ -13,9 +13,10 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
   shift of the same sinogram (``rec.py:210-275``), one TIFF per centre
   ``<out>/recon_slice%04d_center%05.2f.tiff`` (``writer.py:309-313``).
 
-Stripe removal: ``fw`` and ``ti``; zinger removal: ``--dezinger``.  Settings
-outside this path (binning, double-FOV, vo-all stripe removal, phase
-retrieval, ...) are not offered.  Input formats: see
+Stripe removal: ``fw`` and ``ti``; zinger removal: ``--dezinger``; 360-degree
+scans with the axis near the border: ``--file-type double_fov``.  Settings
+outside this path (binning, vo-all stripe removal, phase retrieval, ...) are
+not offered.  Input formats: see
 :mod:`tomocupy_amd.io`.
 """
 import argparse
@@ -40,6 +41,7 @@ def parser():
     r.add_argument('--out-path-name', default=None)
     r.add_argument('--reconstruction-type', default='try', choices=['full', 'try'])
     r.add_argument('--reconstruction-algorithm', default='fourierrec', choices=['fourierrec'])
+    r.add_argument('--file-type', default='standard', choices=['standard', 'double_fov'])
     r.add_argument('--rotation-axis', type=float, default=-1.0)
     r.add_argument('--center-search-width', type=float, default=50.0)
     r.add_argument('--center-search-step', type=float, default=0.5)
@@ -99,11 +101,11 @@ def recon_full(a, ds):
                  fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                  flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads,
                  dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
-                 ti_beta=a.ti_beta, ti_mask=a.ti_mask)
-    ncz = rec.ncz
+                 ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type)
+    ncz, nrec = rec.ncz, rec.n
     stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
                                       dtype=np.float16 if a.dtype == 'float16' else np.float32,
-                                      shape=(nz, ds.ni, ds.ni)) if a.save_format == 'npy' else None
+                                      shape=(nz, nrec, nrec)) if a.save_format == 'npy' else None
 
     def read(k):
         s0 = z0 + k * ncz
@@ -124,7 +126,7 @@ def recon_full(a, ds):
         stack.flush()
     dt = time.perf_counter() - t
     print('reconstructed %d slices of %dx%d in %.2f s (%.1f slices/s) -> %s'
-          % (nz, ds.ni, ds.ni, dt, nz / dt, out))
+          % (nz, nrec, nrec, dt, nz / dt, out))
     return out
 
 
@@ -151,18 +153,26 @@ def recon_try(a, ds):
                        flat_linear=_bool(a.flat_linear), remove_stripe_method=stripe,
                        fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level,
                        dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
-                 ti_beta=a.ti_beta, ti_mask=a.ti_mask)
-    bf = BackprojFunctions(ds.ni, ds.nproj, ncz, theta, centre, a.dtype, a.fbp_filter)
+                       ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type,
+                       center=centre)
+    nrec, rcentre, sgn = ds.ni, centre, 1.0
+    if a.file_type == 'double_fov':
+        # reader.py:143-148 and 233-234: the axis moves to the right half of a
+        # 2 ni wide sinogram; mirrored data take mirrored shifts
+        nrec = 2 * ds.ni
+        if centre < ds.ni // 2:
+            rcentre, sgn = ds.ni - centre, -1.0
+    bf = BackprojFunctions(nrec, ds.nproj, ncz, theta, rcentre, a.dtype, a.fbp_filter)
     written = []
     for idz in ids:
         data, dark, flat = (torch.from_numpy(x).cuda() for x in ds.rows(z0 + idz, z0 + idz + 1))
-        sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, ni)
+        sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, nrec)
         batch = sino.expand(ncz, -1, -1).contiguous()
-        obj = torch.empty((ncz, ds.ni, ds.ni), dtype=td, device='cuda')
+        obj = torch.empty((ncz, nrec, nrec), dtype=td, device='cuda')
         for c0 in range(0, len(shifts), ncz):
             lz = min(ncz, len(shifts) - c0)
             sht = torch.zeros(ncz, dtype=torch.float32, device='cuda')
-            sht[:lz] = torch.from_numpy(shifts[c0:c0 + lz]).cuda()
+            sht[:lz] = torch.from_numpy(sgn * shifts[c0:c0 + lz]).cuda()
             bf.fbp_filter_backprojection(obj, batch, sht)
             host = obj[:lz].float().cpu().numpy()
             for kk in range(lz):

@@ -3,7 +3,9 @@
 Mirrors ``ProcFunctions`` (``src/tomocupy/processing/proc_functions.py:46-198``)
 for the steps on the fourierrec path: ``remove_outliers`` (dezinger),
 ``darkflat_correction``, ``minus_log``, ``proc_sino`` (outliers, dark/flat,
-stripe removal) and ``proc_proj`` (minus log).
+stripe removal), ``proc_proj`` (minus log) and ``pad360`` (double field of
+view: ``file_type='double_fov'`` with the rotation axis ``center`` in detector
+columns doubles the sinogram width).
 Settings the reference reads from the global ``args`` are constructor
 arguments.  ``proc_chunk`` is the fused form used by the conveyor: raw uint16
 chunk -> dark/flat -> [stripe removal] -> minus log -> sinogram order.
@@ -19,7 +21,8 @@ class ProcFunctions:
     def __init__(self, nproj, ncz, n, ndark, nflat, dtype='float32', minus_log=True,
                  bright_ratio=1.0, flat_linear=False, remove_stripe_method='none',
                  fw_sigma=1.0, fw_filter='sym16', fw_level=7, dezinger=0,
-                 dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0):
+                 dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0, file_type='standard',
+                 center=None):
         if remove_stripe_method not in ('none', 'fw', 'ti'):
             raise NotImplementedError('remove_stripe_method %r is not implemented'
                                       % remove_stripe_method)
@@ -36,6 +39,17 @@ class ProcFunctions:
         self.dezinger, self.dezinger_threshold = int(dezinger), int(dezinger_threshold)
         self.ti_beta, self.ti_mask = float(ti_beta), float(ti_mask)
         self._res = None  # float32 transmission chunk of the unfused ti path
+        if file_type not in ('standard', 'double_fov'):
+            raise ValueError('file_type must be standard or double_fov')
+        self.file_type = file_type
+        if file_type == 'double_fov':
+            # reader.py:143-148: the axis is moved to the right half (flip)
+            if center is None:
+                raise ValueError('double_fov needs the rotation axis position (center)')
+            self.flip = float(center) < self.n // 2
+            self.center360 = self.n - float(center) if self.flip else float(center)
+            self._sino_ni = None
+        self.nout = 2 * self.n if file_type == 'double_fov' else self.n
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().pre_create(ctypes.byref(h), self.nproj, self.ncz, self.n,
                                          self.ndark, self.nflat), 'pre_create')
@@ -98,6 +112,21 @@ class ProcFunctions:
             _lib.stream_ptr(None)), 'pre_remove_stripe_ti')
         return res
 
+    def pad360(self, sino, out=None):
+        """Double field of view (``proc_functions.py:109-124``) on sinograms
+        ``(ncz, nproj, n)``: flip when the axis is in the left half, taper the
+        last ``max(1, int(2 (n - center)))`` columns, zero-pad to ``2n``."""
+        import torch
+        td = _lib.torch_dtype(self.dtype)
+        if out is None:
+            out = torch.empty((self.ncz, self.nproj, 2 * self.n), dtype=td, device=sino.device)
+        _lib.check(_lib.lib().pre_pad360(
+            self._h, _lib.dev_ptr(sino, (self.ncz, self.nproj, self.n), td, 'sino'),
+            _lib.dev_ptr(out, (self.ncz, self.nproj, 2 * self.n), td, 'out'),
+            _lib.DTYPES[self.dtype], self.center360, int(self.flip), _lib.stream_ptr(None)),
+            'pre_pad360')
+        return out
+
     def proc_sino(self, data, dark, flat, res=None):
         """outliers, dark/flat correction, then stripe removal
         (``proc_functions.py:140-161``)."""
@@ -125,9 +154,20 @@ class ProcFunctions:
         return out
 
     def proc_chunk(self, data, dark, flat, out=None, res=None):
-        """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, n)``.
-        With ``dezinger > 0`` the outliers of ``data``, ``dark`` and ``flat``
-        are removed in place first, as ``proc_sino`` does."""
+        """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, nout)``
+        (``nout = 2n`` for double field of view, else ``n``).  With
+        ``dezinger > 0`` the outliers of ``data``, ``dark`` and ``flat`` are
+        removed in place first, as ``proc_sino`` does."""
+        import torch
+        if self.file_type == 'double_fov':
+            if self._sino_ni is None or self._sino_ni.device != data.device:
+                self._sino_ni = torch.empty((self.ncz, self.nproj, self.n),
+                                            dtype=_lib.torch_dtype(self.dtype), device=data.device)
+            self._proc_chunk(data, dark, flat, self._sino_ni, res)
+            return self.pad360(self._sino_ni, out)
+        return self._proc_chunk(data, dark, flat, out, res)
+
+    def _proc_chunk(self, data, dark, flat, out, res):
         import torch
         sd, sk, sf, u16 = self._shapes()
         self._outliers(data, dark, flat)

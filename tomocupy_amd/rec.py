@@ -70,21 +70,32 @@ class GPURec:
                  fbp_filter='parzen', remove_stripe_method='none', fw_sigma=1.0,
                  fw_filter='sym16', fw_level=7, minus_log=True, bright_ratio=1.0,
                  flat_linear=False, max_write_threads=8, dezinger=0, dezinger_threshold=5000,
-                 ti_beta=0.022, ti_mask=1.0):
+                 ti_beta=0.022, ti_mask=1.0, file_type='standard'):
+        """``n``: detector width of the data; the reconstruction is ``n x n``,
+        or ``2n x 2n`` for ``file_type='double_fov'`` (reader.py:143-148: the
+        rotation axis ``center`` is mirrored into the right half when it lies in
+        the left one)."""
         import torch
         if ncz == 1:
             ncz = 2  # fourierrec pairs two slices (reader.py:113-115)
+        self.ni = int(n)
+        n = 2 * self.ni if file_type == 'double_fov' else self.ni
+        rec_center = center
+        if file_type == 'double_fov' and float(center) < self.ni // 2:
+            rec_center = self.ni - float(center)
         self.n, self.nproj, self.nz, self.ncz = int(n), int(nproj), int(nz), int(ncz)
         self.ndark, self.nflat = int(ndark), int(nflat)
         self.nzchunk = -(-self.nz // self.ncz)
         self.lzchunk = [min(self.ncz, self.nz - k * self.ncz) for k in range(self.nzchunk)]
-        self.pf = ProcFunctions(self.nproj, self.ncz, self.n, self.ndark, self.nflat, dtype,
+        self.pf = ProcFunctions(self.nproj, self.ncz, self.ni, self.ndark, self.nflat, dtype,
                                 minus_log=minus_log, bright_ratio=bright_ratio,
                                 flat_linear=flat_linear, remove_stripe_method=remove_stripe_method,
                                 fw_sigma=fw_sigma, fw_filter=fw_filter, fw_level=fw_level,
                                 dezinger=dezinger, dezinger_threshold=dezinger_threshold,
-                                ti_beta=ti_beta, ti_mask=ti_mask)
-        self.bf = BackprojFunctions(self.n, self.nproj, self.ncz, theta, center, dtype, fbp_filter)
+                                ti_beta=ti_beta, ti_mask=ti_mask, file_type=file_type,
+                                center=center)
+        self.bf = BackprojFunctions(self.n, self.nproj, self.ncz, theta, rec_center, dtype,
+                                    fbp_filter)
         self.stream1 = torch.cuda.Stream()
         self.stream2 = torch.cuda.Stream()
         self.stream3 = torch.cuda.Stream()
@@ -100,8 +111,8 @@ class GPURec:
         self.bf.fbp_filter_backprojection(rec, out)
 
     def recon_resident(self, raw, dark, flat, rec):
-        """Reconstruct a GPU-resident volume: ``raw (nzchunk, nproj, ncz, n)`` uint16,
-        ``dark``/``flat`` ``(nzchunk, ndark|nflat, ncz, n)``, into
+        """Reconstruct a GPU-resident volume: ``raw (nzchunk, nproj, ncz, ni)`` uint16,
+        ``dark``/``flat`` ``(nzchunk, ndark|nflat, ncz, ni)``, into
         ``rec (nzchunk*ncz, n, n)`` of the plan's dtype, on stream 2."""
         import torch
         with torch.cuda.stream(self.stream2):
@@ -113,18 +124,18 @@ class GPURec:
         """Conveyor over host data.
 
         ``read_chunk(k)`` returns ``(data, dark, flat)`` numpy uint16 arrays of
-        shapes ``(nproj, lz, n)``, ``(ndark, lz, n)``, ``(nflat, lz, n)`` with
+        shapes ``(nproj, lz, ni)``, ``(ndark, lz, ni)``, ``(nflat, lz, ni)`` with
         ``lz = lzchunk[k]``; ``write_chunk(k, rec)`` receives a ``(lz, n, n)``
         numpy view (float32, or float16 for 16-bit plans) that is valid until it
         returns.
         """
         import torch
-        ncz, nproj, n = self.ncz, self.nproj, self.n
+        ncz, nproj, n, ni = self.ncz, self.nproj, self.n, self.ni
         pin = dict(pin_memory=True)
         item_pinned = {
-            'data': torch.zeros((2, nproj, ncz, n), dtype=torch.uint16, **pin),
-            'dark': torch.zeros((2, self.ndark, ncz, n), dtype=torch.uint16, **pin),
-            'flat': torch.ones((2, self.nflat, ncz, n), dtype=torch.uint16, **pin),
+            'data': torch.zeros((2, nproj, ncz, ni), dtype=torch.uint16, **pin),
+            'dark': torch.zeros((2, self.ndark, ncz, ni), dtype=torch.uint16, **pin),
+            'flat': torch.ones((2, self.nflat, ncz, ni), dtype=torch.uint16, **pin),
         }
         item_gpu = {k: torch.empty_like(v, device='cuda') for k, v in item_pinned.items()}
         item_gpu['flat'].copy_(item_pinned['flat'])
@@ -156,7 +167,7 @@ class GPURec:
                     item_pinned['data'][k % 2, :, lz:].zero_()
                     item_pinned['dark'][k % 2, :, lz:].zero_()
                     item_pinned['flat'][k % 2, :, lz:].copy_(
-                        torch.ones((self.nflat, ncz - lz, n), dtype=torch.int32).to(torch.uint16))
+                        torch.ones((self.nflat, ncz - lz, ni), dtype=torch.int32).to(torch.uint16))
                 with torch.cuda.stream(self.stream1):
                     for key in ('data', 'dark', 'flat'):
                         item_gpu[key][k % 2].copy_(item_pinned[key][k % 2], non_blocking=True)
