@@ -127,8 +127,61 @@ __global__ __launch_bounds__(256) void k_fw_rows(const void *__restrict__ src_,
   }
   // linear flat interpolation weight of projection p (proc_functions.py:62-66)
   const float vlin = (MODE == 2 && nproj > 1) ? (float)((double)p / (double)(nproj - 1)) : 0.f;
+  // Unreflected interior of a row with W % 4 == 0 and aligned storage: each
+  // thread loads two quadruples of columns (8- or 16-byte loads of the counts
+  // or values, 16-byte loads of the dark/flat means) before converting them;
+  // only the L-2 reflected border samples are then loaded one by one.
+  const bool vec = MODE != 1 && !zero && (W & 3) == 0 &&
+                   ((MODE == 2 ? (uintptr_t)rawrow & 7 : (uintptr_t)row & 15) == 0);
+  if (vec) {
+    for (int q0 = threadIdx.x; 4 * q0 < W; q0 += 2 * 256) {
+      float v[2][4];
+#pragma unroll
+      for (int h2 = 0; h2 < 2; ++h2) {
+        const int x0 = 4 * (q0 + h2 * 256);
+        if (x0 < W) {
+          if (MODE == 2) {
+            const size_t zx = (size_t)z * W + x0;
+            const ushort4 r4 = *reinterpret_cast<const ushort4 *>(rawrow + x0);
+            const float4 d4 = *reinterpret_cast<const float4 *>(df.d0 + zx);
+            const float4 a4 = *reinterpret_cast<const float4 *>(df.fa + zx);
+            const float4 b4 = df.flat_linear ? *reinterpret_cast<const float4 *>(df.fb + zx)
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+            v[h2][0] = darkflat((float)r4.x, d4.x, a4.x, b4.x, vlin, df.flat_linear);
+            v[h2][1] = darkflat((float)r4.y, d4.y, a4.y, b4.y, vlin, df.flat_linear);
+            v[h2][2] = darkflat((float)r4.z, d4.z, a4.z, b4.z, vlin, df.flat_linear);
+            v[h2][3] = darkflat((float)r4.w, d4.w, a4.w, b4.w, vlin, df.flat_linear);
+          } else {
+            const float4 f4 = *reinterpret_cast<const float4 *>(row + x0);
+            v[h2][0] = f4.x;
+            v[h2][1] = f4.y;
+            v[h2][2] = f4.z;
+            v[h2][3] = f4.w;
+          }
+        }
+      }
+#pragma unroll
+      for (int h2 = 0; h2 < 2; ++h2) {
+        const int x0 = 4 * (q0 + h2 * 256);
+        if (x0 < W)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) xs[pad + x0 + c] = v[h2][c];
+      }
+    }
+    for (int b = threadIdx.x; b < np - W; b += 256) {
+      const int i = b < pad ? b : b + W;  // staged index of a reflected sample
+      const int x = refl_fast(i - pad, W);
+      if (MODE == 2) {
+        const size_t zx = (size_t)z * W + x;
+        xs[i] = darkflat((float)rawrow[x], df.d0[zx], df.fa[zx],
+                         df.flat_linear ? df.fb[zx] : 0.f, vlin, df.flat_linear);
+      } else {
+        xs[i] = row[x];
+      }
+    }
+  }
   constexpr int U = 8;
-  for (int i0 = threadIdx.x; i0 < np; i0 += U * 256) {
+  for (int i0 = vec ? np : threadIdx.x; i0 < np; i0 += U * 256) {
     float v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -506,13 +559,25 @@ __global__ __launch_bounds__(256) void k_fw_syn_rows(const float *__restrict__ l
         }
       }
     }
+    float yo[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const int x0 = 4 * u + c;
-      if (x0 >= nout) continue;
-      float yc = y[c] + xin[c];
-      if (LAST == 2 && take_log) yc = minus_log(yc);
-      orow[x0] = (TO)yc;
+      yo[c] = y[c] + xin[c];
+      if (LAST == 2 && take_log) yo[c] = minus_log(yo[c]);
+    }
+    if (LAST == 2 && vec4 && 4 * u + 3 < nout &&
+        ((uintptr_t)(orow + 4 * u) & (4 * sizeof(TO) - 1)) == 0) {
+      // one 16-byte (float) or 8-byte (16-bit) store per thread
+      if constexpr (sizeof(TO) == 4) {
+        *reinterpret_cast<float4 *>(orow + 4 * u) = make_float4(yo[0], yo[1], yo[2], yo[3]);
+      } else {
+        TO q[4] = {(TO)yo[0], (TO)yo[1], (TO)yo[2], (TO)yo[3]};
+        *reinterpret_cast<uint2 *>(orow + 4 * u) = __builtin_bit_cast(uint2, q);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (4 * u + c < nout) orow[4 * u + c] = (TO)yo[c];
     }
   }
 }
