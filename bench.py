@@ -34,7 +34,9 @@ ISO_CHUNKS = 16  # chunks of the untimed one-stream breakdown pass
 KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_xsum', 'k_fw_syn_cols',
            'k_fw_syn_rows', 'k_pre_log_swap', 'k_flt_weights', 'k_flt_rows', 'k_flt_prepare',
            'k_fr_prep', 'k_fr_filtprep',
-           'k_fr_grid', 'k_fr_grid16', 'k_fr_grid_reduce', 'k_fr_rowfft', 'k_fr_colfft']
+           'k_fr_gscale', 'k_fr_grid_x3', 'k_fr_grid16', 'k_fr_grid_reduce', 'k_fr_rowfft',
+           'k_fr_colfft', 'k_pre_median', 'k_pre_dezinger', 'k_ti_mean', 'k_ti_conv', 'k_ti_add',
+           'k_pre_pad360']
 
 
 def parse():
@@ -118,7 +120,7 @@ def algorithmic_bytes(name, a, s):
     c = 2 * r
     nzc = (ch + 1) // 2
     grp = 4  # complex slices per gridding group
-    if name in ('k_fr_grid', 'k_fr_grid16'):  # read the polar spectra, write the 2n x 2n grid
+    if name in ('k_fr_grid_x3', 'k_fr_grid16'):  # read the polar spectra, write the 2n x 2n grid
         return grp * (nproj * n * c + (2 * n) ** 2 * c)
     if name == 'k_fr_rowfft':    # read the grid rows, write the pruned half
         return nzc * ((2 * n) ** 2 * c + 2 * n * n * c)

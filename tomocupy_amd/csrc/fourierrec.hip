@@ -1115,7 +1115,7 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
         hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, st, P->gmax.as<float>(),
                            nzc * (int)P->nproj, P->gsc.as<int>());
       }
-      ProfScope ps_("k_fr_grid", st);
+      ProfScope ps_("k_fr_grid_x3", st);
       hipLaunchKernelGGL(k_fr_grid_x3, gg, dim3(64 * kGridWaves), 0, st, P->items.as<int4>(),
                          P->nitems, P->ent.as<int4>(), (const float2 *)G, (float2 *)F,
                          P->part.as<float>(), P->gsc.as<int>(), n, P->ntile, P->nslots,
