@@ -241,6 +241,32 @@ int pre_darkflat(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark
                  void *stream);
 
 /*
+ * pre_process and pre_darkflat on float32 counts: the binned frames of
+ * pre_bin (with binning > 0 the reference reads its frames as float and sums
+ * them, dataio/reader.py:96-99, 350-355; the dark/flat correction is then
+ * computed on those sums, proc_functions.py:55-74).  Same arguments and
+ * results otherwise.
+ */
+int pre_process_f32(tomo_pre_plan *plan, const float *data, const float *dark,
+                    const float *flat, void *out, int out_dtype, double bright_ratio,
+                    int flat_linear, int minus_log, void *stream);
+int pre_darkflat_f32(tomo_pre_plan *plan, const float *data, const float *dark,
+                     const float *flat, float *res, double bright_ratio, int flat_linear,
+                     void *stream);
+
+/*
+ * Binning, utils.downsample(data, binning) (src/tomocupy/utils.py:157-167) as
+ * the reader applies it to projections, darks and flats
+ * (dataio/reader.py:350-389): dst[r, z, x] = sum of the 2^b x 2^b block
+ * src[r, z 2^b + a, x 2^b + c], a, c in [0, 2^b).
+ *   src  device uint16 (nrows, nz << binning, n << binning), nz and n of the plan
+ *   dst  device float32 (nrows, nz, n); exact sums (< 2^24 for binning <= 3)
+ * binning in 0..3 (config.py:154-158); 0 converts the counts to float32.
+ */
+int pre_bin(tomo_pre_plan *plan, const uint16_t *src, float *dst, int64_t nrows, int binning,
+            void *stream);
+
+/*
  * minus_log (proc_functions.py:76-83) fused with the swap to sinogram order:
  *   out[z,p,x] = -log(res[p,z,x]) with x<=0 -> 0, NaN -> 6, +-inf -> 0
  * (minus_log != 0), or a plain transpose (minus_log == 0); `out` is stored
@@ -261,6 +287,14 @@ int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, void *out, int out
  */
 int pre_remove_outliers(tomo_pre_plan *plan, uint16_t *data, int64_t nrows, int width,
                         int threshold, void *stream);
+
+/*
+ * pre_remove_outliers on float32 counts (binned frames, see pre_bin): the
+ * same median (exact selection) and replacement rule, data - f compared with
+ * threshold in float32 as the reference does for float frames.
+ */
+int pre_remove_outliers_f32(tomo_pre_plan *plan, float *data, int64_t nrows, int width,
+                            int threshold, void *stream);
 
 /*
  * remove_stripe_ti (src/tomocupy/processing/remove_stripe.py:296-309), the

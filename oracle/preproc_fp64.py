@@ -16,7 +16,8 @@ golden vectors and perfect reconstruction; the damping is a closed form
 (``tests/test_oracle_preproc.py``).  ``remove_outliers`` is
 ``scipy.ndimage.median_filter`` (the CPU twin of the ``cupyx`` routine the
 reference calls) plus the replacement rule, pinned by a brute-force window
-median on small arrays.
+median on small arrays.  ``downsample`` (binning) is pinned by block sums
+through a reshape and by the conservation of the total.
 """
 
 import json
@@ -75,18 +76,32 @@ def pad360(sino, centeri):
 
 
 def remove_outliers(data, width, threshold):
-    """``ProcFunctions.remove_outliers`` (P:96-107) on a uint16 ``(rows, nz, n)``
-    array: median over ``[width, 1, width]`` ('reflect' borders), then
-    ``where(data > f and data - f > threshold, f, data)``.  Returns a copy."""
+    """``ProcFunctions.remove_outliers`` (P:96-107) on a ``(rows, nz, n)``
+    array of counts, uint16 or float (binned): median over ``[width, 1,
+    width]`` ('reflect' borders), then ``where(data > f and data - f >
+    threshold, f, data)``.  Returns a copy."""
     from scipy import ndimage
     data = np.asarray(data)
     if int(width) <= 0:
         return data.copy()
     w = int(width)
     f = ndimage.median_filter(data, [w, 1, w])
-    d = data.astype(np.int64)
-    fi = f.astype(np.int64)
+    wide = np.float64 if data.dtype.kind == 'f' else np.int64
+    d = data.astype(wide)
+    fi = f.astype(wide)
     return np.where((d > fi) & (d - fi > threshold), f, data).astype(data.dtype)
+
+
+def downsample(data, binning):
+    """``utils.downsample`` (``src/tomocupy/utils.py:157-167``): ``binning``
+    halvings of the detector columns (sums of neighbour pairs), then as many of
+    the rows, on ``(frames, rows, cols)``; FP64 sums."""
+    data = np.asarray(data, dtype=np.float64)
+    for _ in range(int(binning)):
+        data = data[:, :, ::2] + data[:, :, 1::2]
+    for _ in range(int(binning)):
+        data = data[:, ::2] + data[:, 1::2]
+    return data
 
 
 def darkflat_correction(data, dark, flat, bright_ratio=1.0, flat_linear=False):
@@ -230,10 +245,12 @@ def remove_stripe_fw(data, sigma, wname, level):
 
 def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym16',
                fw_level=7, take_minus_log=True, bright_ratio=1.0, flat_linear=False,
-               dezinger=0, dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0):
+               dezinger=0, dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0, binning=0):
     """``proc_sino`` then ``proc_proj`` then the swap to sinogram order
-    (P:140-198, ``rec.py:172-174``): [outliers] -> darkflat -> [fw] -> [-log] ->
-    ``(nz, nproj, n)``."""
+    (P:140-198, ``rec.py:172-174``): [binning, as the reader applies it] ->
+    [outliers] -> darkflat -> [fw] -> [-log] -> ``(nz, nproj, n)``."""
+    if binning > 0:
+        data, dark, flat = (downsample(a, binning) for a in (data, dark, flat))
     if dezinger > 0:
         data = remove_outliers(data, dezinger, dezinger_threshold)
         dark = remove_outliers(dark, dezinger, dezinger_threshold)

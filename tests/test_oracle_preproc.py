@@ -95,10 +95,10 @@ def _brute_outliers(data, w, thr):
     for p in range(rows):
         for z in range(nz):
             for x in range(n):
-                win = sorted(int(data[refl(p + a - w // 2, rows), z, refl(x + b - w // 2, n)])
+                win = sorted(data[refl(p + a - w // 2, rows), z, refl(x + b - w // 2, n)].item()
                              for a in range(w) for b in range(w))
                 f = win[(w * w) // 2]
-                v = int(data[p, z, x])
+                v = data[p, z, x].item()
                 if v > f and v - f > thr:
                     out[p, z, x] = f
     return out
@@ -124,6 +124,33 @@ def test_remove_outliers_rank_and_origin_matter():
     assert np.array_equal(got, ref)
     shifted = np.roll(_brute_outliers(np.roll(data, 1, axis=2), 4, 0), -1, axis=2)
     assert not np.array_equal(shifted, ref)
+
+
+@pytest.mark.parametrize("w", [2, 3])
+def test_remove_outliers_float_counts_match_brute_force(w):
+    # binned frames are float sums of counts; same median and rule
+    rng = np.random.default_rng(90 + w)
+    data = rng.integers(3600, 4400, size=(6, 2, 8)).astype(np.float32)
+    data[rng.random(data.shape) < 0.1] = 250000.0
+    got = pp.remove_outliers(data, w, 2000)
+    assert got.dtype == np.float32
+    assert np.array_equal(got, _brute_outliers(data, w, 2000))
+
+
+@pytest.mark.parametrize("b", [1, 2, 3])
+def test_downsample_is_block_sum(b):
+    # utils.downsample: pairwise column sums then pairwise row sums, b times
+    # each = the sum over each 2^b x 2^b block (reshape form, brute force on
+    # one block, total conserved)
+    s = 2 ** b
+    rng = np.random.default_rng(b)
+    x = rng.integers(0, 65536, size=(3, 2 * s, 3 * s)).astype(np.uint16)
+    got = pp.downsample(x, b)
+    ref = x.astype(np.float64).reshape(3, 2, s, 3, s).sum(axis=(2, 4))
+    assert got.shape == (3, 2, 3) and np.array_equal(got, ref)
+    assert got[1, 1, 2] == sum(int(x[1, s + a, 2 * s + c]) for a in range(s) for c in range(s))
+    assert got.sum() == x.astype(np.float64).sum()
+    assert np.array_equal(pp.downsample(x, 0), x.astype(np.float64))
 
 
 def test_remove_outliers_width_zero_is_identity():

@@ -61,6 +61,11 @@ SIGNATURES = {
     'pre_process_fw': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_int, _c_d,
                                 ctypes.c_char_p, _c_int, _c_p]),
     'pre_remove_outliers': (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_int, _c_p]),
+    'pre_remove_outliers_f32': (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_int, _c_p]),
+    'pre_process_f32': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_int,
+                                 _c_p]),
+    'pre_darkflat_f32': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_p]),
+    'pre_bin': (_c_int, [_c_p, _c_p, _c_p, _c_i64, _c_int, _c_p]),
     'pre_remove_stripe_ti': (_c_int, [_c_p, _c_p, _c_d, _c_d, _c_p]),
     'pre_pad360': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_p]),
     'pre_destroy': (None, [_c_p]),

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libtomohip.so')
 BUILD = os.path.join(HERE, '_build')
 SOURCES = ['common.hip', 'fourierrec.hip', 'filter.hip', 'preproc.hip', 'stripe.hip',
-           'outliers.hip', 'stripe_ti.hip', 'pad360.hip']
+           'outliers.hip', 'stripe_ti.hip', 'pad360.hip', 'binning.hip']
 ARCH = os.environ.get('TOMOHIP_ARCH', 'gfx950')
 
 FLAGS = [

@@ -14,9 +14,11 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
   ``<out>/recon_slice%04d_center%05.2f.tiff`` (``writer.py:309-313``).
 
 Stripe removal: ``fw`` and ``ti``; zinger removal: ``--dezinger``; 360-degree
-scans with the axis near the border: ``--file-type double_fov``.  Settings
-outside this path (binning, vo-all stripe removal, phase retrieval, ...) are
-not offered.  Input formats: see
+scans with the axis near the border: ``--file-type double_fov``; binning
+``--binning b`` (2^b x 2^b sums of counts on the GPU, ``reader.py:96-140``).
+As the reference, the width is trimmed to a multiple of ``2^(b+1)``
+(``reader.py:127-133``).  Settings outside this path (vo-all stripe removal,
+phase retrieval, ...) are not offered.  Input formats: see
 :mod:`tomocupy_amd.io`.
 """
 import argparse
@@ -64,6 +66,7 @@ def parser():
     r.add_argument('--bright-ratio', type=float, default=1.0)
     r.add_argument('--dezinger', type=int, default=0)
     r.add_argument('--dezinger-threshold', type=int, default=5000)
+    r.add_argument('--binning', type=int, default=0, choices=[0, 1, 2, 3])
     r.add_argument('--max-write-threads', type=int, default=8)
     r.add_argument('--save-format', default='tiff', choices=['tiff', 'npy'])
     return ap
@@ -79,43 +82,55 @@ def _out_dir(a, kind):
 
 
 def _common(a, ds):
+    """Row range, binned width and centre, stripe method (reader.py:82-140):
+    the raw width is trimmed to a multiple of 2^(b+1), then width, centre and
+    height are divided by 2^b."""
     z0 = a.start_row
     z1 = ds.nz if a.end_row == -1 else a.end_row
     if not (0 <= z0 < z1 <= ds.nz):
         raise SystemExit('invalid --start-row/--end-row for %d rows' % ds.nz)
+    b = a.binning
     centre = ds.ni / 2 if a.rotation_axis == -1 else a.rotation_axis
+    x1 = ds.ni - ds.ni % (2 ** (b + 1))
+    if x1 != ds.ni:
+        print('decreasing projection width by %d pixel to operate with multiple of 2 sizes; '
+              'new width %d' % (ds.ni - x1, x1))
+    if (z1 - z0) >> b < 1:
+        raise SystemExit('fewer than 2^%d rows to bin' % b)
     stripe = a.remove_stripe_method if a.remove_stripe_method in ('fw', 'ti') else 'none'
-    return z0, z1, centre, stripe
+    return z0, z1, x1, x1 >> b, centre / 2 ** b, stripe
 
 
 def recon_full(a, ds):
     from .io import write_tiff
     from .rec import GPURec
-    z0, z1, centre, stripe = _common(a, ds)
-    nz = z1 - z0
+    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    b = a.binning
+    nz = (z1 - z0) >> b
     out = _out_dir(a, 'full')
     os.makedirs(out, exist_ok=True)
-    rec = GPURec(ds.ni, ds.nproj, nz, a.nsino_per_chunk, ds.theta, centre,
+    rec = GPURec(ni, ds.nproj, nz, a.nsino_per_chunk, ds.theta, centre,
                  ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype, fbp_filter=a.fbp_filter,
                  remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
                  fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                  flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads,
                  dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
-                 ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type)
+                 ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type, binning=b)
     ncz, nrec = rec.ncz, rec.n
+    zb = z0 >> b  # first output slice id (rec.py:170)
     stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
                                       dtype=np.float16 if a.dtype == 'float16' else np.float32,
                                       shape=(nz, nrec, nrec)) if a.save_format == 'npy' else None
 
     def read(k):
-        s0 = z0 + k * ncz
-        return ds.rows(s0, min(z1, s0 + ncz))
+        s0 = z0 + ((k * ncz) << b)
+        return ds.rows(s0, s0 + (rec.lzchunk[k] << b), x1)
 
     def write(k, r):
         for kk in range(len(r)):
-            fid = z0 + k * ncz + kk
+            fid = zb + k * ncz + kk
             if stack is not None:
-                stack[fid - z0] = r[kk]
+                stack[fid - zb] = r[kk]
             else:
                 write_tiff(os.path.join(out, 'recon_%05d.tiff' % fid), r[kk])
 
@@ -134,38 +149,42 @@ def recon_try(a, ds):
     import torch
     from . import BackprojFunctions, ProcFunctions, _lib
     from .io import write_tiff
-    z0, z1, centre, stripe = _common(a, ds)
-    nz = z1 - z0
+    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    b = a.binning
+    nz = (z1 - z0) >> b
     out = _out_dir(a, 'try')
     os.makedirs(out, exist_ok=True)
     fractions = ast.literal_eval(a.nsino)
     if not isinstance(fractions, (list, tuple)):
         fractions = [fractions]
-    ids = [int(f * (nz - 1)) for f in fractions]
-    shifts = np.arange(-a.center_search_width, a.center_search_width,
-                       a.center_search_step).astype(np.float32)
-    centres = centre - shifts
+    # raw row of each slice, a multiple of 2^b (reader.py:196-197)
+    ids = [(int(f * (nz * 2 ** b - 1) / 2 ** b)) << b for f in fractions]
+    # shifts in binned pixels, centres saved in raw pixels (reader.py:229-232)
+    shifts = (np.arange(-a.center_search_width, a.center_search_width,
+                        a.center_search_step * 2 ** b).astype(np.float32) / 2 ** b)
+    centres = (centre - shifts) * 2 ** b
     ncz = max(2, a.nsino_per_chunk)
     td = _lib.torch_dtype(a.dtype)
     theta = torch.from_numpy(ds.theta).cuda()
-    pf = ProcFunctions(ds.nproj, 1, ds.ni, ds.dark.shape[0], ds.flat.shape[0], a.dtype,
+    pf = ProcFunctions(ds.nproj, 1, ni, ds.dark.shape[0], ds.flat.shape[0], a.dtype,
                        minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                        flat_linear=_bool(a.flat_linear), remove_stripe_method=stripe,
                        fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level,
                        dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
                        ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type,
-                       center=centre)
-    nrec, rcentre, sgn = ds.ni, centre, 1.0
+                       center=centre, binning=b)
+    nrec, rcentre, sgn = ni, centre, 1.0
     if a.file_type == 'double_fov':
         # reader.py:143-148 and 233-234: the axis moves to the right half of a
         # 2 ni wide sinogram; mirrored data take mirrored shifts
-        nrec = 2 * ds.ni
-        if centre < ds.ni // 2:
-            rcentre, sgn = ds.ni - centre, -1.0
+        nrec = 2 * ni
+        if centre < ni // 2:
+            rcentre, sgn = ni - centre, -1.0
     bf = BackprojFunctions(nrec, ds.nproj, ncz, theta, rcentre, a.dtype, a.fbp_filter)
     written = []
     for idz in ids:
-        data, dark, flat = (torch.from_numpy(x).cuda() for x in ds.rows(z0 + idz, z0 + idz + 1))
+        data, dark, flat = (torch.from_numpy(x).cuda()
+                            for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, x1))
         sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, nrec)
         batch = sino.expand(ncz, -1, -1).contiguous()
         obj = torch.empty((ncz, nrec, nrec), dtype=td, device='cuda')

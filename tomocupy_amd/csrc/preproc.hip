@@ -13,8 +13,10 @@ using namespace tomo;
 namespace {
 
 // dark0 = mean(dark); flat means (one, or two halves for flat_linear)
-// (proc_functions.py:55-72).  Means accumulate in FP64.
-__global__ void k_pre_means(const uint16_t *__restrict__ dark, const uint16_t *__restrict__ flat,
+// (proc_functions.py:55-72).  Means accumulate in FP64.  TI is the type of
+// the counts: uint16_t (raw frames) or float (binned frames).
+template <typename TI>
+__global__ void k_pre_means(const TI *__restrict__ dark, const TI *__restrict__ flat,
                             float *__restrict__ dark0, float *__restrict__ flatA,
                             float *__restrict__ flatB, int zn, int ndark, int nflat,
                             float bright, int flat_linear) {
@@ -39,14 +41,14 @@ __global__ void k_pre_means(const uint16_t *__restrict__ dark, const uint16_t *_
 }
 
 // One workgroup per (p, z) detector row: read data[p][z][:], write out[z][p][:].
-// TO is the storage type of the output.
-template <int MODE, typename TO>  // 0: darkflat (+ optional -log) + swap; 1: darkflat only, projection order
-__global__ void k_pre_rows(const uint16_t *__restrict__ data, const float *__restrict__ dark0,
+// TO is the storage type of the output, TI that of the counts.
+template <int MODE, typename TO, typename TI>  // 0: darkflat (+ optional -log) + swap; 1: darkflat only, projection order
+__global__ void k_pre_rows(const TI *__restrict__ data, const float *__restrict__ dark0,
                            const float *__restrict__ flatA, const float *__restrict__ flatB,
                            TO *__restrict__ out, int nproj, int nz, int n, int flat_linear,
                            int take_log) {
   const int p = blockIdx.x, z = blockIdx.y;
-  const uint16_t *src = data + ((size_t)p * nz + z) * n;
+  const TI *src = data + ((size_t)p * nz + z) * n;
   TO *dst = MODE == 0 ? out + ((size_t)z * nproj + p) * n : out + ((size_t)p * nz + z) * n;
   const float v = nproj > 1 ? (float)((double)p / (double)(nproj - 1)) : 0.f;
   const float *d0 = dark0 + (size_t)z * n;
@@ -103,26 +105,32 @@ extern "C" int pre_create(tomo_pre_plan **plan, int64_t nproj, int64_t nz, int64
   return TOMO_OK;
 }
 
-int tomo::pre_launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *flat,
-                           double bright_ratio, int flat_linear, hipStream_t st) {
+template <typename TI>
+static int launch_means(tomo_pre_plan *P, const TI *dark, const TI *flat, double bright_ratio,
+                        int flat_linear, hipStream_t st) {
   TOMO_REQUIRE(bright_ratio > 0 && std::isfinite(bright_ratio), "bright_ratio must be > 0");
   TOMO_REQUIRE(!flat_linear || P->nflat >= 3, "flat_linear needs at least 3 flat frames");
   const int zn = (int)(P->nz * P->n);
   {
     ProfScope ps_("k_pre_means", st);
-    hipLaunchKernelGGL(k_pre_means, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st, dark, flat,
-                       P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), zn,
+    hipLaunchKernelGGL(k_pre_means<TI>, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st, dark,
+                       flat, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), zn,
                        (int)P->ndark, (int)P->nflat, (float)bright_ratio, flat_linear);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
 }
 
-template <typename TO>
-static void launch_rows0(tomo_pre_plan *P, const uint16_t *data, void *out, int flat_linear,
+int tomo::pre_launch_means(tomo_pre_plan *P, const uint16_t *dark, const uint16_t *flat,
+                           double bright_ratio, int flat_linear, hipStream_t st) {
+  return launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+}
+
+template <typename TO, typename TI>
+static void launch_rows0(tomo_pre_plan *P, const TI *data, void *out, int flat_linear,
                          int take_log, hipStream_t st) {
-  hipLaunchKernelGGL((k_pre_rows<0, TO>), dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0,
-                     st, data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(),
+  hipLaunchKernelGGL((k_pre_rows<0, TO, TI>), dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256),
+                     0, st, data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(),
                      (TO *)out, (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, take_log);
 }
 
@@ -133,16 +141,17 @@ static void launch_log_swap(tomo_pre_plan *P, const float *res, void *out, int t
                      st, res, (TO *)out, (int)P->nproj, (int)P->nz, (int)P->n, take_log);
 }
 
-extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
-                           const uint16_t *flat, void *out, int out_dtype, double bright_ratio,
-                           int flat_linear, int minus_log, void *stream) {
+template <typename TI>
+static int process(const char *who, tomo_pre_plan *P, const TI *data, const TI *dark,
+                   const TI *flat, void *out, int out_dtype, double bright_ratio, int flat_linear,
+                   int minus_log, void *stream) {
   clear_error();
-  TOMO_REQUIRE(P != nullptr, "pre_process: plan is NULL");
-  TOMO_REQUIRE(data && dark && flat && out, "pre_process: NULL array");
+  TOMO_REQUIRE(P != nullptr, "%s: plan is NULL", who);
+  TOMO_REQUIRE(data && dark && flat && out, "%s: NULL array", who);
   TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
-               "pre_process: unknown out_dtype %d", out_dtype);
+               "%s: unknown out_dtype %d", who, out_dtype);
   hipStream_t st = (hipStream_t)stream;
-  int rc = pre_launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
   {
     ProfScope ps_("k_pre_rows", st);
@@ -157,23 +166,52 @@ extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_
   return TOMO_OK;
 }
 
-extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
-                            const uint16_t *flat, float *res, double bright_ratio, int flat_linear,
-                            void *stream) {
+template <typename TI>
+static int darkflat(const char *who, tomo_pre_plan *P, const TI *data, const TI *dark,
+                    const TI *flat, float *res, double bright_ratio, int flat_linear,
+                    void *stream) {
   clear_error();
-  TOMO_REQUIRE(P != nullptr, "pre_darkflat: plan is NULL");
-  TOMO_REQUIRE(data && dark && flat && res, "pre_darkflat: NULL array");
+  TOMO_REQUIRE(P != nullptr, "%s: plan is NULL", who);
+  TOMO_REQUIRE(data && dark && flat && res, "%s: NULL array", who);
   hipStream_t st = (hipStream_t)stream;
-  int rc = pre_launch_means(P, dark, flat, bright_ratio, flat_linear, st);
+  int rc = launch_means(P, dark, flat, bright_ratio, flat_linear, st);
   if (rc) return rc;
   {
     ProfScope ps_("k_pre_rows", st);
-    hipLaunchKernelGGL((k_pre_rows<1, float>), dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0, st,
-                       data, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), res,
-                       (int)P->nproj, (int)P->nz, (int)P->n, flat_linear, 0);
+    hipLaunchKernelGGL((k_pre_rows<1, float, TI>), dim3((unsigned)P->nproj, (unsigned)P->nz),
+                       dim3(256), 0, st, data, P->dark0.as<float>(), P->flatA.as<float>(),
+                       P->flatB.as<float>(), res, (int)P->nproj, (int)P->nz, (int)P->n,
+                       flat_linear, 0);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
+}
+
+extern "C" int pre_process(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
+                           const uint16_t *flat, void *out, int out_dtype, double bright_ratio,
+                           int flat_linear, int minus_log, void *stream) {
+  return process("pre_process", P, data, dark, flat, out, out_dtype, bright_ratio, flat_linear,
+                 minus_log, stream);
+}
+
+extern "C" int pre_process_f32(tomo_pre_plan *P, const float *data, const float *dark,
+                               const float *flat, void *out, int out_dtype, double bright_ratio,
+                               int flat_linear, int minus_log, void *stream) {
+  return process("pre_process_f32", P, data, dark, flat, out, out_dtype, bright_ratio,
+                 flat_linear, minus_log, stream);
+}
+
+extern "C" int pre_darkflat(tomo_pre_plan *P, const uint16_t *data, const uint16_t *dark,
+                            const uint16_t *flat, float *res, double bright_ratio, int flat_linear,
+                            void *stream) {
+  return darkflat("pre_darkflat", P, data, dark, flat, res, bright_ratio, flat_linear, stream);
+}
+
+extern "C" int pre_darkflat_f32(tomo_pre_plan *P, const float *data, const float *dark,
+                                const float *flat, float *res, double bright_ratio,
+                                int flat_linear, void *stream) {
+  return darkflat("pre_darkflat_f32", P, data, dark, flat, res, bright_ratio, flat_linear,
+                  stream);
 }
 
 extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, void *out, int out_dtype,

@@ -5,7 +5,10 @@ for the steps on the fourierrec path: ``remove_outliers`` (dezinger),
 ``darkflat_correction``, ``minus_log``, ``proc_sino`` (outliers, dark/flat,
 stripe removal), ``proc_proj`` (minus log) and ``pad360`` (double field of
 view: ``file_type='double_fov'`` with the rotation axis ``center`` in detector
-columns doubles the sinogram width).
+columns doubles the sinogram width), and the reader's binning
+(``binning=b``: ``downsample``, ``utils.py:157-167``, sums of 2^b x 2^b blocks
+of counts as float32; ``n`` and ``ncz`` are then the binned sizes and the raw
+frames are ``(rows, ncz 2^b, n 2^b)``).
 Settings the reference reads from the global ``args`` are constructor
 arguments.  ``proc_chunk`` is the fused form used by the conveyor: raw uint16
 chunk -> dark/flat -> [stripe removal] -> minus log -> sinogram order.
@@ -22,7 +25,7 @@ class ProcFunctions:
                  bright_ratio=1.0, flat_linear=False, remove_stripe_method='none',
                  fw_sigma=1.0, fw_filter='sym16', fw_level=7, dezinger=0,
                  dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0, file_type='standard',
-                 center=None):
+                 center=None, binning=0):
         if remove_stripe_method not in ('none', 'fw', 'ti'):
             raise NotImplementedError('remove_stripe_method %r is not implemented'
                                       % remove_stripe_method)
@@ -38,7 +41,11 @@ class ProcFunctions:
         self.fw_sigma, self.fw_filter, self.fw_level = float(fw_sigma), fw_filter, int(fw_level)
         self.dezinger, self.dezinger_threshold = int(dezinger), int(dezinger_threshold)
         self.ti_beta, self.ti_mask = float(ti_beta), float(ti_mask)
-        self._res = None  # float32 transmission chunk of the unfused ti path
+        self._res = None  # float32 transmission chunk of the unfused ti/fw paths
+        if int(binning) not in (0, 1, 2, 3):
+            raise ValueError('binning must be 0, 1, 2 or 3 (config.py:154-158)')
+        self.binning = int(binning)
+        self._binned = {}  # float32 binned data/dark/flat of the chunk
         if file_type not in ('standard', 'double_fov'):
             raise ValueError('file_type must be standard or double_fov')
         self.file_type = file_type
@@ -60,20 +67,54 @@ class ProcFunctions:
         return ((self.nproj, self.ncz, self.n), (self.ndark, self.ncz, self.n),
                 (self.nflat, self.ncz, self.n), torch.uint16)
 
+    def downsample(self, data, out=None):
+        """``utils.downsample(data, binning)`` (``utils.py:157-167``) on the
+        GPU: uint16 ``(rows, ncz 2^b, n 2^b)`` -> float32 ``(rows, ncz, n)``
+        block sums (exact)."""
+        import torch
+        b = self.binning
+        if data.dim() != 3:
+            raise ValueError('downsample: expected (rows, %d, %d) uint16 frames, got %s'
+                             % (self.ncz << b, self.n << b, tuple(data.shape)))
+        rows = int(data.shape[0])
+        shape = (rows, self.ncz << b, self.n << b)
+        if out is None:
+            out = torch.empty((rows, self.ncz, self.n), dtype=torch.float32, device=data.device)
+        _lib.check(_lib.lib().pre_bin(
+            self._h, _lib.dev_ptr(data, shape, torch.uint16, 'data'),
+            _lib.dev_ptr(out, (rows, self.ncz, self.n), torch.float32, 'out'), rows, b,
+            _lib.stream_ptr(None)), 'pre_bin')
+        return out
+
+    def _downsample3(self, data, dark, flat):
+        import torch
+        res = []
+        for key, x in (('data', data), ('dark', dark), ('flat', flat)):
+            buf = self._binned.get(key)
+            rows = int(x.shape[0])
+            if buf is None or buf.device != x.device or buf.shape[0] != rows:
+                buf = torch.empty((rows, self.ncz, self.n), dtype=torch.float32, device=x.device)
+                self._binned[key] = buf
+            res.append(self.downsample(x, buf))
+        return res
+
     def remove_outliers(self, data):
-        """Zinger removal in place on a uint16 ``(rows, ncz, n)`` array
-        (``proc_functions.py:96-107``): pixels more than ``dezinger_threshold``
-        above the median of their ``dezinger`` x 1 x ``dezinger`` window
-        (angle, slice, detector) take the median value."""
+        """Zinger removal in place on a ``(rows, ncz, n)`` array of counts,
+        uint16 or float32 (binned) (``proc_functions.py:96-107``): pixels more
+        than ``dezinger_threshold`` above the median of their ``dezinger`` x 1
+        x ``dezinger`` window (angle, slice, detector) take the median value."""
         import torch
         if self.dezinger > 0:
             if data.dim() != 3 or tuple(data.shape[1:]) != (self.ncz, self.n):
                 raise ValueError('remove_outliers: expected (rows, %d, %d), got %s'
                                  % (self.ncz, self.n, tuple(data.shape)))
-            _lib.check(_lib.lib().pre_remove_outliers(
-                self._h, _lib.dev_ptr(data, tuple(data.shape), torch.uint16, 'data'),
+            f32 = data.dtype == torch.float32
+            fn = 'pre_remove_outliers_f32' if f32 else 'pre_remove_outliers'
+            _lib.check(getattr(_lib.lib(), fn)(
+                self._h, _lib.dev_ptr(data, tuple(data.shape),
+                                      torch.float32 if f32 else torch.uint16, 'data'),
                 int(data.shape[0]), self.dezinger, self.dezinger_threshold,
-                _lib.stream_ptr(None)), 'pre_remove_outliers')
+                _lib.stream_ptr(None)), fn)
         return data
 
     def _outliers(self, data, dark, flat):
@@ -84,14 +125,18 @@ class ProcFunctions:
             self.remove_outliers(flat)
 
     def darkflat_correction(self, data, dark, flat, res=None):
+        """``(data - dark0) / flat0`` (``proc_functions.py:55-74``) on uint16
+        or float32 (binned) counts of the plan's ``(rows, ncz, n)``."""
         import torch
         sd, sk, sf, u16 = self._shapes()
         if res is None:
             res = torch.empty(sd, dtype=torch.float32, device=data.device)
-        _lib.check(_lib.lib().pre_darkflat(
-            self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
-            _lib.dev_ptr(flat, sf, u16, 'flat'), _lib.dev_ptr(res, sd, torch.float32, 'res'),
-            self.bright_ratio, int(self.flat_linear), _lib.stream_ptr(None)), 'pre_darkflat')
+        tin = torch.float32 if data.dtype == torch.float32 else u16
+        fn = 'pre_darkflat_f32' if tin == torch.float32 else 'pre_darkflat'
+        _lib.check(getattr(_lib.lib(), fn)(
+            self._h, _lib.dev_ptr(data, sd, tin, 'data'), _lib.dev_ptr(dark, sk, tin, 'dark'),
+            _lib.dev_ptr(flat, sf, tin, 'flat'), _lib.dev_ptr(res, sd, torch.float32, 'res'),
+            self.bright_ratio, int(self.flat_linear), _lib.stream_ptr(None)), fn)
         return res
 
     def remove_stripe_fw(self, res):
@@ -157,7 +202,9 @@ class ProcFunctions:
         """Raw uint16 chunk -> filtered-ready sinograms ``(ncz, nproj, nout)``
         (``nout = 2n`` for double field of view, else ``n``).  With
         ``dezinger > 0`` the outliers of ``data``, ``dark`` and ``flat`` are
-        removed in place first, as ``proc_sino`` does."""
+        removed in place first, as ``proc_sino`` does; with ``binning > 0``
+        the raw frames ``(rows, ncz 2^b, n 2^b)`` are binned first (into
+        float32 buffers of the object; the raw frames are not modified)."""
         import torch
         if self.file_type == 'double_fov':
             if self._sino_ni is None or self._sino_ni.device != data.device:
@@ -169,6 +216,8 @@ class ProcFunctions:
 
     def _proc_chunk(self, data, dark, flat, out, res):
         import torch
+        if self.binning > 0:
+            return self._proc_chunk_f32(*self._downsample3(data, dark, flat), out)
         sd, sk, sf, u16 = self._shapes()
         self._outliers(data, dark, flat)
         td = _lib.torch_dtype(self.dtype)
@@ -199,6 +248,33 @@ class ProcFunctions:
             _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), td, 'out'), _lib.DTYPES[self.dtype],
             self.bright_ratio, int(self.flat_linear), int(self.take_minus_log),
             _lib.stream_ptr(None)), 'pre_process')
+        return out
+
+    def _proc_chunk_f32(self, data, dark, flat, out):
+        # binned float32 counts: dark/flat (+ -log + swap) in one pass, or via
+        # the float32 transmission chunk for stripe removal
+        import torch
+        sd, sk, sf, _ = self._shapes()
+        f32 = torch.float32
+        self._outliers(data, dark, flat)
+        td = _lib.torch_dtype(self.dtype)
+        if out is None:
+            out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=data.device)
+        if self.remove_stripe_method in ('fw', 'ti'):
+            if self._res is None or self._res.device != data.device:
+                self._res = torch.empty(sd, dtype=f32, device=data.device)
+            self.darkflat_correction(data, dark, flat, self._res)
+            if self.remove_stripe_method == 'fw':
+                self.remove_stripe_fw(self._res)
+            else:
+                self.remove_stripe_ti(self._res)
+            return self.proc_proj_swap(self._res, out)
+        _lib.check(_lib.lib().pre_process_f32(
+            self._h, _lib.dev_ptr(data, sd, f32, 'data'), _lib.dev_ptr(dark, sk, f32, 'dark'),
+            _lib.dev_ptr(flat, sf, f32, 'flat'),
+            _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), td, 'out'), _lib.DTYPES[self.dtype],
+            self.bright_ratio, int(self.flat_linear), int(self.take_minus_log),
+            _lib.stream_ptr(None)), 'pre_process_f32')
         return out
 
     def free(self):

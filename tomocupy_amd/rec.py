@@ -70,11 +70,13 @@ class GPURec:
                  fbp_filter='parzen', remove_stripe_method='none', fw_sigma=1.0,
                  fw_filter='sym16', fw_level=7, minus_log=True, bright_ratio=1.0,
                  flat_linear=False, max_write_threads=8, dezinger=0, dezinger_threshold=5000,
-                 ti_beta=0.022, ti_mask=1.0, file_type='standard'):
+                 ti_beta=0.022, ti_mask=1.0, file_type='standard', binning=0):
         """``n``: detector width of the data; the reconstruction is ``n x n``,
         or ``2n x 2n`` for ``file_type='double_fov'`` (reader.py:143-148: the
         rotation axis ``center`` is mirrored into the right half when it lies in
-        the left one)."""
+        the left one).  With ``binning=b`` (reader.py:137-140) ``n``, ``nz`` and
+        ``center`` are the binned sizes; the raw chunks that ``read_chunk``
+        returns hold ``lz 2^b`` rows of ``n 2^b`` columns."""
         import torch
         if ncz == 1:
             ncz = 2  # fourierrec pairs two slices (reader.py:113-115)
@@ -93,7 +95,8 @@ class GPURec:
                                 fw_sigma=fw_sigma, fw_filter=fw_filter, fw_level=fw_level,
                                 dezinger=dezinger, dezinger_threshold=dezinger_threshold,
                                 ti_beta=ti_beta, ti_mask=ti_mask, file_type=file_type,
-                                center=center)
+                                center=center, binning=binning)
+        self.binning = int(binning)
         self.bf = BackprojFunctions(self.n, self.nproj, self.ncz, theta, rec_center, dtype,
                                     fbp_filter)
         self.stream1 = torch.cuda.Stream()
@@ -112,7 +115,8 @@ class GPURec:
 
     def recon_resident(self, raw, dark, flat, rec):
         """Reconstruct a GPU-resident volume: ``raw (nzchunk, nproj, ncz, ni)`` uint16,
-        ``dark``/``flat`` ``(nzchunk, ndark|nflat, ncz, ni)``, into
+        ``dark``/``flat`` ``(nzchunk, ndark|nflat, ncz, ni)`` (rows and columns
+        times ``2^binning`` when binning), into
         ``rec (nzchunk*ncz, n, n)`` of the plan's dtype, on stream 2."""
         import torch
         with torch.cuda.stream(self.stream2):
@@ -125,17 +129,20 @@ class GPURec:
 
         ``read_chunk(k)`` returns ``(data, dark, flat)`` numpy uint16 arrays of
         shapes ``(nproj, lz, ni)``, ``(ndark, lz, ni)``, ``(nflat, lz, ni)`` with
-        ``lz = lzchunk[k]``; ``write_chunk(k, rec)`` receives a ``(lz, n, n)``
+        ``lz = lzchunk[k]`` (``lz 2^b`` rows of ``ni 2^b`` raw columns with
+        binning ``b``); ``write_chunk(k, rec)`` receives a ``(lz, n, n)``
         numpy view (float32, or float16 for 16-bit plans) that is valid until it
         returns.
         """
         import torch
-        ncz, nproj, n, ni = self.ncz, self.nproj, self.n, self.ni
+        ncz, nproj, n = self.ncz, self.nproj, self.n
+        b = self.binning
+        rcz, ni = ncz << b, self.ni << b  # raw rows and columns of a chunk
         pin = dict(pin_memory=True)
         item_pinned = {
-            'data': torch.zeros((2, nproj, ncz, ni), dtype=torch.uint16, **pin),
-            'dark': torch.zeros((2, self.ndark, ncz, ni), dtype=torch.uint16, **pin),
-            'flat': torch.ones((2, self.nflat, ncz, ni), dtype=torch.uint16, **pin),
+            'data': torch.zeros((2, nproj, rcz, ni), dtype=torch.uint16, **pin),
+            'dark': torch.zeros((2, self.ndark, rcz, ni), dtype=torch.uint16, **pin),
+            'flat': torch.ones((2, self.nflat, rcz, ni), dtype=torch.uint16, **pin),
         }
         item_gpu = {k: torch.empty_like(v, device='cuda') for k, v in item_pinned.items()}
         item_gpu['flat'].copy_(item_pinned['flat'])
@@ -159,15 +166,15 @@ class GPURec:
                     rec_pinned[ithread].copy_(rec_gpu[(k - 2) % 2], non_blocking=True)
             if k < nzchunk:
                 data, dark, flat = read_chunk(k)
-                lz = self.lzchunk[k]
+                lz = self.lzchunk[k] << b
                 item_pinned['data'][k % 2, :, :lz].copy_(torch.from_numpy(np.ascontiguousarray(data)))
                 item_pinned['dark'][k % 2, :, :lz].copy_(torch.from_numpy(np.ascontiguousarray(dark)))
                 item_pinned['flat'][k % 2, :, :lz].copy_(torch.from_numpy(np.ascontiguousarray(flat)))
-                if lz < ncz:
+                if lz < rcz:
                     item_pinned['data'][k % 2, :, lz:].zero_()
                     item_pinned['dark'][k % 2, :, lz:].zero_()
                     item_pinned['flat'][k % 2, :, lz:].copy_(
-                        torch.ones((self.nflat, ncz - lz, ni), dtype=torch.int32).to(torch.uint16))
+                        torch.ones((self.nflat, rcz - lz, ni), dtype=torch.int32).to(torch.uint16))
                 with torch.cuda.stream(self.stream1):
                     for key in ('data', 'dark', 'flat'):
                         item_gpu[key][k % 2].copy_(item_pinned[key][k % 2], non_blocking=True)
