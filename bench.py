@@ -36,7 +36,8 @@ KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_xsum', '
            'k_fr_prep', 'k_fr_filtprep',
            'k_fr_gscale', 'k_fr_grid_x3', 'k_fr_grid16', 'k_fr_grid_reduce', 'k_fr_rowfft',
            'k_fr_colfft', 'k_pre_median', 'k_pre_dezinger', 'k_ti_mean', 'k_ti_conv', 'k_ti_add',
-           'k_pre_pad360']
+           'k_pre_pad360', 'k_vo_transpose', 'k_vo_dead', 'k_vo_sort', 'k_vo_xmedian',
+           'k_vo_large', 'k_vo_unsort']
 
 
 def parse():
@@ -50,7 +51,7 @@ def parse():
     ap.add_argument('--chunk', type=int, default=8)
     ap.add_argument('--streams', type=int, default=3)
     ap.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
-    ap.add_argument('--stripe', default='fw', choices=['fw', 'none'])
+    ap.add_argument('--stripe', default='fw', choices=['fw', 'none', 'ti', 'vo-all'])
     ap.add_argument('--filter', default='ramp')
     ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
     ap.add_argument('--distinct', type=int, default=8,
@@ -134,6 +135,10 @@ def algorithmic_bytes(name, a, s):
         return ch * nproj * n * (2 + r)
     if name == 'k_pre_log_swap':  # read float32 transmission, write the sinograms
         return ch * nproj * n * (4 + r)
+    if name == 'k_vo_sort':      # read a float32 column, write sorted values + uint16 ranks
+        return ch * nproj * n * (4 + 4 + 2)
+    if name == 'k_vo_xmedian':   # read the sorted image, write its detector median
+        return ch * nproj * n * (4 + 4)
     return None
 
 
@@ -196,7 +201,7 @@ def cpu_baseline(a, data, dark, flat, theta, center):
             'sample': '%d complex slice pairs (%d slices) of the same workload through the FP64 '
                       'oracle (dark/flat, %s-log, %s filter, USFFT backprojection), one pair '
                       'per process, %.1fs wall' % (pairs, 2 * pairs,
-                                                   'fw stripe removal, ' if a.stripe == 'fw' else '',
+                                                   ('%s stripe removal, ' % a.stripe) if a.stripe != 'none' else '',
                                                    a.filter, dt)}
 
 

@@ -309,6 +309,25 @@ int pre_remove_stripe_ti(tomo_pre_plan *plan, float *res, double beta, double ma
                          void *stream);
 
 /*
+ * remove_all_stripe (src/tomocupy/processing/remove_stripe.py:311-497), the
+ * "vo-all" method of proc_sino (proc_functions.py:157-159; Vo et al. 2018,
+ * algorithms 3-6), in place on the float32 chunk res (nproj, nz, n):
+ * dead stripes (roughness over its detector median, stripe test, linear
+ * interpolation), large stripes (sorted columns, detector median of window
+ * la_size, ratio test, normalisation, replacement) and small stripes (sort,
+ * detector median of window sm_size, unsort).  Sorts are stable as the
+ * reference's argsort; the stripe test's line fit is FP64.
+ *   snr        stripe test ratio (config.py vo-all-snr, default 3)
+ *   la_size    window of the large-stripe median, 1..255 (default 61)
+ *   sm_size    window of the small-stripe median, 1..255 (default 21)
+ *   dim        1 (the reference's default); 2 returns TOMO_EUNSUPPORTED
+ * nproj <= 8192, 8 <= n <= 16384.  Scratch (about 14 bytes per sample of the
+ * chunk) is allocated on the plan's first call.
+ */
+int pre_remove_all_stripe(tomo_pre_plan *plan, float *res, double snr, int la_size,
+                          int sm_size, int dim, void *stream);
+
+/*
  * ProcFunctions.pad360 (src/tomocupy/processing/proc_functions.py:109-124),
  * the double field of view of 360-degree scans (file_type double_fov,
  * :193-194), on sinograms in the plan's layout: sino (nz, nproj, n) ->

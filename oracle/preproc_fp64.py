@@ -17,7 +17,12 @@ golden vectors and perfect reconstruction; the damping is a closed form
 ``scipy.ndimage.median_filter`` (the CPU twin of the ``cupyx`` routine the
 reference calls) plus the replacement rule, pinned by a brute-force window
 median on small arrays.  ``downsample`` (binning) is pinned by block sums
-through a reshape and by the conservation of the total.
+through a reshape and by the conservation of the total.  The ``vo-all``
+functions (``detect_stripe``, ``rs_sort``, ``rs_large``, ``rs_dead``,
+``remove_all_stripe``) use SciPy's ``median_filter``, ``uniform_filter1d`` and
+``binary_dilation`` (the CPU twins of the ``cupyx`` routines the reference
+calls) and a stable argsort; they are pinned by ``np.polyfit``, closed-form
+stripe cases and a pure-Python sort/median/unsort (``tests/test_oracle_vo_all.py``).
 """
 
 import json
@@ -243,9 +248,140 @@ def remove_stripe_fw(data, sigma, wname, level):
     return sli[:, st:st + nproj, :ni].swapaxes(0, 1)
 
 
+def detect_stripe(listdata, snr):
+    """``_detect_stripe_batch`` (R:352-375), Algorithm 4 of Vo et al. 2018, per
+    row of ``listdata (nz, ni)``: sort descending, least-squares line through
+    the middle half (ranks ``ndrop .. ni-ndrop-2``, ``ndrop = int(ni/4)``),
+    noise = the line's drop over the row; a row whose extreme value stands
+    ``>= snr`` noise levels off the line gets a mask on the values beyond
+    ``snr/2`` noise levels.  FP64 throughout (the reference fits in float32)."""
+    listdata = np.asarray(listdata, dtype=np.float64)
+    nz, numdata = listdata.shape
+    srt = np.sort(listdata, axis=1)[:, ::-1]
+    xlist = np.arange(numdata, dtype=np.float64)
+    ndrop = int(0.25 * numdata)
+    x = xlist[ndrop:-ndrop - 1]
+    y = srt[:, ndrop:-ndrop - 1]
+    n = x.size
+    xm = x.mean()
+    ym = y.mean(axis=1)
+    sxy = (y * x).sum(axis=1) - n * ym * xm
+    sxx = (x ** 2).sum() - n * xm ** 2
+    slope = sxy / sxx
+    intercept = ym - slope * xm
+    numt1 = intercept + slope * xlist[-1]
+    noise = np.clip(np.abs(numt1 - intercept), 1e-6, None)
+    val1 = np.abs(srt[:, 0] - intercept) / noise
+    val2 = np.abs(srt[:, -1] - numt1) / noise
+    upper = (intercept + noise * snr * 0.5)[:, None]
+    lower = (numt1 - noise * snr * 0.5)[:, None]
+    mask = np.zeros((nz, numdata))
+    mask = np.where((val1 >= snr)[:, None] & (listdata > upper), 1.0, mask)
+    mask = np.where((val2 >= snr)[:, None] & (listdata <= lower), 1.0, mask)
+    return mask
+
+
+def _dilate3(mask):
+    """``binary_dilation`` with a (1, 3) structure, border 0 (R:421-423)."""
+    from scipy import ndimage
+    return ndimage.binary_dilation(mask > 0, structure=np.ones((1, 3), dtype=bool)).astype(
+        np.float64)
+
+
+def _unsort(vals, ids):
+    """Put sorted-order values back at the positions ``ids`` they came from,
+    along axis 0 (the inverse permutation of R:377-386 applied)."""
+    out = np.empty_like(vals)
+    np.put_along_axis(out, ids, vals, axis=0)
+    return out
+
+
+def rs_sort(tomo, size, dim=1):
+    """``_rs_sort3`` (R:389-401), Algorithm 3 of Vo et al.: sort every detector
+    column of ``tomo (nproj, nz, ni)`` over the angles (stable), median-filter
+    the sorted image along the detector (``dim=1``: window ``size``; ``dim=2``:
+    ``size x size`` over detector and rank), put the values back in the
+    original order."""
+    from scipy import ndimage
+    tomo = np.asarray(tomo, dtype=np.float64)
+    ids = np.argsort(tomo, axis=0, kind='stable')
+    srt = np.take_along_axis(tomo, ids, axis=0)
+    if dim == 1:
+        sm = ndimage.median_filter(srt, (1, 1, size))
+    else:
+        sm = ndimage.median_filter(srt, (size, 1, size))
+    return _unsort(sm, ids)
+
+
+def rs_large(tomo, snr, size, drop_ratio=0.1, norm=True):
+    """``_rs_large3`` (R:404-435), Algorithm 5 of Vo et al.: sorted columns,
+    their detector-median smoothing (window ``size``), the ratio of the two
+    over the central ranks (``ndrop = int(0.5 drop_ratio nproj)`` dropped at
+    each end) locates large stripes (``detect_stripe`` + dilation); ``norm``
+    divides every column by that ratio, then the located columns take the
+    smoothed sorted values in their original order."""
+    from scipy import ndimage
+    tomo = np.asarray(tomo, dtype=np.float64)
+    drop_ratio = max(min(drop_ratio, 0.8), 0)
+    nproj = tomo.shape[0]
+    ndrop = int(0.5 * drop_ratio * nproj)
+    ids = np.argsort(tomo, axis=0, kind='stable')
+    srt = np.take_along_axis(tomo, ids, axis=0)
+    smooth = ndimage.median_filter(srt, (1, 1, size))
+    list1 = np.mean(srt[ndrop:nproj - ndrop], axis=0)
+    list2 = np.mean(smooth[ndrop:nproj - ndrop], axis=0)
+    fact = list1 / list2
+    mask = _dilate3(detect_stripe(fact, snr))
+    if norm:
+        tomo = tomo / fact[None]
+    corrected = _unsort(smooth, ids)
+    return np.where(mask[None] > 0, corrected, tomo)
+
+
+def rs_dead(tomo, snr, size, norm=True):
+    """``_rs_dead3`` (R:438-465), Algorithm 6 of Vo et al.: the summed absolute
+    difference of every column from its 10-angle running mean
+    (``uniform_filter1d``, 'reflect'), over its detector median (window
+    ``size``), locates dead/unresponsive columns (``detect_stripe`` +
+    dilation, two columns at each edge never); they are replaced by the
+    linear interpolation between the nearest unmasked columns; then
+    ``rs_large`` (with ``norm``)."""
+    from scipy import ndimage
+    tomo = np.array(tomo, dtype=np.float64)
+    nproj, nz, ni = tomo.shape
+    smooth = ndimage.uniform_filter1d(tomo, 10, axis=0)
+    listdiff = np.sum(np.abs(tomo - smooth), axis=0)
+    bck = ndimage.median_filter(listdiff, (1, size))
+    fact = listdiff / bck
+    mask = _dilate3(detect_stripe(fact, snr))
+    mask[:, 0:2] = 0.0
+    mask[:, -2:] = 0.0
+    for m in range(nz):
+        listx = np.where(mask[m] < 1.0)[0]
+        miss = np.where(mask[m] > 0.0)[0]
+        if len(miss) > 0:
+            matz = tomo[:, m, listx]
+            ids = np.searchsorted(listx, miss)
+            tomo[:, m, miss] = (matz[:, ids - 1] + (miss - listx[ids - 1]) *
+                                (matz[:, ids] - matz[:, ids - 1]) /
+                                (listx[ids] - listx[ids - 1]))
+    if norm:
+        tomo = rs_large(tomo, snr, size)
+    return tomo
+
+
+def remove_all_stripe(tomo, snr=3, la_size=61, sm_size=21, dim=1):
+    """``remove_all_stripe`` (R:468-497), the ``vo-all`` method of
+    ``proc_sino`` (P:157-159): ``rs_dead`` (with ``rs_large``) then
+    ``rs_sort``, on the transmission chunk ``(nproj, nz, ni)``."""
+    tomo = rs_dead(tomo, snr, la_size)
+    return rs_sort(tomo, sm_size, dim)
+
+
 def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym16',
                fw_level=7, take_minus_log=True, bright_ratio=1.0, flat_linear=False,
-               dezinger=0, dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0, binning=0):
+               dezinger=0, dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0, binning=0,
+               vo_snr=3.0, vo_la_size=61, vo_sm_size=21, vo_dim=1):
     """``proc_sino`` then ``proc_proj`` then the swap to sinogram order
     (P:140-198, ``rec.py:172-174``): [binning, as the reader applies it] ->
     [outliers] -> darkflat -> [fw] -> [-log] -> ``(nz, nproj, n)``."""
@@ -260,6 +396,8 @@ def proc_chain(data, dark, flat, *, stripe='none', fw_sigma=1.0, fw_filter='sym1
         res = remove_stripe_fw(res, fw_sigma, fw_filter, fw_level)
     elif stripe == 'ti':
         res = remove_stripe_ti(res, ti_beta, ti_mask)
+    elif stripe == 'vo-all':
+        res = remove_all_stripe(res, vo_snr, vo_la_size, vo_sm_size, vo_dim)
     elif stripe != 'none':
         raise ValueError(stripe)
     if take_minus_log:

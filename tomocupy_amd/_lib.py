@@ -66,6 +66,7 @@ SIGNATURES = {
                                  _c_p]),
     'pre_darkflat_f32': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_p]),
     'pre_bin': (_c_int, [_c_p, _c_p, _c_p, _c_i64, _c_int, _c_p]),
+    'pre_remove_all_stripe': (_c_int, [_c_p, _c_p, _c_d, _c_int, _c_int, _c_int, _c_p]),
     'pre_remove_stripe_ti': (_c_int, [_c_p, _c_p, _c_d, _c_d, _c_p]),
     'pre_pad360': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_p]),
     'pre_destroy': (None, [_c_p]),

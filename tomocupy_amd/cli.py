@@ -13,12 +13,12 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
   shift of the same sinogram (``rec.py:210-275``), one TIFF per centre
   ``<out>/recon_slice%04d_center%05.2f.tiff`` (``writer.py:309-313``).
 
-Stripe removal: ``fw`` and ``ti``; zinger removal: ``--dezinger``; 360-degree
+Stripe removal: ``fw``, ``ti`` and ``vo-all``; zinger removal: ``--dezinger``; 360-degree
 scans with the axis near the border: ``--file-type double_fov``; binning
 ``--binning b`` (2^b x 2^b sums of counts on the GPU, ``reader.py:96-140``).
 As the reference, the width is trimmed to a multiple of ``2^(b+1)``
-(``reader.py:127-133``).  Settings outside this path (vo-all stripe removal,
-phase retrieval, ...) are not offered.  Input formats: see
+(``reader.py:127-133``).  Settings outside this path (phase retrieval,
+other algorithms, ...) are not offered.  Input formats: see
 :mod:`tomocupy_amd.io`.
 """
 import argparse
@@ -55,12 +55,17 @@ def parser():
                    choices=['none', 'ramp', 'shepp', 'hann', 'hamming', 'parzen', 'cosine',
                             'cosine2'])
     r.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
-    r.add_argument('--remove-stripe-method', default='None', choices=['None', 'none', 'fw', 'ti'])
+    r.add_argument('--remove-stripe-method', default='None',
+                   choices=['None', 'none', 'fw', 'ti', 'vo-all'])
     r.add_argument('--fw-sigma', type=float, default=1.0)
     r.add_argument('--fw-filter', default='sym16', choices=['haar', 'db5', 'sym5', 'sym16'])
     r.add_argument('--fw-level', type=int, default=7)
     r.add_argument('--ti-beta', type=float, default=0.022)
     r.add_argument('--ti-mask', type=float, default=1.0)
+    r.add_argument('--vo-all-snr', type=float, default=3.0)
+    r.add_argument('--vo-all-la-size', type=int, default=61)
+    r.add_argument('--vo-all-sm-size', type=int, default=21)
+    r.add_argument('--vo-all-dim', type=int, default=1, choices=[1, 2])
     r.add_argument('--minus-log', default='True')
     r.add_argument('--flat-linear', default='False')
     r.add_argument('--bright-ratio', type=float, default=1.0)
@@ -81,6 +86,11 @@ def _out_dir(a, kind):
         os.path.join(root, base + '_rec')
 
 
+def _vo(a):
+    return dict(vo_all_snr=a.vo_all_snr, vo_all_la_size=a.vo_all_la_size,
+                vo_all_sm_size=a.vo_all_sm_size, vo_all_dim=a.vo_all_dim)
+
+
 def _common(a, ds):
     """Row range, binned width and centre, stripe method (reader.py:82-140):
     the raw width is trimmed to a multiple of 2^(b+1), then width, centre and
@@ -97,7 +107,7 @@ def _common(a, ds):
               'new width %d' % (ds.ni - x1, x1))
     if (z1 - z0) >> b < 1:
         raise SystemExit('fewer than 2^%d rows to bin' % b)
-    stripe = a.remove_stripe_method if a.remove_stripe_method in ('fw', 'ti') else 'none'
+    stripe = a.remove_stripe_method if a.remove_stripe_method in ('fw', 'ti', 'vo-all') else 'none'
     return z0, z1, x1, x1 >> b, centre / 2 ** b, stripe
 
 
@@ -115,7 +125,8 @@ def recon_full(a, ds):
                  fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                  flat_linear=_bool(a.flat_linear), max_write_threads=a.max_write_threads,
                  dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
-                 ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type, binning=b)
+                 ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type, binning=b,
+                 **_vo(a))
     ncz, nrec = rec.ncz, rec.n
     zb = z0 >> b  # first output slice id (rec.py:170)
     stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
@@ -172,7 +183,7 @@ def recon_try(a, ds):
                        fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level,
                        dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
                        ti_beta=a.ti_beta, ti_mask=a.ti_mask, file_type=a.file_type,
-                       center=centre, binning=b)
+                       center=centre, binning=b, **_vo(a))
     nrec, rcentre, sgn = ni, centre, 1.0
     if a.file_type == 'double_fov':
         # reader.py:143-148 and 233-234: the axis moves to the right half of a

@@ -11,6 +11,25 @@ struct tomo_pre_plan;
 namespace tomo {
 
 constexpr int kMaxDezinger = 9;  // largest median window of pre_remove_outliers
+constexpr int kVoMaxProj = 8192;  // vo-all: angles of a column sorted in LDS
+constexpr int kVoMaxWin = 255;    // vo-all: largest detector median window
+constexpr int kVoMaxN = 16384;    // vo-all: detector width of a slice's stripe test
+
+// Scratch of pre_remove_all_stripe (allocated on the plan's first call)
+struct VoBufs {
+  DevBuf T, S, M;  // (nz, n, nproj) float: columns, sorted columns, smoothed
+  DevBuf I;        // (nz, n, nproj) uint16: angle of each sorted rank
+  DevBuf rowf;     // 5 x (nz, n) float: roughness, its median, two means, ratio
+  DevBuf mask;     // (nz, n) uint8: located columns
+  void release() {
+    T.release();
+    S.release();
+    M.release();
+    I.release();
+    rowf.release();
+    mask.release();
+  }
+};
 
 // One level of the Fourier-wavelet stripe removal: analysis input H x W,
 // bands Ho x Wo; the notch keeps nk angular frequencies of the stripe band.
@@ -65,4 +84,5 @@ struct tomo_pre_plan {
   tomo::DevBuf rowlo, rowhi, synlo, synhi, syn[2];
   tomo::DevBuf outl;  // medians of pre_remove_outliers, allocated on first use
   tomo::DevBuf ti;    // column means and corrections of pre_remove_stripe_ti (first use)
+  tomo::VoBufs vo;    // scratch of pre_remove_all_stripe (first use)
 };

@@ -256,5 +256,6 @@ extern "C" void pre_destroy(tomo_pre_plan *P) {
   P->syn[1].release();
   P->outl.release();
   P->ti.release();
+  P->vo.release();
   delete P;
 }

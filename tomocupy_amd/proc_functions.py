@@ -2,6 +2,7 @@
 
 Mirrors ``ProcFunctions`` (``src/tomocupy/processing/proc_functions.py:46-198``)
 for the steps on the fourierrec path: ``remove_outliers`` (dezinger),
+stripe removal (``fw``, ``ti``, ``vo-all``),
 ``darkflat_correction``, ``minus_log``, ``proc_sino`` (outliers, dark/flat,
 stripe removal), ``proc_proj`` (minus log) and ``pad360`` (double field of
 view: ``file_type='double_fov'`` with the rotation axis ``center`` in detector
@@ -25,8 +26,9 @@ class ProcFunctions:
                  bright_ratio=1.0, flat_linear=False, remove_stripe_method='none',
                  fw_sigma=1.0, fw_filter='sym16', fw_level=7, dezinger=0,
                  dezinger_threshold=5000, ti_beta=0.022, ti_mask=1.0, file_type='standard',
-                 center=None, binning=0):
-        if remove_stripe_method not in ('none', 'fw', 'ti'):
+                 center=None, binning=0, vo_all_snr=3.0, vo_all_la_size=61,
+                 vo_all_sm_size=21, vo_all_dim=1):
+        if remove_stripe_method not in ('none', 'fw', 'ti', 'vo-all'):
             raise NotImplementedError('remove_stripe_method %r is not implemented'
                                       % remove_stripe_method)
         self.nproj, self.ncz, self.n = int(nproj), int(ncz), int(n)
@@ -41,6 +43,8 @@ class ProcFunctions:
         self.fw_sigma, self.fw_filter, self.fw_level = float(fw_sigma), fw_filter, int(fw_level)
         self.dezinger, self.dezinger_threshold = int(dezinger), int(dezinger_threshold)
         self.ti_beta, self.ti_mask = float(ti_beta), float(ti_mask)
+        self.vo_all_snr, self.vo_all_la_size = float(vo_all_snr), int(vo_all_la_size)
+        self.vo_all_sm_size, self.vo_all_dim = int(vo_all_sm_size), int(vo_all_dim)
         self._res = None  # float32 transmission chunk of the unfused ti/fw paths
         if int(binning) not in (0, 1, 2, 3):
             raise ValueError('binning must be 0, 1, 2 or 3 (config.py:154-158)')
@@ -157,6 +161,28 @@ class ProcFunctions:
             _lib.stream_ptr(None)), 'pre_remove_stripe_ti')
         return res
 
+    def remove_all_stripe(self, res):
+        """Vo's combined ring removal (``remove_stripe.py:311-497``, the
+        ``vo-all`` method) in place on the float32 chunk: dead, large and
+        small stripes."""
+        import torch
+        sd = (self.nproj, self.ncz, self.n)
+        _lib.check(_lib.lib().pre_remove_all_stripe(
+            self._h, _lib.dev_ptr(res, sd, torch.float32, 'res'), self.vo_all_snr,
+            self.vo_all_la_size, self.vo_all_sm_size, self.vo_all_dim, _lib.stream_ptr(None)),
+            'pre_remove_all_stripe')
+        return res
+
+    def _remove_stripe_res(self, res):
+        # stripe removal on the float32 transmission chunk (proc_functions.py:150-159)
+        if self.remove_stripe_method == 'fw':
+            self.remove_stripe_fw(res)
+        elif self.remove_stripe_method == 'ti':
+            self.remove_stripe_ti(res)
+        elif self.remove_stripe_method == 'vo-all':
+            self.remove_all_stripe(res)
+        return res
+
     def pad360(self, sino, out=None):
         """Double field of view (``proc_functions.py:109-124``) on sinograms
         ``(ncz, nproj, n)``: flip when the axis is in the left half, taper the
@@ -177,11 +203,7 @@ class ProcFunctions:
         (``proc_functions.py:140-161``)."""
         self._outliers(data, dark, flat)
         res = self.darkflat_correction(data, dark, flat, res)
-        if self.remove_stripe_method == 'fw':
-            self.remove_stripe_fw(res)
-        elif self.remove_stripe_method == 'ti':
-            self.remove_stripe_ti(res)
-        return res
+        return self._remove_stripe_res(res)
 
     def proc_proj_swap(self, data, out=None):
         """minus log (``proc_functions.py:163-198``) fused with the swap to
@@ -234,13 +256,14 @@ class ProcFunctions:
                 int(self.take_minus_log), self.fw_sigma, self.fw_filter.encode(), self.fw_level,
                 _lib.stream_ptr(None)), 'pre_process_fw')
             return out
-        if self.remove_stripe_method == 'ti':
-            # needs the whole transmission chunk (means over angles): dark/flat
-            # into a float32 chunk, ring removal in place, then -log + swap
+        if self.remove_stripe_method in ('ti', 'vo-all'):
+            # needs the whole transmission chunk (means, sorts over angles):
+            # dark/flat into a float32 chunk, ring removal in place, then
+            # -log + swap
             if self._res is None or self._res.device != data.device:
                 self._res = torch.empty(sd, dtype=torch.float32, device=data.device)
             self.darkflat_correction(data, dark, flat, self._res)
-            self.remove_stripe_ti(self._res)
+            self._remove_stripe_res(self._res)
             return self.proc_proj_swap(self._res, out)
         _lib.check(_lib.lib().pre_process(
             self._h, _lib.dev_ptr(data, sd, u16, 'data'), _lib.dev_ptr(dark, sk, u16, 'dark'),
@@ -260,14 +283,11 @@ class ProcFunctions:
         td = _lib.torch_dtype(self.dtype)
         if out is None:
             out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=data.device)
-        if self.remove_stripe_method in ('fw', 'ti'):
+        if self.remove_stripe_method != 'none':
             if self._res is None or self._res.device != data.device:
                 self._res = torch.empty(sd, dtype=f32, device=data.device)
             self.darkflat_correction(data, dark, flat, self._res)
-            if self.remove_stripe_method == 'fw':
-                self.remove_stripe_fw(self._res)
-            else:
-                self.remove_stripe_ti(self._res)
+            self._remove_stripe_res(self._res)
             return self.proc_proj_swap(self._res, out)
         _lib.check(_lib.lib().pre_process_f32(
             self._h, _lib.dev_ptr(data, sd, f32, 'data'), _lib.dev_ptr(dark, sk, f32, 'dark'),

@@ -70,7 +70,8 @@ class GPURec:
                  fbp_filter='parzen', remove_stripe_method='none', fw_sigma=1.0,
                  fw_filter='sym16', fw_level=7, minus_log=True, bright_ratio=1.0,
                  flat_linear=False, max_write_threads=8, dezinger=0, dezinger_threshold=5000,
-                 ti_beta=0.022, ti_mask=1.0, file_type='standard', binning=0):
+                 ti_beta=0.022, ti_mask=1.0, file_type='standard', binning=0, vo_all_snr=3.0,
+                 vo_all_la_size=61, vo_all_sm_size=21, vo_all_dim=1):
         """``n``: detector width of the data; the reconstruction is ``n x n``,
         or ``2n x 2n`` for ``file_type='double_fov'`` (reader.py:143-148: the
         rotation axis ``center`` is mirrored into the right half when it lies in
@@ -95,7 +96,9 @@ class GPURec:
                                 fw_sigma=fw_sigma, fw_filter=fw_filter, fw_level=fw_level,
                                 dezinger=dezinger, dezinger_threshold=dezinger_threshold,
                                 ti_beta=ti_beta, ti_mask=ti_mask, file_type=file_type,
-                                center=center, binning=binning)
+                                center=center, binning=binning, vo_all_snr=vo_all_snr,
+                                vo_all_la_size=vo_all_la_size,
+                                vo_all_sm_size=vo_all_sm_size, vo_all_dim=vo_all_dim)
         self.binning = int(binning)
         self.bf = BackprojFunctions(self.n, self.nproj, self.ncz, theta, rec_center, dtype,
                                     fbp_filter)
