@@ -46,6 +46,7 @@ def _chunk(n, nproj, nz, seed):
     (256, 180, 4, 3.0, 61, 21),
     (200, 97, 2, 3.0, 10, 4),      # even windows, odd nproj
     (128, 64, 1, 2.0, 31, 9),      # one slice
+    (160, 60, 2, 3.0, 101, 65),    # windows above 64: the LDS median kernel
 ])
 def test_remove_all_stripe_matches_oracle(n, nproj, nz, snr, la, sm):
     from tomocupy_amd import ProcFunctions
@@ -67,6 +68,23 @@ def test_remove_all_stripe_matches_oracle(n, nproj, nz, snr, la, sm):
     pf.remove_all_stripe(res2)
     torch.cuda.synchronize()
     assert np.array_equal(res2.cpu().numpy().astype(np.float64), got)
+    pf.free()
+
+
+@pytest.mark.parametrize("n,nproj,nz", [(64, 1500, 2), (32, 3000, 1), (16, 8000, 1)])
+def test_remove_all_stripe_long_columns(n, nproj, nz):
+    # column sorts of 8, 16 and 32 keys per thread (1500, 3000, 8000 angles)
+    from tomocupy_amd import ProcFunctions
+    print('vo-all long columns', n, nproj, nz, flush=True)
+    res, _ = _chunk(n, nproj, nz, 12)
+    ref = pp.remove_all_stripe(res.cpu().numpy().astype(np.float64), 3.0, 9, 5, 1)
+    pf = ProcFunctions(nproj, nz, n, 3, 4, remove_stripe_method='vo-all', vo_all_la_size=9,
+                       vo_all_sm_size=5)
+    pf.remove_all_stripe(res)
+    torch.cuda.synchronize()
+    err = _rel_slices(res.cpu().numpy().astype(np.float64), ref, 1)
+    print('rel-L2', err)
+    assert np.all(err <= RTOL32), err
     pf.free()
 
 

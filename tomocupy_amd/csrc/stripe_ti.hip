@@ -18,42 +18,68 @@ using namespace tomo;
 
 namespace {
 
-// v[z][x] = mean_p res[p][z][x] in FP64 (one thread per column of a slice)
+// Partial column sums over angles, FP64: part[s][z][x] = sum of res[p][z][x]
+// over the s-th of gridDim.z blocks of angles (one thread per column and
+// block, coalesced along the detector; the blocks keep enough loads in
+// flight to stream the chunk).
 __global__ __launch_bounds__(256) void k_ti_mean(const float *__restrict__ res,
-                                                 double *__restrict__ v, int nproj, int nz,
+                                                 double *__restrict__ part, int nproj, int nz,
                                                  int n) {
+  const int x = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y, sb = blockIdx.z;
+  if (x >= n) return;
+  const int len = (nproj + gridDim.z - 1) / gridDim.z;
+  const int p0 = sb * len, p1 = min(nproj, p0 + len);
+  double s = 0.0;
+  for (int p = p0; p < p1; ++p) s += (double)res[((size_t)p * nz + z) * n + x];
+  part[((size_t)sb * nz + z) * n + x] = s;
+}
+
+// v[z][x] = column mean (the nsb partial sums over nproj); gamma (one row,
+// written by the z = 0 blocks): gamma_k = beta r^k' - [k = 0]
+__global__ __launch_bounds__(256) void k_ti_vfin(const double *__restrict__ part, int nsb,
+                                                 int nproj, int nz, int n, double beta,
+                                                 double *__restrict__ v,
+                                                 double *__restrict__ gam) {
   const int x = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
   if (x >= n) return;
   double s = 0.0;
-  for (int p = 0; p < nproj; ++p) s += (double)res[((size_t)p * nz + z) * n + x];
+  for (int b = 0; b < nsb; ++b) s += part[((size_t)b * nz + z) * n + x];
   v[(size_t)z * n + x] = s / nproj;
+  if (z == 0) {
+    const double r = (1.0 - beta) / (1.0 + beta);
+    const int k = x <= n / 2 ? x : n - x;
+    gam[x] = beta * std::pow(r, (double)k) - (x == 0 ? 1.0 : 0.0);
+  }
 }
 
-// c[z][x] = mask(x) * sum_j (v[z][j] - v[z][0]) gamma[(x - j) mod n], FP64;
-// the slice's v row and the kernel gamma are staged in LDS (2n doubles).
+// c[z][x] = mask(x) * sum_j (v[z][j] - v[z][0]) gamma[(x - j) mod n], FP64.
+// The slice's v row and gamma are staged in LDS (2n doubles); 64 outputs per
+// workgroup, four threads (adjacent lanes) per output over quarters of j,
+// combined by lane shuffles.
 __global__ __launch_bounds__(256) void k_ti_conv(const double *__restrict__ v,
-                                                 double *__restrict__ c, int n, double beta,
-                                                 int lo, int hi) {
+                                                 const double *__restrict__ gam_,
+                                                 double *__restrict__ c, int n, int lo, int hi) {
   extern __shared__ double sm[];
   double *vr = sm, *gam = sm + n;
   const int z = blockIdx.y;
-  const double r = (1.0 - beta) / (1.0 + beta);
   const double v0 = v[(size_t)z * n];
   for (int j = threadIdx.x; j < n; j += 256) {
     vr[j] = v[(size_t)z * n + j] - v0;
-    const int k = j <= n / 2 ? j : n - j;
-    gam[j] = beta * std::pow(r, (double)k) - (j == 0 ? 1.0 : 0.0);
+    gam[j] = gam_[j];
   }
   __syncthreads();
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  if (x >= n) return;
+  const int x = blockIdx.x * 64 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+  const int j0 = (int)(((long long)n * q) / 4), j1 = (int)(((long long)n * (q + 1)) / 4);
   double s = 0.0;
-  if (x >= lo && x < hi) {
-    // j <= x: gamma[x - j]; j > x: gamma[x - j + n]
-    for (int j = 0; j <= x; ++j) s += vr[j] * gam[x - j];
-    for (int j = x + 1; j < n; ++j) s += vr[j] * gam[x - j + n];
+  if (x < n && x >= lo && x < hi) {
+    for (int j = j0; j < j1; ++j) {
+      const int d = x - j;
+      s += vr[j] * gam[d >= 0 ? d : d + n];
+    }
   }
-  c[(size_t)z * n + x] = s;
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (x < n && q == 0) c[(size_t)z * n + x] = s;
 }
 
 // res[p][z][x] += c[z][x] (in FP64, rounded once)
@@ -80,20 +106,24 @@ extern "C" int pre_remove_stripe_ti(tomo_pre_plan *P, float *res, double beta, d
   TOMO_REQUIRE(n % 2 == 0 && n >= 2, "pre_remove_stripe_ti: n=%d must be even", n);
   TOMO_REQUIRE((size_t)2 * n * sizeof(double) <= max_lds_per_block(),
                "pre_remove_stripe_ti: n=%d too large for one workgroup", n);
-  if (P->ti.bytes < 2 * sizeof(double) * nz * n) {
+  const int nsb = nproj < 32 ? nproj : 32;  // blocks of angles of the column sums
+  const size_t need = sizeof(double) * ((size_t)(nsb + 2) * nz * n + n);
+  if (P->ti.bytes < need) {
     P->ti.release();
-    int rc = P->ti.alloc(2 * sizeof(double) * nz * n);
+    int rc = P->ti.alloc(need);
     if (rc) return rc;
   }
   // the central mask*n columns: [n/2 - m/2, n/2 + m/2), m = floor(mask*n)
   const int m = (int)std::floor(mask * n);
   const int lo = n / 2 - m / 2, hi = n / 2 + m / 2;
   hipStream_t st = (hipStream_t)stream;
-  double *v = P->ti.as<double>(), *c = v + (size_t)nz * n;
+  double *part = P->ti.as<double>(), *v = part + (size_t)nsb * nz * n;
+  double *c = v + (size_t)nz * n, *gam = c + (size_t)nz * n;
   const dim3 g2((unsigned)ceil_div(n, 256), (unsigned)nz);
   {
     ProfScope ps_("k_ti_mean", st);
-    hipLaunchKernelGGL(k_ti_mean, g2, dim3(256), 0, st, res, v, nproj, nz, n);
+    hipLaunchKernelGGL(k_ti_mean, dim3(g2.x, g2.y, (unsigned)nsb), dim3(256), 0, st, res, part,
+                       nproj, nz, n);
   }
   TOMO_LAUNCH_CHECK();
   {
@@ -101,7 +131,9 @@ extern "C" int pre_remove_stripe_ti(tomo_pre_plan *P, float *res, double beta, d
     const size_t lds = 2 * sizeof(double) * n;
     TOMO_HIP(hipFuncSetAttribute((const void *)k_ti_conv,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_ti_conv, g2, dim3(256), lds, st, v, c, n, beta, lo, hi);
+    hipLaunchKernelGGL(k_ti_vfin, g2, dim3(256), 0, st, part, nsb, nproj, nz, n, beta, v, gam);
+    hipLaunchKernelGGL(k_ti_conv, dim3((unsigned)ceil_div(n, 64), (unsigned)nz), dim3(256), lds,
+                       st, v, gam, c, n, lo, hi);
   }
   TOMO_LAUNCH_CHECK();
   {

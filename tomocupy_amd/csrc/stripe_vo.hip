@@ -121,12 +121,55 @@ __global__ __launch_bounds__(256) void k_vo_roughness(const float *__restrict__ 
   if (threadIdx.x == 0) diff[ri] = (float)acc;
 }
 
+// first index i in [0, w] of the lane's sorted window with key >= k
+__device__ __forceinline__ int lower_bound_win(const uint32_t *wv, int w, uint32_t k) {
+  int lo = 0, hi = w;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (wv[mid * 64] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// window[pn] = key after moving window[po+1 .. pn] down by one (pn >= po)
+__device__ __forceinline__ void shift_down(uint32_t *wv, int po, int pn, uint32_t key) {
+  int i = po;
+  for (; i + 4 <= pn; i += 4) {  // four independent loads, then the stores
+    const uint32_t a0 = wv[(i + 1) * 64], a1 = wv[(i + 2) * 64], a2 = wv[(i + 3) * 64],
+                   a3 = wv[(i + 4) * 64];
+    wv[i * 64] = a0;
+    wv[(i + 1) * 64] = a1;
+    wv[(i + 2) * 64] = a2;
+    wv[(i + 3) * 64] = a3;
+  }
+  for (; i < pn; ++i) wv[i * 64] = wv[(i + 1) * 64];
+  wv[pn * 64] = key;
+}
+
+// window[pn] = key after moving window[pn .. po-1] up by one (pn <= po)
+__device__ __forceinline__ void shift_up(uint32_t *wv, int po, int pn, uint32_t key) {
+  int i = po;
+  for (; i - 4 >= pn; i -= 4) {
+    const uint32_t a0 = wv[(i - 1) * 64], a1 = wv[(i - 2) * 64], a2 = wv[(i - 3) * 64],
+                   a3 = wv[(i - 4) * 64];
+    wv[i * 64] = a0;
+    wv[(i - 1) * 64] = a1;
+    wv[(i - 2) * 64] = a2;
+    wv[(i - 3) * 64] = a3;
+  }
+  for (; i > pn; --i) wv[i * 64] = wv[(i - 1) * 64];
+  wv[pn * 64] = key;
+}
+
 // Median along the detector: out[(z n + x) R + r] = rank-(w/2) value of
 // in[(z n + x') R + r], x' = x - w/2 .. x - w/2 + w - 1 reflected
 // (ndimage.median_filter with a window of w along the detector).  One lane
 // per r (64 per workgroup), xs consecutive outputs per workgroup: the lane
 // keeps its window sorted in LDS (column `lane`, stride 64) and slides it by
-// one removal and one insertion per output.
+// one removal and one insertion per output, both positions found by binary
+// search first so that the moves between them are independent loads and
+// stores.  The keys leaving and entering are loaded one step ahead.
 __global__ __launch_bounds__(64) void k_vo_xmedian(const float *__restrict__ in,
                                                    float *__restrict__ out, int R, int n, int w,
                                                    int xs) {
@@ -139,84 +182,123 @@ __global__ __launch_bounds__(64) void k_vo_xmedian(const float *__restrict__ in,
   const float *base = in + (size_t)z * n * R + r;
   uint32_t *wv = win + lane;
   const int h = w / 2;
-  for (int i = 0; i < w; ++i) {  // insertion sort of the first window
+  for (int i = 0; i < w; ++i) {  // binary insertion sort of the first window
     const uint32_t k = fkey(base[(size_t)refl(x0 - h + i, n) * R]);
-    int j = i;
-    while (j > 0 && wv[(j - 1) * 64] > k) {
-      wv[j * 64] = wv[(j - 1) * 64];
-      --j;
-    }
-    wv[j * 64] = k;
+    shift_up(wv, i, lower_bound_win(wv, i, k), k);
   }
   const int x1 = min(x0 + xs, n);
   float *dst = out + (size_t)z * n * R + r;
+  float vo = base[(size_t)refl(x0 - h, n) * R];      // leaves after output x0
+  float vn = base[(size_t)refl(x0 - h + w, n) * R];  // enters after output x0
   for (int x = x0; x < x1; ++x) {
     dst[(size_t)x * R] = fback(wv[h * 64]);
     if (x + 1 >= x1) break;
-    const uint32_t ko = fkey(base[(size_t)refl(x - h, n) * R]);      // leaves
-    const uint32_t kn = fkey(base[(size_t)refl(x - h + w, n) * R]);  // enters
+    const uint32_t ko = fkey(vo), kn = fkey(vn);
+    if (x + 2 < x1) {
+      vo = base[(size_t)refl(x + 1 - h, n) * R];
+      vn = base[(size_t)refl(x + 1 - h + w, n) * R];
+    }
     if (ko == kn) continue;
-    int lo = 0, hi = w - 1;  // first index holding ko (present by construction)
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (wv[mid * 64] < ko) lo = mid + 1;
-      else hi = mid;
-    }
-    int i = lo;
-    if (kn > ko) {
-      while (i + 1 < w && wv[(i + 1) * 64] < kn) {
-        wv[i * 64] = wv[(i + 1) * 64];
-        ++i;
-      }
-    } else {
-      while (i > 0 && wv[(i - 1) * 64] > kn) {
-        wv[i * 64] = wv[(i - 1) * 64];
-        --i;
-      }
-    }
-    wv[i * 64] = kn;
+    const int po = lower_bound_win(wv, w, ko);  // holds ko (present by construction)
+    const int lb = lower_bound_win(wv, w, kn);
+    if (kn > ko) shift_down(wv, po, lb - 1, kn);  // lb > po
+    else shift_up(wv, po, lb, kn);                // lb <= po
   }
+}
+
+// Median along the detector of one value per column (R = 1, the roughness
+// of the dead-stripe test): one thread per output, bitwise selection of the
+// rank-(w/2) key over the window staged in LDS (the largest t with
+// #{key < t} <= w/2, over 32 bits).
+__global__ __launch_bounds__(256) void k_vo_rowmedian(const float *__restrict__ in,
+                                                      float *__restrict__ out, int n, int w) {
+  extern __shared__ uint32_t seg[];  // 256 + w - 1 keys
+  const int z = blockIdx.y, x0 = blockIdx.x * 256, h = w / 2;
+  const float *row = in + (size_t)z * n;
+  for (int i = threadIdx.x; i < 256 + w - 1; i += 256) seg[i] = fkey(row[refl(x0 - h + i, n)]);
+  __syncthreads();
+  const int x = x0 + threadIdx.x;
+  if (x >= n) return;
+  const uint32_t *s = seg + threadIdx.x;
+  uint32_t m = 0;
+  for (int bit = 31; bit >= 0; --bit) {
+    const uint32_t t = m | (1u << bit);
+    int c = 0;
+    for (int i = 0; i < w; ++i) c += s[i] < t;
+    if (c <= h) m = t;
+  }
+  out[(size_t)z * n + x] = fback(m);
 }
 
 // Stable sort of one column row of T over the angles: S = sorted values,
 // I = their angle indices (argsort + take_along_axis, remove_stripe.py:
-// 392-393, 413-414).  Bitonic network over P2 >= nproj 64-bit keys
-// (fkey(value) << 32 | angle) in LDS.  With mean1 != NULL also the mean of
-// the sorted ranks [r0, r1) (list1, :417), FP64 sum, float32 result.
+// 392-393, 413-414).  Bitonic network over P2 = 256 E 64-bit keys
+// (fkey(value) << 32 | angle), E consecutive keys per thread in registers:
+// compare-exchange steps of stride < E stay in registers, larger strides
+// exchange whole register sets with the partner thread through LDS
+// (element i of thread t at i * 256 + t, conflict-free).  With
+// mean1 != NULL also the mean of the sorted ranks [r0, r1) (list1, :417),
+// FP64 sum, float32 result.
+template <int E>
 __global__ __launch_bounds__(256) void k_vo_sort(const float *__restrict__ T,
                                                  float *__restrict__ S,
                                                  uint16_t *__restrict__ I,
-                                                 float *__restrict__ mean1, int nproj, int P2,
-                                                 int r0, int r1) {
+                                                 float *__restrict__ mean1, int nproj, int r0,
+                                                 int r1) {
   extern __shared__ unsigned long long key[];
   __shared__ double red[4];
+  constexpr int P2 = 256 * E;
   const size_t ri = blockIdx.x;
   const float *src = T + ri * nproj;
-  for (int p = threadIdx.x; p < P2; p += 256)
-    key[p] = p < nproj ? ((unsigned long long)fkey(src[p]) << 32) | (unsigned)p : ~0ull;
-  __syncthreads();
-  const int half = P2 >> 1;
+  const int t = threadIdx.x;
+  unsigned long long v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int g = t * E + i;
+    v[i] = g < nproj ? ((unsigned long long)fkey(src[g]) << 32) | (unsigned)g : ~0ull;
+  }
   for (int k = 2; k <= P2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < half; t += 256) {
-        const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-        const int l = i + j;
-        const unsigned long long a = key[i], b = key[l];
-        if ((a > b) == ((i & k) == 0)) {
-          key[i] = b;
-          key[l] = a;
-        }
+    for (int j = k >> 1; j >= E; j >>= 1) {  // strides across threads
+      const int tp = t ^ (j / E);
+      const bool lower = t < tp;
+#pragma unroll
+      for (int i = 0; i < E; ++i) key[i * 256 + t] = v[i];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        const unsigned long long o = key[i * 256 + tp];
+        const bool asc = ((t * E + i) & k) == 0;
+        v[i] = (lower == asc) ? (v[i] < o ? v[i] : o) : (v[i] < o ? o : v[i]);
       }
       __syncthreads();
     }
+#pragma unroll
+    for (int jj = E / 2; jj >= 1; jj >>= 1) {  // strides within a thread
+      if (jj < k) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          if ((i & jj) == 0) {
+            const bool asc = ((t * E + i) & k) == 0;
+            const unsigned long long a = v[i], b = v[i + jj];
+            if ((a > b) == asc) {
+              v[i] = b;
+              v[i + jj] = a;
+            }
+          }
+        }
+      }
+    }
   }
   double acc = 0.0;
-  for (int q = threadIdx.x; q < nproj; q += 256) {
-    const unsigned long long kv = key[q];
-    const float v = fback((uint32_t)(kv >> 32));
-    S[ri * nproj + q] = v;
-    I[ri * nproj + q] = (uint16_t)(kv & 0xffffu);
-    if (q >= r0 && q < r1) acc += (double)v;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int q = t * E + i;
+    if (q < nproj) {
+      const float val = fback((uint32_t)(v[i] >> 32));
+      S[ri * nproj + q] = val;
+      I[ri * nproj + q] = (uint16_t)(v[i] & 0xffffu);
+      if (q >= r0 && q < r1) acc += (double)val;
+    }
   }
   if (mean1) {
     acc = block_sum(acc, red);
@@ -380,8 +462,12 @@ hipError_t vo_lds_setup() {
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
     const int bytes = 65536;
-    err = hipFuncSetAttribute((const void *)k_vo_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              bytes);
+    const void *sorts[] = {(const void *)k_vo_sort<1>,  (const void *)k_vo_sort<2>,
+                           (const void *)k_vo_sort<4>,  (const void *)k_vo_sort<8>,
+                           (const void *)k_vo_sort<16>, (const void *)k_vo_sort<32>};
+    for (const void *f : sorts)
+      if (err == hipSuccess)
+        err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (err == hipSuccess)
       err = hipFuncSetAttribute((const void *)k_vo_detect,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -448,7 +534,12 @@ extern "C" int pre_remove_all_stripe(tomo_pre_plan *P, float *res, double snr, i
   const int ndrop = (int)(0.5 * 0.1 * (double)nproj);
   const int r0 = ndrop, r1 = nproj - ndrop;
   auto xmedian = [&](const float *in, float *out, int R, int w) {
-    const int xs = R == 1 ? 16 : 128;
+    if (R == 1) {
+      hipLaunchKernelGGL(k_vo_rowmedian, dim3((unsigned)ceil_div(n, 256), (unsigned)nz),
+                         dim3(256), sizeof(uint32_t) * (256 + w - 1), st, in, out, n, w);
+      return;
+    }
+    const int xs = 128;
     const dim3 g((unsigned)ceil_div(R, 64), (unsigned)ceil_div(n, xs), (unsigned)nz);
     hipLaunchKernelGGL(k_vo_xmedian, g, dim3(64), sizeof(uint32_t) * 64 * w, st, in, out, R, n,
                        w, xs);
@@ -457,9 +548,22 @@ extern "C" int pre_remove_all_stripe(tomo_pre_plan *P, float *res, double snr, i
     hipLaunchKernelGGL(k_vo_detect, dim3((unsigned)nz), dim3(1024), sizeof(uint32_t) * N2, st,
                        num, den, fact, mask, n, N2, snr, zero_edges);
   };
+  const int E = P2 <= 256 ? 1 : P2 / 256;  // keys per thread of the column sort
   auto sort = [&](float *m1) {
-    hipLaunchKernelGGL(k_vo_sort, dim3((unsigned)rows), dim3(256),
-                       sizeof(unsigned long long) * P2, st, T, S, I, m1, nproj, P2, r0, r1);
+    const dim3 g((unsigned)rows), b(256);
+    const size_t lds = sizeof(unsigned long long) * 256 * E;
+    switch (E) {
+      case 1: hipLaunchKernelGGL(k_vo_sort<1>, g, b, lds, st, T, S, I, m1, nproj, r0, r1); break;
+      case 2: hipLaunchKernelGGL(k_vo_sort<2>, g, b, lds, st, T, S, I, m1, nproj, r0, r1); break;
+      case 4: hipLaunchKernelGGL(k_vo_sort<4>, g, b, lds, st, T, S, I, m1, nproj, r0, r1); break;
+      case 8: hipLaunchKernelGGL(k_vo_sort<8>, g, b, lds, st, T, S, I, m1, nproj, r0, r1); break;
+      case 16:
+        hipLaunchKernelGGL(k_vo_sort<16>, g, b, lds, st, T, S, I, m1, nproj, r0, r1);
+        break;
+      default:
+        hipLaunchKernelGGL(k_vo_sort<32>, g, b, lds, st, T, S, I, m1, nproj, r0, r1);
+        break;
+    }
   };
   {
     ProfScope ps_("k_vo_transpose", st);
