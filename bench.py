@@ -37,7 +37,7 @@ KERNELS = ['k_pre_means', 'k_pre_rows', 'k_fw_rows', 'k_fw_cols', 'k_fw_xsum', '
            'k_fr_gscale', 'k_fr_grid_x3', 'k_fr_grid16', 'k_fr_grid_reduce', 'k_fr_rowfft',
            'k_fr_colfft', 'k_pre_median', 'k_pre_dezinger', 'k_ti_mean', 'k_ti_conv', 'k_ti_add',
            'k_pre_pad360', 'k_vo_transpose', 'k_vo_dead', 'k_vo_sort', 'k_vo_xmedian',
-           'k_vo_large', 'k_vo_unsort']
+           'k_vo_median2d', 'k_vo_large', 'k_vo_unsort']
 
 
 def parse():
@@ -52,6 +52,7 @@ def parse():
     ap.add_argument('--streams', type=int, default=3)
     ap.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
     ap.add_argument('--stripe', default='fw', choices=['fw', 'none', 'ti', 'vo-all'])
+    ap.add_argument('--vo-all-dim', type=int, default=1, choices=[1, 2])
     ap.add_argument('--filter', default='ramp')
     ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
     ap.add_argument('--distinct', type=int, default=8,
@@ -347,7 +348,8 @@ def main():
         with torch.cuda.stream(s):
             pfs.append(ProcFunctions(a.nproj, a.chunk, a.n, dark.shape[1],
                                      flat.shape[1], a.dtype, remove_stripe_method=a.stripe,
-                                     fw_sigma=1.0, fw_filter='sym16', fw_level=7))
+                                     fw_sigma=1.0, fw_filter='sym16', fw_level=7,
+                                     vo_all_dim=a.vo_all_dim))
             bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, a.dtype, a.filter))
             sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=td, device='cuda'))
     rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=td, device='cuda')

@@ -320,7 +320,9 @@ int pre_remove_stripe_ti(tomo_pre_plan *plan, float *res, double beta, double ma
  *   snr        stripe test ratio (config.py vo-all-snr, default 3)
  *   la_size    window of the large-stripe median, 1..255 (default 61)
  *   sm_size    window of the small-stripe median, 1..255 (default 21)
- *   dim        1 (the reference's default); 2 returns TOMO_EUNSUPPORTED
+ *   dim        1 (the reference's default): small-stripe median along the
+ *              detector; 2: over a sm_size x sm_size window of detector and
+ *              rank (sm_size <= 64 and <= nproj)
  * nproj <= 8192, 8 <= n <= 16384.  Scratch (about 14 bytes per sample of the
  * chunk) is allocated on the plan's first call.
  */

@@ -104,10 +104,35 @@ def test_vo_all_chain_matches_oracle():
     pf.free()
 
 
+@pytest.mark.parametrize("n,nproj,nz,la,sm", [
+    (128, 90, 2, 31, 9),     # 2-D window 9 x 9
+    (96, 30, 1, 9, 21),      # reflected at both rank ends (21 of 30 ranks)
+    (64, 200, 1, 11, 64),    # the largest 2-D window
+])
+def test_remove_all_stripe_dim2_matches_oracle(n, nproj, nz, la, sm):
+    # _rs_sort3 with dim 2: median over a sm x sm window of detector and rank
+    from tomocupy_amd import ProcFunctions
+    print('vo-all dim 2', n, nproj, nz, la, sm, flush=True)
+    res, _ = _chunk(n, nproj, nz, 13)
+    ref = pp.remove_all_stripe(res.cpu().numpy().astype(np.float64), 3.0, la, sm, 2)
+    pf = ProcFunctions(nproj, nz, n, 3, 4, remove_stripe_method='vo-all', vo_all_la_size=la,
+                       vo_all_sm_size=sm, vo_all_dim=2)
+    pf.remove_all_stripe(res)
+    torch.cuda.synchronize()
+    err = _rel_slices(res.cpu().numpy().astype(np.float64), ref, 1)
+    print('rel-L2', err)
+    assert np.all(err <= RTOL32), err
+    pf.free()
+
+
 def test_vo_all_rejects_unsupported():
     from tomocupy_amd import ProcFunctions, _lib
-    pf = ProcFunctions(16, 2, 32, 1, 1, remove_stripe_method='vo-all', vo_all_dim=2)
+    pf = ProcFunctions(16, 2, 32, 1, 1, remove_stripe_method='vo-all', vo_all_dim=2,
+                       vo_all_sm_size=17)
     res = torch.ones((16, 2, 32), dtype=torch.float32, device='cuda')
+    with pytest.raises(_lib.TomoError):   # 2-D window taller than the 16 ranks
+        pf.remove_all_stripe(res)
+    pf.vo_all_dim, pf.vo_all_la_size = 3, 5
     with pytest.raises(_lib.TomoError):
         pf.remove_all_stripe(res)
     pf.vo_all_dim, pf.vo_all_la_size = 1, 256

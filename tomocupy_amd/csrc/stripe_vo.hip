@@ -11,7 +11,8 @@
 //                 of their central-rank means locates stripes; every column
 //                 is divided by that ratio and located ones take the smoothed
 //                 sorted values in their own angle order;
-//   small stripes (_rs_sort3, :389-401): sort, median along the detector,
+//   small stripes (_rs_sort3, :389-401): sort, median along the detector
+//                 (dim 1) or over a window of detector and rank (dim 2),
 //                 unsort, for every column.
 // The stripe test (_detect_stripe_batch, :352-375) fits a line through the
 // middle half of the sorted ratios of a slice, in FP64 here (float32 in the
@@ -23,6 +24,7 @@
 // float bits, angle index), i.e. stable as the reference's argsort; the
 // detector medians keep, per lane, its sorted window in LDS and slide it one
 // insert/remove at a time.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
@@ -204,6 +206,78 @@ __global__ __launch_bounds__(64) void k_vo_xmedian(const float *__restrict__ in,
     if (kn > ko) shift_down(wv, po, lb - 1, kn);  // lb > po
     else shift_up(wv, po, lb, kn);                // lb <= po
   }
+}
+
+// #{col[l .. u-1] < t} for a sorted non-empty run (u - l <= 127): branch-free
+// binary search in seven fixed steps, so that the searches of several
+// columns overlap
+__device__ __forceinline__ int count_below(const uint32_t *col, int l, int u, uint32_t t) {
+  int pos = l;
+#pragma unroll
+  for (int step = 64; step >= 1; step >>= 1) {
+    const int q = pos + step;
+    const bool ok = q <= u && col[min(q, u) - 1] < t;
+    pos = ok ? q : pos;
+  }
+  return pos - l;
+}
+
+// 2-D median of the sorted image (dim 2 of _rs_sort3, remove_stripe.py:
+// 397-398: median_filter with a w x w window over detector and rank):
+// out[(z n + x) R + r] = rank-(w*w/2) key of S over x' in the reflected
+// detector window of x and r' in the reflected rank window of r.  Every
+// column of S is sorted over the ranks, so the reflected rank window of a
+// column is at most three sorted runs ([a, b] clipped, its reflection below 0
+// and above R-1, w <= R): #{key < t} over the window is a sum of binary
+// searches, and the rank-(w*w/2) key follows bit by bit (the largest t with
+// #{key < t} <= w*w/2).  Workgroup: 64 ranks x 4 detector positions, the
+// (4 + w - 1) columns x (64 + w - 1) ranks they need staged in LDS as keys.
+__global__ __launch_bounds__(256) void k_vo_median2d(const float *__restrict__ in,
+                                                     float *__restrict__ out, int R, int n,
+                                                     int w) {
+  extern __shared__ uint32_t tile[];
+  const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * 64, x0 = blockIdx.y * 4, z = blockIdx.z;
+  const int h = w / 2;
+  const int t0 = max(0, r0 - h), t1 = min(R - 1, r0 + 63 - h + w - 1);
+  const int rows = t1 - t0 + 1, cols = 4 + w - 1;
+  const float *src = in + (size_t)z * n * R;
+  for (int idx = threadIdx.x; idx < rows * cols; idx += 256) {
+    const int c = idx / rows, q = idx - c * rows;
+    tile[c * rows + q] = fkey(src[(size_t)refl(x0 - h + c, n) * R + t0 + q]);
+  }
+  __syncthreads();
+  const int r = r0 + lane, x = x0 + ty;
+  if (r >= R || x >= n) return;
+  // rank ranges (absolute) of the reflected window of r
+  const int a = r - h, b = r - h + w - 1;
+  int lo[3], hi[3], nr = 0;
+  lo[nr] = max(a, 0) - t0;
+  hi[nr] = min(b, R - 1) - t0 + 1;
+  ++nr;
+  if (a < 0) {
+    lo[nr] = 0 - t0;
+    hi[nr] = -a - t0;
+    ++nr;
+  }
+  if (b > R - 1) {
+    lo[nr] = 2 * R - 1 - b - t0;
+    hi[nr] = R - t0;
+    ++nr;
+  }
+  const int K = (w * w) / 2;
+  const uint32_t *col0 = tile + ty * rows;
+  uint32_t m = 0;
+  for (int bit = 31; bit >= 0; --bit) {
+    const uint32_t t = m | (1u << bit);
+    int cnt = 0;
+#pragma unroll 4
+    for (int i = 0; i < w; ++i) cnt += count_below(col0 + i * rows, lo[0], hi[0], t);
+    for (int g = 1; g < nr; ++g)
+      for (int i = 0; i < w; ++i) cnt += count_below(col0 + i * rows, lo[g], hi[g], t);
+    if (cnt <= K) m = t;
+  }
+  out[((size_t)z * n + x) * R + r] = fback(m);
 }
 
 // Median along the detector of one value per column (R = 1, the roughness
@@ -493,11 +567,8 @@ extern "C" int pre_remove_all_stripe(tomo_pre_plan *P, float *res, double snr, i
   const int nproj = (int)P->nproj, nz = (int)P->nz, n = (int)P->n;
   TOMO_REQUIRE(nproj <= kVoMaxProj, "pre_remove_all_stripe: nproj=%d > %d", nproj, kVoMaxProj);
   TOMO_REQUIRE(n >= 8 && n <= kVoMaxN, "pre_remove_all_stripe: n=%d not in 8..%d", n, kVoMaxN);
-  if (dim == 2) {
-    set_error("pre_remove_all_stripe: dim=2 (2-D median of the sorted columns) is not "
-              "implemented");
-    return TOMO_EUNSUPPORTED;
-  }
+  TOMO_REQUIRE(dim == 1 || (sm_size <= 64 && sm_size <= nproj),
+               "pre_remove_all_stripe: dim=2 needs sm_size <= 64 and <= nproj (sm %d)", sm_size);
   const size_t cells = (size_t)nz * n * nproj, rows = (size_t)nz * n;
   TOMO_REQUIRE(rows <= 0x7fffffff, "pre_remove_all_stripe: too many columns");
   TOMO_REQUIRE(max_lds_per_block() >= 65536 + 1024,
@@ -605,9 +676,16 @@ extern "C" int pre_remove_all_stripe(tomo_pre_plan *P, float *res, double snr, i
     sort(nullptr);
   }
   TOMO_LAUNCH_CHECK();
-  {
+  if (dim == 1) {
     ProfScope ps_("k_vo_xmedian", st);
     xmedian(S, M, nproj, sm_size);
+  } else {
+    ProfScope ps_("k_vo_median2d", st);
+    const int w = sm_size, h = w / 2;
+    const size_t lds = sizeof(uint32_t) * (4 + w - 1) * (size_t)std::min(nproj, 64 - h + w - 1 + h);
+    hipLaunchKernelGGL(k_vo_median2d,
+                       dim3((unsigned)ceil_div(nproj, 64), (unsigned)ceil_div(n, 4), (unsigned)nz),
+                       dim3(256), lds, st, S, M, nproj, n, w);
   }
   TOMO_LAUNCH_CHECK();
   {
