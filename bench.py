@@ -53,6 +53,8 @@ def parse():
     ap.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
     ap.add_argument('--stripe', default='fw', choices=['fw', 'none', 'ti', 'vo-all'])
     ap.add_argument('--vo-all-dim', type=int, default=1, choices=[1, 2])
+    ap.add_argument('--phase-batch', type=int, default=1,
+                    help='steps form: projections per phase-retrieval launch')
     ap.add_argument('--filter', default='ramp')
     ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
     ap.add_argument('--distinct', type=int, default=8,
@@ -60,7 +62,7 @@ def parse():
     ap.add_argument('--cpu-baseline', default='on', choices=['on', 'off'])
     ap.add_argument('--cpu-workers', type=int, default=0, help='0 = min(16, cpu count)')
     ap.add_argument('--no-prof', action='store_true', help='do not time kernels with events')
-    ap.add_argument('--form', default='pipeline', choices=['pipeline', 'batch', 'cg'],
+    ap.add_argument('--form', default='pipeline', choices=['pipeline', 'batch', 'cg', 'steps'],
                     help='pipeline: the chunked reconstruction (default); batch: 96 independent '
                          'reconstructions of mixed sizes as one HIP graph vs eager launches; '
                          'cg: CG least squares with the autograd projector')
@@ -318,6 +320,63 @@ def bench_cg(a):
     print(json.dumps(line))
 
 
+def bench_steps(a):
+    """recon_steps form: the whole scan resident (n, nproj, nz of the
+    pipeline workload, default 2048 x 1500 x 2048), the reference's
+    recon_steps passes (proc_sino by row chunks, Paganin phase retrieval by
+    projection chunks with its test settings, filtered backprojection by row
+    chunks) into a resident reconstruction; reports slices/s."""
+    import torch
+    from tomocupy_amd import _lib
+    from tomocupy_amd.rec_steps import GPURecSteps
+    theta, h_data, h_dark, h_flat = host_data(a)
+    reps = -(-a.nz // a.chunk)
+    t0 = time.time()
+    raw = torch.from_numpy(h_data).cuda().repeat(1, reps, 1)[:, :a.nz].contiguous()
+    dark = torch.from_numpy(h_dark).cuda().repeat(1, reps, 1)[:, :a.nz].contiguous()
+    flat = torch.from_numpy(h_flat).cuda().repeat(1, reps, 1)[:, :a.nz].contiguous()
+    rec = torch.empty((a.nz, a.n, a.n), dtype=_lib.torch_dtype(a.dtype), device='cuda')
+    center = a.center if a.center is not None else a.n / 2 + 0.5
+    kw = dict(energy=20.0, propagation_distance=60.0, pixel_size=1.17, retrieve_phase_alpha=1e-4)
+    rs = GPURecSteps(a.n, a.nproj, a.nz, a.chunk, torch.from_numpy(theta).cuda(), center,
+                     dark.shape[0], flat.shape[0], a.dtype, a.filter, nproj_per_chunk=8,
+                     retrieve_phase_method='paganin', remove_stripe_method=a.stripe,
+                     phase_batch=a.phase_batch, **kw)
+    torch.cuda.synchronize()
+    print('# resident scan and plans in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        rs.recon_steps_resident(raw, dark, flat, rec)
+    torch.cuda.synchronize()
+    _lib.prof_reset()
+    _lib.prof_enable(True)
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        rs.recon_steps_resident(raw, dark, flat, rec)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / a.steps
+    _lib.prof_enable(False)
+    kern = {}
+    for k in ('k_ph_rows_fwd', 'k_ph_cols', 'k_ph_rows_inv', 'k_pre_log_swap', 'k_fw_rows',
+              'k_fr_grid_x3', 'k_pre_rows'):
+        ms, cnt = _lib.prof_read(k)
+        if cnt:
+            kern[k] = {'avg_ms': round(ms / cnt, 5), 'launches': cnt, 'total_ms': round(ms, 3)}
+    line = {'metric': 'recon_steps slices/s (n=%d, nproj=%d, nz=%d, Paganin)' % (a.n, a.nproj, a.nz),
+            'value': round(a.nz / dt, 2), 'unit': 'slices/s', 'n_gpus': 1, 'steps': a.steps,
+            'warmup': a.warmup, 'ms_per_step': round(dt * 1e3, 3), 'higher_is_better': True,
+            'scaling': 'weak', 'vs_baseline': None, 'dtype': {'float32': 'f32', 'float16': 'f16',
+                                                              'bfloat16': 'bf16'}[a.dtype],
+            'data': 'synthetic ellipse-phantom counts tiled over nz, resident in HBM',
+            'config': {'workload': 'recon_steps form: proc_sino (stripe=%s) by chunks of %d rows, '
+                                   'Paganin (alpha 1e-4, 60 mm, 20 keV, 1.17 um) by chunks of 8 '
+                                   'projections, %s filter + backprojection'
+                                   % (a.stripe, a.chunk, a.filter),
+                       'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk},
+            'kernels_events_timed_steps': kern}
+    rs.free()
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -328,6 +387,8 @@ def main():
         return bench_batch(a)
     if a.form == 'cg':
         return bench_cg(a)
+    if a.form == 'steps':
+        return bench_steps(a)
     center = a.center if a.center is not None else a.n / 2 + 0.5
     theta, h_data, h_dark, h_flat = host_data(a)
     cpu = None

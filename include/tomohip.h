@@ -276,6 +276,16 @@ int pre_minus_log_swap(tomo_pre_plan *plan, const float *res, void *out, int out
                        int minus_log, void *stream);
 
 /*
+ * pre_minus_log_swap reading rows z0 .. z0+lz-1 of every projection of a
+ * resident volume vol (nproj, ld, n) float32 (the recon_steps layout,
+ * rec_steps.py:114-125 and backproj_parallel.py:338-341: -log of proc_proj,
+ * then the swap of the z-chunk to sinograms); rows lz .. nz-1 of out
+ * (nz, nproj, n) are zero (the tail of a partial chunk).
+ */
+int pre_minus_log_swap_rows(tomo_pre_plan *plan, const float *vol, int64_t ld, int64_t z0,
+                            int64_t lz, void *out, int out_dtype, int minus_log, void *stream);
+
+/*
  * ProcFunctions.remove_outliers (src/tomocupy/processing/proc_functions.py:96-107),
  * applied by proc_sino to the projections, darks and flats (:146-148):
  *   f = median over the width x 1 x width window (angle, slice, detector)
@@ -328,6 +338,39 @@ int pre_remove_stripe_ti(tomo_pre_plan *plan, float *res, double beta, double ma
  */
 int pre_remove_all_stripe(tomo_pre_plan *plan, float *res, double snr, int la_size,
                           int sm_size, int dim, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Phase retrieval (ProcFunctions.proc_proj, proc_functions.py:169-181)
+ * ------------------------------------------------------------------------- */
+typedef struct tomo_ph_plan tomo_ph_plan;
+
+/*
+ * Replaces the set-up of retrieve_phase.paganin_filter / farago_filter
+ * (src/tomocupy/processing/retrieve_phase.py:59-141): pad widths
+ * (_calc_pad, :166-200: to the next power of two of size + ceil(pi lambda
+ * dist / ps^2), split on both sides), reciprocal grid (:236-304) and the
+ * filter's maximum, for projections of dy x dz (both even).
+ *   method      0 paganin, 1 Gpaganin, 2 farago (config.py retrieve-phase-method)
+ *   pixel_size  [cm]; dist [cm]; energy [keV] (proc_functions.py:171-174
+ *               converts the CLI's micrometres and millimetres)
+ *   alpha       paganin regularisation; db delta/beta (Gpaganin, farago);
+ *               W [cm] (Gpaganin)
+ *   batch       projections transformed per launch (work buffer
+ *               batch * Ny * Nx complex)
+ * Padded sizes Ny, Nx must be powers of two <= 8192.
+ */
+int ph_create(tomo_ph_plan **plan, int64_t dy, int64_t dz, int64_t batch, int method,
+              double pixel_size, double dist, double energy, double alpha, double db,
+              double W);
+/* Pad widths and padded sizes of the plan. */
+int ph_get_pads(const tomo_ph_plan *plan, int64_t *py, int64_t *pz, int64_t *Ny, int64_t *Nx);
+/*
+ * _retrieve_phase (retrieve_phase.py:143-161) in place on data (nproj, dy, dz)
+ * float32: edge-replicated padding, fft2, times the normalised filter, ifft2,
+ * real part, crop.  Transforms and filter in float32.
+ */
+int ph_apply(tomo_ph_plan *plan, float *data, int64_t nproj, void *stream);
+void ph_destroy(tomo_ph_plan *plan);
 
 /*
  * ProcFunctions.pad360 (src/tomocupy/processing/proc_functions.py:109-124),

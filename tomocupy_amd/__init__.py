@@ -2,7 +2,8 @@
 
 Python classes mirror the reference's interface for this path
 (``FourierRec``, ``FBPFilter``, ``BackprojFunctions``, ``ProcFunctions``,
-``GPURec``) on torch GPU tensors; all compute runs in hand-written HIP kernels
+``GPURec``, ``GPURecSteps``, the phase filters of ``retrieve_phase``) on
+torch GPU tensors; all compute runs in hand-written HIP kernels
 of ``libtomohip.so`` through the C ABI in ``include/tomohip.h``.
 """
 from .fourierrec import FourierRec

@@ -67,6 +67,14 @@ SIGNATURES = {
     'pre_darkflat_f32': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_p]),
     'pre_bin': (_c_int, [_c_p, _c_p, _c_p, _c_i64, _c_int, _c_p]),
     'pre_remove_all_stripe': (_c_int, [_c_p, _c_p, _c_d, _c_int, _c_int, _c_int, _c_p]),
+    'ph_create': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_int, _c_d, _c_d, _c_d,
+                           _c_d, _c_d, _c_d]),
+    'ph_get_pads': (_c_int, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
+                             ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
+    'ph_apply': (_c_int, [_c_p, _c_p, _c_i64, _c_p]),
+    'ph_destroy': (None, [_c_p]),
+    'pre_minus_log_swap_rows': (_c_int, [_c_p, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_int,
+                                         _c_p]),
     'pre_remove_stripe_ti': (_c_int, [_c_p, _c_p, _c_d, _c_d, _c_p]),
     'pre_pad360': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_p]),
     'pre_destroy': (None, [_c_p]),

@@ -7,6 +7,10 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
 * ``--reconstruction-type full``: every row chunk through the conveyor
   (:class:`tomocupy_amd.rec.GPURec`, ``rec.py:113-208``), one TIFF per slice
   ``<out>/recon_%05d.tiff`` (``writer.py:258-264``);
+* ``recon_steps --reconstruction-type full``: the whole scan resident on the
+  GPU through :class:`tomocupy_amd.rec_steps.GPURecSteps` (rec_steps.py:114-125)
+  with phase retrieval (``--retrieve-phase-method paganin|Gpaganin|farago``,
+  ``--energy`` keV, ``--propagation-distance`` mm, ``--pixel-size`` um);
 * ``--reconstruction-type try``: the slices at ``--nsino`` (fractions of the
   height, ``reader.py:192-197``) reconstructed at the centres
   ``centre - arange(-w, w, step)`` (``reader.py:224-246``), each a per-slice
@@ -39,6 +43,23 @@ def parser():
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest='cmd', required=True)
     r = sub.add_parser('recon', help='reconstruct a DXchange-like data set')
+    _recon_args(r)
+    s = sub.add_parser('recon_steps', help='reconstruct by steps (whole scan resident; phase '
+                                           'retrieval)')
+    _recon_args(s)
+    s.add_argument('--retrieve-phase-method', default='none',
+                   choices=['none', 'paganin', 'Gpaganin', 'farago'])
+    s.add_argument('--energy', type=float, default=20.0)
+    s.add_argument('--propagation-distance', type=float, default=0.0)
+    s.add_argument('--pixel-size', type=float, default=1.0)
+    s.add_argument('--retrieve-phase-alpha', type=float, default=0.0)
+    s.add_argument('--retrieve-phase-delta-beta', type=float, default=1500.0)
+    s.add_argument('--retrieve-phase-W', type=float, default=2e-4)
+    s.add_argument('--nproj-per-chunk', type=int, default=8)
+    return ap
+
+
+def _recon_args(r):
     r.add_argument('--file-name', required=True)
     r.add_argument('--out-path-name', default=None)
     r.add_argument('--reconstruction-type', default='try', choices=['full', 'try'])
@@ -74,7 +95,6 @@ def parser():
     r.add_argument('--binning', type=int, default=0, choices=[0, 1, 2, 3])
     r.add_argument('--max-write-threads', type=int, default=8)
     r.add_argument('--save-format', default='tiff', choices=['tiff', 'npy'])
-    return ap
 
 
 def _out_dir(a, kind):
@@ -218,10 +238,55 @@ def recon_try(a, ds):
     return written
 
 
+def recon_steps_full(a, ds):
+    """``tomocupy recon_steps --reconstruction-type full`` (rec_steps.py:114-125):
+    the whole scan through GPURecSteps, one TIFF per slice."""
+    from .io import write_tiff
+    from .rec_steps import GPURecSteps
+    import torch
+    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    b = a.binning
+    nz = (z1 - z0) >> b
+    out = _out_dir(a, 'full')
+    os.makedirs(out, exist_ok=True)
+    rs = GPURecSteps(ni, ds.nproj, nz, a.nsino_per_chunk, torch.from_numpy(ds.theta).cuda(),
+                     centre, ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype,
+                     fbp_filter=a.fbp_filter, nproj_per_chunk=a.nproj_per_chunk,
+                     retrieve_phase_method=a.retrieve_phase_method, energy=a.energy,
+                     propagation_distance=a.propagation_distance, pixel_size=a.pixel_size,
+                     retrieve_phase_alpha=a.retrieve_phase_alpha,
+                     retrieve_phase_delta_beta=a.retrieve_phase_delta_beta,
+                     retrieve_phase_W=a.retrieve_phase_W, file_type=a.file_type,
+                     remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
+                     fw_level=a.fw_level, minus_log=_bool(a.minus_log),
+                     bright_ratio=a.bright_ratio, flat_linear=_bool(a.flat_linear),
+                     dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
+                     ti_beta=a.ti_beta, ti_mask=a.ti_mask, binning=b, **_vo(a))
+    zb, ncz = z0 >> b, rs.ncz
+    data, dark, flat = ds.rows(z0, z0 + (nz << b), x1)
+    t = time.perf_counter()
+
+    def write(k, r):
+        for kk in range(len(r)):
+            write_tiff(os.path.join(out, 'recon_%05d.tiff' % (zb + k * ncz + kk)), r[kk])
+
+    rs.recon_steps_all(data, dark, flat, write)
+    rs.free()
+    dt = time.perf_counter() - t
+    print('reconstructed %d slices of %dx%d by steps in %.2f s -> %s'
+          % (nz, rs.n, rs.n, dt, out))
+    return out
+
+
 def main(argv=None):
     a = parser().parse_args(argv)
     from .io import Dataset
     ds = Dataset(a.file_name)
+    if a.cmd == 'recon_steps':
+        if a.reconstruction_type != 'full':
+            raise SystemExit('recon_steps: only --reconstruction-type full is offered')
+        recon_steps_full(a, ds)
+        return 0
     if a.reconstruction_type == 'full':
         recon_full(a, ds)
     else:

@@ -61,12 +61,18 @@ __global__ void k_pre_rows(const TI *__restrict__ data, const float *__restrict_
   }
 }
 
+// out[z][p][:] = (-log) res[p][z0 + z][:] for z < lz, 0 for lz <= z < gridDim.y;
+// res has ld rows per projection (the chunk itself: ld = nz, z0 = 0).
 template <typename TO>
 __global__ void k_pre_log_swap(const float *__restrict__ res, TO *__restrict__ out, int nproj,
-                               int nz, int n, int take_log) {
+                               int ld, int z0, int lz, int n, int take_log) {
   const int p = blockIdx.x, z = blockIdx.y;
-  const float *src = res + ((size_t)p * nz + z) * n;
   TO *dst = out + ((size_t)z * nproj + p) * n;
+  if (z >= lz) {
+    for (int x = threadIdx.x; x < n; x += blockDim.x) str(dst + x, 0.f);
+    return;
+  }
+  const float *src = res + ((size_t)p * ld + z0 + z) * n;
   for (int x = threadIdx.x; x < n; x += blockDim.x) {
     const float r = src[x];
     str(dst + x, take_log ? minus_log(r) : r);
@@ -135,10 +141,11 @@ static void launch_rows0(tomo_pre_plan *P, const TI *data, void *out, int flat_l
 }
 
 template <typename TO>
-static void launch_log_swap(tomo_pre_plan *P, const float *res, void *out, int take_log,
-                            hipStream_t st) {
+static void launch_log_swap(tomo_pre_plan *P, const float *res, int64_t ld, int64_t z0,
+                            int64_t lz, void *out, int take_log, hipStream_t st) {
   hipLaunchKernelGGL(k_pre_log_swap<TO>, dim3((unsigned)P->nproj, (unsigned)P->nz), dim3(256), 0,
-                     st, res, (TO *)out, (int)P->nproj, (int)P->nz, (int)P->n, take_log);
+                     st, res, (TO *)out, (int)P->nproj, (int)ld, (int)z0, (int)lz, (int)P->n,
+                     take_log);
 }
 
 template <typename TI>
@@ -214,26 +221,42 @@ extern "C" int pre_darkflat_f32(tomo_pre_plan *P, const float *data, const float
                   stream);
 }
 
-extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, void *out, int out_dtype,
-                                  int minus_log, void *stream) {
+static int log_swap(const char *who, tomo_pre_plan *P, const float *res, int64_t ld, int64_t z0,
+                    int64_t lz, void *out, int out_dtype, int minus_log, void *stream) {
   clear_error();
-  TOMO_REQUIRE(P != nullptr, "pre_minus_log_swap: plan is NULL");
-  TOMO_REQUIRE(res && out, "pre_minus_log_swap: NULL array");
+  TOMO_REQUIRE(P != nullptr, "%s: plan is NULL", who);
+  TOMO_REQUIRE(res && out, "%s: NULL array", who);
   TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
-               "pre_minus_log_swap: unknown out_dtype %d", out_dtype);
-  TOMO_REQUIRE(res != out, "pre_minus_log_swap: in-place swap is not supported");
+               "%s: unknown out_dtype %d", who, out_dtype);
+  TOMO_REQUIRE(res != out, "%s: in-place swap is not supported", who);
+  TOMO_REQUIRE(ld >= 1 && ld <= 0x7fffffff && z0 >= 0 && lz >= 0 && lz <= P->nz && z0 + lz <= ld,
+               "%s: rows %lld..%lld outside the %lld rows of the volume", who, (long long)z0,
+               (long long)(z0 + lz), (long long)ld);
   hipStream_t st = (hipStream_t)stream;
   {
     ProfScope ps_("k_pre_log_swap", st);
     if (out_dtype == TOMO_F16)
-      launch_log_swap<_Float16>(P, res, out, minus_log, st);
+      launch_log_swap<_Float16>(P, res, ld, z0, lz, out, minus_log, st);
     else if (out_dtype == TOMO_BF16)
-      launch_log_swap<__bf16>(P, res, out, minus_log, st);
+      launch_log_swap<__bf16>(P, res, ld, z0, lz, out, minus_log, st);
     else
-      launch_log_swap<float>(P, res, out, minus_log, st);
+      launch_log_swap<float>(P, res, ld, z0, lz, out, minus_log, st);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
+}
+
+extern "C" int pre_minus_log_swap(tomo_pre_plan *P, const float *res, void *out, int out_dtype,
+                                  int minus_log, void *stream) {
+  return log_swap("pre_minus_log_swap", P, res, P ? P->nz : 0, 0, P ? P->nz : 0, out, out_dtype,
+                  minus_log, stream);
+}
+
+extern "C" int pre_minus_log_swap_rows(tomo_pre_plan *P, const float *vol, int64_t ld,
+                                       int64_t z0, int64_t lz, void *out, int out_dtype,
+                                       int minus_log, void *stream) {
+  return log_swap("pre_minus_log_swap_rows", P, vol, ld, z0, lz, out, out_dtype, minus_log,
+                  stream);
 }
 
 extern "C" void pre_destroy(tomo_pre_plan *P) {

@@ -200,10 +200,30 @@ class ProcFunctions:
 
     def proc_sino(self, data, dark, flat, res=None):
         """outliers, dark/flat correction, then stripe removal
-        (``proc_functions.py:140-161``)."""
+        (``proc_functions.py:140-161``); the float32 transmission chunk
+        ``(nproj, ncz, n)``.  With binning the raw frames are binned first."""
+        if self.binning > 0:
+            data, dark, flat = self._downsample3(data, dark, flat)
         self._outliers(data, dark, flat)
         res = self.darkflat_correction(data, dark, flat, res)
         return self._remove_stripe_res(res)
+
+    def log_swap_rows(self, vol, z0, lz, out=None):
+        """Sinograms ``(ncz, nproj, n)`` of rows ``z0 .. z0+lz-1`` of a
+        resident float32 volume ``(nproj, nz_total, n)``: minus log
+        (``proc_functions.py:76-83``, as ``proc_proj``) and the swap to
+        sinogram order in one pass; rows from ``lz`` on are zero."""
+        import torch
+        td = _lib.torch_dtype(self.dtype)
+        if out is None:
+            out = torch.empty((self.ncz, self.nproj, self.n), dtype=td, device=vol.device)
+        ld = int(vol.shape[1])
+        _lib.check(_lib.lib().pre_minus_log_swap_rows(
+            self._h, _lib.dev_ptr(vol, (self.nproj, ld, self.n), torch.float32, 'vol'), ld,
+            int(z0), int(lz), _lib.dev_ptr(out, (self.ncz, self.nproj, self.n), td, 'out'),
+            _lib.DTYPES[self.dtype], int(self.take_minus_log), _lib.stream_ptr(None)),
+            'pre_minus_log_swap_rows')
+        return out
 
     def proc_proj_swap(self, data, out=None):
         """minus log (``proc_functions.py:163-198``) fused with the swap to
