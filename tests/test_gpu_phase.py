@@ -138,3 +138,43 @@ def test_cli_recon_steps_paganin(tmp_path):
             err = np.sqrt(((got - ref[k]) ** 2).sum() / (ref[k] ** 2).sum())
             print('slice', c0 + k, err)
             assert err <= RTOL32
+
+
+def test_cli_recon_steps_try_paganin(tmp_path):
+    # recon_steps --reconstruction-type try: the whole scan phase-retrieved,
+    # then one slice at several centres
+    import os
+    from tomocupy_amd import io
+    from tomocupy_amd.cli import main
+    n, nproj, nz = 64, 48, 6
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(9, n, theta, nz, ndark=3, nflat=4)
+    p = tmp_path / 'scan.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    out = str(tmp_path / 'try')
+    main(['recon_steps', '--file-name', str(p), '--reconstruction-type', 'try',
+          '--out-path-name', out, '--rotation-axis', '32', '--center-search-width', '2',
+          '--center-search-step', '1', '--nsino', '0.5', '--retrieve-phase-method', 'paganin',
+          '--retrieve-phase-alpha', '0.0001', '--propagation-distance', '60', '--energy', '20',
+          '--pixel-size', '1.17'])
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    res = ph.retrieve_phase(pp.darkflat_correction(data, dark, flat), 'paganin',
+                            pixel_size=1.17e-4, dist=6.0, energy=20.0, alpha=1e-4)
+    idz = int(0.5 * (nz - 1))
+    sino = np.ascontiguousarray(pp.minus_log(res).swapaxes(0, 1))[idz:idz + 1]
+    shifts = np.arange(-2, 2, 1.0)
+    ncz = 8
+    shv = np.zeros(ncz)
+    shv[:len(shifts)] = shifts
+    ne = 4 * n
+    ref = fr.backprojection(ff.fbp_filter_center(np.concatenate([sino] * ncz),
+                                                 ff.calc_filter(ne, 'parzen'), ne, 32.0, shv), th)
+    files = sorted(f for f in os.listdir(out) if f.endswith('.tiff'))
+    assert len(files) == len(shifts)
+    for k, sh in enumerate(shifts):
+        name = 'recon_slice%04d_center%05.2f.tiff' % (idz, 32.0 - sh)
+        got = io.read_tiff(os.path.join(out, name)).astype(np.float64)
+        err = np.sqrt(((got - ref[k]) ** 2).sum() / (ref[k] ** 2).sum())
+        print(name, err)
+        assert err <= RTOL32

@@ -7,7 +7,7 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
 * ``--reconstruction-type full``: every row chunk through the conveyor
   (:class:`tomocupy_amd.rec.GPURec`, ``rec.py:113-208``), one TIFF per slice
   ``<out>/recon_%05d.tiff`` (``writer.py:258-264``);
-* ``recon_steps --reconstruction-type full``: the whole scan resident on the
+* ``recon_steps --reconstruction-type full|try``: the whole scan resident on the
   GPU through :class:`tomocupy_amd.rec_steps.GPURecSteps` (rec_steps.py:114-125)
   with phase retrieval (``--retrieve-phase-method paganin|Gpaganin|farago``,
   ``--energy`` keV, ``--propagation-distance`` mm, ``--pixel-size`` um);
@@ -176,7 +176,11 @@ def recon_full(a, ds):
     return out
 
 
-def recon_try(a, ds):
+def recon_try(a, ds, steps=False):
+    """Try mode; ``steps`` (``recon_steps --reconstruction-type try``,
+    rec_steps.py:114-125 then backproj_parallel.py:360-413): the whole scan
+    is processed first (phase retrieval mixes rows), then the slices are
+    taken from the processed volume."""
     import torch
     from . import BackprojFunctions, ProcFunctions, _lib
     from .io import write_tiff
@@ -213,10 +217,17 @@ def recon_try(a, ds):
             rcentre, sgn = ni - centre, -1.0
     bf = BackprojFunctions(nrec, ds.nproj, ncz, theta, rcentre, a.dtype, a.fbp_filter)
     written = []
+    if steps:
+        rs = _steps(a, ds, ni, nz, centre, stripe)
+        vol = rs.proc_volume(*(torch.from_numpy(x).cuda()
+                               for x in ds.rows(z0, z0 + (nz << b), x1)))
     for idz in ids:
-        data, dark, flat = (torch.from_numpy(x).cuda()
-                            for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, x1))
-        sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, nrec)
+        if steps:
+            sino = rs.sino_row(vol, idz >> b)
+        else:
+            data, dark, flat = (torch.from_numpy(x).cuda()
+                                for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, x1))
+            sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, nrec)
         batch = sino.expand(ncz, -1, -1).contiguous()
         obj = torch.empty((ncz, nrec, nrec), dtype=td, device='cuda')
         for c0 in range(0, len(shifts), ncz):
@@ -233,23 +244,18 @@ def recon_try(a, ds):
                 written.append(path)
     pf.free()
     bf.free()
+    if steps:
+        rs.free()
     print('wrote %d try images (%d slices x %d centres) -> %s'
           % (len(written), len(ids), len(shifts), out))
     return written
 
 
-def recon_steps_full(a, ds):
-    """``tomocupy recon_steps --reconstruction-type full`` (rec_steps.py:114-125):
-    the whole scan through GPURecSteps, one TIFF per slice."""
-    from .io import write_tiff
-    from .rec_steps import GPURecSteps
+def _steps(a, ds, ni, nz, centre, stripe):
     import torch
-    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    from .rec_steps import GPURecSteps
     b = a.binning
-    nz = (z1 - z0) >> b
-    out = _out_dir(a, 'full')
-    os.makedirs(out, exist_ok=True)
-    rs = GPURecSteps(ni, ds.nproj, nz, a.nsino_per_chunk, torch.from_numpy(ds.theta).cuda(),
+    return GPURecSteps(ni, ds.nproj, nz, a.nsino_per_chunk, torch.from_numpy(ds.theta).cuda(),
                      centre, ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype,
                      fbp_filter=a.fbp_filter, nproj_per_chunk=a.nproj_per_chunk,
                      retrieve_phase_method=a.retrieve_phase_method, energy=a.energy,
@@ -262,6 +268,18 @@ def recon_steps_full(a, ds):
                      bright_ratio=a.bright_ratio, flat_linear=_bool(a.flat_linear),
                      dezinger=a.dezinger, dezinger_threshold=a.dezinger_threshold,
                      ti_beta=a.ti_beta, ti_mask=a.ti_mask, binning=b, **_vo(a))
+
+
+def recon_steps_full(a, ds):
+    """``tomocupy recon_steps --reconstruction-type full`` (rec_steps.py:114-125):
+    the whole scan through GPURecSteps, one TIFF per slice."""
+    from .io import write_tiff
+    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    b = a.binning
+    nz = (z1 - z0) >> b
+    out = _out_dir(a, 'full')
+    os.makedirs(out, exist_ok=True)
+    rs = _steps(a, ds, ni, nz, centre, stripe)
     zb, ncz = z0 >> b, rs.ncz
     data, dark, flat = ds.rows(z0, z0 + (nz << b), x1)
     t = time.perf_counter()
@@ -283,9 +301,10 @@ def main(argv=None):
     from .io import Dataset
     ds = Dataset(a.file_name)
     if a.cmd == 'recon_steps':
-        if a.reconstruction_type != 'full':
-            raise SystemExit('recon_steps: only --reconstruction-type full is offered')
-        recon_steps_full(a, ds)
+        if a.reconstruction_type == 'full':
+            recon_steps_full(a, ds)
+        else:
+            recon_try(a, ds, steps=True)
         return 0
     if a.reconstruction_type == 'full':
         recon_full(a, ds)

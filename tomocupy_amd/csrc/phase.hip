@@ -13,11 +13,15 @@
 // u, v the reference's reciprocal coordinates (k - (N-1)/2) / ((N-1) ps),
 // half a sample off zero for even N.
 //
-// Three kernels per batch of projections, through a complex work buffer:
-// rows (two real rows packed into one complex FFT, split into their two
-// spectra), columns (forward FFT, filter, inverse FFT, scaled by 1/(Ny Nx)),
-// and inverse rows (the Hermitian parts of two row spectra packed into one
-// inverse FFT, whose real and imaginary parts are the two real rows).  The
+// Three kernels per batch of projections, through a complex work buffer of
+// the dy real rows' spectra (the py padded rows above and below a projection
+// are copies of its first and last rows, so their row spectra are those
+// rows' spectra): rows (two real rows packed into one complex FFT, split into
+// their two spectra), columns (the padded column rebuilt from the edge
+// spectra, forward FFT, filter, inverse FFT scaled by 1/(Ny Nx), the dy rows
+// the crop keeps written back), and inverse rows (the Hermitian parts of two
+// row spectra packed into one inverse FFT, whose real and imaginary parts
+// are the two real rows).  The
 // FFTs are the LDS Stockham transforms of fft_lds.h; the filter is evaluated
 // in float32 (the reference's precision) from 1-D tables built in FP64.
 #include <algorithm>
@@ -40,7 +44,7 @@ struct tomo_ph_plan {
   size_t rowLds = 0, colLds = 0;
   FftPlan fy, fx;
   DevBuf ty, tx;  // fftshifted 1-D terms of the filter (rows, columns)
-  DevBuf F;       // (batch, Ny, Nx) float2 work buffer
+  DevBuf F;       // (batch, dy, Nx) float2: spectra of the real rows
 };
 
 namespace {
@@ -78,27 +82,27 @@ __global__ __launch_bounds__(256) void k_ph_fmax(const float *__restrict__ ty,
   if (threadIdx.x == 0) rowmax[ky] = red[0];
 }
 
-// Forward row FFTs of padded rows 2 pr and 2 pr + 1 of projection blockIdx.y:
-// z = row0 + i row1, Z = fft(z); X0 = (Z[k] + conj Z[-k]) / 2,
-// X1 = (Z[k] - conj Z[-k]) / (2i).
+// Forward FFTs of the column-padded real rows 2 pr and 2 pr + 1 of
+// projection blockIdx.y: z = row0 + i row1, Z = fft(z);
+// X0 = (Z[k] + conj Z[-k]) / 2, X1 = (Z[k] - conj Z[-k]) / (2i).
 __global__ __launch_bounds__(512) void k_ph_rows_fwd(const float *__restrict__ data,
                                                       float2 *__restrict__ F, FftDesc fd, int dy,
-                                                      int dz, int py, int pz, int Ny) {
+                                                      int dz, int pz) {
   extern __shared__ float2 sm[];
   const int pr = blockIdx.x, b = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int Nx = fd.N;
-  const int iy0 = 2 * pr, iy1 = iy0 + 1;
+  const int iy0 = 2 * pr;
   const float *img = data + (size_t)b * dy * dz;
-  const float *r0 = img + (size_t)min(max(iy0 - py, 0), dy - 1) * dz;
-  const float *r1 = img + (size_t)min(max(iy1 - py, 0), dy - 1) * dz;
+  const float *r0 = img + (size_t)iy0 * dz;
+  const float *r1 = r0 + dz;
   for (int ix = tid; ix < Nx; ix += nt) {
     const int sx = min(max(ix - pz, 0), dz - 1);
     sm[lds_pad(ix)] = make_float2(r0[sx], r1[sx]);
   }
   __syncthreads();
   lds_fft<-1>(sm, fd, 0, tid, nt);
-  float2 *f0 = F + ((size_t)b * Ny + iy0) * Nx;
+  float2 *f0 = F + ((size_t)b * dy + iy0) * Nx;
   float2 *f1 = f0 + Nx;
   for (int k = tid; k < Nx; k += nt) {
     const float2 zk = sm[lds_pad(k)], zm = cconj(sm[lds_pad(k == 0 ? 0 : Nx - k)]);
@@ -107,10 +111,13 @@ __global__ __launch_bounds__(512) void k_ph_rows_fwd(const float *__restrict__ d
   }
 }
 
-// Column pass over 2^lgC adjacent columns of projection blockIdx.y: forward
-// FFT, times H / max H / (Ny Nx), inverse FFT, in place in F.
+// Column pass over 2^lgC adjacent columns of projection blockIdx.y: the
+// padded column (padded row ky holds real row clamp(ky - py, 0, dy - 1)),
+// forward FFT, times H / max H / (Ny Nx), inverse FFT; rows py .. py+dy-1
+// written back in place.
 __global__ __launch_bounds__(512) void k_ph_cols(float2 *__restrict__ F, FftDesc fd,
-                                                  int lgC, int Nx, const float *__restrict__ ty,
+                                                  int lgC, int Nx, int dy, int py,
+                                                  const float *__restrict__ ty,
                                                   const float *__restrict__ tx, int method,
                                                   float A, float B, float alpha, float db,
                                                   float hmax, float scale) {
@@ -123,10 +130,10 @@ __global__ __launch_bounds__(512) void k_ph_cols(float2 *__restrict__ F, FftDesc
   const int G = gridDim.x;
   const int g = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
   const int kx0 = g << lgC, b = blockIdx.y;
-  float2 *fb = F + (size_t)b * Ny * Nx + kx0;
+  float2 *fb = F + (size_t)b * dy * Nx + kx0;
   for (int e = tid; e < (Ny << lgC); e += nt) {
     const int ky = e >> lgC, c = e & (C - 1);
-    sm[(lds_pad(ky) << lgC) + c] = fb[(size_t)ky * Nx + c];
+    sm[(lds_pad(ky) << lgC) + c] = fb[(size_t)min(max(ky - py, 0), dy - 1) * Nx + c];
   }
   __syncthreads();
   lds_fft<-1>(sm, fd, lgC, tid, nt);
@@ -138,25 +145,25 @@ __global__ __launch_bounds__(512) void k_ph_cols(float2 *__restrict__ F, FftDesc
   }
   __syncthreads();
   lds_fft<1>(sm, fd, lgC, tid, nt);
-  for (int e = tid; e < (Ny << lgC); e += nt) {
-    const int ky = e >> lgC, c = e & (C - 1);
-    fb[(size_t)ky * Nx + c] = sm[(lds_pad(ky) << lgC) + c];
+  for (int e = tid; e < (dy << lgC); e += nt) {
+    const int y = e >> lgC, c = e & (C - 1);
+    fb[(size_t)y * Nx + c] = sm[(lds_pad(y + py) << lgC) + c];
   }
 }
 
-// Inverse row FFTs for output rows 2 pr and 2 pr + 1 (padded rows + py):
+// Inverse row FFTs for output rows 2 pr and 2 pr + 1:
 // Re ifft(G) = ifft(Herm G), Herm G[k] = (G[k] + conj G[-k]) / 2, so
 // ifft(Herm G0 + i Herm G1) = Re ifft(G0) + i Re ifft(G1); cropped columns
 // written back to the projection.
 __global__ __launch_bounds__(512) void k_ph_rows_inv(float *__restrict__ data,
                                                       const float2 *__restrict__ F, FftDesc fd,
-                                                      int dy, int dz, int py, int pz, int Ny) {
+                                                      int dy, int dz, int pz) {
   extern __shared__ float2 sm[];
   const int pr = blockIdx.x, b = blockIdx.y;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int Nx = fd.N;
-  const int oy0 = 2 * pr, oy1 = oy0 + 1;
-  const float2 *g0 = F + ((size_t)b * Ny + oy0 + py) * Nx;
+  const int oy0 = 2 * pr, oy1 = oy0 + 1;  // dy is even: both rows exist
+  const float2 *g0 = F + ((size_t)b * dy + oy0) * Nx;
   const float2 *g1 = g0 + Nx;
   for (int k = tid; k < Nx; k += nt) {
     const int km = k == 0 ? 0 : Nx - k;
@@ -259,7 +266,7 @@ extern "C" int ph_create(tomo_ph_plan **plan, int64_t dy, int64_t dz, int64_t ba
   if (!rc) rc = P->fx.init(P->Nx);
   if (!rc) rc = P->ty.alloc(sizeof(float) * P->Ny);
   if (!rc) rc = P->tx.alloc(sizeof(float) * P->Nx);
-  if (!rc) rc = P->F.alloc(sizeof(float2) * (size_t)batch * P->Ny * P->Nx);
+  if (!rc) rc = P->F.alloc(sizeof(float2) * (size_t)batch * dy * P->Nx);
   if (rc) {
     ph_destroy(P);
     return rc;
@@ -348,15 +355,14 @@ extern "C" int ph_apply(tomo_ph_plan *P, float *data, int64_t nproj, void *strea
     float *img = data + (size_t)b0 * dy * dz;
     {
       ProfScope ps_("k_ph_rows_fwd", st);
-      hipLaunchKernelGGL(k_ph_rows_fwd, dim3((unsigned)(P->Ny / 2), nb), dim3(P->rowThreads),
-                         P->rowLds, st, img, P->F.as<float2>(), P->fx.d, dy, dz, P->py, P->pz,
-                         P->Ny);
+      hipLaunchKernelGGL(k_ph_rows_fwd, dim3((unsigned)(dy / 2), nb), dim3(P->rowThreads),
+                         P->rowLds, st, img, P->F.as<float2>(), P->fx.d, dy, dz, P->pz);
     }
     TOMO_LAUNCH_CHECK();
     {
       ProfScope ps_("k_ph_cols", st);
       hipLaunchKernelGGL(k_ph_cols, dim3((unsigned)(P->Nx >> P->lgC), nb), dim3(P->colThreads),
-                         P->colLds, st, P->F.as<float2>(), P->fy.d, P->lgC, P->Nx,
+                         P->colLds, st, P->F.as<float2>(), P->fy.d, P->lgC, P->Nx, dy, P->py,
                          P->ty.as<float>(), P->tx.as<float>(), P->method, P->A, P->B, P->alpha,
                          P->db, P->hmax, scale);
     }
@@ -364,8 +370,7 @@ extern "C" int ph_apply(tomo_ph_plan *P, float *data, int64_t nproj, void *strea
     {
       ProfScope ps_("k_ph_rows_inv", st);
       hipLaunchKernelGGL(k_ph_rows_inv, dim3((unsigned)(dy / 2), nb), dim3(P->rowThreads),
-                         P->rowLds, st, img, P->F.as<float2>(), P->fx.d, dy, dz, P->py, P->pz,
-                         P->Ny);
+                         P->rowLds, st, img, P->F.as<float2>(), P->fx.d, dy, dz, P->pz);
     }
     TOMO_LAUNCH_CHECK();
   }

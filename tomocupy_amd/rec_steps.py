@@ -88,6 +88,40 @@ class GPURecSteps:
         chunks go to ``rec (nz, n, n)`` on the GPU and/or to
         ``write_chunk(k, numpy)``.  Runs on torch's current stream."""
         import torch
+        vol = self.proc_volume(raw, dark, flat)
+        ncz, dev = self.ncz, vol.device
+        # 3. minus log + swap per row chunk, [pad360], filter + backprojection
+        sino = torch.empty((ncz, self.nproj, self.ni), dtype=self.tdtype, device=dev)
+        out = torch.empty((ncz, self.n, self.n), dtype=self.tdtype, device=dev)
+        for k in range(self.nzchunk):
+            z0, lz = k * ncz, self.lzchunk[k]
+            self.pf.log_swap_rows(vol, z0, lz, sino)
+            s = self.pf.pad360(sino) if self.pf.file_type == 'double_fov' else sino
+            self.bf.fbp_filter_backprojection(out, s)
+            if rec is not None:
+                rec[z0:z0 + lz].copy_(out[:lz])
+            if write_chunk is not None:
+                host = out[:lz].float().cpu().numpy() if self.tdtype == torch.bfloat16 \
+                    else out[:lz].cpu().numpy()
+                write_chunk(k, host)
+        return rec
+
+    def sino_row(self, vol, z):
+        """The sinogram ``(1, nproj, n)`` of row ``z`` of a processed volume
+        (minus log, [pad360]): the try mode's input
+        (``backproj_parallel.py:380``)."""
+        import torch
+        sino = torch.empty((self.ncz, self.nproj, self.ni), dtype=self.tdtype, device=vol.device)
+        self.pf.log_swap_rows(vol, z, 1, sino)
+        if self.pf.file_type == 'double_fov':
+            sino = self.pf.pad360(sino)
+        return sino[:1]
+
+    def proc_volume(self, raw, dark, flat):
+        """Passes 1 and 2 (``proc_sino`` by row chunks, phase retrieval by
+        projection chunks): the float32 transmission volume ``(nproj, nz, ni)``
+        on the GPU."""
+        import torch
         b = self.binning
         ncz, rcz = self.ncz, self.ncz << b
         dev = raw.device
@@ -114,21 +148,7 @@ class GPURecSteps:
         if self.phase is not None:
             for p0 in range(0, self.nproj, self.ncproj):
                 self.phase.apply(vol[p0:p0 + self.ncproj])
-        # 3. minus log + swap per row chunk, [pad360], filter + backprojection
-        sino = torch.empty((ncz, self.nproj, self.ni), dtype=self.tdtype, device=dev)
-        out = torch.empty((ncz, self.n, self.n), dtype=self.tdtype, device=dev)
-        for k in range(self.nzchunk):
-            z0, lz = k * ncz, self.lzchunk[k]
-            self.pf.log_swap_rows(vol, z0, lz, sino)
-            s = self.pf.pad360(sino) if self.pf.file_type == 'double_fov' else sino
-            self.bf.fbp_filter_backprojection(out, s)
-            if rec is not None:
-                rec[z0:z0 + lz].copy_(out[:lz])
-            if write_chunk is not None:
-                host = out[:lz].float().cpu().numpy() if self.tdtype == torch.bfloat16 \
-                    else out[:lz].cpu().numpy()
-                write_chunk(k, host)
-        return rec
+        return vol
 
     def free(self):
         self.pf.free()
