@@ -134,3 +134,29 @@ def test_cli_double_fov_full_and_try(tmp_path):
         err = _rel(got.astype(np.float64), reft[k])
         print('double-fov try centre', centeri - sh, err)
         assert err <= 1e-5
+
+
+def test_cli_full_float16_crops_to_power_of_two(tmp_path):
+    # reader.py:155-165: float16 crops the width to 2^int(log2 ni) (centred)
+    # and moves the centre by (2^k - ni)/2; here 96 -> 64 columns from 16
+    ni, nproj, nz = 96, 60, 2
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(3, ni, theta, nz, ndark=3, nflat=4)
+    p = tmp_path / 'scan.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    out = str(tmp_path / 'rec16')
+    main(['recon', '--file-name', str(p), '--reconstruction-type', 'full', '--nsino-per-chunk',
+          '2', '--out-path-name', out, '--dtype', 'float16', '--rotation-axis', '50'])
+    n, st, centre = 64, 16, 50 + (64 - 96) / 2
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    sino = pp.proc_chain(data[:, :, st:st + n], dark[:, :, st:st + n], flat[:, :, st:st + n])
+    ne = 256  # 4n, a power of two already
+    ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne, centre),
+                            th)
+    for k in range(nz):
+        got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k))
+        assert got.shape == (n, n) and got.dtype == np.float16
+        err = _rel(got.astype(np.float64), ref[k])
+        print('slice', k, err)
+        assert err <= 2e-3

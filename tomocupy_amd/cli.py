@@ -112,9 +112,12 @@ def _vo(a):
 
 
 def _common(a, ds):
-    """Row range, binned width and centre, stripe method (reader.py:82-140):
-    the raw width is trimmed to a multiple of 2^(b+1), then width, centre and
-    height are divided by 2^b."""
+    """Row range, raw column range, binned width and centre, stripe method
+    (reader.py:82-165): the raw width is trimmed to a multiple of 2^(b+1),
+    then width, centre and height are divided by 2^b; for float16 the width
+    is cropped to its largest power of two, centred, and the centre moves by
+    (2^k - ni)/2 (reader.py:155-165; the crop is applied to the binned
+    columns, which equals the reference's raw crop without binning)."""
     z0 = a.start_row
     z1 = ds.nz if a.end_row == -1 else a.end_row
     if not (0 <= z0 < z1 <= ds.nz):
@@ -128,13 +131,23 @@ def _common(a, ds):
     if (z1 - z0) >> b < 1:
         raise SystemExit('fewer than 2^%d rows to bin' % b)
     stripe = a.remove_stripe_method if a.remove_stripe_method in ('fw', 'ti', 'vo-all') else 'none'
-    return z0, z1, x1, x1 >> b, centre / 2 ** b, stripe
+    ni, centre = x1 >> b, centre / 2 ** b
+    x0 = 0
+    if a.dtype == 'float16':
+        p2 = 2 ** int(np.log2(ni))
+        st = (ni - p2) // 2
+        centre += (p2 - ni) / 2
+        if p2 != ni:
+            print('crop data to the power of 2 sizes to work with 16bit precision, output size '
+                  'in x dimension %d' % p2)
+        x0, x1, ni = st << b, (st + p2) << b, p2
+    return z0, z1, (x0, x1), ni, centre, stripe
 
 
 def recon_full(a, ds):
     from .io import write_tiff
     from .rec import GPURec
-    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    z0, z1, cols, ni, centre, stripe = _common(a, ds)
     b = a.binning
     nz = (z1 - z0) >> b
     out = _out_dir(a, 'full')
@@ -155,7 +168,7 @@ def recon_full(a, ds):
 
     def read(k):
         s0 = z0 + ((k * ncz) << b)
-        return ds.rows(s0, s0 + (rec.lzchunk[k] << b), x1)
+        return ds.rows(s0, s0 + (rec.lzchunk[k] << b), cols)
 
     def write(k, r):
         for kk in range(len(r)):
@@ -184,7 +197,7 @@ def recon_try(a, ds, steps=False):
     import torch
     from . import BackprojFunctions, ProcFunctions, _lib
     from .io import write_tiff
-    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    z0, z1, cols, ni, centre, stripe = _common(a, ds)
     b = a.binning
     nz = (z1 - z0) >> b
     out = _out_dir(a, 'try')
@@ -197,7 +210,10 @@ def recon_try(a, ds, steps=False):
     # shifts in binned pixels, centres saved in raw pixels (reader.py:229-232)
     shifts = (np.arange(-a.center_search_width, a.center_search_width,
                         a.center_search_step * 2 ** b).astype(np.float32) / 2 ** b)
-    centres = (centre - shifts) * 2 ** b
+    # saved centres (reader.py:231-232): (uncropped centre - shift) 2^b + st,
+    # st the float16 crop start (0 otherwise), as the reference names them
+    st = cols[0] >> b
+    centres = (centre + st - shifts) * 2 ** b + st
     ncz = max(2, a.nsino_per_chunk)
     td = _lib.torch_dtype(a.dtype)
     theta = torch.from_numpy(ds.theta).cuda()
@@ -220,13 +236,13 @@ def recon_try(a, ds, steps=False):
     if steps:
         rs = _steps(a, ds, ni, nz, centre, stripe)
         vol = rs.proc_volume(*(torch.from_numpy(x).cuda()
-                               for x in ds.rows(z0, z0 + (nz << b), x1)))
+                               for x in ds.rows(z0, z0 + (nz << b), cols)))
     for idz in ids:
         if steps:
             sino = rs.sino_row(vol, idz >> b)
         else:
             data, dark, flat = (torch.from_numpy(x).cuda()
-                                for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, x1))
+                                for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, cols))
             sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, nrec)
         batch = sino.expand(ncz, -1, -1).contiguous()
         obj = torch.empty((ncz, nrec, nrec), dtype=td, device='cuda')
@@ -274,14 +290,14 @@ def recon_steps_full(a, ds):
     """``tomocupy recon_steps --reconstruction-type full`` (rec_steps.py:114-125):
     the whole scan through GPURecSteps, one TIFF per slice."""
     from .io import write_tiff
-    z0, z1, x1, ni, centre, stripe = _common(a, ds)
+    z0, z1, cols, ni, centre, stripe = _common(a, ds)
     b = a.binning
     nz = (z1 - z0) >> b
     out = _out_dir(a, 'full')
     os.makedirs(out, exist_ok=True)
     rs = _steps(a, ds, ni, nz, centre, stripe)
     zb, ncz = z0 >> b, rs.ncz
-    data, dark, flat = ds.rows(z0, z0 + (nz << b), x1)
+    data, dark, flat = ds.rows(z0, z0 + (nz << b), cols)
     t = time.perf_counter()
 
     def write(k, r):

@@ -58,13 +58,13 @@ class Dataset:
         else:
             self.theta = np.linspace(0, np.pi, self.nproj).astype(np.float32)
 
-    def rows(self, z0, z1, x1=None):
+    def rows(self, z0, z1, cols=None):
         """``(data, dark, flat)`` uint16 for detector rows [z0, z1) and
-        columns [0, x1) (all columns by default)."""
-        x1 = self.ni if x1 is None else int(x1)
-        return (np.ascontiguousarray(self.data[:, z0:z1, :x1], dtype=np.uint16),
-                np.ascontiguousarray(self.dark[:, z0:z1, :x1], dtype=np.uint16),
-                np.ascontiguousarray(self.flat[:, z0:z1, :x1], dtype=np.uint16))
+        columns ``cols = (x0, x1)`` (all columns by default)."""
+        x0, x1 = (0, self.ni) if cols is None else (int(cols[0]), int(cols[1]))
+        return (np.ascontiguousarray(self.data[:, z0:z1, x0:x1], dtype=np.uint16),
+                np.ascontiguousarray(self.dark[:, z0:z1, x0:x1], dtype=np.uint16),
+                np.ascontiguousarray(self.flat[:, z0:z1, x0:x1], dtype=np.uint16))
 
 
 def write_tiff(path, img):
