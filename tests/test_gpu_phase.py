@@ -178,3 +178,38 @@ def test_cli_recon_steps_try_paganin(tmp_path):
         err = np.sqrt(((got - ref[k]) ** 2).sum() / (ref[k] ** 2).sum())
         print(name, err)
         assert err <= RTOL32
+
+
+def test_cli_recon_steps_paganin_float16(tmp_path):
+    # the reference's test_steps_full.py:13 case: Paganin by steps in float16
+    # (width 80 cropped to 64 columns, centre moved by -8)
+    import os
+    from tomocupy_amd import io
+    from tomocupy_amd.cli import main
+    ni, nproj, nz = 80, 48, 4
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(10, ni, theta, nz, ndark=3, nflat=4)
+    p = tmp_path / 'scan.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    out = str(tmp_path / 'rec')
+    main(['recon_steps', '--file-name', str(p), '--reconstruction-type', 'full',
+          '--nsino-per-chunk', '4', '--out-path-name', out, '--rotation-axis', '40',
+          '--dtype', 'float16', '--retrieve-phase-method', 'paganin',
+          '--retrieve-phase-alpha', '0.0001', '--propagation-distance', '60', '--energy', '20',
+          '--pixel-size', '1.17'])
+    n, st, centre = 64, 8, 40 - 8
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    sl = slice(st, st + n)
+    res = ph.retrieve_phase(pp.darkflat_correction(data[:, :, sl], dark[:, :, sl], flat[:, :, sl]),
+                            'paganin', pixel_size=1.17e-4, dist=6.0, energy=20.0, alpha=1e-4)
+    sino = np.ascontiguousarray(pp.minus_log(res).swapaxes(0, 1))
+    ne = 4 * n
+    ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne, centre),
+                            th)
+    for k in range(nz):
+        got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k))
+        assert got.dtype == np.float16 and got.shape == (n, n)
+        err = np.sqrt(((got.astype(np.float64) - ref[k]) ** 2).sum() / (ref[k] ** 2).sum())
+        print('slice', k, err)
+        assert err <= 2e-3
