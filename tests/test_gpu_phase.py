@@ -213,3 +213,35 @@ def test_cli_recon_steps_paganin_float16(tmp_path):
         err = np.sqrt(((got.astype(np.float64) - ref[k]) ** 2).sum() / (ref[k] ** 2).sum())
         print('slice', k, err)
         assert err <= 2e-3
+
+
+def test_cli_recon_steps_binning_paganin(tmp_path):
+    # binning 1 then Paganin by steps: 8 raw rows x 128 raw columns -> 4 x 64
+    import os
+    from tomocupy_amd import io
+    from tomocupy_amd.cli import main
+    ni_raw, nproj, nz_raw = 128, 40, 8
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(11, ni_raw, theta, nz_raw, ndark=3, nflat=4)
+    p = tmp_path / 'scan.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    out = str(tmp_path / 'rec')
+    main(['recon_steps', '--file-name', str(p), '--reconstruction-type', 'full',
+          '--nsino-per-chunk', '4', '--out-path-name', out, '--binning', '1',
+          '--retrieve-phase-method', 'paganin', '--retrieve-phase-alpha', '0.0001',
+          '--propagation-distance', '60', '--energy', '20', '--pixel-size', '1.17'])
+    n, nz = 64, 4
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    b = [pp.downsample(a, 1) for a in (data, dark, flat)]
+    res = ph.retrieve_phase(pp.darkflat_correction(*b), 'paganin', pixel_size=1.17e-4, dist=6.0,
+                            energy=20.0, alpha=1e-4)
+    sino = np.ascontiguousarray(pp.minus_log(res).swapaxes(0, 1))
+    ne = 4 * n
+    ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne,
+                                                 ni_raw / 2 / 2), th)
+    for k in range(nz):
+        got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k)).astype(np.float64)
+        err = np.sqrt(((got - ref[k]) ** 2).sum() / (ref[k] ** 2).sum())
+        print('slice', k, err)
+        assert err <= RTOL32
