@@ -160,3 +160,35 @@ def test_cli_full_float16_crops_to_power_of_two(tmp_path):
         err = _rel(got.astype(np.float64), ref[k])
         print('slice', k, err)
         assert err <= 2e-3
+
+
+def test_cli_full_projection_and_column_selection(tmp_path):
+    # reader.py:124-181: --start-proj/--end-proj, --blocked-views (angles of
+    # [0.5, 1.2] modulo pi dropped), --start-column/--end-column (width 70 ->
+    # 70 - 70 % 2 = 70, centre moved by the start column)
+    n0, nproj, nz = 100, 90, 2
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(4, n0, theta, nz, ndark=3, nflat=4)
+    p = tmp_path / 'scan.npz'
+    np.savez(p, data=data, data_white=flat, data_dark=dark,
+             theta=theta.astype(np.float64) * 180 / np.pi)
+    out = str(tmp_path / 'rec')
+    main(['recon', '--file-name', str(p), '--reconstruction-type', 'full', '--nsino-per-chunk',
+          '2', '--out-path-name', out, '--start-proj', '5', '--end-proj', '80',
+          '--blocked-views', '[0.5,1.2]', '--start-column', '20', '--end-column', '90',
+          '--rotation-axis', '55'])
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)[5:80]
+    keep = ((th % np.pi) < 0.5) | (((th - 0.5) % np.pi) > 0.7)
+    ids = np.arange(5, 80)[keep]
+    assert 0 < len(ids) < 75
+    n = 70
+    sino = pp.proc_chain(data[ids][:, :, 20:90], dark[:, :, 20:90], flat[:, :, 20:90])
+    ne = 4 * n
+    ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne, 35.0),
+                            th[keep])
+    for k in range(nz):
+        got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k)).astype(np.float64)
+        assert got.shape == (n, n)
+        err = _rel(got, ref[k])
+        print('slice', k, err)
+        assert err <= 1e-5

@@ -72,6 +72,11 @@ def _recon_args(r):
     r.add_argument('--nsino-per-chunk', type=int, default=8)
     r.add_argument('--start-row', type=int, default=0)
     r.add_argument('--end-row', type=int, default=-1)
+    r.add_argument('--start-column', type=int, default=0)
+    r.add_argument('--end-column', type=int, default=-1)
+    r.add_argument('--start-proj', type=int, default=0)
+    r.add_argument('--end-proj', type=int, default=-1)
+    r.add_argument('--blocked-views', default='none')
     r.add_argument('--fbp-filter', default='parzen',
                    choices=['none', 'ramp', 'shepp', 'hann', 'hamming', 'parzen', 'cosine',
                             'cosine2'])
@@ -124,15 +129,19 @@ def _common(a, ds):
         raise SystemExit('invalid --start-row/--end-row for %d rows' % ds.nz)
     b = a.binning
     centre = ds.ni / 2 if a.rotation_axis == -1 else a.rotation_axis
-    x1 = ds.ni - ds.ni % (2 ** (b + 1))
-    if x1 != ds.ni:
+    x0 = a.start_column
+    x1 = ds.ni if a.end_column == -1 else a.end_column
+    if not (0 <= x0 < x1 <= ds.ni):
+        raise SystemExit('invalid --start-column/--end-column for %d columns' % ds.ni)
+    nsh = (x1 - x0) % (2 ** (b + 1))
+    if nsh:
+        x1 -= nsh
         print('decreasing projection width by %d pixel to operate with multiple of 2 sizes; '
-              'new width %d' % (ds.ni - x1, x1))
+              'new width %d' % (nsh, x1 - x0))
     if (z1 - z0) >> b < 1:
         raise SystemExit('fewer than 2^%d rows to bin' % b)
     stripe = a.remove_stripe_method if a.remove_stripe_method in ('fw', 'ti', 'vo-all') else 'none'
-    ni, centre = x1 >> b, centre / 2 ** b
-    x0 = 0
+    ni, centre = (x1 - x0) >> b, (centre - x0) / 2 ** b
     if a.dtype == 'float16':
         p2 = 2 ** int(np.log2(ni))
         st = (ni - p2) // 2
@@ -140,19 +149,39 @@ def _common(a, ds):
         if p2 != ni:
             print('crop data to the power of 2 sizes to work with 16bit precision, output size '
                   'in x dimension %d' % p2)
-        x0, x1, ni = st << b, (st + p2) << b, p2
+        x0, x1, ni = x0 + (st << b), x0 + ((st + p2) << b), p2
     return z0, z1, (x0, x1), ni, centre, stripe
+
+
+def _projections(a, ds):
+    """Projection selection (reader.py:167-181): [start-proj, end-proj), then
+    the angles outside every blocked range [st, end] (modulo pi).  Returns
+    (index array into the data, angles)."""
+    p1 = ds.nproj if a.end_proj == -1 else a.end_proj
+    if not (0 <= a.start_proj < p1 <= ds.nproj):
+        raise SystemExit('invalid --start-proj/--end-proj for %d projections' % ds.nproj)
+    theta = ds.theta[a.start_proj:p1]
+    ids = np.arange(len(theta))
+    if a.blocked_views != 'none':
+        tmp = ast.literal_eval(a.blocked_views)
+        if not isinstance(tmp[0], (list, tuple)):
+            tmp = [tmp]
+        for st, end in tmp:
+            keep = np.where(((theta % np.pi) < st) | (((theta - st) % np.pi) > end - st))[0]
+            ids = np.intersect1d(ids, keep)
+    return np.arange(a.start_proj, p1)[ids], np.ascontiguousarray(theta[ids])
 
 
 def recon_full(a, ds):
     from .io import write_tiff
     from .rec import GPURec
     z0, z1, cols, ni, centre, stripe = _common(a, ds)
+    pids, theta = _projections(a, ds)
     b = a.binning
     nz = (z1 - z0) >> b
     out = _out_dir(a, 'full')
     os.makedirs(out, exist_ok=True)
-    rec = GPURec(ni, ds.nproj, nz, a.nsino_per_chunk, ds.theta, centre,
+    rec = GPURec(ni, len(pids), nz, a.nsino_per_chunk, theta, centre,
                  ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype, fbp_filter=a.fbp_filter,
                  remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
                  fw_level=a.fw_level, minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
@@ -168,7 +197,7 @@ def recon_full(a, ds):
 
     def read(k):
         s0 = z0 + ((k * ncz) << b)
-        return ds.rows(s0, s0 + (rec.lzchunk[k] << b), cols)
+        return ds.rows(s0, s0 + (rec.lzchunk[k] << b), cols, pids)
 
     def write(k, r):
         for kk in range(len(r)):
@@ -210,14 +239,17 @@ def recon_try(a, ds, steps=False):
     # shifts in binned pixels, centres saved in raw pixels (reader.py:229-232)
     shifts = (np.arange(-a.center_search_width, a.center_search_width,
                         a.center_search_step * 2 ** b).astype(np.float32) / 2 ** b)
-    # saved centres (reader.py:231-232): (uncropped centre - shift) 2^b + st,
-    # st the float16 crop start (0 otherwise), as the reference names them
-    st = cols[0] >> b
-    centres = (centre + st - shifts) * 2 ** b + st
+    # saved centres (reader.py:231-232): (centre before the float16 crop -
+    # shift) 2^b + st_n, st_n the start column, or for float16 the crop start
+    # (binned; the reference replaces st_n with it, reader.py:157)
+    st16 = (cols[0] - a.start_column) >> b if a.dtype == 'float16' else 0
+    st_n = st16 if a.dtype == 'float16' else a.start_column
+    centres = (centre + st16 - shifts) * 2 ** b + st_n
     ncz = max(2, a.nsino_per_chunk)
     td = _lib.torch_dtype(a.dtype)
-    theta = torch.from_numpy(ds.theta).cuda()
-    pf = ProcFunctions(ds.nproj, 1, ni, ds.dark.shape[0], ds.flat.shape[0], a.dtype,
+    pids, th = _projections(a, ds)
+    theta = torch.from_numpy(th).cuda()
+    pf = ProcFunctions(len(pids), 1, ni, ds.dark.shape[0], ds.flat.shape[0], a.dtype,
                        minus_log=_bool(a.minus_log), bright_ratio=a.bright_ratio,
                        flat_linear=_bool(a.flat_linear), remove_stripe_method=stripe,
                        fw_sigma=a.fw_sigma, fw_filter=a.fw_filter, fw_level=a.fw_level,
@@ -231,18 +263,18 @@ def recon_try(a, ds, steps=False):
         nrec = 2 * ni
         if centre < ni // 2:
             rcentre, sgn = ni - centre, -1.0
-    bf = BackprojFunctions(nrec, ds.nproj, ncz, theta, rcentre, a.dtype, a.fbp_filter)
+    bf = BackprojFunctions(nrec, len(pids), ncz, theta, rcentre, a.dtype, a.fbp_filter)
     written = []
     if steps:
-        rs = _steps(a, ds, ni, nz, centre, stripe)
+        rs = _steps(a, ds, ni, nz, centre, stripe, pids, th)
         vol = rs.proc_volume(*(torch.from_numpy(x).cuda()
-                               for x in ds.rows(z0, z0 + (nz << b), cols)))
+                               for x in ds.rows(z0, z0 + (nz << b), cols, pids)))
     for idz in ids:
         if steps:
             sino = rs.sino_row(vol, idz >> b)
         else:
             data, dark, flat = (torch.from_numpy(x).cuda()
-                                for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, cols))
+                                for x in ds.rows(z0 + idz, z0 + idz + 2 ** b, cols, pids))
             sino = pf.proc_chunk(data, dark, flat)  # (1, nproj, nrec)
         batch = sino.expand(ncz, -1, -1).contiguous()
         obj = torch.empty((ncz, nrec, nrec), dtype=td, device='cuda')
@@ -267,11 +299,11 @@ def recon_try(a, ds, steps=False):
     return written
 
 
-def _steps(a, ds, ni, nz, centre, stripe):
+def _steps(a, ds, ni, nz, centre, stripe, pids, theta):
     import torch
     from .rec_steps import GPURecSteps
     b = a.binning
-    return GPURecSteps(ni, ds.nproj, nz, a.nsino_per_chunk, torch.from_numpy(ds.theta).cuda(),
+    return GPURecSteps(ni, len(pids), nz, a.nsino_per_chunk, torch.from_numpy(theta).cuda(),
                      centre, ds.dark.shape[0], ds.flat.shape[0], dtype=a.dtype,
                      fbp_filter=a.fbp_filter, nproj_per_chunk=a.nproj_per_chunk,
                      retrieve_phase_method=a.retrieve_phase_method, energy=a.energy,
@@ -295,9 +327,10 @@ def recon_steps_full(a, ds):
     nz = (z1 - z0) >> b
     out = _out_dir(a, 'full')
     os.makedirs(out, exist_ok=True)
-    rs = _steps(a, ds, ni, nz, centre, stripe)
+    pids, theta = _projections(a, ds)
+    rs = _steps(a, ds, ni, nz, centre, stripe, pids, theta)
     zb, ncz = z0 >> b, rs.ncz
-    data, dark, flat = ds.rows(z0, z0 + (nz << b), cols)
+    data, dark, flat = ds.rows(z0, z0 + (nz << b), cols, pids)
     t = time.perf_counter()
 
     def write(k, r):

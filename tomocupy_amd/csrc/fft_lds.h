@@ -1,4 +1,4 @@
-// LDS-resident Stockham autosort FFT for sizes N = 2^a 3^b (N <= 16384 per
+// LDS-resident Stockham autosort FFT for sizes N = 2^a 3^b 5^c 7^d (N <= 16384 per
 // transform), executed cooperatively by a workgroup on 2^lgH interleaved
 // transforms held in LDS.
 //
@@ -14,6 +14,7 @@
 // both directions.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "common.h"
@@ -34,7 +35,7 @@ struct FftDesc {
 struct FftPlan {
   FftDesc d{};
   DevBuf tw;
-  int init(int N);  // N = 2^a 3^b
+  int init(int N);  // N = 2^a 3^b 5^c 7^d
 };
 
 template <int DIR>
@@ -58,6 +59,54 @@ __device__ __forceinline__ void butterfly(float2 *a) {
     a[0] = cadd(a[0], t1);
     a[1] = cadd(m, r);
     a[2] = csub(m, r);
+  } else if constexpr (R == 5) {
+    // y_k = sum_j a_j W^(jk), W = exp(-+2 pi i/5): pairs a_j +- a_(5-j) with
+    // c_m = cos(2 pi m/5), s_m = sin(2 pi m/5)
+    const float c1 = 0.30901699437494745f, c2 = -0.80901699437494734f;
+    const float s1 = 0.95105651629515353f, s2 = 0.58778525229247320f;
+    const float2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+    const float2 t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
+    const float2 b1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+    const float2 b2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+    const float2 d1 = mul_i<DIR>(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+    const float2 d2 = mul_i<DIR>(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(b1, d1);
+    a[4] = csub(b1, d1);
+    a[2] = cadd(b2, d2);
+    a[3] = csub(b2, d2);
+  } else if constexpr (R == 7) {
+    // pairs a_j +- a_(7-j), j = 1..3; y_m = a0 + sum_j cos(2 pi jm/7) t_j
+    // -+ i sum_j sin(2 pi jm/7) u_j, y_(7-m) its conjugate combination
+    const float C[4] = {1.f, 0.62348980185873353f, -0.22252093395631440f,
+                        -0.90096886790241913f};
+    const float S[4] = {0.f, 0.78183148246802981f, 0.97492791218182361f,
+                        0.43388373911755812f};
+    float2 t[4], u[4];
+#pragma unroll
+    for (int j = 1; j <= 3; ++j) {
+      t[j] = cadd(a[j], a[7 - j]);
+      u[j] = csub(a[j], a[7 - j]);
+    }
+    float2 y[7];
+    y[0] = cadd(a[0], cadd(t[1], cadd(t[2], t[3])));
+#pragma unroll
+    for (int m = 1; m <= 3; ++m) {
+      float2 b = a[0], d = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 1; j <= 3; ++j) {
+        const int q = (j * m) % 7;  // cos(2 pi q/7) = C[min(q, 7-q)], sin = +-S[min(q, 7-q)]
+        const float c = C[q <= 3 ? q : 7 - q];
+        const float sg = q <= 3 ? S[q] : -S[7 - q];
+        b = make_float2(b.x + c * t[j].x, b.y + c * t[j].y);
+        d = make_float2(d.x + sg * u[j].x, d.y + sg * u[j].y);
+      }
+      const float2 di = mul_i<DIR>(d);
+      y[m] = cadd(b, di);
+      y[7 - m] = csub(b, di);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) a[k] = y[k];
   } else if constexpr (R == 4) {
     float2 t0 = cadd(a[0], a[2]);
     float2 t1 = csub(a[0], a[2]);
@@ -147,6 +196,20 @@ __device__ __forceinline__ void twiddle(float2 *a, const float2 *__restrict__ tw
   } else if constexpr (R == 3) {
     a[1] = cmul(a[1], ld(1));
     a[2] = cmul(a[2], ld(2));
+  } else if constexpr (R == 5) {
+    const float2 w1 = ld(1), w2 = ld(2), w4 = ld(4);
+    a[1] = cmul(a[1], w1);
+    a[2] = cmul(a[2], w2);
+    a[3] = cmul(a[3], cmul(w2, w1));
+    a[4] = cmul(a[4], w4);
+  } else if constexpr (R == 7) {
+    const float2 w1 = ld(1), w2 = ld(2), w4 = ld(4);
+    a[1] = cmul(a[1], w1);
+    a[2] = cmul(a[2], w2);
+    a[3] = cmul(a[3], cmul(w2, w1));
+    a[4] = cmul(a[4], w4);
+    a[5] = cmul(a[5], cmul(w4, w1));
+    a[6] = cmul(a[6], cmul(w4, w2));
   } else if constexpr (R == 4) {
     const float2 w1 = ld(1), w2 = ld(2);
     a[1] = cmul(a[1], w1);
@@ -232,7 +295,9 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
 // 2^lgH * N <= 15 * nt (16 * nt without radix-3 passes); see fft_threads().
 // The caller must __syncthreads() after filling buf; the function ends with a
 // barrier.
-template <int DIR, int EPT = kFftEPT>
+// R57: also radix-5 and radix-7 passes (kernels for such sizes only: the
+// extra passes raise the register demand of every instantiation).
+template <int DIR, int EPT = kFftEPT, bool R57 = false>
 __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt) {
   int Ns = 1;
   for (int s = 0; s < d.nr; ++s) {
@@ -246,8 +311,14 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
       fft_pass<4, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 2)
       fft_pass<2, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-    else
+    else if (R == 3)
       fft_pass<3, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    else if constexpr (R57) {
+      if (R == 5)
+        fft_pass<5, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else
+        fft_pass<7, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    }
     Ns *= R;
   }
 }
@@ -334,6 +405,7 @@ inline void with_static_size(int N, F &&f) {
     case 2048: f(std::integral_constant<int, 2048>()); break;
     case 4096: f(std::integral_constant<int, 4096>()); break;
     case 8192: f(std::integral_constant<int, 8192>()); break;
+    case -1: f(std::integral_constant<int, -1>()); break;  // run-time sizes with 5s or 7s
     default: f(std::integral_constant<int, 0>()); break;
   }
 }
@@ -369,24 +441,27 @@ __device__ __forceinline__ void stage_lds(int count, int tid, int nt, Ld ld, St 
 }
 
 // Workgroup size for transforming `elems` complex values (all interleaved
-// sequences) with plan d: 16 values per thread for radices 2/4/8, 15 when a
-// radix-3 pass is present; at least 256 and a multiple of 64.  Returns -1 if
+// sequences) with plan d: 16 values per thread for radices 2/4/8/16, fewer
+// with a radix-3, 5 or 7 pass (15, 15, 14: whole butterflies); at least 256
+// and a multiple of 64.  Returns -1 if
 // more than 1024 threads would be needed.
 inline int fft_threads(const FftDesc &d, int64_t elems, int ept = kFftEPT) {
-  bool r3 = false;
-  for (int i = 0; i < d.nr; ++i) r3 |= (d.r[i] == 3);
-  const int64_t per = r3 ? (ept / 3) * 3 : ept;
+  int64_t per = ept;  // values a thread covers in every pass: whole butterflies
+  for (int i = 0; i < d.nr; ++i) per = std::min<int64_t>(per, (ept / d.r[i]) * d.r[i]);
   int64_t t = (elems + per - 1) / per;
   t = ((t + 63) / 64) * 64;
   if (t > 1024) return -1;
   return (int)(t < 256 ? 256 : t);
 }
 
-// True if N = 2^a 3^b with N >= 1.
+// True if N has a factor 5 or 7 (its plans need the R57 transforms).
+inline bool fft_has57(int64_t N) { return N > 0 && (N % 5 == 0 || N % 7 == 0); }
+
+// True if N = 2^a 3^b 5^c 7^d with N >= 1.
 inline bool fft_size_ok(int64_t N) {
   if (N < 1) return false;
-  while (N % 2 == 0) N /= 2;
-  while (N % 3 == 0) N /= 3;
+  for (int p : {2, 3, 5, 7})
+    while (N % p == 0) N /= p;
   return N == 1;
 }
 

@@ -58,10 +58,16 @@ class Dataset:
         else:
             self.theta = np.linspace(0, np.pi, self.nproj).astype(np.float32)
 
-    def rows(self, z0, z1, cols=None):
-        """``(data, dark, flat)`` uint16 for detector rows [z0, z1) and
-        columns ``cols = (x0, x1)`` (all columns by default)."""
+    def rows(self, z0, z1, cols=None, proj=None):
+        """``(data, dark, flat)`` uint16 for detector rows [z0, z1), columns
+        ``cols = (x0, x1)`` (all by default) and projections ``proj`` (an index
+        array or a slice; all by default, dark and flat frames always all)."""
         x0, x1 = (0, self.ni) if cols is None else (int(cols[0]), int(cols[1]))
+        if proj is not None:
+            d = np.asarray(self.data[:, z0:z1, x0:x1])[proj]
+            return (np.ascontiguousarray(d, dtype=np.uint16),
+                    np.ascontiguousarray(self.dark[:, z0:z1, x0:x1], dtype=np.uint16),
+                    np.ascontiguousarray(self.flat[:, z0:z1, x0:x1], dtype=np.uint16))
         return (np.ascontiguousarray(self.data[:, z0:z1, x0:x1], dtype=np.uint16),
                 np.ascontiguousarray(self.dark[:, z0:z1, x0:x1], dtype=np.uint16),
                 np.ascontiguousarray(self.flat[:, z0:z1, x0:x1], dtype=np.uint16))
