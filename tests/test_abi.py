@@ -35,7 +35,7 @@ def test_version():
 @pytest.mark.parametrize("n,nproj,nz,dtype,msg", [
     (21, 10, 2, 0, 'even'), (18, 10, 2, 0, '>= 20'), (64, 0, 2, 0, 'nproj'),
     (64, 10, 0, 0, 'nz'), (64, 10, 2, 7, 'dtype'), (8192, 10, 2, 0, 'maximum'),
-    (70, 10, 2, 0, '2^a 3^b')])
+    (88, 10, 2, 0, '2^a 3^b 5^c 7^d')])  # 88 = 8 x 11
 def test_fr_create_validation(n, nproj, nz, dtype, msg):
     h = ctypes.c_void_p()
     rc = _lib.lib().fr_create(ctypes.byref(h), n, nproj, nz, ctypes.c_void_p(16), dtype, None)
