@@ -63,7 +63,7 @@ def test_flt_create_validation():
     h = ctypes.c_void_p()
     L = _lib.lib()
     assert L.flt_create(ctypes.byref(h), 64, 60, 10, 2, 0) == _lib.TOMO_EINVAL  # ne < n
-    assert L.flt_create(ctypes.byref(h), 64, 256 * 7, 10, 2, 0) == _lib.TOMO_EINVAL  # 7 | ne
+    assert L.flt_create(ctypes.byref(h), 64, 256 * 11, 10, 2, 0) == _lib.TOMO_EINVAL  # 11 | ne
     assert L.flt_create(ctypes.byref(h), 64, 256, 10, 2, 9) == _lib.TOMO_EINVAL
 
 
