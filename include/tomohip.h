@@ -342,6 +342,19 @@ int pre_remove_stripe_ti(tomo_pre_plan *plan, float *res, double beta, double ma
 int pre_remove_all_stripe(tomo_pre_plan *plan, float *res, double snr, int la_size,
                           int sm_size, int dim, void *stream);
 
+/*
+ * ProcFunctions.rotate_proj (src/tomocupy/processing/proc_functions.py:126-138),
+ * applied by proc_proj (:182-184): ndimage.rotate(data, angle, reshape=False,
+ * order, axes=(2, 1), mode='nearest') on projections src (nproj, ny, nx)
+ * float32 into dst (same shape, not in place): dst[p,y,x] = src[p] sampled
+ * at R (y, x) + centre - R centre, R = [[cos, sin], [-sin, cos]] of the angle
+ * in degrees, centre ((ny-1)/2, (nx-1)/2), neighbours clamped.  order 0
+ * (nearest) or 1 (bilinear, the reference CLI's default); the spline orders
+ * 2-5 return TOMO_EUNSUPPORTED.
+ */
+int pre_rotate_proj(const float *src, float *dst, int64_t nproj, int64_t ny, int64_t nx,
+                    double angle, int order, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Phase retrieval (ProcFunctions.proc_proj, proc_functions.py:169-181)
  * ------------------------------------------------------------------------- */

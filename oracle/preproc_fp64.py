@@ -248,6 +248,15 @@ def remove_stripe_fw(data, sigma, wname, level):
     return sli[:, st:st + nproj, :ni].swapaxes(0, 1)
 
 
+def rotate_proj(data, angle, order=1):
+    """``ProcFunctions.rotate_proj`` (P:126-138): ``ndimage.rotate(data, angle,
+    reshape=False, order=order, axes=(2, 1), mode='nearest')`` (SciPy, the CPU
+    twin of the ``cupyx`` routine), FP64."""
+    from scipy import ndimage
+    return ndimage.rotate(np.asarray(data, dtype=np.float64), float(angle), reshape=False,
+                          order=order, axes=(2, 1), mode='nearest')
+
+
 def detect_stripe(listdata, snr):
     """``_detect_stripe_batch`` (R:352-375), Algorithm 4 of Vo et al. 2018, per
     row of ``listdata (nz, ni)``: sort descending, least-squares line through

@@ -242,3 +242,34 @@ def test_pad360_flip_rule():
     assert flip and center == ni - 5.0
     w = int(2 * (ni - center))
     assert np.array_equal(out[0, 0, :ni - w], x[0, 0, ::-1][:ni - w])
+
+
+def test_rotate_proj_conventions():
+    # ndimage.rotate(..., axes=(2, 1), reshape=False): identity at 0, a
+    # quarter turn of a square projection is np.rot90 (k = 1 in the (y, x)
+    # plane), and at any angle the bilinear value at (y, x) is the one at
+    # R (y, x) + c - R c with R = [[cos, sin], [-sin, cos]], c the centre,
+    # neighbours clamped (brute force on a small projection)
+    rng = np.random.default_rng(12)
+    a = rng.uniform(0, 1, (2, 5, 7))
+    assert np.allclose(pp.rotate_proj(a, 0.0), a, atol=1e-15)
+    sq = rng.uniform(0, 1, (1, 6, 6))
+    assert np.allclose(pp.rotate_proj(sq, 90.0), np.rot90(sq, 1, axes=(1, 2)), atol=1e-12)
+    ang = 17.0
+    c, s = np.cos(np.deg2rad(ang)), np.sin(np.deg2rad(ang))
+    ny, nx = a.shape[1:]
+    cy, cx = (ny - 1) / 2, (nx - 1) / 2
+    got = pp.rotate_proj(a, ang, 1)
+    for p in range(2):
+        for y in range(ny):
+            for x in range(nx):
+                iy = c * y + s * x + cy - (c * cy + s * cx)
+                ix = -s * y + c * x + cx - (-s * cy + c * cx)
+                y0, x0 = int(np.floor(iy)), int(np.floor(ix))
+                wy, wx = iy - y0, ix - x0
+
+                def at(yy, xx):
+                    return a[p, min(max(yy, 0), ny - 1), min(max(xx, 0), nx - 1)]
+                v = ((1 - wy) * ((1 - wx) * at(y0, x0) + wx * at(y0, x0 + 1)) +
+                     wy * ((1 - wx) * at(y0 + 1, x0) + wx * at(y0 + 1, x0 + 1)))
+                assert abs(got[p, y, x] - v) < 1e-12, (p, y, x)

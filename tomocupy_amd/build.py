@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, 'libtomohip.so')
 BUILD = os.path.join(HERE, '_build')
 SOURCES = ['common.hip', 'fourierrec.hip', 'filter.hip', 'preproc.hip', 'stripe.hip',
            'outliers.hip', 'stripe_ti.hip', 'pad360.hip', 'binning.hip',
-           'stripe_vo.hip', 'phase.hip']
+           'stripe_vo.hip', 'phase.hip', 'rotate.hip']
 ARCH = os.environ.get('TOMOHIP_ARCH', 'gfx950')
 
 FLAGS = [

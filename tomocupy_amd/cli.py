@@ -56,6 +56,8 @@ def parser():
     s.add_argument('--retrieve-phase-delta-beta', type=float, default=1500.0)
     s.add_argument('--retrieve-phase-W', type=float, default=2e-4)
     s.add_argument('--nproj-per-chunk', type=int, default=8)
+    s.add_argument('--rotate-proj-angle', type=float, default=0.0)
+    s.add_argument('--rotate-proj-order', type=int, default=1)
     return ap
 
 
@@ -311,6 +313,7 @@ def _steps(a, ds, ni, nz, centre, stripe, pids, theta):
                      retrieve_phase_alpha=a.retrieve_phase_alpha,
                      retrieve_phase_delta_beta=a.retrieve_phase_delta_beta,
                      retrieve_phase_W=a.retrieve_phase_W, file_type=a.file_type,
+                     rotate_proj_angle=a.rotate_proj_angle, rotate_proj_order=a.rotate_proj_order,
                      remove_stripe_method=stripe, fw_sigma=a.fw_sigma, fw_filter=a.fw_filter,
                      fw_level=a.fw_level, minus_log=_bool(a.minus_log),
                      bright_ratio=a.bright_ratio, flat_linear=_bool(a.flat_linear),

@@ -161,6 +161,30 @@ class ProcFunctions:
             _lib.stream_ptr(None)), 'pre_remove_stripe_ti')
         return res
 
+    @staticmethod
+    def rotate_proj(data, angle, order=1, out=None):
+        """``rotate_proj`` (``proc_functions.py:126-138``): every projection of
+        the float32 ``data (nproj, ny, nx)`` rotated by ``angle`` degrees
+        (``ndimage.rotate(..., reshape=False, axes=(2, 1), mode='nearest')``),
+        interpolation order 0 or 1; in place (through a temporary) unless
+        ``out`` is given."""
+        import torch
+        if abs(float(angle)) == 0.0:
+            if out is not None:
+                out.copy_(data)
+                return out
+            return data
+        dst = torch.empty_like(data) if out is None else out
+        shape = tuple(data.shape)
+        _lib.check(_lib.lib().pre_rotate_proj(
+            _lib.dev_ptr(data, shape, torch.float32, 'data'),
+            _lib.dev_ptr(dst, shape, torch.float32, 'out'), shape[0], shape[1], shape[2],
+            float(angle), int(order), _lib.stream_ptr(None)), 'pre_rotate_proj')
+        if out is None:
+            data.copy_(dst)
+            return data
+        return dst
+
     def remove_all_stripe(self, res):
         """Vo's combined ring removal (``remove_stripe.py:311-497``, the
         ``vo-all`` method) in place on the float32 chunk: dead, large and

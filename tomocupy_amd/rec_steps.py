@@ -7,8 +7,8 @@ reference's three passes over a whole scan:
 
 1. ``proc_sino`` by chunks of rows (outliers, dark/flat, stripe removal),
    ``rec_steps.py:127-192``;
-2. ``proc_proj`` by chunks of projections (phase retrieval, then minus log),
-   ``rec_steps.py:194-243``;
+2. ``proc_proj`` by chunks of projections (phase retrieval, projection
+   rotation, then minus log), ``rec_steps.py:194-243``;
 3. filtered backprojection by chunks of rows
    (``backproj_parallel.py:315-358``).
 
@@ -36,7 +36,7 @@ class GPURecSteps:
                  fbp_filter='parzen', nproj_per_chunk=8, retrieve_phase_method='none',
                  energy=20.0, propagation_distance=0.0, pixel_size=1.0, retrieve_phase_alpha=0.0,
                  retrieve_phase_delta_beta=1500.0, retrieve_phase_W=2e-4, file_type='standard',
-                 phase_batch=1, **proc_kwargs):
+                 phase_batch=1, rotate_proj_angle=0.0, rotate_proj_order=1, **proc_kwargs):
         """``n``, ``nz``, ``center``: binned detector width, rows and axis
         (as ``GPURec``).  Phase retrieval settings in the reference CLI's
         units: energy [keV], propagation distance [mm], pixel size [um], W
@@ -69,6 +69,10 @@ class GPURecSteps:
                                    pixel_size * 1e-4, propagation_distance / 10, energy,
                                    retrieve_phase_alpha, retrieve_phase_delta_beta,
                                    retrieve_phase_W * 1e-4, batch=int(phase_batch))
+        self.rotate_angle, self.rotate_order = float(rotate_proj_angle), int(rotate_proj_order)
+        if self.rotate_angle != 0.0 and self.rotate_order not in (0, 1):
+            raise NotImplementedError('rotate_proj_order %d: orders 0 and 1 are implemented'
+                                      % self.rotate_order)
         self.stream = torch.cuda.Stream()
         self.tdtype = _lib.torch_dtype(dtype)
 
@@ -144,10 +148,20 @@ class GPURecSteps:
             self.pf.proc_sino(cd, ck, cf, res)
             vol[:, z0:z0 + lz].copy_(res[:, :lz])
         del cd, ck, cf, res
-        # 2. proc_proj by projection chunks: phase retrieval in place
-        if self.phase is not None:
+        # 2. proc_proj by projection chunks: phase retrieval, then the
+        # rotation of the projections, in place (proc_functions.py:169-184)
+        rot = None
+        if self.rotate_angle != 0.0:
+            rot = torch.empty((self.ncproj, self.nz, self.ni), dtype=torch.float32, device=dev)
+        if self.phase is not None or rot is not None:
             for p0 in range(0, self.nproj, self.ncproj):
-                self.phase.apply(vol[p0:p0 + self.ncproj])
+                blk = vol[p0:p0 + self.ncproj]
+                if self.phase is not None:
+                    self.phase.apply(blk)
+                if rot is not None:
+                    ProcFunctions.rotate_proj(blk, self.rotate_angle, self.rotate_order,
+                                              out=rot[:blk.shape[0]])
+                    blk.copy_(rot[:blk.shape[0]])
         return vol
 
     def free(self):
