@@ -10,7 +10,8 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
 * ``recon_steps --reconstruction-type full|try``: the whole scan resident on the
   GPU through :class:`tomocupy_amd.rec_steps.GPURecSteps` (rec_steps.py:114-125)
   with phase retrieval (``--retrieve-phase-method paganin|Gpaganin|farago``,
-  ``--energy`` keV, ``--propagation-distance`` mm, ``--pixel-size`` um);
+  ``--energy`` keV, ``--propagation-distance`` mm, ``--pixel-size`` um) and
+  projection rotation (``--rotate-proj-angle``, ``--rotate-proj-order`` 0 or 1);
 * ``--reconstruction-type try``: the slices at ``--nsino`` (fractions of the
   height, ``reader.py:192-197``) reconstructed at the centres
   ``centre - arange(-w, w, step)`` (``reader.py:224-246``), each a per-slice
@@ -21,9 +22,12 @@ Stripe removal: ``fw``, ``ti`` and ``vo-all``; zinger removal: ``--dezinger``; 3
 scans with the axis near the border: ``--file-type double_fov``; binning
 ``--binning b`` (2^b x 2^b sums of counts on the GPU, ``reader.py:96-140``).
 As the reference, the width is trimmed to a multiple of ``2^(b+1)``
-(``reader.py:127-133``).  Settings outside this path (phase retrieval,
-other algorithms, ...) are not offered.  Input formats: see
-:mod:`tomocupy_amd.io`.
+(``reader.py:127-133``); ``--start-column/--end-column``,
+``--start-proj/--end-proj`` and ``--blocked-views`` select the data
+(``reader.py:124-181``); float16 crops the width to a power of two
+(``reader.py:155-165``).  Settings outside this path (other algorithms,
+laminography, automatic centre search, HDF5 output) are not offered.  Input
+formats: see :mod:`tomocupy_amd.io`.
 """
 import argparse
 import ast
