@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("angle,order", [(1.5, 1), (-7.25, 1), (90.0, 1), (200.0, 1),
-                                         (1.5, 0), (33.0, 0)])
+                                         (1.5, 0), (33.0, 0), (1.5, 3), (-7.25, 3), (90.0, 3),
+                                         (200.0, 2), (4.0, 2)])
 def test_rotate_proj_matches_oracle(angle, order):
     from tomocupy_amd import ProcFunctions
     rng = np.random.default_rng(int(abs(angle) * 4) + order)
@@ -41,14 +42,31 @@ def test_rotate_proj_matches_oracle(angle, order):
         assert err <= 1e-6
 
 
+@pytest.mark.parametrize("order", [2, 3])
+def test_rotate_proj_spline_large(order):
+    # padded planes wider and taller than one 256-thread workgroup
+    from tomocupy_amd import ProcFunctions
+    rng = np.random.default_rng(order)
+    a = rng.uniform(0.2, 1.0, (2, 300, 420)).astype(np.float32)
+    ref = pp.rotate_proj(a.astype(np.float64), 2.5, order)
+    g = torch.from_numpy(a.copy()).cuda()
+    ProcFunctions.rotate_proj(g, 2.5, order)
+    torch.cuda.synchronize()
+    got = g.cpu().numpy().astype(np.float64)
+    err = np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())
+    print('order', order, 'rel-L2', err, 'max', np.abs(got - ref).max())
+    assert err <= 1e-6
+
+
 def test_rotate_proj_rejects_spline_orders():
     from tomocupy_amd import ProcFunctions, _lib
     g = torch.ones((1, 8, 8), dtype=torch.float32, device='cuda')
     with pytest.raises(_lib.TomoError):
-        ProcFunctions.rotate_proj(g, 3.0, 3)
+        ProcFunctions.rotate_proj(g, 3.0, 4)
 
 
-def test_cli_recon_steps_rotate(tmp_path):
+@pytest.mark.parametrize("order", [1, 3])
+def test_cli_recon_steps_rotate(tmp_path, order):
     from tomocupy_amd import io
     from tomocupy_amd.cli import main
     n, nproj, nz = 64, 40, 8
@@ -59,9 +77,10 @@ def test_cli_recon_steps_rotate(tmp_path):
              theta=theta.astype(np.float64) * 180 / np.pi)
     out = str(tmp_path / 'rec')
     main(['recon_steps', '--file-name', str(p), '--reconstruction-type', 'full',
-          '--nsino-per-chunk', '4', '--out-path-name', out, '--rotate-proj-angle', '1.5'])
+          '--nsino-per-chunk', '4', '--out-path-name', out, '--rotate-proj-angle', '1.5',
+          '--rotate-proj-order', str(order)])
     th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
-    res = pp.rotate_proj(pp.darkflat_correction(data, dark, flat), 1.5, 1)
+    res = pp.rotate_proj(pp.darkflat_correction(data, dark, flat), 1.5, order)
     sino = np.ascontiguousarray(pp.minus_log(res).swapaxes(0, 1))
     ne = 4 * n
     for c0 in range(0, nz, 4):

@@ -273,3 +273,29 @@ def test_rotate_proj_conventions():
                 v = ((1 - wy) * ((1 - wx) * at(y0, x0) + wx * at(y0, x0 + 1)) +
                      wy * ((1 - wx) * at(y0 + 1, x0) + wx * at(y0 + 1, x0 + 1)))
                 assert abs(got[p, y, x] - v) < 1e-12, (p, y, x)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_rotate_proj_spline_reproduces_polynomials(order):
+    # B-splines of order k reproduce polynomials of degree <= k: away from the
+    # edges (where the 'nearest' extension bends the image), the rotated
+    # value at (y, x) is the polynomial at R (y, x) + c - R c
+    ny, nx = 48, 56
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+
+    def f(yy, xx):
+        return 0.3 + 0.02 * yy - 0.015 * xx + 1e-3 * yy * xx - 4e-4 * xx * xx
+
+    ang = 11.0
+    got = pp.rotate_proj(f(y, x)[None], ang, order)[0]
+    c, s = np.cos(np.deg2rad(ang)), np.sin(np.deg2rad(ang))
+    cy, cx = (ny - 1) / 2, (nx - 1) / 2
+    iy = c * y + s * x + cy - (c * cy + s * cx)
+    ix = -s * y + c * x + cx - (-s * cy + c * cx)
+    inner = (iy > 14) & (iy < ny - 15) & (ix > 14) & (ix < nx - 15)
+    assert inner.sum() > 200
+    assert np.max(np.abs(got - f(iy, ix))[inner]) < 1e-6
+    # and a spline interpolates: the identity at angle 0
+    rng = np.random.default_rng(4)
+    a = rng.uniform(0, 1, (1, 9, 11))
+    assert np.allclose(pp.rotate_proj(a, 0.0, order), a, atol=1e-12)

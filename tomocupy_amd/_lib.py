@@ -73,7 +73,7 @@ SIGNATURES = {
                              ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
     'ph_apply': (_c_int, [_c_p, _c_p, _c_i64, _c_p]),
     'ph_destroy': (None, [_c_p]),
-    'pre_rotate_proj': (_c_int, [_c_p, _c_p, _c_i64, _c_i64, _c_i64, _c_d, _c_int, _c_p]),
+    'pre_rotate_proj': (_c_int, [_c_p, _c_p, _c_i64, _c_i64, _c_i64, _c_d, _c_int, _c_p, _c_p]),
     'pre_minus_log_swap_rows': (_c_int, [_c_p, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_int,
                                          _c_p]),
     'pre_remove_stripe_ti': (_c_int, [_c_p, _c_p, _c_d, _c_d, _c_p]),

@@ -166,7 +166,7 @@ class ProcFunctions:
         """``rotate_proj`` (``proc_functions.py:126-138``): every projection of
         the float32 ``data (nproj, ny, nx)`` rotated by ``angle`` degrees
         (``ndimage.rotate(..., reshape=False, axes=(2, 1), mode='nearest')``),
-        interpolation order 0 or 1; in place (through a temporary) unless
+        interpolation order 0 to 3; in place (through a temporary) unless
         ``out`` is given."""
         import torch
         if abs(float(angle)) == 0.0:
@@ -176,10 +176,15 @@ class ProcFunctions:
             return data
         dst = torch.empty_like(data) if out is None else out
         shape = tuple(data.shape)
+        work = None
+        if int(order) >= 2:  # spline coefficients of the 12-padded planes
+            work = torch.empty((shape[0], shape[1] + 24, shape[2] + 24), dtype=torch.float32,
+                               device=data.device)
         _lib.check(_lib.lib().pre_rotate_proj(
             _lib.dev_ptr(data, shape, torch.float32, 'data'),
             _lib.dev_ptr(dst, shape, torch.float32, 'out'), shape[0], shape[1], shape[2],
-            float(angle), int(order), _lib.stream_ptr(None)), 'pre_rotate_proj')
+            float(angle), int(order), None if work is None else work.data_ptr(),
+            _lib.stream_ptr(None)), 'pre_rotate_proj')
         if out is None:
             data.copy_(dst)
             return data
