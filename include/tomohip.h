@@ -349,10 +349,10 @@ int pre_remove_all_stripe(tomo_pre_plan *plan, float *res, double snr, int la_si
  * float32 into dst (same shape, not in place): dst[p,y,x] = src[p] sampled
  * at R (y, x) + centre - R centre, R = [[cos, sin], [-sin, cos]] of the angle
  * in degrees, centre ((ny-1)/2, (nx-1)/2), neighbours clamped.  order 0
- * (nearest), 1 (bilinear, the reference CLI's default), 2 or 3 (quadratic /
- * cubic B-spline on the prefiltered plane, edge-padded by 12 as SciPy's
- * spline path); 4 and 5 return TOMO_EUNSUPPORTED.
- *   work  orders 2, 3: device float32 of nproj (ny + 24) (nx + 24) values
+ * (nearest), 1 (bilinear, the reference CLI's default), 2 to 5 (B-spline of
+ * that order on the prefiltered plane, edge-padded by 12 as SciPy's spline
+ * path).
+ *   work  orders 2-5: device float32 of nproj (ny + 24) (nx + 24) values
  *         (the spline coefficients); may be NULL for orders 0, 1
  */
 int pre_rotate_proj(const float *src, float *dst, int64_t nproj, int64_t ny, int64_t nx,

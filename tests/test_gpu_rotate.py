@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("angle,order", [(1.5, 1), (-7.25, 1), (90.0, 1), (200.0, 1),
                                          (1.5, 0), (33.0, 0), (1.5, 3), (-7.25, 3), (90.0, 3),
-                                         (200.0, 2), (4.0, 2)])
+                                         (200.0, 2), (4.0, 2), (-3.0, 4), (130.0, 4),
+                                         (2.0, 5), (-45.0, 5)])
 def test_rotate_proj_matches_oracle(angle, order):
     from tomocupy_amd import ProcFunctions
     rng = np.random.default_rng(int(abs(angle) * 4) + order)
@@ -42,7 +43,7 @@ def test_rotate_proj_matches_oracle(angle, order):
         assert err <= 1e-6
 
 
-@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
 def test_rotate_proj_spline_large(order):
     # padded planes wider and taller than one 256-thread workgroup
     from tomocupy_amd import ProcFunctions
@@ -58,11 +59,12 @@ def test_rotate_proj_spline_large(order):
     assert err <= 1e-6
 
 
-def test_rotate_proj_rejects_spline_orders():
+def test_rotate_proj_rejects_orders_above_5():
+    # as ndimage ('spline order not supported')
     from tomocupy_amd import ProcFunctions, _lib
     g = torch.ones((1, 8, 8), dtype=torch.float32, device='cuda')
     with pytest.raises(_lib.TomoError):
-        ProcFunctions.rotate_proj(g, 3.0, 4)
+        ProcFunctions.rotate_proj(g, 3.0, 6)
 
 
 @pytest.mark.parametrize("order", [1, 3])

@@ -275,7 +275,7 @@ def test_rotate_proj_conventions():
                 assert abs(got[p, y, x] - v) < 1e-12, (p, y, x)
 
 
-@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
 def test_rotate_proj_spline_reproduces_polynomials(order):
     # B-splines of order k reproduce polynomials of degree <= k: away from the
     # edges (where the 'nearest' extension bends the image), the rotated

@@ -11,7 +11,7 @@ reference's command (``src/tomocupy/__main__.py``, flags and defaults of
   GPU through :class:`tomocupy_amd.rec_steps.GPURecSteps` (rec_steps.py:114-125)
   with phase retrieval (``--retrieve-phase-method paganin|Gpaganin|farago``,
   ``--energy`` keV, ``--propagation-distance`` mm, ``--pixel-size`` um) and
-  projection rotation (``--rotate-proj-angle``, ``--rotate-proj-order`` 0 to 3);
+  projection rotation (``--rotate-proj-angle``, ``--rotate-proj-order`` 0 to 5);
 * ``--reconstruction-type try``: the slices at ``--nsino`` (fractions of the
   height, ``reader.py:192-197``) reconstructed at the centres
   ``centre - arange(-w, w, step)`` (``reader.py:224-246``), each a per-slice

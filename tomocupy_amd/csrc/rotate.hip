@@ -6,8 +6,8 @@
 //   offset = centre - R centre,  centre = ((ny - 1)/2, (nx - 1)/2),
 // c, s = cos, sin of the angle in degrees, with order 0 (nearest sample),
 // 1 (bilinear), neighbours clamped to the image ('nearest' extension), or 2
-// and 3 (quadratic, cubic B-splines on the prefiltered, edge-padded plane, as
-// SciPy's spline path).  Coordinates in FP64, interpolation in float32.
+// to 5 (B-splines on the prefiltered, edge-padded plane, as SciPy's spline
+// path).  Coordinates in FP64, interpolation in float32.
 #include <cmath>
 #include <cstdint>
 
@@ -45,64 +45,148 @@ __global__ __launch_bounds__(256) void k_pre_rotate(const float *__restrict__ sr
   dst[((size_t)p * ny + y) * nx + x] = v;
 }
 
-// B-spline prefilter of order 2 or 3 (single pole z: sqrt(8) - 3, sqrt(3) - 2)
-// on the plane edge-padded by kSplinePad samples (as SciPy's
-// _prepad_for_spline_filter does for mode 'nearest'), the recursive filter of
-// Unser, Aldroubi and Eden (IEEE TSP 41, 1993) with mirror (whole-sample
-// symmetric) initial values as in Thevenaz, Blu and Unser (IEEE TMI 19,
-// 2000): gain (1 - z)(1 - 1/z), causal c[k] += z c[k-1] from a truncated
-// mirror sum, anticausal c[k] = z (c[k+1] - c[k]) from
-// c[N-1] = z/(z^2 - 1) (z c[N-2] + c[N-1]).  The initial values only reach the
-// data through |z|^12 (<= 1.4e-7).
+// B-spline prefilter of order 2 to 5 on the plane edge-padded by kSplinePad
+// samples (as SciPy's _prepad_for_spline_filter does for mode 'nearest'):
+// the inverse of the sampled B-spline, whose poles z in (-1, 0) are one for
+// orders 2, 3 and two for 4, 5 (Unser, Aldroubi and Eden, IEEE TSP 41, 1993).
+// Its impulse response is, per pole, g[k] = (1 - z)/(1 + z) z^|k| (the
+// causal/anticausal pair with gain (1 - z)(1 - 1/z)), and for two poles the
+// convolution of the two.  It is applied as a symmetric FIR truncated where
+// |z_max|^M < 1e-8, separably along rows then columns of an LDS tile, with
+// whole-sample mirror extension at the padded plane's edges: the solution the
+// recursive filter with mirror initial values (Thevenaz, Blu and Unser, IEEE
+// TMI 19, 2000) computes.  The padding keeps the boundary |z|^12 from the data.
 constexpr int kSplinePad = 12;
+constexpr int kSplineTy = 32, kSplineTx = 64;  // output tile of the prefilter
+constexpr double kSplinePoles[4][2] = {{-0.17157287525380990, 0.0},   // order 2: sqrt(8) - 3
+                                       {-0.26794919243112270, 0.0},   // order 3: sqrt(3) - 2
+                                       {-0.36134122590022033, -0.013725429297339118},
+                                       {-0.43057534709997360, -0.043096288203264665}};
+// FIR half-lengths ceil(log(1e-8) / log|z_max|) for orders 2..5
+constexpr int kSplineHalf[4] = {11, 14, 19, 22};
 
-__device__ __forceinline__ float spline_pole(int order) {
-  return order == 2 ? -0.17157287525380990f : -0.26794919243112270f;
-}
+struct SplineTaps {
+  float h[24];  // h[0..M]
+};
 
-// in place along a strided sequence c[0], c[st], ... of length N (gain applied)
-__device__ void spline_1d(float *c, int N, size_t st, float z) {
-  const float lambda = (1.f - z) * (1.f - 1.f / z);
-  if (N < 2) return;
-  for (int k = 0; k < N; ++k) c[k * st] *= lambda;
-  // causal initial value: sum_k z^k c[k] until z^k < 1e-9
-  float zk = 1.f, sum = 0.f;
-  for (int k = 0; k < N && fabsf(zk) > 1e-9f; ++k) {
-    sum += zk * c[k * st];
-    zk *= z;
+// h[k] of the prefilter of `order` for k = 0..M (FP64 on the host)
+SplineTaps spline_taps(int order) {
+  const double *z = kSplinePoles[order - 2];
+  const int M = kSplineHalf[order - 2];
+  auto g = [](double zz, int k) { return (1 - zz) / (1 + zz) * std::pow(zz, std::abs(k)); };
+  SplineTaps t{};
+  for (int k = 0; k <= M; ++k) {
+    double v;
+    if (z[1] == 0.0) {
+      v = g(z[0], k);
+    } else {
+      v = 0.0;  // |z| < 0.44: 0.44^120 vanishes in FP64
+      for (int j = -120; j <= 120; ++j) v += g(z[0], j) * g(z[1], k - j);
+    }
+    t.h[k] = (float)v;
   }
-  c[0] = sum;
-  for (int k = 1; k < N; ++k) c[k * st] += z * c[(k - 1) * st];
-  c[(N - 1) * st] = (z / (z * z - 1.f)) * (z * c[(N - 2) * st] + c[(N - 1) * st]);
-  for (int k = N - 2; k >= 0; --k) c[k * st] = z * (c[(k + 1) * st] - c[k * st]);
+  return t;
 }
 
-// columns of the padded plane: work[p][yc][xc] = spline along yc of
-// data[p][clamp(yc - pad)][clamp(xc - pad)], one thread per padded column
-__global__ __launch_bounds__(256) void k_pre_spline_cols(const float *__restrict__ src,
+// whole-sample mirror into [0, N), then clamped (indices that only feed
+// outputs outside the plane may reflect more than once)
+__device__ __forceinline__ int spline_mirror(int q, int N) {
+  q = q < 0 ? -q : q;
+  q = q > N - 1 ? 2 * (N - 1) - q : q;
+  return min(max(q, 0), N - 1);
+}
+
+// spline coefficients of one kSplineTy x kSplineTx tile of the padded plane
+// of projection blockIdx.z: the tile and an M halo (mirrored, then edge
+// padding read from the projection) into LDS, the row FIR into a second LDS
+// array, the column FIR out to work[p][Py][Px]
+template <int M>
+__global__ __launch_bounds__(256) void k_pre_spline_coef(const float *__restrict__ src,
                                                          float *__restrict__ work, int ny,
-                                                         int nx, int order) {
+                                                         int nx, SplineTaps taps) {
+  constexpr int AH = kSplineTy + 2 * M, AW = kSplineTx + 2 * M;
+  __shared__ float A[AH * AW];
+  __shared__ float B[AH * kSplineTx];
   const int Py = ny + 2 * kSplinePad, Px = nx + 2 * kSplinePad;
-  const int xc = blockIdx.x * 256 + threadIdx.x, p = blockIdx.y;
-  if (xc >= Px) return;
-  const int x = min(max(xc - kSplinePad, 0), nx - 1);
+  const int X0 = blockIdx.x * kSplineTx, Y0 = blockIdx.y * kSplineTy, p = blockIdx.z;
   const float *img = src + (size_t)p * ny * nx;
-  float *w = work + (size_t)p * Py * Px + xc;
-  for (int yc = 0; yc < Py; ++yc)
-    w[(size_t)yc * Px] = img[(size_t)min(max(yc - kSplinePad, 0), ny - 1) * nx + x];
-  spline_1d(w, Py, (size_t)Px, spline_pole(order));
+  for (int i = threadIdx.x; i < AH * AW; i += 256) {
+    const int r = i / AW, cc = i - r * AW;
+    const int yy = min(max(spline_mirror(Y0 - M + r, Py) - kSplinePad, 0), ny - 1);
+    const int xx = min(max(spline_mirror(X0 - M + cc, Px) - kSplinePad, 0), nx - 1);
+    A[i] = img[(size_t)yy * nx + xx];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < AH * kSplineTx; i += 256) {
+    const int r = i / kSplineTx, j = i % kSplineTx;
+    const float *a = A + r * AW + j + M;
+    float v = taps.h[0] * a[0];
+#pragma unroll
+    for (int k = 1; k <= M; ++k) v += taps.h[k] * (a[k] + a[-k]);
+    B[i] = v;
+  }
+  __syncthreads();
+  float *w = work + (size_t)p * Py * Px;
+  for (int i = threadIdx.x; i < kSplineTy * kSplineTx; i += 256) {
+    const int r = i / kSplineTx, j = i % kSplineTx;
+    const int y = Y0 + r, x = X0 + j;
+    if (y >= Py || x >= Px) continue;
+    const float *bb = B + (r + M) * kSplineTx + j;
+    float v = taps.h[0] * bb[0];
+#pragma unroll
+    for (int k = 1; k <= M; ++k) v += taps.h[k] * (bb[k * kSplineTx] + bb[-k * kSplineTx]);
+    w[(size_t)y * Px + x] = v;
+  }
 }
 
-// rows of the padded plane, in place, one thread per padded row
-__global__ __launch_bounds__(256) void k_pre_spline_rows(float *__restrict__ work, int ny,
-                                                         int nx, int order) {
-  const int Py = ny + 2 * kSplinePad, Px = nx + 2 * kSplinePad;
-  const int yc = blockIdx.x * 256 + threadIdx.x, p = blockIdx.y;
-  if (yc >= Py) return;
-  spline_1d(work + ((size_t)p * Py + yc) * Px, Px, 1, spline_pole(order));
+// the ORDER + 1 weights beta_ORDER(u - b - k), k = 0..ORDER, of the samples
+// b, b + 1, ... around u (closed forms for 2 and 3; for 4 and 5 the
+// truncated-power sum beta_n(x) = 1/n! sum_k (-1)^k C(n+1, k)
+// max(x + (n+1)/2 - k, 0)^n, in FP64)
+template <int ORDER>
+__device__ __forceinline__ void spline_weights(double u, float *w, double &b) {
+  if (ORDER == 3) {
+    const double f = floor(u);
+    const float t = (float)(u - f);
+    b = f - 1.0;
+    const float t2 = t * t, t3 = t2 * t, m = 1.f - t;
+    w[0] = m * m * m / 6.f;
+    w[1] = (4.f - 6.f * t2 + 3.f * t3) / 6.f;
+    w[2] = (1.f + 3.f * t + 3.f * t2 - 3.f * t3) / 6.f;
+    w[3] = t3 / 6.f;
+  } else if (ORDER == 2) {
+    const double f = floor(u + 0.5);
+    const float t = (float)(u - f);
+    b = f - 1.0;
+    w[0] = 0.5f * (0.5f - t) * (0.5f - t);
+    w[1] = 0.75f - t * t;
+    w[2] = 0.5f * (0.5f + t) * (0.5f + t);
+  } else {
+    b = (ORDER & 1) ? floor(u) - (ORDER - 1) / 2 : floor(u + 0.5) - ORDER / 2;
+    constexpr double binom[7] = {1, ORDER + 1.0, (ORDER + 1.0) * ORDER / 2,
+                                 (ORDER + 1.0) * ORDER * (ORDER - 1) / 6,
+                                 (ORDER + 1.0) * ORDER * (ORDER - 1) * (ORDER - 2) / 24,
+                                 (ORDER + 1.0) * ORDER * (ORDER - 1) * (ORDER - 2) * (ORDER - 3) / 120,
+                                 1};
+    constexpr double inv_fact = ORDER == 4 ? 1.0 / 24 : 1.0 / 120;
+#pragma unroll
+    for (int k = 0; k <= ORDER; ++k) {
+      const double x = u - (b + k) + 0.5 * (ORDER + 1);
+      double sum = 0.0;
+#pragma unroll
+      for (int j = 0; j <= ORDER + 1; ++j) {
+        const double t = fmax(x - j, 0.0);
+        double tp = t;
+#pragma unroll
+        for (int e = 1; e < ORDER; ++e) tp *= t;
+        sum += ((j & 1) ? -binom[j] : binom[j]) * tp;
+      }
+      w[k] = (float)(sum * inv_fact);
+    }
+  }
 }
 
-// B-spline evaluation of order 2 or 3 at the rotated coordinates, on the
+// B-spline evaluation of order 2 to 5 at the rotated coordinates, on the
 // coefficients of the padded plane (indices clamped to it)
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_pre_rotate_spline(const float *__restrict__ work,
@@ -117,27 +201,8 @@ __global__ __launch_bounds__(256) void k_pre_rotate_spline(const float *__restri
   constexpr int K = ORDER + 1;
   float wy[K], wx[K];
   double by, bx;  // first neighbour
-  auto weights = [&](double u, float *w, double &b) {
-    if (ORDER == 3) {
-      const double f = floor(u);
-      const float t = (float)(u - f);
-      b = f - 1.0;
-      const float t2 = t * t, t3 = t2 * t, m = 1.f - t;
-      w[0] = m * m * m / 6.f;
-      w[1] = (4.f - 6.f * t2 + 3.f * t3) / 6.f;
-      w[2] = (1.f + 3.f * t + 3.f * t2 - 3.f * t3) / 6.f;
-      w[3] = t3 / 6.f;
-    } else {
-      const double f = floor(u + 0.5);
-      const float t = (float)(u - f);
-      b = f - 1.0;
-      w[0] = 0.5f * (0.5f - t) * (0.5f - t);
-      w[1] = 0.75f - t * t;
-      w[2] = 0.5f * (0.5f + t) * (0.5f + t);
-    }
-  };
-  weights(iy, wy, by);
-  weights(ix, wx, bx);
+  spline_weights<ORDER>(iy, wy, by);
+  spline_weights<ORDER>(ix, wx, bx);
   float v = 0.f;
 #pragma unroll
   for (int a = 0; a < K; ++a) {
@@ -178,13 +243,10 @@ extern "C" int pre_rotate_proj(const float *src, float *dst, int64_t nproj, int6
                    nx <= 0x7fffffff,
                "pre_rotate_proj: sizes out of range");
   TOMO_REQUIRE(std::isfinite(angle), "pre_rotate_proj: angle must be finite");
-  if (order < 0 || order > 3) {
-    set_error("pre_rotate_proj: interpolation order %d not supported (0 to 3; the spline "
-              "orders 4 and 5 are not implemented)", order);
-    return order < 0 ? TOMO_EINVAL : TOMO_EUNSUPPORTED;
-  }
+  TOMO_REQUIRE(order >= 0 && order <= 5,
+               "pre_rotate_proj: interpolation order must be in 0..5");
   TOMO_REQUIRE(order < 2 || work != nullptr,
-               "pre_rotate_proj: orders 2 and 3 need the work buffer");
+               "pre_rotate_proj: the spline orders 2 to 5 need the work buffer");
   TOMO_REQUIRE(order < 2 || (work != src && work != dst),
                "pre_rotate_proj: work must not alias src or dst");
   TOMO_REQUIRE(order < 2 || nx <= 0x7fffffff - 2 * kSplinePad,
@@ -199,21 +261,25 @@ extern "C" int pre_rotate_proj(const float *src, float *dst, int64_t nproj, int6
   if (order >= 2) {
     const int Py = (int)ny + 2 * kSplinePad, Px = (int)nx + 2 * kSplinePad;
     {
-      ProfScope ps_("k_pre_spline", st);
-      hipLaunchKernelGGL(k_pre_spline_cols, dim3((unsigned)ceil_div(Px, 256), (unsigned)nproj),
-                         dim3(256), 0, st, src, work, (int)ny, (int)nx, order);
-      hipLaunchKernelGGL(k_pre_spline_rows, dim3((unsigned)ceil_div(Py, 256), (unsigned)nproj),
-                         dim3(256), 0, st, work, (int)ny, (int)nx, order);
+      ProfScope ps_("k_pre_spline_coef", st);
+      const dim3 g2((unsigned)ceil_div(Px, kSplineTx), (unsigned)ceil_div(Py, kSplineTy),
+                    (unsigned)nproj);
+      const SplineTaps taps = spline_taps(order);
+      auto kern = order == 2   ? k_pre_spline_coef<kSplineHalf[0]>
+                  : order == 3 ? k_pre_spline_coef<kSplineHalf[1]>
+                  : order == 4 ? k_pre_spline_coef<kSplineHalf[2]>
+                               : k_pre_spline_coef<kSplineHalf[3]>;
+      hipLaunchKernelGGL(kern, g2, dim3(256), 0, st, src, work, (int)ny, (int)nx, taps);
     }
     TOMO_LAUNCH_CHECK();
     {
       ProfScope ps_("k_pre_rotate", st);
-      if (order == 2)
-        hipLaunchKernelGGL(k_pre_rotate_spline<2>, grid, dim3(256), 0, st, work, dst, (int)ny,
-                           (int)nx, c, s, oy, ox);
-      else
-        hipLaunchKernelGGL(k_pre_rotate_spline<3>, grid, dim3(256), 0, st, work, dst, (int)ny,
-                           (int)nx, c, s, oy, ox);
+      auto kern = order == 2   ? k_pre_rotate_spline<2>
+                  : order == 3 ? k_pre_rotate_spline<3>
+                  : order == 4 ? k_pre_rotate_spline<4>
+                               : k_pre_rotate_spline<5>;
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, work, dst, (int)ny, (int)nx, c, s, oy,
+                         ox);
     }
     TOMO_LAUNCH_CHECK();
     return TOMO_OK;

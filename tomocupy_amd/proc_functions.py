@@ -166,7 +166,7 @@ class ProcFunctions:
         """``rotate_proj`` (``proc_functions.py:126-138``): every projection of
         the float32 ``data (nproj, ny, nx)`` rotated by ``angle`` degrees
         (``ndimage.rotate(..., reshape=False, axes=(2, 1), mode='nearest')``),
-        interpolation order 0 to 3; in place (through a temporary) unless
+        interpolation order 0 to 5; in place (through a temporary) unless
         ``out`` is given."""
         import torch
         if abs(float(angle)) == 0.0:

@@ -70,9 +70,9 @@ class GPURecSteps:
                                    retrieve_phase_alpha, retrieve_phase_delta_beta,
                                    retrieve_phase_W * 1e-4, batch=int(phase_batch))
         self.rotate_angle, self.rotate_order = float(rotate_proj_angle), int(rotate_proj_order)
-        if self.rotate_angle != 0.0 and self.rotate_order not in (0, 1, 2, 3):
-            raise NotImplementedError('rotate_proj_order %d: orders 0 to 3 are implemented'
-                                      % self.rotate_order)
+        if self.rotate_angle != 0.0 and self.rotate_order not in range(6):
+            raise ValueError('rotate_proj_order %d: the interpolation order is 0 to 5'
+                             % self.rotate_order)
         self.stream = torch.cuda.Stream()
         self.tdtype = _lib.torch_dtype(dtype)
 
