@@ -68,8 +68,9 @@ typedef struct tomo_flt_plan tomo_flt_plan;
  * Replaces cfunc_fourierrec::cfunc_fourierrec(nproj, nz/2, n, theta)
  * (src/cuda/cfunc_fourierrec.cu:4-50) together with FourierRec.__init__
  * (src/tomocupy/reconstruction/fourierrec.py:49-59).
- *   n        detector width = image size; even, >= 20, of the form
- *            2^a 3^b 5^c 7^d (the reference's cuFFT takes any length)
+ *   n        detector width = image size; even, 20..4096, with no prime
+ *            factor above 127 (the reference's cuFFT takes any length; e.g.
+ *            2448 = 2^4 3^2 17 is accepted)
  *   nproj    number of angles, >= 1
  *   nz       number of REAL slices per call (the reference's ncz), >= 1;
  *            odd nz is accepted and paired with an implicit zero slice
@@ -145,7 +146,7 @@ int fr_filter_backprojection_prepared(tomo_fr_plan *plan, tomo_flt_plan *flt, vo
  *   data  output, device float32 (nz, nproj, n)
  *   obj   input, device float32 (nz, n, n); NOT modified
  * float32 plans with n = 2^a 3^b only (TOMO_EUNSUPPORTED for 16-bit plans and
- * for n with a factor 5 or 7).
+ * for n with a prime factor above 3).
  */
 int fr_forward(tomo_fr_plan *plan, void *data, const void *obj, void *stream);
 
@@ -164,7 +165,7 @@ int fr_get_sizes(const tomo_fr_plan *plan, int64_t *n, int64_t *nproj, int64_t *
  * and FBPFilter.__init__ (src/tomocupy/reconstruction/fbp_filter.py:47-52).
  *   n      unpadded detector width; even, >= 20
  *   ne     padded FFT length (4n in float32, backproj_functions.py:50-54); ne >= n, even,
- *          2^a 3^b 5^c 7^d <= 16384
+ *          <= 16384, with no prime factor above 127
  *   nproj, nz  rows per call
  *   dtype  storage of the filtered rows: TOMO_F32, TOMO_F16 (cfunc_filterfp16.i) or
  *          TOMO_BF16; the arithmetic is float32

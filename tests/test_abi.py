@@ -35,7 +35,7 @@ def test_version():
 @pytest.mark.parametrize("n,nproj,nz,dtype,msg", [
     (21, 10, 2, 0, 'even'), (18, 10, 2, 0, '>= 20'), (64, 0, 2, 0, 'nproj'),
     (64, 10, 0, 0, 'nz'), (64, 10, 2, 7, 'dtype'), (8192, 10, 2, 0, 'maximum'),
-    (88, 10, 2, 0, '2^a 3^b 5^c 7^d')])  # 88 = 8 x 11
+    (262, 10, 2, 0, 'prime factor')])  # 262 = 2 x 131
 def test_fr_create_validation(n, nproj, nz, dtype, msg):
     h = ctypes.c_void_p()
     rc = _lib.lib().fr_create(ctypes.byref(h), n, nproj, nz, ctypes.c_void_p(16), dtype, None)
@@ -63,7 +63,8 @@ def test_flt_create_validation():
     h = ctypes.c_void_p()
     L = _lib.lib()
     assert L.flt_create(ctypes.byref(h), 64, 60, 10, 2, 0) == _lib.TOMO_EINVAL  # ne < n
-    assert L.flt_create(ctypes.byref(h), 64, 256 * 11, 10, 2, 0) == _lib.TOMO_EINVAL  # 11 | ne
+    assert L.flt_create(ctypes.byref(h), 64, 64 * 131, 10, 2, 0) == _lib.TOMO_EINVAL  # 131 | ne
+    assert 'prime factors' in _lib.last_error()
     assert L.flt_create(ctypes.byref(h), 64, 256, 10, 2, 9) == _lib.TOMO_EINVAL
 
 

@@ -1,8 +1,9 @@
-"""Detector widths with factors 5 and 7 (the reference's cuFFT takes any
-length; here the LDS Stockham transforms have radix-5 and radix-7 passes):
-backprojection, the fused filter + backprojection and the unfused filter
-against the FP64 oracle (per-slice rel-L2 <= 1e-5).  The forward projection
-(autograd) rejects such widths."""
+"""Detector widths with prime factors above 3 (the reference's cuFFT takes any
+length; here the LDS Stockham transforms have butterflies for the primes
+5..23 and direct-DFT passes for the primes 29..127): backprojection, the fused
+filter + backprojection and the unfused filter against the FP64 oracle
+(per-slice rel-L2 <= 1e-5).  The forward projection (autograd) rejects such
+widths."""
 import numpy as np
 import pytest
 
@@ -31,7 +32,9 @@ def _sinos(rng, nz, nproj, n):
 
 
 @pytest.mark.parametrize("n,nproj,nz", [(80, 60, 4), (70, 48, 2), (100, 75, 3), (140, 90, 2),
-                                        (350, 64, 2)])
+                                        (350, 64, 2), (68, 40, 2), (88, 50, 3), (286, 40, 2),
+                                        (254, 30, 2), (228, 40, 2), (184, 36, 2),
+                                        (2448, 16, 1)])
 def test_backprojection_5_7(n, nproj, nz):
     from tomocupy_amd import FourierRec
     print('n', n, flush=True)
@@ -48,7 +51,8 @@ def test_backprojection_5_7(n, nproj, nz):
     assert np.all(err <= RTOL32), err
 
 
-@pytest.mark.parametrize("n,filt", [(70, 'shepp'), (100, 'parzen')])
+@pytest.mark.parametrize("n,filt", [(70, 'shepp'), (100, 'parzen'), (68, 'parzen'),
+                                    (286, 'hann'), (184, 'cosine')])
 def test_filter_backprojection_5_7(n, filt):
     from tomocupy_amd import BackprojFunctions
     nproj, nz = 60, 4

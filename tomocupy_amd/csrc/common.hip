@@ -33,11 +33,13 @@ size_t max_lds_per_block() {
 }
 
 int FftPlan::init(int N) {
-  TOMO_REQUIRE(fft_size_ok(N) && N <= 16384, "FFT length %d is not 2^a 3^b 5^c 7^d <= 16384", N);
+  TOMO_REQUIRE(fft_size_ok(N) && N <= 16384,
+               "FFT length %d is not <= 16384 with prime factors <= %d", N, kFftMaxPrime);
   d.N = N;
   d.nr = 0;
   int m = N;
-  // radix 16 first (fewest passes), then 8, 4, 2, then 3s, 5s, 7s
+  // radix 16 first (fewest passes), then 8, 4, 2, then 3s, 5s, 7s, then the
+  // larger primes
   while (m % 16 == 0) { d.r[d.nr++] = 16; m /= 16; }
   while (m % 8 == 0) { d.r[d.nr++] = 8; m /= 8; }
   while (m % 4 == 0) { d.r[d.nr++] = 4; m /= 4; }
@@ -45,6 +47,8 @@ int FftPlan::init(int N) {
   while (m % 3 == 0) { d.r[d.nr++] = 3; m /= 3; }
   while (m % 5 == 0) { d.r[d.nr++] = 5; m /= 5; }
   while (m % 7 == 0) { d.r[d.nr++] = 7; m /= 7; }
+  for (int p = 11; p <= kFftMaxPrime && m > 1; p += 2)  // generic prime passes
+    while (m % p == 0) { d.r[d.nr++] = p; m /= p; }
   std::vector<float2> h(N);
   for (int k = 0; k < N; ++k) {
     double a = -2.0 * M_PI * (double)k / (double)N;

@@ -1,6 +1,8 @@
-// LDS-resident Stockham autosort FFT for sizes N = 2^a 3^b 5^c 7^d (N <= 16384 per
-// transform), executed cooperatively by a workgroup on 2^lgH interleaved
-// transforms held in LDS.
+// LDS-resident Stockham autosort FFT for sizes N <= 16384 whose prime factors
+// are at most kFftMaxPrime (per transform), executed cooperatively by a
+// workgroup on 2^lgH interleaved transforms held in LDS.  Radices 2, 3, 4, 5,
+// 7, 8, 11, 13, 16, 17, 19 and 23 run as butterflies in registers; any other
+// prime p runs as a direct length-p DFT per output (fft_pass_gen).
 //
 // Pass with radix R and current sub-transform length Ns (1, R1, R1*R2, ...):
 //   for j in [0, N/R):  k = j mod Ns
@@ -23,6 +25,7 @@ namespace tomo {
 
 constexpr int kFftMaxRadices = 16;
 constexpr int kFftEPT = 16;  // complex elements staged per thread per pass (max)
+constexpr int kFftMaxPrime = 127;  // largest prime factor of a transform length
 
 struct FftDesc {
   int N;
@@ -35,7 +38,7 @@ struct FftDesc {
 struct FftPlan {
   FftDesc d{};
   DevBuf tw;
-  int init(int N);  // N = 2^a 3^b 5^c 7^d
+  int init(int N);  // N <= 16384, prime factors <= kFftMaxPrime
 };
 
 template <int DIR>
@@ -290,13 +293,164 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
   __syncthreads();
 }
 
+// Radix-R pass for an odd prime 11 <= R <= 23 with the butterfly in
+// registers: inputs twiddled by W_N^(r k N/(Ns R)) from the table, then the
+// DFT from the cosine/sine pairs P_j = a_j + a_(R-j), M_j = a_j - a_(R-j):
+//   b_q, b_(R-q) = a_0 + sum_j cos(2 pi j q/R) P_j  -/+ i sum_j sin(2 pi j q/R) M_j
+// (signs of i swapped for the inverse), cos and sin read from the table
+// (W_R^m = W_N^(m N/R)).  One butterfly per thread and round when R > EPT.
+template <int R, int DIR, int EPT>
+__device__ __noinline__ void fft_pass_prime(float2 *buf, int N, int Ns, const float2 *tw,
+                                            int lgH, int tid, int nt) {
+  constexpr int MAXB = EPT / R > 0 ? EPT / R : 1;
+  constexpr int HR = (R - 1) / 2;
+  const int H = 1 << lgH;
+  const int nbPer = N / R;
+  const int nb = nbPer << lgH;
+  const int twstep = N / (Ns * R);
+  float cs[HR + 1], sn[HR + 1];  // cos, sin(2 pi m / R), m = 1..HR
+#pragma unroll
+  for (int m = 1; m <= HR; ++m) {
+    const float2 w = tw[m * nbPer];
+    cs[m] = w.x;
+    sn[m] = -w.y;
+  }
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> lgH;
+      const int k = j % Ns;
+      const unsigned base = (unsigned)(k * twstep);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = buf[(lds_pad(j + r * nbPer) << lgH) + t];
+        if (r > 0 && Ns > 1) {
+          float2 w = tw[base * r];
+          if (DIR > 0) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[q][r] = x;
+      }
+      float2 P[HR + 1], M[HR + 1];
+      float2 b0 = v[q][0];
+#pragma unroll
+      for (int jj = 1; jj <= HR; ++jj) {
+        P[jj] = make_float2(v[q][jj].x + v[q][R - jj].x, v[q][jj].y + v[q][R - jj].y);
+        M[jj] = make_float2(v[q][jj].x - v[q][R - jj].x, v[q][jj].y - v[q][R - jj].y);
+        b0.x += P[jj].x;
+        b0.y += P[jj].y;
+      }
+#pragma unroll
+      for (int qq = 1; qq <= HR; ++qq) {
+        float2 re = v[q][0], im = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int jj = 1; jj <= HR; ++jj) {
+          const int m = (jj * qq) % R;  // compile-time after unrolling
+          const float c = m <= HR ? cs[m] : cs[R - m];
+          const float sv = m <= HR ? sn[m] : -sn[R - m];
+          re.x += c * P[jj].x;
+          re.y += c * P[jj].y;
+          im.x += sv * M[jj].x;
+          im.y += sv * M[jj].y;
+        }
+        const float2 mi = mul_i<DIR>(im);
+        v[q][qq] = make_float2(re.x + mi.x, re.y + mi.y);
+        v[q][R - qq] = make_float2(re.x - mi.x, re.y - mi.y);
+      }
+      v[q][0] = b0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> lgH;
+      const int jq = j / Ns, k = j - jq * Ns;
+      const int o = jq * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(lds_pad(o + r * Ns) << lgH) + t] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Pass with a prime radix 29 <= R <= kFftMaxPrime:
+// each output y[o], o = jq Ns R + q Ns + k, j = jq Ns + k, is the direct sum
+//   y[o] = sum_r x[j + r N/R] W_N^(r e mod N),  e = k N/(Ns R) + q N/R,
+// (the twiddle W_N^(r k N/(Ns R)) and the DFT_R factor W_R^(r q) in one
+// factor).  Four outputs run side by side; their factors advance by products
+// with W_N^e and are reloaded from the FP64-built table every fourth r (at
+// most three float32 complex products from a table entry).  At most EPT
+// outputs per thread are held in registers across the barrier.
+template <int DIR, int EPT>
+__device__ __noinline__ void fft_pass_gen(float2 *buf, int N, int Ns, int R, const float2 *tw,
+                                          int lgH, int tid, int nt) {
+  constexpr int G = 4;
+  static_assert(EPT % G == 0, "EPT must be a multiple of 4");
+  const int H = 1 << lgH;
+  const int nbPer = N / R;
+  const int total = N << lgH;
+  const int twstep = N / (Ns * R);
+  auto ld = [&](int m) {
+    float2 w = tw[m];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+  };
+  float2 acc[EPT];
+#pragma unroll
+  for (int q0 = 0; q0 < EPT; q0 += G) {
+    int base[G], e[G], m[G];
+    float2 we[G], w[G], s[G];
+    bool live[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int g = tid + (q0 + u) * nt;
+      live[u] = g < total;
+      const int o = live[u] ? g >> lgH : 0, t = g & (H - 1);
+      const int k = o % Ns, rest = o / Ns;
+      const int qq = rest % R, jq = rest / R;
+      base[u] = (jq * Ns + k) * H + t;  // unpadded interleaved index of x[j]
+      e[u] = k * twstep + qq * nbPer;   // < N
+      m[u] = 0;
+      we[u] = ld(e[u]);
+      w[u] = make_float2(1.f, 0.f);
+      s[u] = make_float2(0.f, 0.f);
+    }
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        if ((r & 3) == 0) w[u] = ld(m[u]);
+        const int idx = base[u] + r * nbPer * H;
+        const float2 x = buf[lds_pad(idx >> lgH) * H + (idx & (H - 1))];
+        s[u].x += x.x * w[u].x - x.y * w[u].y;
+        s[u].y += x.x * w[u].y + x.y * w[u].x;
+        w[u] = cmul(w[u], we[u]);
+        m[u] += e[u];
+        if (m[u] >= N) m[u] -= N;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) acc[q0 + u] = s[u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int g = tid + q * nt;
+    if (g < total) buf[(lds_pad(g >> lgH) << lgH) + (g & (H - 1))] = acc[q];
+  }
+  __syncthreads();
+}
+
 // In-place transform of 2^lgH interleaved sequences of length d.N held in LDS:
 // element i of sequence t lives at buf[lds_pad(i) * 2^lgH + t].  Requires
 // 2^lgH * N <= 15 * nt (16 * nt without radix-3 passes); see fft_threads().
 // The caller must __syncthreads() after filling buf; the function ends with a
 // barrier.
-// R57: also radix-5 and radix-7 passes (kernels for such sizes only: the
-// extra passes raise the register demand of every instantiation).
+// R57: also radix-5, radix-7 and generic prime passes (kernels for such sizes
+// only: the extra passes raise the register demand of every instantiation).
 template <int DIR, int EPT = kFftEPT, bool R57 = false>
 __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt) {
   int Ns = 1;
@@ -316,8 +470,20 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
     else if constexpr (R57) {
       if (R == 5)
         fft_pass<5, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-      else
+      else if (R == 7)
         fft_pass<7, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else if (R == 11)
+        fft_pass_prime<11, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else if (R == 13)
+        fft_pass_prime<13, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else if (R == 17)
+        fft_pass_prime<17, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else if (R == 19)
+        fft_pass_prime<19, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else if (R == 23)
+        fft_pass_prime<23, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      else
+        fft_pass_gen<DIR, EPT>(buf, d.N, Ns, R, d.tw, lgH, tid, nt);
     }
     Ns *= R;
   }
@@ -442,25 +608,36 @@ __device__ __forceinline__ void stage_lds(int count, int tid, int nt, Ld ld, St 
 
 // Workgroup size for transforming `elems` complex values (all interleaved
 // sequences) with plan d: 16 values per thread for radices 2/4/8/16, fewer
-// with a radix-3, 5 or 7 pass (15, 15, 14: whole butterflies); at least 256
-// and a multiple of 64.  Returns -1 if
-// more than 1024 threads would be needed.
+// with a radix-3, 5, 7, 11 or 13 pass (15, 15, 14, 11, 13: whole butterflies;
+// the other prime passes take any count); at least 256 and a multiple of 64.  Returns -1
+// if more than 1024 threads would be needed.
 inline int fft_threads(const FftDesc &d, int64_t elems, int ept = kFftEPT) {
   int64_t per = ept;  // values a thread covers in every pass: whole butterflies
-  for (int i = 0; i < d.nr; ++i) per = std::min<int64_t>(per, (ept / d.r[i]) * d.r[i]);
+  for (int i = 0; i < d.nr; ++i) {
+    const int R = d.r[i];
+    // whole butterflies; a register-butterfly prime pass above ept runs one
+    // per thread (nt >= N/16 > N/R butterflies), the direct-sum pass any count
+    if (R <= 8 || R == 16 || (R <= 23 && R <= ept)) per = std::min<int64_t>(per, (ept / R) * R);
+  }
   int64_t t = (elems + per - 1) / per;
   t = ((t + 63) / 64) * 64;
   if (t > 1024) return -1;
   return (int)(t < 256 ? 256 : t);
 }
 
-// True if N has a factor 5 or 7 (its plans need the R57 transforms).
-inline bool fft_has57(int64_t N) { return N > 0 && (N % 5 == 0 || N % 7 == 0); }
+// True if N has a prime factor other than 2 and 3 (its plans need the R57
+// transforms: radix-5, radix-7 and generic prime passes).
+inline bool fft_general(int64_t N) {
+  if (N < 1) return false;
+  while (N % 2 == 0) N /= 2;
+  while (N % 3 == 0) N /= 3;
+  return N != 1;
+}
 
-// True if N = 2^a 3^b 5^c 7^d with N >= 1.
+// True if N >= 1 and every prime factor of N is at most kFftMaxPrime.
 inline bool fft_size_ok(int64_t N) {
   if (N < 1) return false;
-  for (int p : {2, 3, 5, 7})
+  for (int64_t p = 2; p <= kFftMaxPrime; ++p)
     while (N % p == 0) N /= p;
   return N == 1;
 }

@@ -430,8 +430,9 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   *plan = nullptr;
   TOMO_REQUIRE(n >= 20 && n % 2 == 0, "flt_create: n=%lld must be even and >= 20", (long long)n);
   TOMO_REQUIRE(ne >= n && ne % 2 == 0, "flt_create: ne=%lld must be even and >= n", (long long)ne);
-  TOMO_REQUIRE(fft_size_ok(ne) && ne <= 16384, "flt_create: ne=%lld must be 2^a 3^b 5^c 7^d <= 16384",
-               (long long)ne);
+  TOMO_REQUIRE(fft_size_ok(ne) && ne <= 16384,
+               "flt_create: ne=%lld must be <= 16384 with prime factors <= %d", (long long)ne,
+               kFftMaxPrime);
   TOMO_REQUIRE(nproj >= 1 && nz >= 1, "flt_create: nproj and nz must be >= 1");
   TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
                "flt_create: unknown dtype %d", dtype);
@@ -493,7 +494,7 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
 template <int MODE, typename T>
 static void launch_rows(tomo_flt_plan *P, void *g, const float2 *w, hipStream_t st) {
   const dim3 grid((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz);
-  if (fft_has57(P->ne))  // transforms with radix-5/7 passes
+  if (fft_general(P->ne))  // transforms with radix-5/7 and generic prime passes
     hipLaunchKernelGGL((k_flt_rows<MODE, T, true>), grid, dim3(P->threads), P->lds, st, (T *)g,
                        w, P->fft.d, (int)P->ne, (int)P->n, (int)P->nproj);
   else

@@ -45,7 +45,7 @@ constexpr int col_lg(int N) { return ilog2c(16384 / N) > 4 ? 4 : ilog2c(16384 / 
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
 
 // NS: n as a compile-time constant (0 = sizes known at run time only, -1 the
-// same for sizes with factors 5 or 7), as in
+// same for sizes with a prime factor above 3), as in
 // every FFT kernel of this file.
 // max over the workgroup of the threads' m; thread 0 gets the result
 // (LDS for the per-wave maxima; every thread must call it)
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft(const cpx<T> 
   if constexpr (NS > 0)
     lds_fft_c<NS, 1, kFftEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
   else
-    lds_fft<1, kColEPT, (NS < 0)>(sm, fd, lgCB, tid, nt);
+    lds_fft<1, (NS < 0 ? kFftEPT : kColEPT), (NS < 0)>(sm, fd, lgCB, tid, nt);
   const int z2 = c + nzc;
   const int h = n / 2;
   for (int idx = tid; idx < CB * n; idx += nt) {
@@ -975,7 +975,8 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   *plan = nullptr;
   TOMO_REQUIRE(n >= 20 && n % 2 == 0, "fr_create: n=%lld must be even and >= 20", (long long)n);
   TOMO_REQUIRE(n <= 4096, "fr_create: n=%lld exceeds the supported maximum 4096", (long long)n);
-  TOMO_REQUIRE(fft_size_ok(n), "fr_create: n=%lld must be of the form 2^a 3^b 5^c 7^d", (long long)n);
+  TOMO_REQUIRE(fft_size_ok(n), "fr_create: n=%lld must have no prime factor above %d",
+               (long long)n, kFftMaxPrime);
   TOMO_REQUIRE(nproj >= 1 && nproj <= (1 << 20), "fr_create: nproj=%lld out of range", (long long)nproj);
   TOMO_REQUIRE(nz >= 1 && nz <= (1 << 16), "fr_create: nz=%lld out of range", (long long)nz);
   TOMO_REQUIRE(theta != nullptr, "fr_create: theta is NULL");
@@ -1033,16 +1034,19 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   // column pass: CB columns of length 2n in LDS
   const size_t maxlds = max_lds_per_block();
   const size_t col1 = (size_t)lds_padded_len((int)n2) * sizeof(float2);
+  // the general-size column kernel (NS = -1) stages 16 values per thread: its
+  // radix-5/7 and prime passes spill at 32
+  const int colEPT = fft_general(n) ? kFftEPT : kColEPT;
   int CB = 1, lg = 0;
   while (CB * 2 <= 16 && (size_t)(CB * 2) * col1 <= maxlds &&
-         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2, kColEPT) > 0 &&
-         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2, kColEPT) <= 512)
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2, colEPT) > 0 &&
+         fft_threads(P->fft2.d, (int64_t)(CB * 2) * n2, colEPT) <= 512)
     CB *= 2, lg += 1;
   P->colCB = CB;
   P->lgCB = lg;
   P->colLds = (size_t)CB * col1;
   TOMO_REQUIRE(P->colLds <= maxlds, "fr_create: 2n=%lld exceeds the LDS of one workgroup", (long long)n2);
-  P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2, kColEPT);
+  P->colThreads = fft_threads(P->fft2.d, (int64_t)CB * n2, colEPT);
   P->prepThreads = fft_threads(P->fft1.d, n);
   P->rowThreads = fft_threads(P->fft2.d, n2);
   if (P->colThreads < 0 || P->prepThreads < 0 || P->rowThreads < 0 || P->colThreads > 512 ||
@@ -1064,7 +1068,7 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   P->nsPrep = fft_generic ? 0 : (int)n;
   with_static_size(P->nsPrep, [&](auto ns) { P->nsPrep = decltype(ns)::value; });
   if (P->nsRow == 0 || 2 * P->nsPrep != P->nsRow) P->nsPrep = 0;
-  if (fft_has57(n)) P->nsRow = P->nsCol = P->nsPrep = -1;  // transforms with 5 and 7 passes
+  if (fft_general(n)) P->nsRow = P->nsCol = P->nsPrep = -1;  // radix-5/7 and generic prime passes
   hipError_t ea = hipSuccess;
   auto lds_attr = [&](const void *k, size_t bytes) {
     if (ea == hipSuccess)
@@ -1256,9 +1260,9 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
     set_error("fr_forward: the forward projection is implemented for float32 plans only");
     return TOMO_EUNSUPPORTED;
   }
-  if (fft_has57(P->n)) {
-    set_error("fr_forward: n=%lld with a factor 5 or 7 is not supported by the forward "
-              "projection (backprojection is)", (long long)P->n);
+  if (fft_general(P->n)) {
+    set_error("fr_forward: n=%lld with a prime factor above 3 is not supported by the "
+              "forward projection (backprojection is)", (long long)P->n);
     return TOMO_EUNSUPPORTED;
   }
   float *data = (float *)data_;
