@@ -1,8 +1,8 @@
 // LDS-resident Stockham autosort FFT for sizes N <= 16384 whose prime factors
 // are at most kFftMaxPrime (per transform), executed cooperatively by a
 // workgroup on 2^lgH interleaved transforms held in LDS.  Radices 2, 3, 4, 5,
-// 7, 8, 11, 13, 16, 17, 19 and 23 run as butterflies in registers; any other
-// prime p runs as a direct length-p DFT per output (fft_pass_gen).
+// 7, 8, 11, 13, 16 and 17 run as butterflies in registers; any other prime p
+// runs as a direct length-p DFT per output (fft_pass_gen).
 //
 // Pass with radix R and current sub-transform length Ns (1, R1, R1*R2, ...):
 //   for j in [0, N/R):  k = j mod Ns
@@ -293,7 +293,7 @@ __device__ __forceinline__ void fft_pass(float2 *buf, int N, int Ns, const float
   __syncthreads();
 }
 
-// Radix-R pass for an odd prime 11 <= R <= 23 with the butterfly in
+// Radix-R pass for an odd prime 11 <= R <= 17 with the butterfly in
 // registers: inputs twiddled by W_N^(r k N/(Ns R)) from the table, then the
 // DFT from the cosine/sine pairs P_j = a_j + a_(R-j), M_j = a_j - a_(R-j):
 //   b_q, b_(R-q) = a_0 + sum_j cos(2 pi j q/R) P_j  -/+ i sum_j sin(2 pi j q/R) M_j
@@ -377,63 +377,41 @@ __device__ __noinline__ void fft_pass_prime(float2 *buf, int N, int Ns, const fl
   __syncthreads();
 }
 
-// Pass with a prime radix 29 <= R <= kFftMaxPrime:
-// each output y[o], o = jq Ns R + q Ns + k, j = jq Ns + k, is the direct sum
+// Pass with a prime radix 19 <= R <= kFftMaxPrime: each output y[o],
+// o = jq Ns R + q Ns + k, j = jq Ns + k, is the direct sum
 //   y[o] = sum_r x[j + r N/R] W_N^(r e mod N),  e = k N/(Ns R) + q N/R,
-// (the twiddle W_N^(r k N/(Ns R)) and the DFT_R factor W_R^(r q) in one
-// factor).  Four outputs run side by side; their factors advance by products
-// with W_N^e and are reloaded from the FP64-built table every fourth r (at
-// most three float32 complex products from a table entry).  At most EPT
-// outputs per thread are held in registers across the barrier.
+// (the twiddle W_N^(r k N/(Ns R)) and the DFT_R factor W_R^(r q) in one table
+// entry), at most EPT outputs per thread held in registers across the barrier.
 template <int DIR, int EPT>
 __device__ __noinline__ void fft_pass_gen(float2 *buf, int N, int Ns, int R, const float2 *tw,
                                           int lgH, int tid, int nt) {
-  constexpr int G = 4;
-  static_assert(EPT % G == 0, "EPT must be a multiple of 4");
   const int H = 1 << lgH;
   const int nbPer = N / R;
   const int total = N << lgH;
   const int twstep = N / (Ns * R);
-  auto ld = [&](int m) {
-    float2 w = tw[m];
-    if (DIR > 0) w.y = -w.y;
-    return w;
-  };
   float2 acc[EPT];
 #pragma unroll
-  for (int q0 = 0; q0 < EPT; q0 += G) {
-    int base[G], e[G], m[G];
-    float2 we[G], w[G], s[G];
-    bool live[G];
-#pragma unroll
-    for (int u = 0; u < G; ++u) {
-      const int g = tid + (q0 + u) * nt;
-      live[u] = g < total;
-      const int o = live[u] ? g >> lgH : 0, t = g & (H - 1);
+  for (int q = 0; q < EPT; ++q) {
+    const int g = tid + q * nt;
+    if (g < total) {
+      const int t = g & (H - 1), o = g >> lgH;
       const int k = o % Ns, rest = o / Ns;
       const int qq = rest % R, jq = rest / R;
-      base[u] = (jq * Ns + k) * H + t;  // unpadded interleaved index of x[j]
-      e[u] = k * twstep + qq * nbPer;   // < N
-      m[u] = 0;
-      we[u] = ld(e[u]);
-      w[u] = make_float2(1.f, 0.f);
-      s[u] = make_float2(0.f, 0.f);
-    }
-    for (int r = 0; r < R; ++r) {
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        if ((r & 3) == 0) w[u] = ld(m[u]);
-        const int idx = base[u] + r * nbPer * H;
-        const float2 x = buf[lds_pad(idx >> lgH) * H + (idx & (H - 1))];
-        s[u].x += x.x * w[u].x - x.y * w[u].y;
-        s[u].y += x.x * w[u].y + x.y * w[u].x;
-        w[u] = cmul(w[u], we[u]);
-        m[u] += e[u];
-        if (m[u] >= N) m[u] -= N;
+      const int j = jq * Ns + k;
+      const int e = k * twstep + qq * nbPer;  // < N
+      float2 s = make_float2(0.f, 0.f);
+      int m = 0;
+      for (int r = 0; r < R; ++r) {
+        const float2 x = buf[(lds_pad(j + r * nbPer) << lgH) + t];
+        float2 w = tw[m];
+        if (DIR > 0) w.y = -w.y;
+        s.x += x.x * w.x - x.y * w.y;
+        s.y += x.x * w.y + x.y * w.x;
+        m += e;
+        if (m >= N) m -= N;
       }
+      acc[q] = s;
     }
-#pragma unroll
-    for (int u = 0; u < G; ++u) acc[q0 + u] = s[u];
   }
   __syncthreads();
 #pragma unroll
@@ -444,45 +422,52 @@ __device__ __noinline__ void fft_pass_gen(float2 *buf, int N, int Ns, int R, con
   __syncthreads();
 }
 
+// fft_pass behind a call: the run-time-plan transform calls one per pass, so
+// the register demand of the largest radix does not add to the caller's.
+template <int R, int DIR, int EPT>
+__device__ __noinline__ void fft_pass_call(float2 *buf, int N, int Ns, const float2 *tw, int lgH,
+                                           int tid, int nt) {
+  fft_pass<R, DIR, EPT>(buf, N, Ns, tw, lgH, tid, nt);
+}
+
 // In-place transform of 2^lgH interleaved sequences of length d.N held in LDS:
 // element i of sequence t lives at buf[lds_pad(i) * 2^lgH + t].  Requires
 // 2^lgH * N <= 15 * nt (16 * nt without radix-3 passes); see fft_threads().
 // The caller must __syncthreads() after filling buf; the function ends with a
 // barrier.
-// R57: also radix-5, radix-7 and generic prime passes (kernels for such sizes
-// only: the extra passes raise the register demand of every instantiation).
-template <int DIR, int EPT = kFftEPT, bool R57 = false>
+// LV (fft_level of the sizes a kernel serves): 0 = radices 2, 3, 4, 8, 16;
+// 1 = also the butterflies of 5, 7, 11, 13, 17; 2 = also the direct-sum
+// primes 19..127.  Each level is compiled only into the kernels instantiated
+// for such sizes: the extra passes raise the register demand of every
+// instantiation.
+template <int DIR, int EPT = kFftEPT, int LV = 0>
 __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt) {
   int Ns = 1;
   for (int s = 0; s < d.nr; ++s) {
     const int R = d.r[s];
     if (R == 16) {
-      if constexpr (EPT >= 16) fft_pass<16, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      if constexpr (EPT >= 16) fft_pass_call<16, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     } else if (R == 8) {
-      fft_pass<8, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass_call<8, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     }
     else if (R == 4)
-      fft_pass<4, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass_call<4, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 2)
-      fft_pass<2, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+      fft_pass_call<2, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
     else if (R == 3)
-      fft_pass<3, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-    else if constexpr (R57) {
+      fft_pass_call<3, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+    else if constexpr (LV >= 1) {
       if (R == 5)
-        fft_pass<5, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+        fft_pass_call<5, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
       else if (R == 7)
-        fft_pass<7, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
+        fft_pass_call<7, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
       else if (R == 11)
         fft_pass_prime<11, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
       else if (R == 13)
         fft_pass_prime<13, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
       else if (R == 17)
         fft_pass_prime<17, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-      else if (R == 19)
-        fft_pass_prime<19, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-      else if (R == 23)
-        fft_pass_prime<23, DIR, EPT>(buf, d.N, Ns, d.tw, lgH, tid, nt);
-      else
+      else if constexpr (LV >= 2)
         fft_pass_gen<DIR, EPT>(buf, d.N, Ns, R, d.tw, lgH, tid, nt);
     }
     Ns *= R;
@@ -571,7 +556,8 @@ inline void with_static_size(int N, F &&f) {
     case 2048: f(std::integral_constant<int, 2048>()); break;
     case 4096: f(std::integral_constant<int, 4096>()); break;
     case 8192: f(std::integral_constant<int, 8192>()); break;
-    case -1: f(std::integral_constant<int, -1>()); break;  // run-time sizes with 5s or 7s
+    case -1: f(std::integral_constant<int, -1>()); break;  // run-time sizes, fft_level 1
+    case -2: f(std::integral_constant<int, -2>()); break;  // run-time sizes, fft_level 2
     default: f(std::integral_constant<int, 0>()); break;
   }
 }
@@ -617,7 +603,7 @@ inline int fft_threads(const FftDesc &d, int64_t elems, int ept = kFftEPT) {
     const int R = d.r[i];
     // whole butterflies; a register-butterfly prime pass above ept runs one
     // per thread (nt >= N/16 > N/R butterflies), the direct-sum pass any count
-    if (R <= 8 || R == 16 || (R <= 23 && R <= ept)) per = std::min<int64_t>(per, (ept / R) * R);
+    if (R <= 8 || R == 16 || (R <= 17 && R <= ept)) per = std::min<int64_t>(per, (ept / R) * R);
   }
   int64_t t = (elems + per - 1) / per;
   t = ((t + 63) / 64) * 64;
@@ -625,14 +611,24 @@ inline int fft_threads(const FftDesc &d, int64_t elems, int ept = kFftEPT) {
   return (int)(t < 256 ? 256 : t);
 }
 
-// True if N has a prime factor other than 2 and 3 (its plans need the R57
-// transforms: radix-5, radix-7 and generic prime passes).
-inline bool fft_general(int64_t N) {
-  if (N < 1) return false;
+// Transform level of a length N (the LV of lds_fft): 0 if N = 2^a 3^b, 1 if
+// its other prime factors are at most 17, 2 otherwise.
+inline int fft_level(int64_t N) {
+  if (N < 1) return 0;
   while (N % 2 == 0) N /= 2;
   while (N % 3 == 0) N /= 3;
-  return N != 1;
+  if (N == 1) return 0;
+  for (int64_t p : {5, 7, 11, 13, 17})
+    while (N % p == 0) N /= p;
+  return N == 1 ? 1 : 2;
 }
+
+// True if N has a prime factor other than 2 and 3 (its plans need level 1 or
+// 2 transforms).
+inline bool fft_general(int64_t N) { return fft_level(N) > 0; }
+
+// lds_fft level of the run-time-size kernels NS = -1 and NS = -2.
+constexpr int fft_lv(int NS) { return NS == -2 ? 2 : (NS == -1 ? 1 : 0); }
 
 // True if N >= 1 and every prime factor of N is at most kFftMaxPrime.
 inline bool fft_size_ok(int64_t N) {

@@ -42,7 +42,7 @@ __global__ void k_flt_weights(const float *__restrict__ wfilter, const float *__
 
 // mode 0: rows of width ne already padded (cfunc_filter::filter);
 // mode 1: rows of width n, edge-replicate padded on load and cropped on store
-template <int MODE, typename T, bool R57 = false>
+template <int MODE, typename T, int LV = 0>
 __global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const float2 *__restrict__ w,
                                                    FftDesc fd, int ne, int n, int nproj) {
   extern __shared__ float2 sm[];
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const floa
       },
       [&](int j, cpx<T> v) { sm[lds_pad(j)] = tof2(v); });
   __syncthreads();
-  lds_fft<-1, kFftEPT, R57>(sm, fd, 0, tid, nt);
+  lds_fft<-1, kFftEPT, LV>(sm, fd, 0, tid, nt);
   const float2 *wz = w + (size_t)z * (ne / 2 + 1);
   const int nh = ne / 2;
   for (int k = tid; k < ne; k += nt) {
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(1024) void k_flt_rows(T *__restrict__ g, const floa
     sm[lds_pad(k)] = cmul(sm[lds_pad(k)], W);
   }
   __syncthreads();
-  lds_fft<1, kFftEPT, R57>(sm, fd, 0, tid, nt);
+  lds_fft<1, kFftEPT, LV>(sm, fd, 0, tid, nt);
   if (MODE == 0) {
     for (int j = tid; j < ne; j += nt) {
       const float2 y = sm[lds_pad(j)];
@@ -476,9 +476,12 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
       (const void *)k_flt_rows<0, float>, (const void *)k_flt_rows<1, float>,
       (const void *)k_flt_rows<0, _Float16>, (const void *)k_flt_rows<1, _Float16>,
       (const void *)k_flt_rows<0, __bf16>, (const void *)k_flt_rows<1, __bf16>,
-      (const void *)k_flt_rows<0, float, true>, (const void *)k_flt_rows<1, float, true>,
-      (const void *)k_flt_rows<0, _Float16, true>, (const void *)k_flt_rows<1, _Float16, true>,
-      (const void *)k_flt_rows<0, __bf16, true>, (const void *)k_flt_rows<1, __bf16, true>};
+      (const void *)k_flt_rows<0, float, 1>, (const void *)k_flt_rows<1, float, 1>,
+      (const void *)k_flt_rows<0, _Float16, 1>, (const void *)k_flt_rows<1, _Float16, 1>,
+      (const void *)k_flt_rows<0, __bf16, 1>, (const void *)k_flt_rows<1, __bf16, 1>,
+      (const void *)k_flt_rows<0, float, 2>, (const void *)k_flt_rows<1, float, 2>,
+      (const void *)k_flt_rows<0, _Float16, 2>, (const void *)k_flt_rows<1, _Float16, 2>,
+      (const void *)k_flt_rows<0, __bf16, 2>, (const void *)k_flt_rows<1, __bf16, 2>};
   hipError_t e = hipSuccess;
   for (const void *k : rows_kernels)
     if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds);
@@ -494,9 +497,13 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
 template <int MODE, typename T>
 static void launch_rows(tomo_flt_plan *P, void *g, const float2 *w, hipStream_t st) {
   const dim3 grid((unsigned)ceil_div(P->nproj, 2), (unsigned)P->nz);
-  if (fft_general(P->ne))  // transforms with radix-5/7 and generic prime passes
-    hipLaunchKernelGGL((k_flt_rows<MODE, T, true>), grid, dim3(P->threads), P->lds, st, (T *)g,
-                       w, P->fft.d, (int)P->ne, (int)P->n, (int)P->nproj);
+  const int lv = fft_level(P->ne);  // transforms with the level-1 or level-2 passes
+  if (lv == 2)
+    hipLaunchKernelGGL((k_flt_rows<MODE, T, 2>), grid, dim3(P->threads), P->lds, st, (T *)g, w,
+                       P->fft.d, (int)P->ne, (int)P->n, (int)P->nproj);
+  else if (lv == 1)
+    hipLaunchKernelGGL((k_flt_rows<MODE, T, 1>), grid, dim3(P->threads), P->lds, st, (T *)g, w,
+                       P->fft.d, (int)P->ne, (int)P->n, (int)P->nproj);
   else
     hipLaunchKernelGGL((k_flt_rows<MODE, T>), grid, dim3(P->threads), P->lds, st, (T *)g, w,
                        P->fft.d, (int)P->ne, (int)P->n, (int)P->nproj);

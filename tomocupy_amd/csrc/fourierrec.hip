@@ -44,8 +44,8 @@ constexpr int col_lg(int N) { return ilog2c(16384 / N) > 4 ? 4 : ilog2c(16384 / 
 // Sign 1 - 2*((t+1) % 2): -1 for even t, +1 for odd t (kernels_fourierrec.cuh:121-132).
 __device__ __forceinline__ float shift_sign(int t) { return (t & 1) ? 1.f : -1.f; }
 
-// NS: n as a compile-time constant (0 = sizes known at run time only, -1 the
-// same for sizes with a prime factor above 3), as in
+// NS: n as a compile-time constant (0 = sizes known at run time only, -1 and
+// -2 the same for sizes of fft_level 1 and 2), as in
 // every FFT kernel of this file.
 // max over the workgroup of the threads' m; thread 0 gets the result
 // (LDS for the per-wave maxima; every thread must call it)
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx
   if constexpr (NS > 0)
     lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd.tw, tid, nt);
   else
-    lds_fft<-1, kFftEPT, (NS < 0)>(sm, fd, 0, tid, nt);
+    lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
   store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
 }
 
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   if constexpr (NS > 0)
     lds_fft_c<2 * NS, -1, kFftEPT, 0>(sm, fd2.tw, tid, nt);
   else
-    lds_fft<-1, kFftEPT, (NS < 0)>(sm, fd2, 0, tid, nt);
+    lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd2, 0, tid, nt);
   const float2 *Ha = H2 + (size_t)c * n2;
   const float2 *Hb = H2 + (size_t)zb * n2;
   for (int k = tid; k <= n; k += nt) {
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   if constexpr (NS > 0)
     lds_fft_c<2 * NS, 1, kFftEPT, 0>(sm, fd2.tw, tid, nt);
   else
-    lds_fft<1, kFftEPT, (NS < 0)>(sm, fd2, 0, tid, nt);
+    lds_fft<1, kFftEPT, fft_lv(NS)>(sm, fd2, 0, tid, nt);
   const float *hl1 = HL + (size_t)c * n, *hr1 = HR + (size_t)c * n;
   const float *hl2 = HL + (size_t)zb * n, *hr2 = HR + (size_t)zb * n;
   const float e0 = edge[0], e1 = edge[1], e2 = edge[2], e3 = edge[3];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   if constexpr (NS > 0)
     lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd1.tw, tid, nt);
   else
-    lds_fft<-1, kFftEPT, (NS < 0)>(sm, fd1, 0, tid, nt);
+    lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd1, 0, tid, nt);
   store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
 }
 
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
   if constexpr (NS > 0)
     lds_fft_c<NS, 1, kFftEPT, 0>(sm, fd.tw, tid, nt);
   else
-    lds_fft<1, kFftEPT, (NS < 0)>(sm, fd, 0, tid, nt);
+    lds_fft<1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
   const int h = n / 2;
   if (lgB > 0) {
     const int B = 1 << lgB;
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft(const cpx<T> 
   if constexpr (NS > 0)
     lds_fft_c<NS, 1, kFftEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
   else
-    lds_fft<1, (NS < 0 ? kFftEPT : kColEPT), (NS < 0)>(sm, fd, lgCB, tid, nt);
+    lds_fft<1, (NS < 0 ? kFftEPT : kColEPT), fft_lv(NS)>(sm, fd, lgCB, tid, nt);
   const int z2 = c + nzc;
   const int h = n / 2;
   for (int idx = tid; idx < CB * n; idx += nt) {
@@ -1068,7 +1068,8 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   P->nsPrep = fft_generic ? 0 : (int)n;
   with_static_size(P->nsPrep, [&](auto ns) { P->nsPrep = decltype(ns)::value; });
   if (P->nsRow == 0 || 2 * P->nsPrep != P->nsRow) P->nsPrep = 0;
-  if (fft_general(n)) P->nsRow = P->nsCol = P->nsPrep = -1;  // radix-5/7 and generic prime passes
+  if (fft_general(n))  // run-time kernels with the level-1 or level-2 passes
+    P->nsRow = P->nsCol = P->nsPrep = fft_level(n) == 2 ? -2 : -1;
   hipError_t ea = hipSuccess;
   auto lds_attr = [&](const void *k, size_t bytes) {
     if (ea == hipSuccess)
