@@ -145,8 +145,8 @@ int fr_filter_backprojection_prepared(tomo_fr_plan *plan, tomo_flt_plan *flt, vo
  * sum(fr_backprojection(d) * o) == sum(d * fr_forward(o)) to rounding.
  *   data  output, device float32 (nz, nproj, n)
  *   obj   input, device float32 (nz, n, n); NOT modified
- * float32 plans with n = 2^a 3^b only (TOMO_EUNSUPPORTED for 16-bit plans and
- * for n with a prime factor above 3).
+ * float32 plans only (TOMO_EUNSUPPORTED for 16-bit plans); every n that
+ * fr_create accepts.
  */
 int fr_forward(tomo_fr_plan *plan, void *data, const void *obj, void *stream);
 

@@ -627,6 +627,17 @@ inline int fft_level(int64_t N) {
 // 2 transforms).
 inline bool fft_general(int64_t N) { return fft_level(N) > 0; }
 
+// f(std::integral_constant<int, L>()) for the transform level L in 0..2.
+template <typename F>
+inline void with_fft_level(int L, F &&f) {
+  if (L == 2)
+    f(std::integral_constant<int, 2>());
+  else if (L == 1)
+    f(std::integral_constant<int, 1>());
+  else
+    f(std::integral_constant<int, 0>());
+}
+
 // lds_fft level of the run-time-size kernels NS = -1 and NS = -2.
 constexpr int fft_lv(int NS) { return NS == -2 ? 2 : (NS == -1 ? 1 : 0); }
 

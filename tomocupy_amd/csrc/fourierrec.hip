@@ -643,6 +643,7 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft(const cpx<T> 
 
 // ---- adjoint (forward projection) kernels --------------------------------
 
+template <int LV>
 __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
                                 const float *__restrict__ phix, const float *__restrict__ phiy,
                                 const int2 *__restrict__ circ, FftDesc fd, int n, int nz,
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
         sm[(lds_pad(u) << lgCB) + t] = make_float2(v.x * ph, v.y * ph);
       });
   __syncthreads();
-  lds_fft<-1, kColEPT>(sm, fd, lgCB, tid, nt);
+  lds_fft<-1, (LV > 0 ? kFftEPT : kColEPT), LV>(sm, fd, lgCB, tid, nt);
   float2 *dst = R + (size_t)c * n2 * n;
   for (int idx = tid; idx < CB * n2; idx += nt) {
     const int a = idx >> lgCB, t = idx & (CB - 1);
@@ -686,6 +687,7 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
   }
 }
 
+template <int LV>
 __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict__ R, float2 *__restrict__ F, FftDesc fd,
                                 int n, int nzc) {
   extern __shared__ float2 sm[];
@@ -699,7 +701,7 @@ __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict_
       [&](int v) { return (v >= h && v < h + n) ? src[v - h] : make_float2(0.f, 0.f); },
       [&](int v, float2 x) { sm[lds_pad(v)] = (v & 1) ? make_float2(-x.x, -x.y) : x; });
   __syncthreads();
-  lds_fft<-1>(sm, fd, 0, tid, nt);
+  lds_fft<-1, kFftEPT, LV>(sm, fd, 0, tid, nt);
   float2 *dst = F + ((size_t)c * n2 + a) * n2;
   for (int b = tid; b < n2; b += nt) {
     const float2 x = sm[lds_pad(b)];
@@ -753,6 +755,7 @@ __global__ void k_fr_interp(const float4 *__restrict__ geom, const float2 *__res
     if (c0 + c < nzc) G[(size_t)(c0 + c) * nsamp + s] = acc[c];
 }
 
+template <int LV>
 __global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ data, FftDesc fd,
                               int n, int nproj, int nz, int nzc, float scale) {
   extern __shared__ float2 sm[];
@@ -764,7 +767,7 @@ __global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ 
       n, tid, nt, [&](int k) { return in[k]; },
       [&](int k, float2 v) { sm[lds_pad(k)] = cscale(v, shift_sign(k) * scale); });
   __syncthreads();
-  lds_fft<1>(sm, fd, 0, tid, nt);
+  lds_fft<1, kFftEPT, LV>(sm, fd, 0, tid, nt);
   const int z2 = c + nzc;
   float *r1 = data + ((size_t)c * nproj + p) * n;
   float *r2 = data + ((size_t)z2 * nproj + p) * n;
@@ -1085,8 +1088,10 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
     set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
     return fail(TOMO_EHIP);
   }
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_colfft_adj,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->colLds));
+  with_fft_level((int)fft_level(n), [&](auto lv) {
+    constexpr int LV = decltype(lv)::value;
+    lds_attr((const void *)k_fr_colfft_adj<LV>, P->colLds);
+  });
   const int rowLds = (int)(lds_padded_len((int)n2) * sizeof(float2));
   P->rowLds = rowLds;
   P->prepLds = (int)(lds_padded_len((int)n) * sizeof(float2));
@@ -1100,8 +1105,14 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
     set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
     return fail(TOMO_EHIP);
   }
-  TOMO_HIP(hipFuncSetAttribute((const void *)k_fr_rowfft_adj,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, rowLds));
+  with_fft_level((int)fft_level(n), [&](auto lv) {
+    constexpr int LV = decltype(lv)::value;
+    lds_attr((const void *)k_fr_rowfft_adj<LV>, rowLds);
+  });
+  if (ea != hipSuccess) {
+    set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
+    return fail(TOMO_EHIP);
+  }
   *plan = P;
   return TOMO_OK;
 }
@@ -1261,28 +1272,31 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
     set_error("fr_forward: the forward projection is implemented for float32 plans only");
     return TOMO_EUNSUPPORTED;
   }
-  if (fft_general(P->n)) {
-    set_error("fr_forward: n=%lld with a prime factor above 3 is not supported by the "
-              "forward projection (backprojection is)", (long long)P->n);
-    return TOMO_EUNSUPPORTED;
-  }
   float *data = (float *)data_;
   const float *obj = (const float *)obj_;
   hipStream_t st = (hipStream_t)stream;
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   const int n2 = 2 * n;
+  const int level = fft_level(n);
   {
     ProfScope ps_("k_fr_colfft_adj", st);
-    hipLaunchKernelGGL(k_fr_colfft_adj, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
-                       dim3(P->colThreads), P->colLds, st, obj, P->R.as<float2>(), P->phix.as<float>(),
-                       P->phiy.as<float>(), P->circ.as<int2>(), P->fft2.d, n, nz, nzc, P->lgCB);
+    with_fft_level(level, [&](auto lv) {
+      constexpr int LV = decltype(lv)::value;
+      hipLaunchKernelGGL(k_fr_colfft_adj<LV>, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
+                         dim3(P->colThreads), P->colLds, st, obj, P->R.as<float2>(),
+                         P->phix.as<float>(), P->phiy.as<float>(), P->circ.as<int2>(),
+                         P->fft2.d, n, nz, nzc, P->lgCB);
+    });
   }
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_rowfft_adj", st);
-    hipLaunchKernelGGL(k_fr_rowfft_adj, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds,
-                       st, P->R.as<float2>(), P->F.as<float2>(), P->fft2.d, n, nzc);
+    with_fft_level(level, [&](auto lv) {
+      constexpr int LV = decltype(lv)::value;
+      hipLaunchKernelGGL(k_fr_rowfft_adj<LV>, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds,
+                         st, P->R.as<float2>(), P->F.as<float2>(), P->fft2.d, n, nzc);
+    });
   }
   TOMO_LAUNCH_CHECK();
   {
@@ -1294,8 +1308,12 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_prep_adj", st);
-    hipLaunchKernelGGL(k_fr_prep_adj, dim3(nproj, ng * 4), dim3(P->prepThreads), P->prepLds, st,
-                       P->G.as<float2>(), data, P->fft1.d, n, nproj, nz, nzc, P->scale1);
+    with_fft_level(level, [&](auto lv) {
+      constexpr int LV = decltype(lv)::value;
+      hipLaunchKernelGGL(k_fr_prep_adj<LV>, dim3(nproj, ng * 4), dim3(P->prepThreads),
+                         P->prepLds, st, P->G.as<float2>(), data, P->fft1.d, n, nproj, nz, nzc,
+                         P->scale1);
+    });
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
