@@ -4,7 +4,9 @@
   <= 1e-5; the chunking pairs slices within a chunk of nsino-per-chunk, as the
   conveyor does);
 * try: the TIFF of each trial centre against the oracle reconstructed at that
-  centre (<= 1e-5).
+  centre (<= 1e-5);
+* a width with a prime factor 17 through fw stripe removal and the fused
+  filter + backprojection (<= 1e-5).
 """
 import os
 
@@ -190,5 +192,26 @@ def test_cli_full_projection_and_column_selection(tmp_path):
         got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k)).astype(np.float64)
         assert got.shape == (n, n)
         err = _rel(got, ref[k])
+        print('slice', k, err)
+        assert err <= 1e-5
+
+
+def test_cli_full_width_with_prime_17(tmp_path):
+    # a detector width with a prime factor 17 (as 2448 = 2^4 3^2 17) through
+    # recon with fw stripe removal: raw counts -> fw -> fused filter +
+    # backprojection, every slice against the oracle chain
+    path, data, dark, flat, theta = _dataset(tmp_path, n=136, nproj=60, nz=4)
+    out = str(tmp_path / 'rec')
+    main(['recon', '--file-name', path, '--reconstruction-type', 'full', '--nsino-per-chunk', '4',
+          '--out-path-name', out, '--remove-stripe-method', 'fw', '--fw-level', '3',
+          '--rotation-axis', '67.5'])
+    n, nz = data.shape[2], data.shape[1]
+    ne = 4 * n
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    sino = pp.proc_chain(data, dark, flat, stripe='fw', fw_level=3)
+    ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne, 67.5), th)
+    for k in range(nz):
+        got = io.read_tiff(os.path.join(out, 'recon_%05d.tiff' % k))
+        err = _rel(got.astype(np.float64), ref[k])
         print('slice', k, err)
         assert err <= 1e-5
