@@ -90,3 +90,18 @@ def test_calc_filter_matches_oracle(name, ne):
 def test_calc_filter_rejects_unknown():
     with pytest.raises(_lib.TomoError):
         _lib.calc_filter(256, 'bogus')
+
+
+def test_rotate_proj_validation():
+    # argument checks of pre_rotate_proj run before any HIP call
+    L = _lib.lib()
+    a, b, w = ctypes.c_void_p(16), ctypes.c_void_p(4096), ctypes.c_void_p(1 << 20)
+    assert L.pre_rotate_proj(a, b, 1, 8, 8, 3.0, 6, w, None) == _lib.TOMO_EINVAL
+    assert 'order' in _lib.last_error()
+    assert L.pre_rotate_proj(a, b, 1, 8, 8, 3.0, -1, w, None) == _lib.TOMO_EINVAL
+    assert L.pre_rotate_proj(a, b, 1, 8, 8, 3.0, 3, None, None) == _lib.TOMO_EINVAL
+    assert 'work' in _lib.last_error()
+    assert L.pre_rotate_proj(a, b, 1, 8, 8, 3.0, 2, a, None) == _lib.TOMO_EINVAL
+    assert L.pre_rotate_proj(a, b, 1, 8, 8, 3.0, 5, b, None) == _lib.TOMO_EINVAL
+    assert L.pre_rotate_proj(a, a, 1, 8, 8, 3.0, 1, None, None) == _lib.TOMO_EINVAL
+    assert L.pre_rotate_proj(a, b, 1, 8, 8, float('nan'), 1, None, None) == _lib.TOMO_EINVAL
