@@ -55,6 +55,10 @@ def parse():
     ap.add_argument('--vo-all-dim', type=int, default=1, choices=[1, 2])
     ap.add_argument('--phase-batch', type=int, default=1,
                     help='steps form: projections per phase-retrieval launch')
+    ap.add_argument('--rotate-proj-angle', type=float, default=0.0,
+                    help='steps form: rotation of the projections in degrees')
+    ap.add_argument('--rotate-proj-order', type=int, default=1,
+                    help='steps form: interpolation order of the rotation (0..5)')
     ap.add_argument('--filter', default='ramp')
     ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
     ap.add_argument('--distinct', type=int, default=8,
@@ -341,7 +345,8 @@ def bench_steps(a):
     rs = GPURecSteps(a.n, a.nproj, a.nz, a.chunk, torch.from_numpy(theta).cuda(), center,
                      dark.shape[0], flat.shape[0], a.dtype, a.filter, nproj_per_chunk=8,
                      retrieve_phase_method='paganin', remove_stripe_method=a.stripe,
-                     phase_batch=a.phase_batch, **kw)
+                     phase_batch=a.phase_batch, rotate_proj_angle=a.rotate_proj_angle,
+                     rotate_proj_order=a.rotate_proj_order, **kw)
     torch.cuda.synchronize()
     print('# resident scan and plans in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
     for _ in range(a.warmup):
@@ -373,8 +378,11 @@ def bench_steps(a):
             'data': 'synthetic ellipse-phantom counts tiled over nz, resident in HBM',
             'config': {'workload': 'recon_steps form: proc_sino (stripe=%s) by chunks of %d rows, '
                                    'Paganin (alpha 1e-4, 60 mm, 20 keV, 1.17 um) by chunks of 8 '
-                                   'projections, %s filter + backprojection'
-                                   % (a.stripe, a.chunk, a.filter),
+                                   'projections, %s%s filter + backprojection'
+                                   % (a.stripe, a.chunk,
+                                      ('rotation %g deg order %d, ' % (a.rotate_proj_angle,
+                                                                        a.rotate_proj_order)
+                                       if a.rotate_proj_angle else ''), a.filter),
                        'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk},
             'kernels_untimed_breakdown_step': kern}
     rs.free()
