@@ -234,20 +234,15 @@ def bench_batch(a):
     import torch
     from tomocupy_amd import synth
     from tomocupy_amd.batch import BatchRec
-    rng = np.random.default_rng(2024)
-    sizes = [256, 512, 768, 1024, 1536, 2048]
-    reqs = []
+    sizes = list(synth.BATCH_SIZES)
     t0 = time.time()
     cache = {}
-    for k in range(a.requests):
-        n = int(rng.choice(sizes))
-        nproj = int(round(1.5 * n))
-        th0 = rng.uniform(0, np.pi)
-        theta = (th0 + np.arange(nproj) * np.pi / nproj).astype(np.float32)
-        nz = int(rng.integers(1, 33))
+    reqs = synth.batch_requests(a.requests, 2024)
+    for r in reqs:
+        n = r['n']
         if n not in cache:
-            cache[n] = synth.sinograms(4, n, synth.angles(nproj), 1)[0].astype(np.float32)
-        reqs.append({'n': n, 'nz': nz, 'theta': theta, 'center': n / 2, 'sino': cache[n]})
+            cache[n] = synth.sinograms(4, n, synth.angles(len(r['theta'])), 1)[0].astype(np.float32)
+        r['sino'] = cache[n]
     print('# %d requests generated in %.1fs' % (len(reqs), time.time() - t0), file=sys.stderr,
           flush=True)
     opts = {}

@@ -15,9 +15,38 @@ detector-constant and angle-independent sinograms, an off-centre point source)
 and by the swap symmetry of the pair packing (``tests/test_oracle_fourierrec.py``).
 """
 
+import ctypes
+import os
+
 import numpy as np
 
 EPS = 1e-3
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_CLIB = None
+
+
+def _clib():
+    """The C gridding of ``grid_fp64.c`` (``libgrid_fp64.so``, built by
+    ``oracle/Makefile``), or None when it is not built: the NumPy loops below
+    then do the same FP64 arithmetic, more slowly."""
+    global _CLIB
+    if _CLIB is None:
+        path = os.path.join(_HERE, 'libgrid_fp64.so')
+        _CLIB = False
+        if os.environ.get('ORACLE_NUMPY_GRID') != '1' and os.path.exists(path):
+            lib = ctypes.CDLL(path)
+            dp = ctypes.POINTER(ctypes.c_double)
+            i64 = ctypes.c_int64
+            for f in (lib.grid_scatter, lib.grid_interp):
+                f.argtypes = [dp, dp, i64, i64, i64, ctypes.c_int, ctypes.c_double, dp]
+                f.restype = None
+            _CLIB = lib
+    return _CLIB or None
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 def params(n):
@@ -109,12 +138,22 @@ def backprojection(data, theta, chunk=64):
     G = s * np.fft.fft(s * g, axis=-1)
 
     # gather (K:60-119): scatter-add of each sample onto its 9x9 footprint
-    x, y = takexy(theta, n)
-    b0, b1, e0, e1 = gather_weights(x, y, n, m, mu)
-    c0 = np.pi / (mu * (4.0 * n * n))
-    v = G * (4.0 / n)
     fde = np.zeros((nzc, S * S), dtype=np.complex128)
-    for p0 in range(0, nproj, chunk):
+    lib = _clib()
+    if lib is not None:
+        Gc = np.ascontiguousarray(G, dtype=np.complex128)
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
+        lib.grid_scatter(_dptr(Gc.view(np.float64)), _dptr(th), nzc, nproj, n, m, mu,
+                         _dptr(fde.view(np.float64)))
+        chunk = nproj  # the loop below then has nothing left to do
+        p_start = nproj
+    else:
+        x, y = takexy(theta, n)
+        b0, b1, e0, e1 = gather_weights(x, y, n, m, mu)
+        c0 = np.pi / (mu * (4.0 * n * n))
+        v = G * (4.0 / n)
+        p_start = 0
+    for p0 in range(p_start, nproj, chunk):
         p1 = min(nproj, p0 + chunk)
         rows = (n + m + b1[p0:p1])[..., None, None] + np.arange(2 * m + 1)[:, None]
         cols = (n + m + b0[p0:p1])[..., None, None] + np.arange(2 * m + 1)[None, :]
@@ -207,11 +246,20 @@ def forward(obj, theta, nproj=None, chunk=64):
     fde[:, rr, qq] = fde[:, r0, q0]
     fde = fde.reshape(nzc, S * S)
     # gather^T: interpolate the grid at the polar samples with the same weights
-    x, y = takexy(theta, n)
-    b0, b1, e0, e1 = gather_weights(x, y, n, m, mu)
-    c0 = np.pi / (mu * (4.0 * n * n))
     G = np.zeros((nzc, nproj, n), dtype=np.complex128)
-    for p0 in range(0, nproj, chunk):
+    lib = _clib()
+    if lib is not None:
+        Fc = np.ascontiguousarray(fde)
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
+        lib.grid_interp(_dptr(Fc.view(np.float64)), _dptr(th), nzc, nproj, n, m, mu,
+                        _dptr(G.view(np.float64)))
+        p_start = nproj
+    else:
+        x, y = takexy(theta, n)
+        b0, b1, e0, e1 = gather_weights(x, y, n, m, mu)
+        c0 = np.pi / (mu * (4.0 * n * n))
+        p_start = 0
+    for p0 in range(p_start, nproj, chunk):
         p1 = min(nproj, p0 + chunk)
         rows = (n + m + b1[p0:p1])[..., None, None] + np.arange(2 * m + 1)[:, None]
         cols = (n + m + b0[p0:p1])[..., None, None] + np.arange(2 * m + 1)[None, :]

@@ -141,3 +141,26 @@ def test_ellipse_phantom_loose():
     rec = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'ramp'), ne, n / 2), theta)[0]
     img = np.roll(synth.phantom_image(ell, n) * fr.circ_mask(n), -1, axis=0)
     assert np.linalg.norm(rec - img) / np.linalg.norm(img) < 0.35
+
+
+@pytest.mark.parametrize("n,nproj,nz", [(64, 40, 3), (96, 50, 4), (130, 33, 2)])
+def test_c_gridding_equals_numpy_gridding(n, nproj, nz, monkeypatch):
+    """The C gridding (``grid_fp64.c``, the fast path of the oracle at large n)
+    against the NumPy scatter/gather it replaces, which the checks above pin:
+    same FP64 arithmetic, so agreement to summation-order rounding."""
+    import os
+    if not os.path.exists(os.path.join(os.path.dirname(fr.__file__), 'libgrid_fp64.so')):
+        pytest.skip('oracle C gridding not built (make -C oracle)')
+    rng = np.random.default_rng(n)
+    theta = _theta(nproj, 0.3)
+    d = rng.standard_normal((nz, nproj, n))
+    o = rng.standard_normal((nz, n, n))
+    monkeypatch.setattr(fr, '_CLIB', None)
+    monkeypatch.setenv('ORACLE_NUMPY_GRID', '1')
+    a, fa = fr.backprojection(d, theta), fr.forward(o, theta)
+    monkeypatch.setattr(fr, '_CLIB', None)
+    monkeypatch.setenv('ORACLE_NUMPY_GRID', '0')
+    assert fr._clib() is not None
+    b, fb = fr.backprojection(d, theta), fr.forward(o, theta)
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(a).max()
+    assert np.abs(fa - fb).max() <= 1e-14 * np.abs(fa).max()

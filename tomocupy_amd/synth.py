@@ -100,3 +100,23 @@ def raw_counts(cfg, n, theta, nz, ndark=10, nflat=20, stripes=False, z0=0, atten
         flat[:, z] = np.clip(rng.poisson(np.broadcast_to(gain + dk, (nflat, n))), 0, 65535)
         dark[:, z] = np.clip(rng.poisson(np.broadcast_to(dk, (ndark, n))), 0, 65535)
     return data, dark, flat, sino
+
+
+BATCH_SIZES = (256, 512, 768, 1024, 1536, 2048)
+
+
+def batch_requests(count=96, seed=2024, sizes=BATCH_SIZES):
+    """The batching form's mixed request list: per request ``n`` drawn from
+    ``sizes``, ``nproj = round(1.5 n)`` angles ``theta0 + p pi/nproj`` from a
+    random start ``theta0`` in [0, pi), ``nz`` in [1, 32], centre ``n/2``.
+    Returns dicts with ``n``, ``nz``, ``theta`` (float32) and ``center``."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(count):
+        n = int(rng.choice(sizes))
+        nproj = int(round(1.5 * n))
+        th0 = rng.uniform(0, np.pi)
+        theta = (th0 + np.arange(nproj) * np.pi / nproj).astype(np.float32)
+        nz = int(rng.integers(1, 33))
+        reqs.append({'n': n, 'nz': nz, 'theta': theta, 'center': n / 2})
+    return reqs
