@@ -66,6 +66,12 @@ def parse():
     ap.add_argument('--cpu-baseline', default='on', choices=['on', 'off'])
     ap.add_argument('--cpu-workers', type=int, default=0, help='0 = min(16, cpu count)')
     ap.add_argument('--no-prof', action='store_true', help='do not time kernels with events')
+    ap.add_argument('--half', default='float16', choices=['float16', 'bfloat16', 'off'],
+                    help='pipeline form at --dtype float32: also time the 16-bit storage form '
+                         'on the same workload (the "half" object)')
+    ap.add_argument('--conveyor', default='on', choices=['on', 'off'],
+                    help='pipeline form: also time GPURec.recon_all over host memory (H2D, '
+                         'D2H through pinned buffers)')
     ap.add_argument('--form', default='pipeline', choices=['pipeline', 'batch', 'cg', 'steps'],
                     help='pipeline: the chunked reconstruction (default); batch: 96 independent '
                          'reconstructions of mixed sizes as one HIP graph vs eager launches; '
@@ -178,8 +184,26 @@ def device_data(a, torch, data, dark, flat):
     return raw, dk, fl
 
 
+ACC_PAIRS = 2  # slice pairs of chunk 0 whose oracle outputs are kept for the accuracy check
+
+
+def _cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return ''
+
+
 def cpu_baseline(a, data, dark, flat, theta, center):
-    """FP64 oracle chain on the host: a process pool over complex slice pairs."""
+    """FP64 oracle chain on the host: a process pool over slice pairs (one
+    pair per worker per round), one small warm-up round, then the median of
+    three timed rounds; plus one pair in a single process with every core
+    (the oracle's C gridding is OpenMP-parallel).  Returns the baseline
+    object and the oracle reconstructions of the first ACC_PAIRS pairs
+    (slices 0..2*ACC_PAIRS-1 of the chunk), float32, for the accuracy check."""
     import multiprocessing as mp
     workers = a.cpu_workers or min(16, os.cpu_count() or 1)
     pairs = workers
@@ -188,41 +212,104 @@ def cpu_baseline(a, data, dark, flat, theta, center):
         z = (2 * i) % data.shape[1]
         idx = [z, (z + 1) % data.shape[1]]
         sl.append((data[:, idx], dark[:, idx], flat[:, idx]))
-    args = [(s, theta, center, a) for s in sl]
+    args = [(s, theta, center, a, i < ACC_PAIRS, 1) for i, s in enumerate(sl)]
+    # warm-up: module imports and the C gridding library, on a tiny problem
+    small = synth_small(a)
     # runs before this process initialises the GPU, so forked workers are plain CPU processes
     ctx = mp.get_context('fork')
-    t0 = time.time()
+    times, outs = [], None
     with ctx.Pool(workers) as pool:
-        pool.map(_cpu_pair, args)
-    dt = time.time() - t0
-    model = ''
-    try:
-        for line in open('/proc/cpuinfo'):
-            if line.startswith('model name'):
-                model = line.split(':', 1)[1].strip()
-                break
-    except OSError:
-        pass
+        pool.map(_cpu_pair, [small] * workers)
+        for r in range(3):
+            t0 = time.time()
+            res = pool.map(_cpu_pair, args)
+            times.append(time.time() - t0)
+            if outs is None:
+                outs = [o for o in res if o is not None]
+            print('# cpu baseline round %d: %.1fs' % (r, times[-1]), file=sys.stderr, flush=True)
+    dt = float(np.median(times))
+    with ctx.Pool(1) as pool:
+        pool.map(_cpu_pair, [small])
+        t0 = time.time()
+        pool.map(_cpu_pair, [(sl[0], theta, center, a, False, os.cpu_count() or 1)])
+        dt1 = time.time() - t0
+    oracle = np.concatenate(outs) if outs else None
     return {'value': round(2 * pairs / dt, 3), 'unit': 'slices/s', 'cores': workers,
-            'kind': 'port', 'host': {'cpu_count': os.cpu_count(), 'model': model},
-            'sample': '%d complex slice pairs (%d slices) of the same workload through the FP64 '
+            'kind': 'port', 'host': {'cpu_count': os.cpu_count(), 'model': _cpu_model()},
+            'rounds_s': [round(t, 2) for t in times],
+            'single_process': {'value': round(2 / dt1, 3), 'threads': os.cpu_count(),
+                               'note': 'one pair in one process, OpenMP gridding on every '
+                                       'thread'},
+            'sample': '%d slice pairs (%d slices) of the same workload through the FP64 '
                       'oracle (dark/flat, %s-log, %s filter, USFFT backprojection), one pair '
-                      'per process, %.1fs wall' % (pairs, 2 * pairs,
-                                                   ('%s stripe removal, ' % a.stripe) if a.stripe != 'none' else '',
-                                                   a.filter, dt)}
+                      'per process per round; warm-up round on a tiny problem, median of 3 '
+                      'rounds (%.1fs)' % (pairs, 2 * pairs,
+                                          ('%s stripe removal, ' % a.stripe) if a.stripe != 'none' else '',
+                                          a.filter, dt)}, oracle
+
+
+def synth_small(a):
+    from tomocupy_amd import synth
+    th = synth.angles(24)
+    d, dk, fl, _ = synth.raw_counts(1, 64, th, 2, ndark=2, nflat=2)
+    return ((d, dk, fl), th, 32.0, a, False, 1)
 
 
 def _cpu_pair(arg):
-    (data, dark, flat), theta, center, a = arg
-    os.environ['OMP_NUM_THREADS'] = '1'
+    (data, dark, flat), theta, center, a, keep, threads = arg
+    os.environ['OMP_NUM_THREADS'] = str(threads)
     from oracle import filter_fp64 as ff
     from oracle import fourierrec_fp64 as fr
     from oracle import preproc_fp64 as pp
     sino = pp.proc_chain(data, dark, flat, stripe=a.stripe)
-    ne = 4 * a.n
+    n = data.shape[-1]
+    ne = 4 * n
     filt = ff.fbp_filter_center(sino, ff.calc_filter(ne, a.filter), ne, center)
-    fr.backprojection(filt, theta)
-    return 0
+    rec = fr.backprojection(filt, theta)
+    return rec.astype(np.float32) if keep else None
+
+
+def rel_l2(got, ref):
+    """Per-slice relative L2 error and max |error| / max |ref| (FP64)."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    num = np.sqrt(((got - ref) ** 2).reshape(len(ref), -1).sum(1))
+    den = np.sqrt((ref ** 2).reshape(len(ref), -1).sum(1))
+    return num / np.maximum(den, 1e-300), float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+def accuracy(a, torch, rec, oracle, pf, bf, raw, dark, flat, theta, center, sino):
+    """Accuracy of the GPU output against the FP64 oracle on the same slices.
+
+    * end_to_end: the GPU's slices 0..3 of chunk 0 against the oracle chain run
+      on the same uint16 counts (the CPU baseline's outputs);
+    * stagewise: the GPU's reconstruction of slices 0 and 1 against the
+      oracle filter + backprojection of the GPU's own sinograms of those
+      slices.  With photon-starved rays the -log clamp (x <= 0 -> 1) flips
+      single sinogram values where the transmission after stripe removal is
+      near 0, so the end-to-end figure also carries those flips; the
+      stagewise figure isolates the filter and backprojection."""
+    from oracle import filter_fp64 as ff
+    from oracle import fourierrec_fp64 as fr
+    out = {}
+    k = len(oracle)
+    got = rec[:k].float().cpu().numpy()
+    e, mx = rel_l2(got, oracle)
+    out['end_to_end'] = {'slices': k, 'rel_l2_max': float(e.max()),
+                         'rel_l2_median': float(np.median(e)), 'max_abs_over_max': mx}
+    s = pf.proc_chunk(raw[0], dark[0], flat[0], out=sino)
+    r = torch.empty((a.chunk, a.n, a.n), dtype=rec.dtype, device='cuda')
+    bf.fbp_filter_backprojection(r, s)
+    torch.cuda.synchronize()
+    s2 = s[:2].double().cpu().numpy()
+    ne = 4 * a.n
+    ref = fr.backprojection(ff.fbp_filter_center(s2, ff.calc_filter(ne, a.filter), ne, center),
+                            theta)
+    e, mx = rel_l2(r[:2].double().cpu().numpy(), ref)
+    out['stagewise'] = {'slices': 2, 'rel_l2_max': float(e.max()),
+                        'rel_l2_median': float(np.median(e)), 'max_abs_over_max': mx}
+    del r
+    return out
 
 
 def bench_batch(a):
@@ -384,6 +471,91 @@ def bench_steps(a):
     print(json.dumps(line))
 
 
+class Pipeline:
+    """Per-stream plans and buffers of the chunk pipeline in one storage dtype."""
+
+    def __init__(self, a, torch, dtype, th, center, nchunks, ndark, nflat):
+        from tomocupy_amd import BackprojFunctions, ProcFunctions, _lib
+        self.a, self.torch, self.dtype = a, torch, dtype
+        self.streams = [torch.cuda.Stream() for _ in range(a.streams)]
+        td = _lib.torch_dtype(dtype)
+        self.pfs, self.bfs, self.sinos = [], [], []
+        for st in self.streams:
+            with torch.cuda.stream(st):
+                self.pfs.append(ProcFunctions(a.nproj, a.chunk, a.n, ndark, nflat, dtype,
+                                              remove_stripe_method=a.stripe, fw_sigma=1.0,
+                                              fw_filter='sym16', fw_level=7,
+                                              vo_all_dim=a.vo_all_dim))
+                self.bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, dtype,
+                                                  a.filter))
+                self.sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=td, device='cuda'))
+        self.rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=td, device='cuda')
+        torch.cuda.synchronize()
+
+    def step(self, raw, dark, flat, nstreams=None, count=None):
+        a, torch = self.a, self.torch
+        nstreams = nstreams or len(self.streams)
+        count = raw.shape[0] if count is None else count
+        for k in range(count):
+            i = k % nstreams
+            with torch.cuda.stream(self.streams[i]):
+                out = self.pfs[i].proc_chunk(raw[k], dark[k], flat[k], out=self.sinos[i])
+                self.bfs[i].fbp_filter_backprojection(self.rec[k * a.chunk:(k + 1) * a.chunk],
+                                                      out)
+
+    def timed(self, raw, dark, flat, steps, warmup, label):
+        torch = self.torch
+        for _ in range(warmup):
+            self.step(raw, dark, flat)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            self.step(raw, dark, flat)
+            torch.cuda.synchronize()
+            print('# %s step %d done' % (label, i), file=sys.stderr, flush=True)
+        return (time.perf_counter() - t0) / steps
+
+    def free(self):
+        for pf, bf in zip(self.pfs, self.bfs):
+            pf.free()
+            bf.free()
+        del self.rec
+
+
+def conveyor(a, torch, theta, h_data, h_dark, h_flat, center, dtype):
+    """The reference's three-stream conveyor (GPURec.recon_all) over host
+    data: every chunk is copied from a host array into pinned memory,
+    uploaded, processed and downloaded into pinned memory (no disk; the
+    writer callback does nothing).  Returns slices/s over the whole volume."""
+    from tomocupy_amd.rec import GPURec
+    g = GPURec(a.n, a.nproj, a.nz, a.chunk, torch.from_numpy(theta).cuda(), center,
+               h_dark.shape[0], h_flat.shape[0], dtype, fbp_filter=a.filter,
+               remove_stripe_method=a.stripe, fw_sigma=1.0, fw_filter='sym16', fw_level=7)
+
+    def read_chunk(k):
+        return h_data, h_dark, h_flat
+
+    def write_chunk(k, rec):
+        pass
+
+    g.recon_all(read_chunk, write_chunk)  # warm-up (plans, pinned buffers)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.recon_all(read_chunk, write_chunk)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    g.free()
+    up = a.nproj * a.chunk * a.n * 2
+    down = a.chunk * a.n * a.n * (4 if dtype == 'float32' else 2)
+    nch = -(-a.nz // a.chunk)
+    return {'value': round(a.nz / dt, 2), 'unit': 'slices/s', 'seconds': round(dt, 3),
+            'dtype': dtype, 'h2d_bytes': up * nch, 'd2h_bytes': down * nch,
+            'pcie_gbs': round((up + down) * nch / dt / 1e9, 2),
+            'note': 'GPURec.recon_all: host array -> pinned -> H2D (stream 1), chunk '
+                    'processing (stream 2), D2H to pinned (stream 3), writer threads; '
+                    'no disk; second of two runs'}
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -398,38 +570,19 @@ def main():
         return bench_steps(a)
     center = a.center if a.center is not None else a.n / 2 + 0.5
     theta, h_data, h_dark, h_flat = host_data(a)
-    cpu = None
+    cpu, oracle = None, None
     if a.cpu_baseline == 'on':
         print('# cpu baseline...', file=sys.stderr, flush=True)
-        cpu = cpu_baseline(a, h_data, h_dark, h_flat, theta, center)
+        cpu, oracle = cpu_baseline(a, h_data, h_dark, h_flat, theta, center)
         print('# cpu baseline %s' % cpu, file=sys.stderr, flush=True)
     import torch
-    from tomocupy_amd import BackprojFunctions, ProcFunctions, _lib
+    from tomocupy_amd import _lib
     raw, dark, flat = device_data(a, torch, h_data, h_dark, h_flat)
     nchunks = raw.shape[0]
     th = torch.from_numpy(theta).cuda()
-    streams = [torch.cuda.Stream() for _ in range(a.streams)]
-    td = _lib.torch_dtype(a.dtype)
-    pfs, bfs, sinos = [], [], []
     t0 = time.time()
-    for s in streams:
-        with torch.cuda.stream(s):
-            pfs.append(ProcFunctions(a.nproj, a.chunk, a.n, dark.shape[1],
-                                     flat.shape[1], a.dtype, remove_stripe_method=a.stripe,
-                                     fw_sigma=1.0, fw_filter='sym16', fw_level=7,
-                                     vo_all_dim=a.vo_all_dim))
-            bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, a.dtype, a.filter))
-            sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=td, device='cuda'))
-    rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=td, device='cuda')
-    torch.cuda.synchronize()
+    pl = Pipeline(a, torch, a.dtype, th, center, nchunks, dark.shape[1], flat.shape[1])
     print('# plans ready in %.1fs' % (time.time() - t0), file=sys.stderr, flush=True)
-
-    def step(nstreams=len(streams), count=nchunks):
-        for k in range(count):
-            i = k % nstreams
-            with torch.cuda.stream(streams[i]):
-                out = pfs[i].proc_chunk(raw[k], dark[k], flat[k], out=sinos[i])
-                bfs[i].fbp_filter_backprojection(rec[k * a.chunk:(k + 1) * a.chunk], out)
 
     if a.copy_calibration_mb > 0:
         nb = int(a.copy_calibration_mb * 1e6) // 4
@@ -439,7 +592,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     for _ in range(a.warmup):
-        step()
+        pl.step(raw, dark, flat)
     torch.cuda.synchronize()
     # per-kernel breakdown (untimed): the first ISO_CHUNKS chunks on ONE
     # stream with every kernel bracketed by events, i.e. isolated kernel
@@ -455,7 +608,7 @@ def main():
         _lib.prof_only(None)
         _lib.prof_reset()
         _lib.prof_enable(True)
-        step(1, min(nchunks, ISO_CHUNKS))
+        pl.step(raw, dark, flat, 1, min(nchunks, ISO_CHUNKS))
         torch.cuda.synchronize()
         _lib.prof_enable(False)
         for k in KERNELS:
@@ -488,6 +641,9 @@ def main():
                 roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(ach, 1),
                         'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                         'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': traffic,
+                        'traffic_source': 'profiles/traffic.json (rocprofv3 FETCH_SIZE + '
+                                          'WRITE_SIZE of this kernel and workload, scaled by '
+                                          'a copy kernel of known size)',
                         'algorithmic_bytes_per_launch': ab,
                         'avg_launch_ms': round(kern[dom]['avg_ms'], 5),
                         'launches': kern[dom]['launches'], 'end_to_end': e2e,
@@ -499,18 +655,10 @@ def main():
     if dom is not None:
         _lib.prof_only(dom)
         _lib.prof_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        step()
-        torch.cuda.synchronize()
-        print('# step %d done' % i, file=sys.stderr, flush=True)
-    t1 = time.perf_counter()
+    dt = pl.timed(raw, dark, flat, a.steps, 0, a.dtype)
     _lib.prof_enable(False)
     _lib.prof_only(None)
-    dt = (t1 - t0) / a.steps
-    slices = a.nz
-    value = slices / dt
+    value = a.nz / dt
 
     roof_ov = None
     if dom is not None:
@@ -523,6 +671,32 @@ def main():
                        'note': 'events over the timed %d-stream steps: start-to-end on the '
                                'launch stream, including waits behind other streams'
                                % a.streams}
+    acc = None
+    if oracle is not None:
+        print('# accuracy against the FP64 oracle', file=sys.stderr, flush=True)
+        acc = accuracy(a, torch, pl.rec, oracle, pl.pfs[0], pl.bfs[0], raw, dark, flat, theta,
+                       center, pl.sinos[0])
+    pl.free()
+    del pl
+    torch.cuda.empty_cache()
+    # the 16-bit storage form on the same workload, same run
+    half = None
+    if a.half != 'off' and a.dtype == 'float32':
+        hd = a.half
+        p16 = Pipeline(a, torch, hd, th, center, nchunks, dark.shape[1], flat.shape[1])
+        dt16 = p16.timed(raw, dark, flat, a.steps, max(1, a.warmup), hd)
+        half = {'dtype': hd, 'value': round(a.nz / dt16, 2), 'unit': 'slices/s',
+                'ms_per_step': round(dt16 * 1e3, 3), 'speedup_vs_f32': round(dt / dt16, 3)}
+        if oracle is not None:
+            half['accuracy'] = accuracy(a, torch, p16.rec, oracle, p16.pfs[0], p16.bfs[0], raw,
+                                        dark, flat, theta, center, p16.sinos[0])
+        p16.free()
+        del p16
+        torch.cuda.empty_cache()
+    conv = None
+    if a.conveyor == 'on':
+        print('# conveyor', file=sys.stderr, flush=True)
+        conv = conveyor(a, torch, theta, h_data, h_dark, h_flat, center, a.dtype)
     line = {
         'metric': METRIC, 'value': round(value, 2), 'unit': 'slices/s', 'n_gpus': 1,
         'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(dt * 1e3, 3),
@@ -537,7 +711,8 @@ def main():
                    'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk,
                    'streams': a.streams, 'stripe': a.stripe, 'filter': a.filter,
                    'timed_with_kernel_events': 'dominant kernel only' if dom else False},
-        'roofline': roof, 'cpu_baseline': cpu, 'roofline_overlapped_events': roof_ov,
+        'roofline': roof, 'cpu_baseline': cpu, 'accuracy': acc, 'half': half,
+        'conveyor': conv, 'roofline_overlapped_events': roof_ov,
         'kernels_untimed_breakdown_step': kern,
     }
     print(json.dumps(line))

@@ -152,8 +152,12 @@ class GPURec:
         # bfloat16 has no numpy type: it is widened on the GPU before the copy
         host_dt = torch.float32 if self.tdtype == torch.bfloat16 else self.tdtype
         rec_pinned = torch.zeros((self.max_write_threads, ncz, n, n), dtype=host_dt, **pin)
-        rec_gpu = torch.zeros((2, ncz, n, n), dtype=self.tdtype, device='cuda')
+        rec_gpu = torch.empty((2, ncz, n, n), dtype=self.tdtype, device='cuda')
         writers = [_WRThread() for _ in range(self.max_write_threads)]
+        # the set-up copies above ran on the current stream; order the three
+        # conveyor streams after them
+        for s in (self.stream1, self.stream2, self.stream3):
+            s.wait_stream(torch.cuda.current_stream())
         nzchunk = self.nzchunk
         for k in range(nzchunk + 2):
             if 0 < k < nzchunk + 1:
