@@ -180,21 +180,68 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// Gaussian factors of one list entry {sample, footprint offsets (cols |
-// rows << 8, signed bytes), du bits, dv bits} for tile column / row i:
-// e0 = 2^(K2 (i - rb0 - 4 - du)^2) inside the 9-cell footprint, else 0, and
-// e1 likewise for the rows.  Branch-free (padding entries, sample -1, give
-// 0); K2 x^2 >= -26 for |x| <= 5, far from the range where the hardware
-// exponential would flush, so v_exp_f32 is used directly.
-__device__ __forceinline__ void gauss_pair(const int4 &e, int i, float K2, float &e0, float &e1) {
-  const int d0 = i - (int)(signed char)(e.y & 0xff);
-  const int d1 = i - (int)(signed char)((e.y >> 8) & 0xff);
-  const float x0 = (float)(d0 - 4) - __int_as_float(e.z);
-  const float x1 = (float)(d1 - 4) - __int_as_float(e.w);
-  const bool ok = e.x >= 0;
-  e0 = (ok && (unsigned)d0 <= 8u) ? __builtin_amdgcn_exp2f(K2 * x0 * x0) : 0.f;
-  e1 = (ok && (unsigned)d1 <= 8u) ? __builtin_amdgcn_exp2f(K2 * x1 * x1) : 0.f;
+// Gaussian factor of one list entry for tile column (or row) i.  Entries are
+// {sample, footprint starts, du bits, dv bits}; the footprint starts are two
+// halves (columns low, rows high) holding b = start + 8, the start of the
+// 9-cell footprint relative to the tile plus 8, in [0, 24); padding entries
+// (sample -1) hold b = 255, outside every tile.  With x0 = i - start - 4 - du
+// the tap offset, x = x0 + 0.5 - kGaussDelta = (fiA - b) - du with the lane
+// constant fiA = i + 4.5 - kGaussDelta, and the footprint (start <= i <=
+// start + 8, i.e. x0 in (-5, 4]) is |x| <= 4.5 - kGaussDelta up to taps with
+// x0 in (-5, -5 + 2 kGaussDelta), whose weights (< 2e-8 of the centre tap)
+// are dropped.  The exponent K2 x0^2 + s is evaluated as
+// fma(fma(K2, x, m2), x, cs) with m2 = -2 K2 cB, cs = K2 cB^2 + s,
+// cB = 0.5 - kGaussDelta.  K2 x0^2 >= -26 inside the footprint, far from
+// where the hardware exponential flushes, so v_exp_f32 is used directly.
+constexpr float kGaussDelta = 1.f / 1024.f;
+constexpr float kGaussLim = 4.5f - kGaussDelta;
+constexpr float kGaussCB = 0.5f - kGaussDelta;
+
+struct GaussLane {
+  float fiA, K2, m2, cs0, cs1;
+  __device__ GaussLane(int i, float k2, float s0, float s1) {
+    fiA = (float)i + 4.5f - kGaussDelta;
+    K2 = k2;
+    m2 = -2.f * k2 * kGaussCB;
+    const double q = (double)k2 * (double)kGaussCB * (double)kGaussCB;
+    cs0 = (float)(q + (double)s0);
+    cs1 = (float)(q + (double)s1);
+  }
+  // fiA - b with b the low (HI = 0) or high (HI = 1) half of bb, widened
+  // inside one fused multiply-add (v_fma_mix_f32)
+  template <int HI>
+  __device__ __forceinline__ float offset(int bb) const {
+    float r;
+    if constexpr (HI)
+      asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+          : "=v"(r)
+          : "v"(bb), "v"(fiA));
+    else
+      asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(bb), "v"(fiA));
+    return r;
+  }
+  __device__ __forceinline__ float one(float fb, float du, float cs) const {
+    const float x = fb - du;
+    const float arg = fmaf(fmaf(K2, x, m2), x, cs);
+    return fabsf(x) <= kGaussLim ? __builtin_amdgcn_exp2f(arg) : 0.f;
+  }
+  // e0: column factor (times 2^s0), e1: row factor (times 2^s1)
+  __device__ __forceinline__ void pair(const int4 &e, float &e0, float &e1) const {
+    e0 = one(offset<0>(e.y), __int_as_float(e.z), cs0);
+    e1 = one(offset<1>(e.y), __int_as_float(e.w), cs1);
+  }
+};
+
+// Footprint-start field of a list entry (see GaussLane).
+inline int pack_starts(int col_start, int row_start) {
+  const _Float16 c = (_Float16)(float)(col_start + 8), r = (_Float16)(float)(row_start + 8);
+  unsigned short uc, ur;
+  std::memcpy(&uc, &c, 2);
+  std::memcpy(&ur, &r, 2);
+  return (int)((unsigned)uc | ((unsigned)ur << 16));
 }
+inline int pad_starts() { return pack_starts(255 - 8, 255 - 8); }
+constexpr int kPadStarts = 0x5BF85BF8;  // pad_starts(): 255.0 in both halves
 
 // Gridding as a matrix product on the matrix cores.  For one 16x16 cell tile
 // (rows a, columns b of the periodic 2n x 2n grid) and a batch of polar
@@ -274,17 +321,18 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
   const int g4 = lane >> 4, i = lane & 15;
   const int nlive = min(4, nzc - 4 * grp);
   const cpx<T> *Gg = G + (size_t)(4 * grp) * nsamp;
+  const GaussLane gl(i, K2, 0.f, 0.f);
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
   __shared__ int4 se[kGridWaves][2][kGridBatch];
   __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
   const int eend = ebeg + cnt;
-  int4 re = make_int4(-1, 0, 0, 0);
+  int4 re = make_int4(-1, kPadStarts, 0, 0);
   cpx<T> rg[4];
   auto fetch = [&](int base) {
     const int idx = base + lane;
-    re = idx < eend ? ent[idx] : make_int4(-1, 0, 0, 0);
+    re = idx < eend ? ent[idx] : make_int4(-1, kPadStarts, 0, 0);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       rg[c] = (re.x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + re.x] : cpx<T>{T(0), T(0)};
@@ -303,8 +351,7 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
       // 32q + 8 g4 + u (u < 8) to both operands
       float e0v[8], e1v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        gauss_pair(se[wave][buf][32 * q + 8 * g4 + u], i, K2, e0v[u], e1v[u]);
+      for (int u = 0; u < 8; ++u) gl.pair(se[wave][buf][32 * q + 8 * g4 + u], e0v[u], e1v[u]);
       const typename M::v8 b = M::pack8(e0v);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -375,53 +422,47 @@ __global__ __launch_bounds__(256) void k_fr_gscale(const float *__restrict__ gma
 // store.  On gfx950 the float32 matrix instruction costs as much as 2.4
 // half-precision ones and takes a quarter of the samples, and matrix and
 // vector work do not overlap.
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef __fp16 half2_t __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ float hi_part(float x) {
-  return __int_as_float(__float_as_int(x) & (int)0xffffe000);
-}
-
-__device__ __forceinline__ half4_t pack_rtz(float a, float b, float c, float d) {
-  const half2_t p = __builtin_amdgcn_cvt_pkrtz(a, b), q = __builtin_amdgcn_cvt_pkrtz(c, d);
-  uint2 u;
-  u.x = __builtin_bit_cast(unsigned, p);
-  u.y = __builtin_bit_cast(unsigned, q);
-  return __builtin_bit_cast(half4_t, u);
-}
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ half8_t pack8_rtz(const float *x) {
-  uint4 u;
-  u.x = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[0], x[1]));
-  u.y = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[2], x[3]));
-  u.z = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[4], x[5]));
-  u.w = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[6], x[7]));
-  return __builtin_bit_cast(half8_t, u);
+// Two halves {a0*b0 - h.lo, a1*b1 - h.hi}: fused multiply-adds with the
+// half operand widened and the result rounded to half (v_fma_mixlo_f16 writes
+// the low half of the destination, v_fma_mixhi_f16 the high half).
+__device__ __forceinline__ unsigned mix_lo2(float a0, float b0, float a1, float b1, unsigned h) {
+  unsigned r;
+  asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %0, %4, %5, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+      : "=&v"(r)
+      : "v"(a0), "v"(b0), "v"(h), "v"(a1), "v"(b1));
+  return r;
 }
 
-__device__ __forceinline__ void split8(const float *x, half8_t &hi, half8_t &lo) {
-  float h[8], l[8];
+// x = hi + lo for 8 float32 products x[u] = f[u] * g[u]: hi is the product
+// rounded toward zero to half precision, lo = f*g - hi (exact product minus
+// hi, rounded once to half).
+__device__ __forceinline__ void split8_prod(const float *f, const float *g, half8_t &hi,
+                                           half8_t &lo) {
+  unsigned h[4], l[4];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    h[u] = hi_part(x[u]);
-    l[u] = x[u] - h[u];
-  }
-  hi = pack8_rtz(h);
-  lo = pack8_rtz(l);
+  for (int u = 0; u < 4; ++u)
+    h[u] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(f[2 * u] * g[2 * u],
+                                                                  f[2 * u + 1] * g[2 * u + 1]));
+#pragma unroll
+  for (int u = 0; u < 4; ++u) l[u] = mix_lo2(f[2 * u], g[2 * u], f[2 * u + 1], g[2 * u + 1], h[u]);
+  hi = __builtin_bit_cast(half8_t, make_uint4(h[0], h[1], h[2], h[3]));
+  lo = __builtin_bit_cast(half8_t, make_uint4(l[0], l[1], l[2], l[3]));
 }
 
-// Gaussian factors as gauss_pair, times 2^s0 (columns) and 2^s1 (rows).
-__device__ __forceinline__ void gauss_pair_s(const int4 &e, int i, float K2, float s0, float s1,
-                                             float &e0, float &e1) {
-  const int d0 = i - (int)(signed char)(e.y & 0xff);
-  const int d1 = i - (int)(signed char)((e.y >> 8) & 0xff);
-  const float x0 = (float)(d0 - 4) - __int_as_float(e.z);
-  const float x1 = (float)(d1 - 4) - __int_as_float(e.w);
-  const bool ok = e.x >= 0;
-  e0 = (ok && (unsigned)d0 <= 8u) ? __builtin_amdgcn_exp2f(fmaf(K2 * x0, x0, s0)) : 0.f;
-  e1 = (ok && (unsigned)d1 <= 8u) ? __builtin_amdgcn_exp2f(fmaf(K2 * x1, x1, s1)) : 0.f;
+// The same split of 8 float32 values.
+__device__ __forceinline__ void split8(const float *x, half8_t &hi, half8_t &lo) {
+  unsigned h[4], l[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    h[u] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(x[2 * u], x[2 * u + 1]));
+#pragma unroll
+  for (int u = 0; u < 4; ++u) l[u] = mix_lo2(x[2 * u], 1.f, x[2 * u + 1], 1.f, h[u]);
+  hi = __builtin_bit_cast(half8_t, make_uint4(h[0], h[1], h[2], h[3]));
+  lo = __builtin_bit_cast(half8_t, make_uint4(l[0], l[1], l[2], l[3]));
 }
 
 __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
@@ -438,18 +479,18 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
   const int nlive = min(4, nzc - 4 * grp);
   const float2 *Gg = G + (size_t)(4 * grp) * nsamp;
   const int kA = ksc[0];
-  const float sA = (float)kA, sB = 10.f;
+  const GaussLane gl(i, K2, 10.f, (float)kA);
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
   __shared__ int4 se[kGridWaves][2][kGridBatch];
   __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
   const int eend = ebeg + cnt;
-  int4 re = make_int4(-1, 0, 0, 0);
+  int4 re = make_int4(-1, kPadStarts, 0, 0);
   float2 rg[4];
   auto fetch = [&](int base) {
     const int idx = base + lane;
-    re = idx < eend ? ent[idx] : make_int4(-1, 0, 0, 0);
+    re = idx < eend ? ent[idx] : make_int4(-1, kPadStarts, 0, 0);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       rg[c] = (re.x >= 0 && c < nlive) ? Gg[(size_t)c * nsamp + re.x] : make_float2(0.f, 0.f);
@@ -468,25 +509,24 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
       // operands, so the K order inside the instruction does not matter
       float e0v[8], e1v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        gauss_pair_s(se[wave][buf][32 * q + 8 * g4 + u], i, K2, sB, sA, e0v[u], e1v[u]);
+      for (int u = 0; u < 8; ++u) gl.pair(se[wave][buf][32 * q + 8 * g4 + u], e0v[u], e1v[u]);
       half8_t bh, bl;
       split8(e0v, bh, bl);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        float ar[8], ai[8];
+        float vr[8], vi[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const float2 v = sg[wave][buf][c][32 * q + 8 * g4 + u];
-          ar[u] = e1v[u] * v.x;
-          ai[u] = e1v[u] * v.y;
+          vr[u] = v.x;
+          vi[u] = v.y;
         }
         half8_t ah, al;
-        split8(ar, ah, al);
+        split8_prod(e1v, vr, ah, al);
         acc[2 * c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[2 * c], 0, 0, 0);
         acc[2 * c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[2 * c], 0, 0, 0);
         acc[2 * c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[2 * c], 0, 0, 0);
-        split8(ai, ah, al);
+        split8_prod(e1v, vi, ah, al);
         acc[2 * c + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[2 * c + 1], 0, 0, 0);
         acc[2 * c + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[2 * c + 1], 0, 0, 0);
         acc[2 * c + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[2 * c + 1], 0, 0, 0);
@@ -871,7 +911,8 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   std::vector<int64_t> off(ntiles + 1, 0);
   const int64_t gpad = 32;  // samples per half-precision matrix instruction
   for (size_t t = 0; t < ntiles; ++t) off[t + 1] = off[t] + (counts[t] + gpad - 1) / gpad * gpad;
-  std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, 0, 0, 0));
+  TOMO_REQUIRE(pad_starts() == kPadStarts, "fr_create: internal padding encoding mismatch");
+  std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, pad_starts(), 0, 0));
   std::vector<int64_t> fill(off.begin(), off.end() - 1);
   for (int64_t s = 0; s < S; ++s) {
     int tr[4], rr[4], tc[4], rc2[4];
@@ -887,7 +928,7 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
         int dub, dvb;
         std::memcpy(&dub, &du, 4);
         std::memcpy(&dvb, &dv, 4);
-        ent[fill[t]++] = make_int4((int)s, (rc2[v] & 0xff) | ((rr[u] & 0xff) << 8), dub, dvb);
+        ent[fill[t]++] = make_int4((int)s, pack_starts(rc2[v], rr[u]), dub, dvb);
       }
   }
   TOMO_REQUIRE(off[ntiles] < ((int64_t)1 << 31), "fr_create: tile lists too large");
