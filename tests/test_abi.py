@@ -35,7 +35,11 @@ def test_version():
 @pytest.mark.parametrize("n,nproj,nz,dtype,msg", [
     (21, 10, 2, 0, 'even'), (18, 10, 2, 0, '>= 20'), (64, 0, 2, 0, 'nproj'),
     (64, 10, 0, 0, 'nz'), (64, 10, 2, 7, 'dtype'), (8192, 10, 2, 0, 'maximum'),
-    (262, 10, 2, 0, 'prime factor')])  # 262 = 2 x 131
+    (262, 10, 2, 0, 'prime factor'),  # 262 = 2 x 131
+    # 2n = 8000 = 2^6 5^3 and 8160 = 2^5 3 5 17: 15 values per thread in the
+    # radix-3/5 passes -> 576 and 576 threads in the row pass (limit 512)
+    (4000, 10, 2, 0, 'more than 512 threads'), (4080, 10, 2, 0, 'more than 512 threads'),
+    (2838, 10, 2, 0, 'more than 512 threads')])  # the smallest rejected width
 def test_fr_create_validation(n, nproj, nz, dtype, msg):
     h = ctypes.c_void_p()
     rc = _lib.lib().fr_create(ctypes.byref(h), n, nproj, nz, ctypes.c_void_p(16), dtype, None)

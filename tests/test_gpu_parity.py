@@ -193,7 +193,10 @@ def test_config1_full_chain():
 @pytest.mark.parametrize("wname,level,sigma,shape", [('sym16', 7, 1.0, (180, 3, 256)),
                                                      ('db5', 4, 2.0, (97, 2, 130)),
                                                      ('haar', 3, 1.0, (64, 2, 64)),
-                                                     ('sym5', 5, 1.5, (120, 1, 100))])
+                                                     ('sym5', 5, 1.5, (120, 1, 100)),
+                                                     # 27 and 52 kept notch frequencies
+                                                     ('db5', 5, 4.0, (300, 2, 128)),
+                                                     ('sym16', 3, 8.0, (400, 2, 256))])
 def test_remove_stripe_fw_matches_oracle(wname, level, sigma, shape):
     from tomocupy_amd import ProcFunctions
     nproj, nz, n = shape

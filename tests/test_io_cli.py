@@ -43,3 +43,28 @@ def test_cli_defaults_match_reference():
     assert a.center_search_width == 50.0 and a.center_search_step == 0.5
     assert a.nsino_per_chunk == 8 and a.rotation_axis == -1.0 and a.dtype == 'float32'
     assert a.minus_log == 'True' and a.max_write_threads == 8
+
+
+def test_console_entry_is_tomocupy():
+    """setup.py installs the reference's command name (reference setup.py:9)."""
+    import ast
+    import os
+    src = open(os.path.join(os.path.dirname(__file__), '..', 'setup.py')).read()
+    call = next(n for n in ast.walk(ast.parse(src))
+                if isinstance(n, ast.Call) and getattr(n.func, 'id', '') == 'setup')
+    kw = {k.arg: ast.literal_eval(k.value) for k in call.keywords if k.arg == 'entry_points'}
+    scripts = dict(s.replace(' ', '').split('=') for s in kw['entry_points']['console_scripts'])
+    assert scripts['tomocupy'] == 'tomocupy_amd.cli:main'
+    mod, fn = scripts['tomocupy'].split(':')
+    import importlib
+    main = getattr(importlib.import_module(mod), fn)
+    with pytest.raises(SystemExit) as e:
+        main(['recon', '--help'])
+    assert e.value.code == 0
+
+
+def test_cli_rejects_fw_sigma_beyond_notch_limit(capsys):
+    from tomocupy_amd.cli import main
+    with pytest.raises(SystemExit) as e:
+        main(['recon', '--file-name', 'x.npz', '--remove-stripe-method', 'fw', '--fw-sigma', '12'])
+    assert e.value.code == 2 and '--fw-sigma' in capsys.readouterr().err

@@ -43,7 +43,10 @@ def _bool(s):
 
 
 def parser():
-    ap = argparse.ArgumentParser(prog='tomocupy_amd', description=__doc__,
+    prog = os.path.basename(sys.argv[0]) if sys.argv and sys.argv[0] else 'tomocupy'
+    if prog in ('__main__.py', '-c', '-m', ''):
+        prog = 'python -m tomocupy_amd'
+    ap = argparse.ArgumentParser(prog=prog, description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest='cmd', required=True)
     r = sub.add_parser('recon', help='reconstruct a DXchange-like data set')
@@ -352,8 +355,17 @@ def recon_steps_full(a, ds):
     return out
 
 
+# Largest --fw-sigma: the notch keeps the frequencies whose damping
+# exp(-k^2/(2 sigma^2)) is >= 1e-9, at most 64 per level (about 6.44 sigma + 1).
+FW_SIGMA_MAX = 9.9
+
+
 def main(argv=None):
-    a = parser().parse_args(argv)
+    ap = parser()
+    a = ap.parse_args(argv)
+    if getattr(a, 'remove_stripe_method', None) == 'fw' and not 0 < a.fw_sigma <= FW_SIGMA_MAX:
+        ap.error('--fw-sigma must be in (0, %g] (the notch keeps at most 64 frequencies per '
+                 'wavelet level)' % FW_SIGMA_MAX)
     from .io import Dataset
     ds = Dataset(a.file_name)
     if a.cmd == 'recon_steps':

@@ -32,11 +32,12 @@ size_t max_lds_per_block() {
   return (size_t)v;
 }
 
-int FftPlan::init(int N) {
+int FftPlan::radices(int N, FftDesc &d) {
   TOMO_REQUIRE(fft_size_ok(N) && N <= 16384,
                "FFT length %d is not <= 16384 with prime factors <= %d", N, kFftMaxPrime);
   d.N = N;
   d.nr = 0;
+  d.tw = nullptr;
   int m = N;
   // radix 16 first (fewest passes), then 8, 4, 2, then 3s, 5s, 7s, then the
   // larger primes
@@ -49,12 +50,18 @@ int FftPlan::init(int N) {
   while (m % 7 == 0) { d.r[d.nr++] = 7; m /= 7; }
   for (int p = 11; p <= kFftMaxPrime && m > 1; p += 2)  // generic prime passes
     while (m % p == 0) { d.r[d.nr++] = p; m /= p; }
+  return TOMO_OK;
+}
+
+int FftPlan::init(int N) {
+  int rc = radices(N, d);
+  if (rc) return rc;
   std::vector<float2> h(N);
   for (int k = 0; k < N; ++k) {
     double a = -2.0 * M_PI * (double)k / (double)N;
     h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
   }
-  int rc = tw.alloc(sizeof(float2) * N);
+  rc = tw.alloc(sizeof(float2) * N);
   if (rc) return rc;
   TOMO_HIP(hipMemcpy(tw.p, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
   d.tw = tw.as<float2>();

@@ -39,6 +39,8 @@ struct FftPlan {
   FftDesc d{};
   DevBuf tw;
   int init(int N);  // N <= 16384, prime factors <= kFftMaxPrime
+  // the radix sequence of init() alone (no device memory, no HIP call)
+  static int radices(int N, FftDesc &d);
 };
 
 template <int DIR>

@@ -436,6 +436,16 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   TOMO_REQUIRE(nproj >= 1 && nz >= 1, "flt_create: nproj and nz must be >= 1");
   TOMO_REQUIRE(dtype == TOMO_F32 || dtype == TOMO_F16 || dtype == TOMO_BF16,
                "flt_create: unknown dtype %d", dtype);
+  {
+    // workgroup of the ne-point row kernels, from the radix sequence alone
+    FftDesc d{};
+    const int rc0 = FftPlan::radices((int)ne, d);
+    if (rc0) return rc0;
+    TOMO_REQUIRE(fft_threads(d, ne) > 0,
+                 "flt_create: ne=%lld needs more than 1024 threads per FFT workgroup "
+                 "(its radix-3/5/7/11/13 passes cover fewer than 16 values per thread)",
+                 (long long)ne);
+  }
   auto *P = new (std::nothrow) tomo_flt_plan();
   if (!P) {
     set_error("flt_create: host allocation failed");

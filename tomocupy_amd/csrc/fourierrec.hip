@@ -1004,6 +1004,31 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
 }
 
 
+// Workgroup shapes of the FFT passes for width n, from the radix sequences
+// alone (no HIP call): the detector pass (n points), the row pass (2n) and
+// the column pass (2n, one column per workgroup at least) each need at most
+// 512 threads at their values per thread.  Rejects e.g. n = 4000
+// (2n = 2^6 5^3: 15 values per thread in the radix-5 passes, 576 threads).
+int fr_fft_shapes_ok(int64_t n) {
+  FftDesc d1{}, d2{};
+  int rc = FftPlan::radices((int)n, d1);
+  if (!rc) rc = FftPlan::radices((int)(2 * n), d2);
+  if (rc) return rc;
+  const int colEPT = fft_general(n) ? kFftEPT : kColEPT;
+  const struct {
+    const char *pass;
+    int threads;
+  } need[] = {{"detector (n-point)", fft_threads(d1, n)},
+              {"row (2n-point)", fft_threads(d2, 2 * n)},
+              {"column (2n-point)", fft_threads(d2, 2 * n, colEPT)}};
+  for (const auto &q : need)
+    TOMO_REQUIRE(q.threads > 0 && q.threads <= 512,
+                 "fr_create: n=%lld: the %s FFT pass needs more than 512 threads per workgroup "
+                 "(its radix-3/5/7/11/13 passes cover fewer than 16 values per thread)",
+                 (long long)n, q.pass);
+  return TOMO_OK;
+}
+
 }  // namespace
 
 extern "C" int fr_create(tomo_fr_plan **plan, int64_t n, int64_t nproj, int64_t nz,
@@ -1031,6 +1056,10 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
                "fr_create: grid_chunk=%d must be 0 (default) or a multiple of 64 in [64, 8192]",
                grid_chunk);
   TOMO_REQUIRE(fft_generic == 0 || fft_generic == 1, "fr_create: fft_generic must be 0 or 1");
+  {
+    const int rc0 = fr_fft_shapes_ok(n);
+    if (rc0) return rc0;
+  }
   auto *P = new (std::nothrow) tomo_fr_plan();
   if (!P) {
     set_error("fr_create: host allocation failed");
@@ -1095,7 +1124,7 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   P->rowThreads = fft_threads(P->fft2.d, n2);
   if (P->colThreads < 0 || P->prepThreads < 0 || P->rowThreads < 0 || P->colThreads > 512 ||
       P->prepThreads > 512 || P->rowThreads > 512) {
-    set_error("fr_create: n=%lld needs more than 1024 threads per FFT workgroup", (long long)n);
+    set_error("fr_create: n=%lld needs more than 512 threads per FFT workgroup", (long long)n);
     return fail(TOMO_EINVAL);
   }
   // size-specialised kernels where 2n is a compiled power of two (and the

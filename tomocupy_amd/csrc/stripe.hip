@@ -25,7 +25,7 @@ using namespace tomo;
 
 namespace {
 
-constexpr int kMaxNotch = 32;
+constexpr int kMaxNotch = 64;  // kept notch frequencies per level: sigma up to about 9.8
 
 struct Taps {
   float a[32], b[32];
@@ -73,8 +73,10 @@ void with_km(int nk, F &&f) {
     f(IC<8>());
   else if (nk <= 16)
     f(IC<16>());
-  else
+  else if (nk <= 32)
     f(IC<32>());
+  else
+    f(IC<64>());
 }
 
 template <typename F>
@@ -644,7 +646,7 @@ int setup_fw(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level) {
     // column analysis: 4R output rows per workgroup (R = 8 for tall bands,
     // fewer, for more workgroups, in the short bands of the deep levels);
     // partial notch DFTs per row block, kept until the synthesis
-    l.KM = l.nk <= 8 ? 8 : (l.nk <= 16 ? 16 : 32);
+    l.KM = l.nk <= 8 ? 8 : (l.nk <= 16 ? 16 : (l.nk <= 32 ? 32 : 64));
     l.R = l.Ho >= 256 ? 8 : (l.Ho >= 64 ? 4 : 2);
     l.nrb = (int)ceil_div(l.Ho, 4 * l.R);
     if (l.nk > 0 && ((rc = l.part.alloc(sizeof(float2) * nz * l.nrb * l.KM * l.Wo)) ||
