@@ -215,3 +215,41 @@ def test_cli_full_width_with_prime_17(tmp_path):
         err = _rel(got.astype(np.float64), ref[k])
         print('slice', k, err)
         assert err <= 1e-5
+
+
+@pytest.mark.parametrize("fmt", ['h5', 'h5nolinks'])
+def test_cli_full_hdf5_in_and_out(tmp_path, fmt):
+    """DXchange HDF5 input (written by tomocupy_amd.h5) and the reference's
+    h5 (virtual dataset over part files) and h5nolinks outputs, rows 2..6 of 8:
+    the output covers the whole height with the rows at their absolute
+    positions (writer.py:130-141, 172-175); each slice against the oracle."""
+    from tomocupy_amd import h5
+    n, nproj, nz = 128, 90, 8
+    theta = synth.angles(nproj)
+    data, dark, flat, _ = synth.raw_counts(9, n, theta, nz, ndark=3, nflat=4)
+    p = str(tmp_path / 'scan.h5')
+    with h5.File(p, 'w') as f:
+        f.create_dataset('/exchange/data', data=data)
+        f.create_dataset('/exchange/data_white', data=flat)
+        f.create_dataset('/exchange/data_dark', data=dark)
+        f.create_dataset('/exchange/theta', data=theta.astype(np.float64) * 180 / np.pi)
+    out = str(tmp_path / 'rec')
+    main(['recon', '--file-name', p, '--reconstruction-type', 'full', '--nsino-per-chunk', '2',
+          '--start-row', '2', '--end-row', '6', '--out-path-name', out, '--fbp-filter', 'parzen',
+          '--save-format', fmt])
+    ne = 4 * n
+    th = (theta.astype(np.float64) * 180 / np.pi / 180 * np.pi).astype(np.float32)
+    with h5.File(out + '.h5') as f:
+        d = f['/exchange/data']
+        assert d.shape == (nz, n, n)
+        vol = d[...]
+    assert not vol[:2].any() and not vol[6:].any()
+    for c0 in (2, 4):
+        s = slice(c0, c0 + 2)
+        sino = pp.proc_chain(data[:, s], dark[:, s], flat[:, s])
+        ref = fr.backprojection(ff.fbp_filter_center(sino, ff.calc_filter(ne, 'parzen'), ne, n / 2),
+                                th)
+        for k in range(2):
+            err = _rel(vol[c0 + k].astype(np.float64), ref[k])
+            print(fmt, 'slice', c0 + k, err)
+            assert err <= 1e-5

@@ -108,7 +108,7 @@ def _recon_args(r):
     r.add_argument('--dezinger-threshold', type=int, default=5000)
     r.add_argument('--binning', type=int, default=0, choices=[0, 1, 2, 3])
     r.add_argument('--max-write-threads', type=int, default=8)
-    r.add_argument('--save-format', default='tiff', choices=['tiff', 'npy'])
+    r.add_argument('--save-format', default='tiff', choices=['tiff', 'h5', 'h5nolinks', 'npy'])
 
 
 def _out_dir(a, kind):
@@ -200,15 +200,26 @@ def recon_full(a, ds):
                  **_vo(a))
     ncz, nrec = rec.ncz, rec.n
     zb = z0 >> b  # first output slice id (rec.py:170)
-    stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+',
-                                      dtype=np.float16 if a.dtype == 'float16' else np.float32,
+    odt = np.float16 if a.dtype == 'float16' else np.float32
+    stack = np.lib.format.open_memmap(os.path.join(out, 'recon.npy'), mode='w+', dtype=odt,
                                       shape=(nz, nrec, nrec)) if a.save_format == 'npy' else None
+    h5w = None
+    if a.save_format in ('h5', 'h5nolinks'):
+        from .io import H5RecWriter
+        # the whole height of the file, chunks at their absolute rows
+        # (writer.py:130-141, 172-175)
+        rows = [(zb + k * ncz, rec.lzchunk[k]) for k in range(rec.nzchunk)]
+        h5w = H5RecWriter(out, a.save_format, ds.nz >> b, nrec, odt, rows,
+                          ' '.join([os.path.basename(sys.argv[0])] + sys.argv[1:]))
 
     def read(k):
         s0 = z0 + ((k * ncz) << b)
         return ds.rows(s0, s0 + (rec.lzchunk[k] << b), cols, pids)
 
     def write(k, r):
+        if h5w is not None:
+            h5w.write_chunk(k, zb + k * ncz, r)
+            return
         for kk in range(len(r)):
             fid = zb + k * ncz + kk
             if stack is not None:
@@ -221,6 +232,9 @@ def recon_full(a, ds):
     rec.free()
     if stack is not None:
         stack.flush()
+    if h5w is not None:
+        h5w.close()
+        out = h5w.fname
     dt = time.perf_counter() - t
     print('reconstructed %d slices of %dx%d in %.2f s (%.1f slices/s) -> %s'
           % (nz, nrec, nrec, dt, nz / dt, out))

@@ -2,14 +2,17 @@
 
 Input (reference: ``dataio/reader.py:274-359``, DXchange layout
 ``/exchange/data``, ``data_white``, ``data_dark``, ``theta`` in degrees):
-* HDF5 through ``h5py`` when it is importable;
+* HDF5 files, read by :mod:`tomocupy_amd.h5` (no h5py needed; contiguous,
+  chunked, compressed and virtual datasets);
 * ``.npz`` archives with the same four keys (``exchange/`` prefix optional).
 When ``theta`` is absent the angles are ``linspace(0, pi, nproj)`` including
 the endpoint, as ``reader.py:317``.
 
 Output (reference: ``dataio/writer.py``): one uncompressed single-strip
 little-endian TIFF per slice (float32, or float16 for half plans, sample
-format IEEE float), written by :func:`write_tiff`, or one ``.npy`` stack.
+format IEEE float), written by :func:`write_tiff`; the reference's ``h5``
+(a virtual dataset over one part file per chunk) and ``h5nolinks`` (one
+dataset) files, written by :class:`H5RecWriter`; or one ``.npy`` stack.
 """
 import os
 import struct
@@ -27,12 +30,8 @@ class Dataset:
         self.path = path
         ext = os.path.splitext(path)[1].lower()
         if ext in ('.h5', '.hdf5', '.hdf', '.nxs'):
-            try:
-                import h5py
-            except ImportError as e:  # pragma: no cover - depends on the environment
-                raise RuntimeError('reading HDF5 needs h5py, which is not installed; convert '
-                                   'the file to .npz with keys %s' % (_KEYS,)) from e
-            self._f = h5py.File(path, 'r')
+            from . import h5
+            self._f = h5.File(path)
             self._get = lambda k: self._f['/exchange/' + k] if ('/exchange/' + k) in self._f else None
         elif ext == '.npz':
             self._f = np.load(path, allow_pickle=False, mmap_mode='r')
@@ -117,3 +116,54 @@ def read_tiff(path):
     dt = np.float32 if bits == 32 else np.float16
     return np.frombuffer(b, dtype=np.dtype(dt).newbyteorder('<'), count=w * h,
                          offset=tags[273]).reshape(h, w).astype(dt)
+
+
+class H5RecWriter:
+    """The reference's HDF5 outputs of a full reconstruction
+    (``dataio/writer.py:127-194, 277-285``).
+
+    ``h5``: ``<base>.h5`` holds ``/exchange/data`` ``(nz, n, n)`` as a virtual
+    dataset over ``<base>_parts/p%04d.h5`` (one ``/exchange/data`` per chunk,
+    written as the chunk arrives); ``h5nolinks``: ``<base>.h5`` holds
+    ``/exchange/data`` itself, filled chunk by chunk.  Both carry the
+    attributes ``command``, ``axes`` = ``z:y:x``, ``description`` =
+    ``ReconData`` and ``units`` = ``counts`` (writer.py:146-155).
+    """
+
+    def __init__(self, base, fmt, nz, n, dtype, chunk_rows, command):
+        from . import h5
+        self.base, self.fmt, self.dtype = base, fmt, np.dtype(dtype)
+        attrs = {'command': command, 'axes': 'z:y:x', 'description': 'ReconData',
+                 'units': 'counts'}
+        self.fname = base + '.h5'
+        if fmt == 'h5':
+            parts = base + '_parts'
+            os.makedirs(parts, exist_ok=True)
+            rel = os.path.basename(parts)
+            srcs = []
+            for k, (st, lz) in enumerate(chunk_rows):
+                srcs.append(('%s/p%04d.h5' % (rel, k), '/exchange/data',
+                             ((st, 0, 0), (st + lz, n, n))))
+            with h5.File(self.fname, 'w') as f:
+                f.create_virtual_dataset('/exchange/data', (nz, n, n), self.dtype, srcs, attrs)
+            self._f = None
+        elif fmt == 'h5nolinks':
+            self._f = h5.File(self.fname, 'w')
+            self._d = self._f.create_dataset('/exchange/data', shape=(nz, n, n),
+                                             dtype=self.dtype, attrs=attrs)
+        else:
+            raise ValueError('unknown HDF5 output format %r' % fmt)
+
+    def write_chunk(self, k, st, rec):
+        """Chunk ``k`` (``rec`` of shape ``(lz, n, n)``) at output slice ``st``."""
+        from . import h5
+        if self.fmt == 'h5':
+            with h5.File('%s_parts/p%04d.h5' % (self.base, k), 'w') as f:
+                f.create_dataset('/exchange/data', data=np.asarray(rec, self.dtype))
+        else:
+            self._d[st:st + len(rec)] = np.asarray(rec, self.dtype)
+
+    def close(self):
+        if self._f is not None:
+            self._f.close()
+            self._f = None
