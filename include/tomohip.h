@@ -237,6 +237,21 @@ int pre_process(tomo_pre_plan *plan, const uint16_t *data, const uint16_t *dark,
                 int flat_linear, int minus_log, void *stream);
 
 /*
+ * pre_process without a plan and without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * dark/flat means live in a static device array of each GPU (12 MiB, held
+ * until the process ends), so nz*n <= 1048576 (larger sizes return -1); the
+ * sizes of pre_create and the arguments and results of pre_process
+ * otherwise.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (different streams, different threads) race.
+ */
+int pre_process_noalloc(int64_t nproj, int64_t nz, int64_t n, int64_t ndark, int64_t nflat,
+                        const uint16_t *data, const uint16_t *dark, const uint16_t *flat,
+                        void *out, int out_dtype, double bright_ratio, int flat_linear,
+                        int minus_log, void *stream);
+
+/*
  * Dark/flat correction only, in tomocupy's projection layout:
  *   res (nproj, nz, n) float32 = (data - dark0)/flat0  (proc_functions.py:55-74)
  */

@@ -55,6 +55,8 @@ SIGNATURES = {
     'flt_calc_filter': (_c_int, [_c_i64, ctypes.c_char_p, _c_p]),
     'pre_create': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_i64, _c_i64]),
     'pre_process': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_int, _c_d, _c_int, _c_int, _c_p]),
+    'pre_process_noalloc': (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p,
+                                     _c_p, _c_int, _c_d, _c_int, _c_int, _c_p]),
     'pre_darkflat': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_d, _c_int, _c_p]),
     'pre_minus_log_swap': (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_int, _c_p]),
     'pre_remove_stripe_fw': (_c_int, [_c_p, _c_p, _c_d, ctypes.c_char_p, _c_int, _c_p]),

@@ -2,6 +2,7 @@
 // from projection order (nproj, nz, n) to sinogram order (nz, nproj, n)
 // (tomocupy processing/proc_functions.py:55-83, 140-198; rec.py:172-174).
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -257,6 +258,91 @@ extern "C" int pre_minus_log_swap_rows(tomo_pre_plan *P, const float *vol, int64
                                        int minus_log, void *stream) {
   return log_swap("pre_minus_log_swap_rows", P, vol, ld, z0, lz, out, out_dtype, minus_log,
                   stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free preprocessing: pre_process without a plan, its dark/flat
+// means in a static device array instead of plan-owned scratch.
+
+constexpr int64_t kNoAllocMaxZN = (int64_t)1 << 20;  // nz * n of one call
+
+// dark0, flatA, flatB of one call, kNoAllocMaxZN floats each (12 MiB per
+// GPU, held until the process ends).  Host code never passes this name to a
+// kernel: it is a host-side stand-in; the per-device address comes from
+// hipGetSymbolAddress below.
+static __device__ __align__(256) float scratch[3 * kNoAllocMaxZN];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratch_address(float **out) {
+  int dev = -1;
+  TOMO_HIP(hipGetDevice(&dev));
+  TOMO_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "pre_process_noalloc: device %d out of range",
+               dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    TOMO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TOMO_REQUIRE(p != nullptr, "pre_process_noalloc: no address for the scratch array");
+    g_scratch_dev[dev] = static_cast<float *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return TOMO_OK;
+}
+
+}  // namespace
+
+extern "C" int pre_process_noalloc(int64_t nproj, int64_t nz, int64_t n, int64_t ndark,
+                                   int64_t nflat, const uint16_t *data, const uint16_t *dark,
+                                   const uint16_t *flat, void *out, int out_dtype,
+                                   double bright_ratio, int flat_linear, int minus_log,
+                                   void *stream) {
+  clear_error();
+  TOMO_REQUIRE(nproj >= 1 && nz >= 1 && n >= 1, "pre_process_noalloc: sizes must be >= 1");
+  TOMO_REQUIRE(ndark >= 1 && nflat >= 1,
+               "pre_process_noalloc: need at least one dark and one flat frame");
+  TOMO_REQUIRE(nproj <= 65535 && nz <= 65535, "pre_process_noalloc: nproj and nz must be <= 65535");
+  TOMO_REQUIRE(nz * n <= kNoAllocMaxZN,
+               "pre_process_noalloc: nz*n=%lld exceeds the static scratch (nz*n <= %lld)",
+               (long long)(nz * n), (long long)kNoAllocMaxZN);
+  TOMO_REQUIRE(data && dark && flat && out, "pre_process_noalloc: NULL array");
+  TOMO_REQUIRE(out_dtype == TOMO_F32 || out_dtype == TOMO_F16 || out_dtype == TOMO_BF16,
+               "pre_process_noalloc: unknown out_dtype %d", out_dtype);
+  TOMO_REQUIRE(bright_ratio > 0 && std::isfinite(bright_ratio),
+               "pre_process_noalloc: bright_ratio must be > 0");
+  TOMO_REQUIRE(!flat_linear || nflat >= 3,
+               "pre_process_noalloc: flat_linear needs at least 3 flat frames");
+  float *sc = nullptr;
+  int rc = scratch_address(&sc);
+  if (rc) return rc;
+  // k_pre_means writes every mean it produces (flatB only with flat_linear,
+  // the only case that reads it), so nothing needs clearing between calls
+  float *d0 = sc, *fa = sc + kNoAllocMaxZN, *fb = sc + 2 * kNoAllocMaxZN;
+  hipStream_t st = (hipStream_t)stream;
+  const int zn = (int)(nz * n);
+  hipLaunchKernelGGL(k_pre_means<uint16_t>, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st,
+                     dark, flat, d0, fa, fb, zn, (int)ndark, (int)nflat, (float)bright_ratio,
+                     flat_linear);
+  TOMO_LAUNCH_CHECK();
+  const dim3 grid((unsigned)nproj, (unsigned)nz);
+  if (out_dtype == TOMO_F16)
+    hipLaunchKernelGGL((k_pre_rows<0, _Float16, uint16_t>), grid, dim3(256), 0, st, data, d0, fa,
+                       fb, (_Float16 *)out, (int)nproj, (int)nz, (int)n, flat_linear, minus_log);
+  else if (out_dtype == TOMO_BF16)
+    hipLaunchKernelGGL((k_pre_rows<0, __bf16, uint16_t>), grid, dim3(256), 0, st, data, d0, fa,
+                       fb, (__bf16 *)out, (int)nproj, (int)nz, (int)n, flat_linear, minus_log);
+  else
+    hipLaunchKernelGGL((k_pre_rows<0, float, uint16_t>), grid, dim3(256), 0, st, data, d0, fa, fb,
+                       (float *)out, (int)nproj, (int)nz, (int)n, flat_linear, minus_log);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
 }
 
 extern "C" void pre_destroy(tomo_pre_plan *P) {
