@@ -187,6 +187,13 @@ def device_data(a, torch, data, dark, flat):
 ACC_PAIRS = 2  # slice pairs of chunk 0 whose oracle outputs are kept for the accuracy check
 
 
+def acc_pair(a, i):
+    """Slices of chunk 0 in the GPU's complex pair number i mod chunk/2."""
+    h = max(1, a.chunk // 2)
+    c = i % h
+    return [c, c + h] if c + h < a.chunk else [c, c]
+
+
 def _cpu_model():
     try:
         for line in open('/proc/cpuinfo'):
@@ -202,15 +209,21 @@ def cpu_baseline(a, data, dark, flat, theta, center):
     pair per worker per round), one small warm-up round, then the median of
     three timed rounds; plus one pair in a single process with every core
     (the oracle's C gridding is OpenMP-parallel).  Returns the baseline
-    object and the oracle reconstructions of the first ACC_PAIRS pairs
-    (slices 0..2*ACC_PAIRS-1 of the chunk), float32, for the accuracy check."""
+    object and the oracle reconstructions of the first ACC_PAIRS pairs,
+    float32, for the accuracy check.
+
+    Pair i is the GPU's complex pair (c, c + chunk/2), c = i mod chunk/2: the
+    reference packs those two slices into one complex slice
+    (fourierrec.py:61-68), and since the backprojection of a real sinogram
+    is not exactly real (the row shift of divphi breaks the symmetry), each
+    slice's output carries a small part of its partner's; the oracle has to
+    see the same partner."""
     import multiprocessing as mp
     workers = a.cpu_workers or min(16, os.cpu_count() or 1)
     pairs = workers
     sl = []
     for i in range(pairs):
-        z = (2 * i) % data.shape[1]
-        idx = [z, (z + 1) % data.shape[1]]
+        idx = acc_pair(a, i)
         sl.append((data[:, idx], dark[:, idx], flat[:, idx]))
     args = [(s, theta, center, a, i < ACC_PAIRS, 1) for i, s in enumerate(sl)]
     # warm-up: module imports and the C gridding library, on a tiny problem
@@ -281,9 +294,10 @@ def rel_l2(got, ref):
 def accuracy(a, torch, rec, oracle, pf, bf, raw, dark, flat, theta, center, sino):
     """Accuracy of the GPU output against the FP64 oracle on the same slices.
 
-    * end_to_end: the GPU's slices 0..3 of chunk 0 against the oracle chain run
-      on the same uint16 counts (the CPU baseline's outputs);
-    * stagewise: the GPU's reconstruction of slices 0 and 1 against the
+    * end_to_end: the GPU's slices of the first ACC_PAIRS complex pairs of
+      chunk 0 against the oracle chain run on the same uint16 counts with the
+      same pairing (the CPU baseline's outputs);
+    * stagewise: the GPU's reconstruction of the first pair against the
       oracle filter + backprojection of the GPU's own sinograms of those
       slices.  With photon-starved rays the -log clamp (x <= 0 -> 1) flips
       single sinogram values where the transmission after stripe removal is
@@ -293,20 +307,22 @@ def accuracy(a, torch, rec, oracle, pf, bf, raw, dark, flat, theta, center, sino
     from oracle import fourierrec_fp64 as fr
     out = {}
     k = len(oracle)
-    got = rec[:k].float().cpu().numpy()
+    idx = [z for i in range(k // 2) for z in acc_pair(a, i)]
+    got = rec[idx].float().cpu().numpy()
     e, mx = rel_l2(got, oracle)
-    out['end_to_end'] = {'slices': k, 'rel_l2_max': float(e.max()),
+    out['end_to_end'] = {'slices': idx, 'rel_l2_max': float(e.max()),
                          'rel_l2_median': float(np.median(e)), 'max_abs_over_max': mx}
     s = pf.proc_chunk(raw[0], dark[0], flat[0], out=sino)
     r = torch.empty((a.chunk, a.n, a.n), dtype=rec.dtype, device='cuda')
     bf.fbp_filter_backprojection(r, s)
     torch.cuda.synchronize()
-    s2 = s[:2].double().cpu().numpy()
+    p0 = acc_pair(a, 0)
+    s2 = s[p0].double().cpu().numpy()
     ne = 4 * a.n
     ref = fr.backprojection(ff.fbp_filter_center(s2, ff.calc_filter(ne, a.filter), ne, center),
                             theta)
-    e, mx = rel_l2(r[:2].double().cpu().numpy(), ref)
-    out['stagewise'] = {'slices': 2, 'rel_l2_max': float(e.max()),
+    e, mx = rel_l2(r[p0].double().cpu().numpy(), ref)
+    out['stagewise'] = {'slices': p0, 'rel_l2_max': float(e.max()),
                         'rel_l2_median': float(np.median(e)), 'max_abs_over_max': mx}
     del r
     return out
