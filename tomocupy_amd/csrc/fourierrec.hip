@@ -374,15 +374,27 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
     const int TA = tile / ntile, TB = tile - TA * ntile;
     const int bcol = 16 * TB + j;
     const size_t plane = (size_t)n2 * n2;
+    const bool whole = (n2 & 15) == 0;
+    const int o0 = (16 * TA + ib) * n2 + bcol;
+    if (whole && grp * 4 + 4 <= nzc) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (grp * 4 + c >= nzc) continue;
-      cpx<T> *Fc = F + (size_t)(grp * 4 + c) * plane;
+      for (int c = 0; c < 4; ++c) {
+        cpx<T> *Fc = F + (size_t)(grp * 4 + c) * plane;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int a = 16 * TA + ib + r;
-        if (a < n2 && bcol < n2)
-          stc(Fc + (size_t)a * n2 + bcol, make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
+        for (int r = 0; r < 4; ++r)
+          stc(Fc + (o0 + r * n2), make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (grp * 4 + c >= nzc) continue;
+        cpx<T> *Fc = F + (size_t)(grp * 4 + c) * plane;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int a = 16 * TA + ib + r;
+          if (a < n2 && bcol < n2)
+            stc(Fc + (o0 + r * n2), make_float2(acc[2 * c][r], acc[2 * c + 1][r]));
+        }
       }
     }
   } else {
@@ -541,16 +553,29 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
     const int TA = tile / ntile, TB = tile - TA * ntile;
     const int bcol = 16 * TB + j;
     const size_t plane = (size_t)n2 * n2;
+    // whole tiles when 16 divides 2n: no bounds tests; 32-bit offsets from
+    // each plane's base
+    const bool whole = (n2 & 15) == 0;
+    const int o0 = (16 * TA + ib) * n2 + bcol;
+    if (whole && grp * 4 + 4 <= nzc) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (grp * 4 + c >= nzc) continue;
-      float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+      for (int c = 0; c < 4; ++c) {
+        float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int a = 16 * TA + ib + r;
-        if (a < n2 && bcol < n2)
-          Fc[(size_t)a * n2 + bcol] =
-              make_float2(acc[2 * c][r] * unscale, acc[2 * c + 1][r] * unscale);
+        for (int r = 0; r < 4; ++r)
+          Fc[o0 + r * n2] = make_float2(acc[2 * c][r] * unscale, acc[2 * c + 1][r] * unscale);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (grp * 4 + c >= nzc) continue;
+        float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int a = 16 * TA + ib + r;
+          if (a < n2 && bcol < n2)
+            Fc[o0 + r * n2] = make_float2(acc[2 * c][r] * unscale, acc[2 * c + 1][r] * unscale);
+        }
       }
     }
   } else {
@@ -594,7 +619,8 @@ template <typename T, int NS>
 // writes.  lgB = 0: R[c][a][v'].
 __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
                                                    cpx<T> *__restrict__ R, FftDesc fd, int n,
-                                                   int nzc, int lgB) {
+                                                   int nzc, int lgB,
+                                                   const int2 *__restrict__ span) {
   extern __shared__ float2 sm[];
   const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
   const int a = L % gridDim.x, c = L / gridDim.x;
@@ -602,8 +628,11 @@ __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n2 = 2 * n;
   const cpx<T> *src = F + ((size_t)c * n2 + a) * n2;
+  // only the row's occupied span was gridded; the rest of the row is zero
+  const int2 sp = span[a];
   stage_lds<8, cpx<T>>(
-      n2, tid, nt, [&](int b) { return src[b]; },
+      n2, tid, nt,
+      [&](int b) { return (b >= sp.x && b < sp.y) ? src[b] : cpx<T>{T(0), T(0)}; },
       [&](int b, cpx<T> v) {
         const float2 x = tof2(v);
         sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
@@ -749,50 +778,111 @@ __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict_
   }
 }
 
-// Interpolation of the periodic grid at the polar samples (gather^T); one
-// thread per sample, taps summed in a fixed order.
-__global__ void k_fr_interp(const float4 *__restrict__ geom, const float2 *__restrict__ F,
-                            float2 *__restrict__ G, int n, int nsamp, int nzc, float K2) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// Interpolation of the periodic grid at the polar samples (gather^T), tile
+// by tile over the gridding's sample lists.  A wave takes one work item (a
+// 16x16 tile's list, or a chunk of it), stages the tile's values of the 4
+// complex slices of its group in LDS (16 x 16 x 4 float2, 8 KB), and each
+// lane takes one list entry: the part of the sample's 9x9 footprint that lies
+// in the tile, sum over taps of e1(row) e0(column) F, with the separable
+// Gaussian factors e(i) = 2^(K2 (i - 4 - d)^2) (d the sample's fractional
+// offset) set to zero outside the tile.  The partial sums go to ipart[entry];
+// k_fr_interp_sum adds each sample's (at most 4) partials in a fixed order,
+// so the result is deterministic.
+__global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
+    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
+    const float2 *__restrict__ F, float4 *__restrict__ ipart, int n, int ntile, int nzc,
+    float K2) {
+  __shared__ float4 ft[kGridWaves][16 * 16 * 2];  // [row][col][slice pair]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int it = blockIdx.x * kGridWaves + wave;
   const int grp = blockIdx.y;
-  if (s >= nsamp) return;
+  if (it >= nitems) return;
+  const int4 item = items[it];
+  const int tile = item.x, ebeg = item.y, cnt = item.z;
   const int n2 = 2 * n;
-  const float4 g = geom[s];
-  const int c0 = grp * 4;
-  float2 acc[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+  const int TA = tile / ntile, TB = tile - TA * ntile;
   const size_t plane = (size_t)n2 * n2;
-  const float2 *Fg = F + (size_t)c0 * plane;
-  float ex[9];
+  float4 *T = ft[wave];
+  // stage the tile: 256 cells x 4 slices, 16 values per lane
 #pragma unroll
-  for (int i0 = 0; i0 < 9; ++i0) {
-    const float x0 = (float)(i0 - 4) - g.x;
-    ex[i0] = x0 * x0;
+  for (int q = 0; q < 4; ++q) {
+    const int cell = lane + 64 * q;
+    const int r = cell >> 4, cc = cell & 15;
+    const int a = 16 * TA + r, bcol = 16 * TB + cc;
+    float2 v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      v[c] = make_float2(0.f, 0.f);
+      if (grp * 4 + c < nzc && a < n2 && bcol < n2)
+        v[c] = F[(size_t)(grp * 4 + c) * plane + (size_t)a * n2 + bcol];
+    }
+    T[2 * cell] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+    T[2 * cell + 1] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
   }
-  for (int i1 = 0; i1 < 9; ++i1) {
-    int a = n + (int)g.w + i1;
-    a += (a < 0) ? n2 : 0;
-    a -= (a >= n2) ? n2 : 0;
-    const float x1 = (float)(i1 - 4) - g.y;
+  __builtin_amdgcn_wave_barrier();
+  const int eend = ebeg + cnt;
+  for (int e0 = ebeg; e0 < eend; e0 += 64) {
+    const int e = e0 + lane;
+    const int4 en = e < eend ? ent[e] : make_int4(-1, kPadStarts, 0, 0);
+    if (en.x < 0) continue;  // list padding
+    // footprint start relative to the tile (stored as half-precision start + 8)
+    const int cs = (int)(float)__builtin_bit_cast(_Float16, (unsigned short)(en.y & 0xffff)) - 8;
+    const int rs = (int)(float)__builtin_bit_cast(_Float16, (unsigned short)(en.y >> 16)) - 8;
+    const float du = __int_as_float(en.z), dv = __int_as_float(en.w);
+    float ex[9], ey[9];
 #pragma unroll
-    for (int i0 = 0; i0 < 9; ++i0) {
-      int b = n + (int)g.z + i0;
-      b += (b < 0) ? n2 : 0;
-      b -= (b >= n2) ? n2 : 0;
-      const float w = exp2f(K2 * (ex[i0] + x1 * x1));
-      const size_t o = (size_t)a * n2 + b;
+    for (int i = 0; i < 9; ++i) {
+      const float x0 = (float)(i - 4) - du, x1 = (float)(i - 4) - dv;
+      ex[i] = (cs + i >= 0 && cs + i < 16) ? exp2f(K2 * x0 * x0) : 0.f;
+      ey[i] = (rs + i >= 0 && rs + i < 16) ? exp2f(K2 * x1 * x1) : 0.f;
+    }
+    float4 acc0 = make_float4(0.f, 0.f, 0.f, 0.f), acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c0 + c < nzc) {
-          const float2 v = Fg[o + c * plane];
-          acc[c].x += w * v.x;
-          acc[c].y += w * v.y;
-        }
+    for (int i1 = 0; i1 < 9; ++i1) {
+      const int r = min(max(rs + i1, 0), 15);
+#pragma unroll
+      for (int i0 = 0; i0 < 9; ++i0) {
+        const int cc = min(max(cs + i0, 0), 15);
+        const float w = ey[i1] * ex[i0];
+        const float4 x = T[2 * (r * 16 + cc)], y = T[2 * (r * 16 + cc) + 1];
+        acc0.x = fmaf(w, x.x, acc0.x);
+        acc0.y = fmaf(w, x.y, acc0.y);
+        acc0.z = fmaf(w, x.z, acc0.z);
+        acc0.w = fmaf(w, x.w, acc0.w);
+        acc1.x = fmaf(w, y.x, acc1.x);
+        acc1.y = fmaf(w, y.y, acc1.y);
+        acc1.z = fmaf(w, y.z, acc1.z);
+        acc1.w = fmaf(w, y.w, acc1.w);
       }
     }
+    ipart[2 * (size_t)e] = acc0;
+    ipart[2 * (size_t)e + 1] = acc1;
   }
+}
+
+// G[c][s] = sum of sample s's tile partials (entries ient[s], -1 = none, in
+// list order).
+__global__ __launch_bounds__(256) void k_fr_interp_sum(const int4 *__restrict__ ient,
+                                                       const float4 *__restrict__ ipart,
+                                                       float2 *__restrict__ G, int nsamp,
+                                                       int grp, int nzc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsamp) return;
+  const int4 q = ient[s];
+  const int idx[4] = {q.x, q.y, q.z, q.w};
+  float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (idx[u] < 0) continue;
+    const float4 x = ipart[2 * (size_t)idx[u]], y = ipart[2 * (size_t)idx[u] + 1];
+    a0.x += x.x, a0.y += x.y, a0.z += x.z, a0.w += x.w;
+    a1.x += y.x, a1.y += y.y, a1.z += y.z, a1.w += y.w;
+  }
+  const float2 v[4] = {make_float2(a0.x, a0.y), make_float2(a0.z, a0.w),
+                       make_float2(a1.x, a1.y), make_float2(a1.z, a1.w)};
 #pragma unroll
   for (int c = 0; c < 4; ++c)
-    if (c0 + c < nzc) G[(size_t)(c0 + c) * nsamp + s] = acc[c];
+    if (grp * 4 + c < nzc) G[(size_t)(grp * 4 + c) * nsamp + s] = v[c];
 }
 
 template <int LV>
@@ -826,7 +916,10 @@ struct tomo_fr_plan {
   size_t csz = sizeof(float2);  // bytes of one stored complex value
   int64_t nsamp = 0;
   FftPlan fft1, fft2;
-  DevBuf geom, ent, items, red, part, phix, phiy, circ;
+  DevBuf ient, ent, items, red, part, phix, phiy, circ;
+  DevBuf ipart;      // (list entries) x 4 complex: tile partials of fr_forward (first use)
+  int64_t nent = 0;  // list entries, padding included
+  DevBuf span;  // (2n,) int2: columns [x, y) of grid row a that hold samples
   DevBuf gmax, gsc;  // float32 plans: per-row spectrum maxima, gridding scale exponent
   int ntile = 0, nitems = 0, nred = 0, nslots = 0;
   DevBuf G, F, R;
@@ -914,6 +1007,9 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   TOMO_REQUIRE(pad_starts() == kPadStarts, "fr_create: internal padding encoding mismatch");
   std::vector<int4> ent(std::max<int64_t>(1, off[ntiles]), make_int4(-1, pad_starts(), 0, 0));
   std::vector<int64_t> fill(off.begin(), off.end() - 1);
+  // entries of each sample (at most 2 x 2 tiles), for fr_forward's partial sums
+  std::vector<int4> ient(S, make_int4(-1, -1, -1, -1));
+  std::vector<unsigned char> nient(S, 0);
   for (int64_t s = 0; s < S; ++s) {
     int tr[4], rr[4], tc[4], rc2[4];
     const int nr = tiles_of(b1v[s], tr, rr), nc = tiles_of(b0v[s], tc, rc2);
@@ -928,15 +1024,52 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
         int dub, dvb;
         std::memcpy(&dub, &du, 4);
         std::memcpy(&dvb, &dv, 4);
-        ent[fill[t]++] = make_int4((int)s, pack_starts(rc2[v], rr[u]), dub, dvb);
+        const int64_t e = fill[t]++;
+        ent[e] = make_int4((int)s, pack_starts(rc2[v], rr[u]), dub, dvb);
+        int *slot = &ient[s].x;
+        if (nient[s] >= 4) {
+          set_error("fr_create: internal: a sample in more than 4 tiles");
+          return TOMO_EINVAL;
+        }
+        slot[nient[s]++] = (int)e;
       }
   }
   TOMO_REQUIRE(off[ntiles] < ((int64_t)1 << 31), "fr_create: tile lists too large");
+  // Occupied span of each tile row: the column tiles [lo, hi) between its
+  // first and last tile with samples (the samples fill a disc, so the tiles
+  // in between have samples too; if one does not, the row's span is the whole
+  // row).  Tiles outside their row's span are never gridded (they are zero)
+  // and the row pass reads only the span of each grid row.
+  std::vector<int> tlo(nt, 0), thi(nt, 0);
+  for (int ta = 0; ta < nt; ++ta) {
+    int lo = nt, hi = -1;
+    for (int tb = 0; tb < nt; ++tb)
+      if (counts[(size_t)ta * nt + tb] > 0) lo = std::min(lo, tb), hi = std::max(hi, tb);
+    if (hi < lo) continue;  // no samples in this tile row
+    bool gap = false;
+    for (int tb = lo; tb <= hi; ++tb) gap |= counts[(size_t)ta * nt + tb] == 0;
+    if (gap) lo = 0, hi = nt - 1;
+    tlo[ta] = lo;
+    thi[ta] = hi + 1;
+  }
+  {
+    std::vector<int2> span(n2);
+    for (int64_t a = 0; a < n2; ++a) {
+      const int ta = (int)(a / 16);
+      span[a] = make_int2((int)std::min<int64_t>(16 * (int64_t)tlo[ta], n2),
+                          (int)std::min<int64_t>(16 * (int64_t)thi[ta], n2));
+    }
+    int rc0 = P->span.alloc(sizeof(int2) * n2);
+    if (rc0) return rc0;
+    TOMO_HIP(hipMemcpy(P->span.p, span.data(), sizeof(int2) * n2, hipMemcpyHostToDevice));
+  }
   // work items: whole tiles, or chunks of gridChunk entries for heavy tiles
   const int64_t kChunk = P->gridChunk;
   std::vector<int4> items, red;
   int nslots = 0;
   for (size_t t = 0; t < ntiles; ++t) {
+    const int ta = (int)(t / nt), tb = (int)(t % nt);
+    if (tb < tlo[ta] || tb >= thi[ta]) continue;  // outside the occupied span
     const int64_t L = off[t + 1] - off[t];
     if (L <= kChunk) {
       items.push_back(make_int4((int)t, (int)off[t], (int)L, -1));
@@ -966,8 +1099,9 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
     if ((rc = P->red.alloc(sizeof(int4) * red.size()))) return rc;
     TOMO_HIP(hipMemcpy(P->red.p, red.data(), sizeof(int4) * red.size(), hipMemcpyHostToDevice));
   }
-  if ((rc = P->geom.alloc(sizeof(float4) * S))) return rc;
-  TOMO_HIP(hipMemcpy(P->geom.p, geom.data(), sizeof(float4) * S, hipMemcpyHostToDevice));
+  P->nent = off[ntiles];
+  if ((rc = P->ient.alloc(sizeof(int4) * S))) return rc;
+  TOMO_HIP(hipMemcpy(P->ient.p, ient.data(), sizeof(int4) * S, hipMemcpyHostToDevice));
 
   // deapodisation (kernels_fourierrec.cuh:14-15), FP64, separable in x and y;
   // the 1/nproj and (1 - n%4) factors ride on the y table
@@ -1226,7 +1360,8 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
     ProfScope ps_("k_fr_rowfft", st);
     with_static_size(P->nsRow, [&](auto ns) {
       hipLaunchKernelGGL((k_fr_rowfft<T, decltype(ns)::value>), dim3(n2, ng * 4),
-                         dim3(P->rowThreads), P->rowLds, st, F, R, P->fft2.d, n, nzc, P->lgB);
+                         dim3(P->rowThreads), P->rowLds, st, F, R, P->fft2.d, n, nzc, P->lgB,
+                         P->span.as<int2>());
     });
   }
   TOMO_LAUNCH_CHECK();
@@ -1342,6 +1477,10 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
     set_error("fr_forward: the forward projection is implemented for float32 plans only");
     return TOMO_EUNSUPPORTED;
   }
+  if (!P->ipart.p) {  // tile partials of the interpolation, allocated on the first call
+    const int rc0 = P->ipart.alloc(2 * sizeof(float4) * (size_t)std::max<int64_t>(1, P->nent));
+    if (rc0) return rc0;
+  }
   float *data = (float *)data_;
   const float *obj = (const float *)obj_;
   hipStream_t st = (hipStream_t)stream;
@@ -1369,13 +1508,24 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
     });
   }
   TOMO_LAUNCH_CHECK();
-  {
-    ProfScope ps_("k_fr_interp", st);
-    hipLaunchKernelGGL(k_fr_interp, dim3((unsigned)ceil_div(P->nsamp, 256), ng), dim3(256), 0, st,
-                       P->geom.as<float4>(), P->F.as<float2>(), P->G.as<float2>(), n, (int)P->nsamp, nzc,
-                       P->K2);
+  for (int g = 0; g < ng; ++g) {
+    {
+      ProfScope ps_("k_fr_interp_tiles", st);
+      hipLaunchKernelGGL(k_fr_interp_tiles, dim3((unsigned)ceil_div(P->nitems, kGridWaves), 1, 1),
+                         dim3(64 * kGridWaves), 0, st, P->items.as<int4>(), P->nitems,
+                         P->ent.as<int4>(), P->F.as<float2>() + (size_t)g * 4 * n2 * n2,
+                         P->ipart.as<float4>(), n, P->ntile, nzc - 4 * g, P->K2);
+    }
+    TOMO_LAUNCH_CHECK();
+    {
+      ProfScope ps_("k_fr_interp_sum", st);
+      hipLaunchKernelGGL(k_fr_interp_sum, dim3((unsigned)ceil_div(P->nsamp, 256)), dim3(256), 0,
+                         st, P->ient.as<int4>(), P->ipart.as<float4>(),
+                         P->G.as<float2>() + (size_t)g * 4 * P->nsamp, (int)P->nsamp, 0,
+                         nzc - 4 * g);
+    }
+    TOMO_LAUNCH_CHECK();
   }
-  TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_prep_adj", st);
     with_fft_level(level, [&](auto lv) {
@@ -1391,7 +1541,9 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
 
 extern "C" void fr_destroy(tomo_fr_plan *P) {
   if (!P) return;
-  P->geom.release();
+  P->ient.release();
+  P->ipart.release();
+  P->span.release();
   P->ent.release();
   P->items.release();
   P->red.release();

@@ -6,8 +6,13 @@ Usage (paths are rocprofv3 ``-f csv`` outputs)::
   python -m tomocupy_amd.profiling traffic FETCH_COUNTER_CSV WRITE_COUNTER_CSV
          --copy-bytes B [--key-suffix S] [--json OUT.json] [--title T] [--cmd C]
   python -m tomocupy_amd.profiling window KERNEL_TRACE_CSV --kernel K --skip N --count M
+  python -m tomocupy_amd.profiling counters COUNTER_CSV [COUNTER_CSV ...] [--kernels K1,K2]
+         [--title T] [--cmd C]
 
-``stats`` prints a markdown table of the kernel-trace statistics.  ``window``
+``stats`` prints a markdown table of the kernel-trace statistics.
+``counters`` averages every ``--pmc`` counter per dispatch and kernel over
+one or more counter-collection files (one per pass) and prints the raw
+averages plus per-wave and utilisation figures derived from them.  ``window``
 prints the mean duration of dispatches N .. N+M-1 (in start order) of the
 kernels whose base name is K, e.g. the launches of one pass of a run.  ``traffic``
 averages ``FETCH_SIZE`` and ``WRITE_SIZE`` (KiB per dispatch) per kernel from
@@ -170,6 +175,60 @@ def cmd_traffic(a):
         json.dump(old, open(a.json, 'w'), indent=1, sort_keys=True)
 
 
+def cmd_counters(a):
+    acc = defaultdict(lambda: defaultdict(list))  # kernel -> counter -> per-dispatch values
+    for path in a.csv:
+        per = defaultdict(float)  # (dispatch, kernel, counter) -> value summed over dimensions
+        for r in csv.DictReader(open(path)):
+            key = (r.get('Dispatch_Id', r.get('Correlation_Id')), short_name(r['Kernel_Name']),
+                   r['Counter_Name'])
+            per[key] += float(r['Counter_Value'])
+        for (_, k, c), v in per.items():
+            acc[k][c].append(v)
+    want = a.kernels.split(',') if a.kernels else None
+    out = []
+    if a.title:
+        out.append('# %s\n' % a.title)
+    if a.cmd:
+        out.append('Commands:\n\n```\n%s\n```\n' % a.cmd)
+    for k in sorted(acc):
+        if not k.startswith('k_') or (want and k.split('<')[0] not in want and k not in want):
+            continue
+        c = {n: sum(v) / len(v) for n, v in acc[k].items()}
+        out.append('## %s (%d dispatches)\n' % (k, max(len(v) for v in acc[k].values())))
+        out.append('| counter | mean per dispatch |')
+        out.append('|---|---|')
+        for n in sorted(c):
+            out.append('| %s | %.4g |' % (n, c[n]))
+        waves = c.get('SQ_WAVES')
+        der = []
+        if waves:
+            for n in ('SQ_INSTS_VALU', 'SQ_INSTS_MFMA', 'SQ_INSTS_LDS', 'SQ_INSTS_VMEM_RD',
+                      'SQ_INSTS_VMEM_WR', 'SQ_INSTS_SALU', 'SQ_INSTS_SMEM'):
+                if n in c:
+                    der.append(('%s per wave' % n, c[n] / waves))
+        busy = c.get('SQ_BUSY_CYCLES')
+        wc = c.get('SQ_WAVE_CYCLES')
+        if wc:
+            for n in ('SQ_ACTIVE_INST_VALU', 'SQ_ACTIVE_INST_LDS', 'SQ_ACTIVE_INST_VMEM',
+                      'SQ_ACTIVE_INST_ANY', 'SQ_WAIT_INST_ANY', 'SQ_WAIT_ANY', 'SQ_WAIT_INST_LDS'):
+                if n in c:
+                    der.append(('%s / SQ_WAVE_CYCLES' % n, c[n] / wc))
+        if 'SQ_LDS_BANK_CONFLICT' in c and 'SQ_ACTIVE_INST_LDS' in c and c['SQ_ACTIVE_INST_LDS']:
+            der.append(('SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS',
+                        c['SQ_LDS_BANK_CONFLICT'] / c['SQ_ACTIVE_INST_LDS']))
+        if busy and 'SQ_VALU_MFMA_BUSY_CYCLES' in c:
+            der.append(('SQ_VALU_MFMA_BUSY_CYCLES / SQ_BUSY_CYCLES',
+                        c['SQ_VALU_MFMA_BUSY_CYCLES'] / busy))
+        if der:
+            out.append('\n| derived | value |')
+            out.append('|---|---|')
+            for n, v in der:
+                out.append('| %s | %.4g |' % (n, v))
+        out.append('')
+    print('\n'.join(out))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest='cmd_name', required=True)
@@ -190,11 +249,18 @@ def main(argv=None):
     w.add_argument('--kernel', required=True)
     w.add_argument('--skip', type=int, default=0)
     w.add_argument('--count', type=int, required=True)
+    c = sub.add_parser('counters')
+    c.add_argument('csv', nargs='+')
+    c.add_argument('--kernels', default='')
+    c.add_argument('--title', default='')
+    c.add_argument('--cmd', default='')
     a = ap.parse_args(argv)
     if a.cmd_name == 'stats':
         cmd_stats(a)
     elif a.cmd_name == 'window':
         cmd_window(a)
+    elif a.cmd_name == 'counters':
+        cmd_counters(a)
     else:
         cmd_traffic(a)
 
