@@ -58,6 +58,15 @@ int tomo_prof_only(const char *kernel);
 int tomo_prof_reset(void);
 int tomo_prof_read(const char *kernel, double *total_ms, int64_t *launches);
 
+/*
+ * Inner product for the iterative solvers of the differentiation form (no
+ * counterpart in the reference): out[0] = sum_i a[i] b[i] over n float32
+ * values, accumulated in FP64 in a fixed order (deterministic).  out is a
+ * device array of at least 1025 doubles: out[1..1024] hold the per-block
+ * partial sums.  Asynchronous on `stream`.
+ */
+int tomo_dot_f64(const float *a, const float *b, int64_t n, double *out, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* USFFT backprojector                                                      */
 /* ------------------------------------------------------------------------ */

@@ -142,3 +142,21 @@ def test_cgls_matches_fp64_iteration():
     assert err5 < 1e-4
     assert rel_obj.max() < 2e-2
     assert got_obj[-1] < 0.05 * got_obj[0]
+
+
+@pytest.mark.parametrize("n", [1, 1000, 3000001])
+def test_dot_f64_matches_numpy(n):
+    """The CG inner product (tomo_dot_f64): FP64 accumulation of float32
+    products, against numpy in FP64, and bitwise equal on repetition."""
+    from tomocupy_amd.autograd import _dot
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    got = _dot(ta, tb)
+    again = _dot(ta, tb)
+    ref = float(np.dot(a.astype(np.float64), b.astype(np.float64)))
+    scale = float(np.abs(a.astype(np.float64) * b.astype(np.float64)).sum())
+    assert got.dtype == torch.float64
+    assert abs(float(got) - ref) <= 1e-13 * scale
+    assert float(got) == float(again)

@@ -13,6 +13,7 @@ pair of HIP launches sequences on the plan's stream.
 """
 import torch
 
+from . import _lib
 from .fourierrec import FourierRec
 
 
@@ -62,8 +63,16 @@ def backproject(data, rec):
     return Backproject.apply(data, rec)
 
 
-def _dot(a, b):
-    return torch.sum(a.double() * b.double())
+def _dot(a, b, out=None):
+    """sum(a * b) of two float32 tensors, accumulated in FP64 in a fixed
+    order on the GPU (``tomo_dot_f64``); a 0-d float64 tensor."""
+    if out is None:
+        out = torch.empty(1025, dtype=torch.float64, device=a.device)
+    a = a.contiguous()
+    b = b.contiguous()
+    _lib.check(_lib.lib().tomo_dot_f64(a.data_ptr(), b.data_ptr(), a.numel(), out.data_ptr(),
+                                       _lib.stream_ptr(None)), 'tomo_dot_f64')
+    return out[0].clone()
 
 
 def cgls(rec, data, lam=0.0, niter=50, x0=None, callback=None):
@@ -93,15 +102,16 @@ def cgls(rec, data, lam=0.0, niter=50, x0=None, callback=None):
         x = x0.detach().clone().to(torch.float32)
         r = _apply_backprojection(rec, data) - normal(x)
     p = r.clone()
-    rr = _dot(r, r)
+    work = torch.empty(1025, dtype=torch.float64, device=data.device)
+    rr = _dot(r, r, work)
     zero = torch.zeros((), dtype=torch.float64, device=data.device)
     for k in range(niter):
         q = normal(p)
-        pq = _dot(p, q)
+        pq = _dot(p, q, work)
         alpha = torch.where(pq > 0, rr / torch.where(pq > 0, pq, 1.0), zero).to(torch.float32)
-        x.add_(p * alpha)
-        r.sub_(q * alpha)
-        rr_new = _dot(r, r)
+        x.addcmul_(p, alpha)
+        r.addcmul_(q, alpha, value=-1.0)
+        rr_new = _dot(r, r, work)
         if callback is not None:
             callback(k, x, rr_new)
         beta = torch.where(rr > 0, rr_new / torch.where(rr > 0, rr, 1.0), zero).to(torch.float32)

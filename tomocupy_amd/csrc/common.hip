@@ -154,3 +154,54 @@ extern "C" int tomo_prof_read(const char *kernel, double *total_ms, int64_t *lau
 
 extern "C" const char *tomo_last_error(void) { return tomo::g_err; }
 extern "C" const char *tomo_version(void) { return "tomohip 0.1.0 (gfx950)"; }
+
+namespace {
+
+constexpr int kDotBlocks = 1024;
+
+// Block b sums a[i] b[i] for i = b*256 + t + k*(1024*256), in FP64, then a
+// fixed-order tree over the block's threads.
+__global__ __launch_bounds__(256) void k_dot_partial(const float *__restrict__ a,
+                                                     const float *__restrict__ b, int64_t n,
+                                                     double *__restrict__ part) {
+  __shared__ double sd[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kDotBlocks * 256)
+    s = fma((double)a[i], (double)b[i], s);
+  sd[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sd[threadIdx.x] += sd[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sd[0];
+}
+
+__global__ __launch_bounds__(256) void k_dot_final(double *__restrict__ out) {
+  __shared__ double sd[256];
+  const double *part = out + 1;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kDotBlocks; i += 256) s += part[i];
+  sd[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sd[threadIdx.x] += sd[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sd[0];
+}
+
+}  // namespace
+
+extern "C" int tomo_dot_f64(const float *a, const float *b, int64_t n, double *out,
+                            void *stream) {
+  tomo::clear_error();
+  TOMO_REQUIRE(a != nullptr && b != nullptr && out != nullptr, "tomo_dot_f64: NULL array");
+  TOMO_REQUIRE(n >= 0, "tomo_dot_f64: n < 0");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(256), 0, st, a, b, n, out + 1);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, out);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}

@@ -712,8 +712,11 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft(const cpx<T> 
 
 // ---- adjoint (forward projection) kernels --------------------------------
 
-template <int LV>
-__global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
+// NS as k_fr_colfft: a compiled power-of-two 2n (1024 threads, col_lg(NS)
+// columns per workgroup) or 0 / -1 / -2 for the run-time plans of fft_level
+// 0 / 1 / 2.
+template <int NS>
+__global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
                                 const float *__restrict__ phix, const float *__restrict__ phiy,
                                 const int2 *__restrict__ circ, FftDesc fd, int n, int nz,
                                 int nzc, int lgCB) {
@@ -746,7 +749,10 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
         sm[(lds_pad(u) << lgCB) + t] = make_float2(v.x * ph, v.y * ph);
       });
   __syncthreads();
-  lds_fft<-1, (LV > 0 ? kFftEPT : kColEPT), LV>(sm, fd, lgCB, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<NS, -1, kFftEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
+  else
+    lds_fft<-1, (NS < 0 ? kFftEPT : kColEPT), fft_lv(NS)>(sm, fd, lgCB, tid, nt);
   float2 *dst = R + (size_t)c * n2 * n;
   for (int idx = tid; idx < CB * n2; idx += nt) {
     const int a = idx >> lgCB, t = idx & (CB - 1);
@@ -756,7 +762,7 @@ __global__ __launch_bounds__(512) void k_fr_colfft_adj(const float *__restrict__
   }
 }
 
-template <int LV>
+template <int NS>
 __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict__ R, float2 *__restrict__ F, FftDesc fd,
                                 int n, int nzc) {
   extern __shared__ float2 sm[];
@@ -770,7 +776,10 @@ __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict_
       [&](int v) { return (v >= h && v < h + n) ? src[v - h] : make_float2(0.f, 0.f); },
       [&](int v, float2 x) { sm[lds_pad(v)] = (v & 1) ? make_float2(-x.x, -x.y) : x; });
   __syncthreads();
-  lds_fft<-1, kFftEPT, LV>(sm, fd, 0, tid, nt);
+  if constexpr (NS > 0)
+    lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd.tw, tid, nt);
+  else
+    lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
   float2 *dst = F + ((size_t)c * n2 + a) * n2;
   for (int b = tid; b < n2; b += nt) {
     const float2 x = sm[lds_pad(b)];
@@ -1292,9 +1301,8 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
     set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
     return fail(TOMO_EHIP);
   }
-  with_fft_level((int)fft_level(n), [&](auto lv) {
-    constexpr int LV = decltype(lv)::value;
-    lds_attr((const void *)k_fr_colfft_adj<LV>, P->colLds);
+  with_static_size(P->nsCol, [&](auto ns) {
+    lds_attr((const void *)k_fr_colfft_adj<decltype(ns)::value>, P->colLds);
   });
   const int rowLds = (int)(lds_padded_len((int)n2) * sizeof(float2));
   P->rowLds = rowLds;
@@ -1309,9 +1317,8 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
     set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
     return fail(TOMO_EHIP);
   }
-  with_fft_level((int)fft_level(n), [&](auto lv) {
-    constexpr int LV = decltype(lv)::value;
-    lds_attr((const void *)k_fr_rowfft_adj<LV>, rowLds);
+  with_static_size(P->nsRow, [&](auto ns) {
+    lds_attr((const void *)k_fr_rowfft_adj<decltype(ns)::value>, rowLds);
   });
   if (ea != hipSuccess) {
     set_error("fr_create: hipFuncSetAttribute failed: %s", hipGetErrorString(ea));
@@ -1490,10 +1497,11 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
   const int level = fft_level(n);
   {
     ProfScope ps_("k_fr_colfft_adj", st);
-    with_fft_level(level, [&](auto lv) {
-      constexpr int LV = decltype(lv)::value;
-      hipLaunchKernelGGL(k_fr_colfft_adj<LV>, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
-                         dim3(P->colThreads), P->colLds, st, obj, P->R.as<float2>(),
+    with_static_size(P->nsCol, [&](auto ns) {
+      constexpr int NS = decltype(ns)::value;
+      hipLaunchKernelGGL(k_fr_colfft_adj<NS>, dim3((unsigned)ceil_div(n, P->colCB), ng * 4),
+                         dim3(NS > 0 ? P->colThreadsStatic : P->colThreads), P->colLds, st, obj,
+                         P->R.as<float2>(),
                          P->phix.as<float>(), P->phiy.as<float>(), P->circ.as<int2>(),
                          P->fft2.d, n, nz, nzc, P->lgCB);
     });
@@ -1501,9 +1509,9 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
   TOMO_LAUNCH_CHECK();
   {
     ProfScope ps_("k_fr_rowfft_adj", st);
-    with_fft_level(level, [&](auto lv) {
-      constexpr int LV = decltype(lv)::value;
-      hipLaunchKernelGGL(k_fr_rowfft_adj<LV>, dim3(n2, ng * 4), dim3(P->rowThreads), P->rowLds,
+    with_static_size(P->nsRow, [&](auto ns) {
+      hipLaunchKernelGGL(k_fr_rowfft_adj<decltype(ns)::value>, dim3(n2, ng * 4),
+                         dim3(P->rowThreads), P->rowLds,
                          st, P->R.as<float2>(), P->F.as<float2>(), P->fft2.d, n, nzc);
     });
   }
