@@ -374,6 +374,20 @@ def bench_batch(a):
     b.capture()
     t_graph = timed(b.run)
     slices = b.slices
+    t_bgrid = None
+    if a.dtype == 'float32':
+        # the same batch with one gridding launch for all requests (graph)
+        ref = [o.clone() for o in b.outputs()]
+        b.free()
+        del b
+        b = BatchRec(reqs, dtype=a.dtype, fbp_filter='parzen', grid='batched')
+        for r, d in zip(reqs, b.inputs()):
+            d.copy_(torch.from_numpy(r['sino'])[None].expand(r['nz'], -1, -1).to(td))
+        b.capture()
+        t_bgrid = timed(b.run)
+        same = all(torch.equal(x, y) for x, y in zip(ref, b.outputs()))
+        del ref
+    b.free()
     line = {'metric': 'slices/s, %d independent mixed-size reconstructions (filter + '
                       'backprojection), one HIP graph' % len(reqs),
             'value': round(slices / t_graph, 2), 'unit': 'slices/s', 'n_gpus': 1,
@@ -388,6 +402,11 @@ def bench_batch(a):
             'sequential_eager': {'value': round(slices / t_seq, 2),
                                  'ms_per_step': round(t_seq * 1e3, 3)},
             'graph_speedup': round(t_seq / t_graph, 3)}
+    if t_bgrid is not None:
+        line['batched_grid'] = {'value': round(slices / t_bgrid, 2),
+                                'ms_per_step': round(t_bgrid * 1e3, 3),
+                                'bitwise_equal_to_graph': bool(same),
+                                'note': 'one gridding launch for all requests, one stream, graph'}
     print(json.dumps(line))
 
 

@@ -148,6 +148,30 @@ int fr_filter_backprojection_prepared(tomo_fr_plan *plan, tomo_flt_plan *flt, vo
                                       const void *data, void *stream);
 
 /*
+ * Batches of independent requests (no counterpart in the reference, which
+ * reconstructs one chunk at a time: rec.py:167-189).  A batch groups the
+ * float32 plans of `count` requests (sizes may differ, each plan in at most
+ * one live batch at a time); fr_batch_filter_backprojection then does
+ * fr_filter_backprojection_prepared for every request r,
+ *   objs[r] <- filtered backprojection of datas[r] with flts[r],
+ * but grids all requests in ONE gridding launch, whose work items are sorted
+ * longest first over the whole batch, so that the small requests of a
+ * mixed batch fill the machine beside the large ones.  Each result equals
+ * that of fr_filter_backprojection_prepared on the same plan bit for bit.
+ * The plans' scratch is used; plans must not run elsewhere meanwhile.
+ *   batch  out: the batch handle (fr_batch_destroy frees it, not the plans)
+ *   plans  host array of `count` fr plans, float32, distinct
+ *   flts   host array of prepared filter plans (flt_prepare), one per plan
+ *   objs   host array of device float32 (nz, n, n) outputs
+ *   datas  host array of device float32 (nz, nproj, n) inputs, NOT modified
+ */
+typedef struct tomo_fr_batch tomo_fr_batch;
+int fr_batch_create(tomo_fr_batch **batch, tomo_fr_plan *const *plans, int count);
+int fr_batch_filter_backprojection(tomo_fr_batch *batch, tomo_flt_plan *const *flts,
+                                   void *const *objs, const void *const *datas, void *stream);
+void fr_batch_destroy(tomo_fr_batch *batch);
+
+/*
  * Adjoint of fr_backprojection (USFFT forward projection); no counterpart in
  * the reference's tomography path, structured like cfunc_usfft2d::fwd
  * (src/cuda/cfunc_usfft2d.cu:53-70).  For real arrays

@@ -63,6 +63,20 @@ def test_null_arguments():
     L.pre_destroy(None)
 
 
+def test_fr_batch_validation():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.fr_batch_create(None, None, 1) == _lib.TOMO_EINVAL
+    assert L.fr_batch_create(ctypes.byref(h), None, 1) == _lib.TOMO_EINVAL
+    plans = (ctypes.c_void_p * 2)(None, None)
+    assert L.fr_batch_create(ctypes.byref(h), plans, 0) == _lib.TOMO_EINVAL
+    assert L.fr_batch_create(ctypes.byref(h), plans, 2) == _lib.TOMO_EINVAL
+    assert 'plan 0 is NULL' in _lib.last_error()
+    assert not h.value
+    assert L.fr_batch_filter_backprojection(None, None, None, None, None) == _lib.TOMO_EINVAL
+    L.fr_batch_destroy(None)
+
+
 def test_flt_create_validation():
     h = ctypes.c_void_p()
     L = _lib.lib()

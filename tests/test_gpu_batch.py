@@ -5,6 +5,8 @@
   matches the FP64 oracle per request (per-slice rel-L2 <= 1e-5).
 * The captured HIP graph replayed three times, with other GPU work in
   between, reproduces the eager outputs bitwise.
+* The batched gridding (one gridding launch for all requests) gives the
+  per-request launches' outputs bitwise, eagerly and over graph replays.
 * Autotuner options change only rounding: outputs within 1e-6 rel-L2 of the
   default plan.
 """
@@ -75,6 +77,42 @@ def test_batch_graph_replays_match_eager():
         for e, o in zip(eager, b.outputs()):
             assert torch.equal(e, o), 'replay %d differs from the eager run' % rep
     b.free()
+
+
+def test_batched_gridding_equals_per_request():
+    from tomocupy_amd.batch import BatchRec
+    reqs = _requests(13)
+    print('batch of', [(r['n'], r['nz']) for r in reqs], flush=True)
+    outs = {}
+    for mode in ('streams', 'batched'):
+        b = BatchRec(reqs, fbp_filter='parzen', streams=2, grid=mode)
+        for r, d in zip(reqs, b.inputs()):
+            d.copy_(torch.from_numpy(r['sino'].astype(np.float32)))
+        b.run_eager()
+        torch.cuda.synchronize()
+        outs[mode] = [o.clone() for o in b.outputs()]
+        if mode == 'batched':
+            b.capture()
+            for rep in range(3):
+                for o in b.outputs():
+                    o.fill_(float('nan'))
+                torch.randn(1 << 20, device='cuda').mul_(2.0)
+                torch.cuda.synchronize()
+                b.run()
+                torch.cuda.synchronize()
+                bad = [i for i, (e, o) in enumerate(zip(outs[mode], b.outputs()))
+                       if not torch.equal(e, o)]
+                assert not bad, 'replay %d: requests %s differ' % (rep, bad)
+        b.free()
+    for i, (e, o) in enumerate(zip(outs['streams'], outs['batched'])):
+        assert torch.isfinite(o).all()
+        assert torch.equal(e, o), 'request %d differs' % i
+
+
+def test_batched_gridding_rejects_16bit():
+    from tomocupy_amd.batch import BatchRec
+    with pytest.raises(ValueError):
+        BatchRec(_requests()[:1], dtype='float16', grid='batched')
 
 
 def test_autotune_options_change_only_rounding(tmp_path, monkeypatch):

@@ -193,6 +193,16 @@ def test_batch_96_requests_graph_equals_eager():
         print('replay %d: %d of %d requests differ' % (rep, len(bad), len(eager)), flush=True)
         assert not bad
     b.free()
+    # the same batch with one gridding launch for all 96 requests
+    b = BatchRec(reqs, fbp_filter='parzen', grid='batched')
+    for r, d in zip(reqs, b.inputs()):
+        d.copy_(torch.from_numpy(cache[r['n']])[None].expand(r['nz'], -1, -1))
+    b.run()
+    torch.cuda.synchronize()
+    bad = [i for i, (e, o) in enumerate(zip(eager, b.outputs())) if not torch.equal(e, o)]
+    print('batched gridding: %d of %d requests differ' % (len(bad), len(eager)), flush=True)
+    assert not bad
+    b.free()
 
 
 # ---- differentiation form --------------------------------------------------

@@ -42,6 +42,10 @@ SIGNATURES = {
     'fr_create_opts': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_int,
                                 _c_int, _c_p]),
     'fr_filter_backprojection_prepared': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p]),
+    'fr_batch_create': (_c_int, [ctypes.POINTER(_c_p), ctypes.POINTER(_c_p), _c_int]),
+    'fr_batch_filter_backprojection': (_c_int, [_c_p, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
+                                                ctypes.POINTER(_c_p), _c_p]),
+    'fr_batch_destroy': (None, [_c_p]),
     'flt_prepare': (_c_int, [_c_p, _c_p, _c_d, _c_p, _c_p]),
     'fr_backprojection': (_c_int, [_c_p, _c_p, _c_p, _c_p]),
     'fr_forward': (_c_int, [_c_p, _c_p, _c_p, _c_p]),

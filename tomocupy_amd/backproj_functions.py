@@ -70,21 +70,27 @@ class BackprojFunctions:
                 self.cl_rec._handle(), self.cl_filter._h, po, pd, self.wfilter.data_ptr(),
                 self.center, ps, sp), 'fr_filter_backprojection')
             return obj
-        # a scalar shift: the filter kernels are built once per (centre, shift)
-        key = (self.center, float(sht))
-        if getattr(self, '_prepared', None) != key:
-            ps = None
-            if sht != 0:
-                self._shv = torch.full((self.ncz,), float(sht), dtype=torch.float32,
-                                       device=data.device)
-                ps = self._shv.data_ptr()
-            _lib.check(L.flt_prepare(self.cl_filter._h, self.wfilter.data_ptr(), self.center, ps,
-                                     sp), 'flt_prepare')
-            self._prepared = key
+        self.prepare(sht, stream)
         _lib.check(L.fr_filter_backprojection_prepared(
             self.cl_rec._handle(), self.cl_filter._h, po, pd, sp),
             'fr_filter_backprojection_prepared')
         return obj
+
+    def prepare(self, sht=0, stream=None):
+        """Builds the filter kernels for the centre and the scalar shift
+        ``sht`` unless the last call built them (``flt_prepare``)."""
+        import torch
+        from . import _lib
+        key = (self.center, float(sht))
+        if getattr(self, '_prepared', None) == key:
+            return
+        ps = None
+        if sht != 0:
+            self._shv = torch.full((self.ncz,), float(sht), dtype=torch.float32, device='cuda')
+            ps = self._shv.data_ptr()
+        _lib.check(_lib.lib().flt_prepare(self.cl_filter._h, self.wfilter.data_ptr(), self.center,
+                                          ps, _lib.stream_ptr(stream)), 'flt_prepare')
+        self._prepared = key
 
     def free(self):
         self.cl_rec.free()

@@ -10,8 +10,16 @@ least loaded stream) and joins them, so one graph launch replaces the host
 launch sequence of the whole batch.  Plans own their scratch, so requests on
 different streams never share memory.
 
+With ``grid='batched'`` (float32) the requests share ONE gridding launch
+instead: ``fr_batch_filter_backprojection`` runs every request's filter +
+spectrum stage, then grids all requests together from one work list sorted
+longest first (the small requests' tiles fill the machine beside the large
+ones' instead of each small request launching a grid that leaves most CUs
+idle), then every request's FFT and output stages, all on one stream.
+
 This is the request-level scheduler the reference does not have (it runs one
-volume per process); the per-request arithmetic is exactly the chunk step.
+volume per process); the per-request arithmetic is exactly the chunk step,
+and both modes give the same result bit for bit.
 """
 import numpy as np
 
@@ -30,14 +38,22 @@ def request_bytes(n, nproj, nz, dtype='float32'):
 
 class BatchRec:
     def __init__(self, requests, dtype='float32', fbp_filter='parzen', streams=4,
-                 rec_options=None):
+                 rec_options=None, grid='streams'):
         """``requests``: list of dicts with ``n``, ``nz``, ``theta`` (angles in
         radians, array-like) and optionally ``center`` (default n/2) and
-        ``rec_options`` (FourierRec tuning options, e.g. from the autotuner)."""
+        ``rec_options`` (FourierRec tuning options, e.g. from the autotuner).
+        ``grid``: 'streams' (per-request launches fanned out over ``streams``
+        streams) or 'batched' (one gridding launch for the whole batch,
+        float32 only)."""
         import torch
         if not requests:
             raise ValueError('empty batch')
+        if grid not in ('streams', 'batched'):
+            raise ValueError("grid must be 'streams' or 'batched'")
+        if grid == 'batched' and dtype != 'float32':
+            raise ValueError("grid='batched' needs float32 storage")
         self.dtype = dtype
+        self.grid = grid
         td = _lib.torch_dtype(dtype)
         self.items = []
         for r in requests:
@@ -62,6 +78,17 @@ class BatchRec:
             self.assign[s].append(i)
             load[s] += self.items[i]['cost']
         self.graph = None
+        self._batch = None
+        if grid == 'batched':
+            import ctypes
+            k = len(self.items)
+            plans = (ctypes.c_void_p * k)(*[it['bf'].cl_rec._handle() for it in self.items])
+            h = ctypes.c_void_p()
+            _lib.check(_lib.lib().fr_batch_create(ctypes.byref(h), plans, k), 'fr_batch_create')
+            self._batch = h
+            self._flts = (ctypes.c_void_p * k)(*[it['bf'].cl_filter._h for it in self.items])
+            self._objs = (ctypes.c_void_p * k)(*[it['obj'].data_ptr() for it in self.items])
+            self._datas = (ctypes.c_void_p * k)(*[it['data'].data_ptr() for it in self.items])
 
     @property
     def slices(self):
@@ -78,6 +105,13 @@ class BatchRec:
     def _launch(self):
         import torch
         cur = torch.cuda.current_stream()
+        if self._batch is not None:
+            for it in self.items:
+                it['bf'].prepare(0, cur)
+            _lib.check(_lib.lib().fr_batch_filter_backprojection(
+                self._batch, self._flts, self._objs, self._datas, _lib.stream_ptr(cur)),
+                'fr_batch_filter_backprojection')
+            return
         for s, idx in zip(self.streams, self.assign):
             if not idx:
                 continue
@@ -120,6 +154,9 @@ class BatchRec:
 
     def free(self):
         self.graph = None
+        if self._batch is not None:
+            _lib.lib().fr_batch_destroy(self._batch)
+            self._batch = None
         for it in self.items:
             it['bf'].free()
         self.items = []

@@ -477,20 +477,16 @@ __device__ __forceinline__ void split8(const float *x, half8_t &hi, half8_t &lo)
   lo = __builtin_bit_cast(half8_t, make_uint4(l[0], l[1], l[2], l[3]));
 }
 
-__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
-    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
-    const float2 *__restrict__ G, float2 *__restrict__ F, float *__restrict__ part,
-    const int *__restrict__ ksc, int n, int ntile, int nslots, int nsamp, int nzc, float K2) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int it = blockIdx.x * kGridWaves + wave;
-  const int grp = blockIdx.y;
-  if (it >= nitems) return;
-  const int4 item = items[it];
+// One work item of the float32 gridding for one group of (up to) 4 complex
+// slices: Gg/Fg/Pg are the group's spectra, grid planes and partial tiles,
+// nlive of its 4 slices exist, kA is the group's scale exponent.
+__device__ __forceinline__ void grid_x3_item(const int4 item, const int4 *__restrict__ ent,
+                                             const float2 *__restrict__ Gg,
+                                             float2 *__restrict__ Fg, float *__restrict__ Pg,
+                                             int kA, int n, int ntile, int nsamp, int nlive,
+                                             float K2, int wave, int lane) {
   const int tile = item.x, ebeg = item.y, cnt = item.z, slot = item.w;
   const int g4 = lane >> 4, i = lane & 15;
-  const int nlive = min(4, nzc - 4 * grp);
-  const float2 *Gg = G + (size_t)(4 * grp) * nsamp;
-  const int kA = ksc[0];
   const GaussLane gl(i, K2, 10.f, (float)kA);
   floatx4 acc[8];
 #pragma unroll
@@ -557,10 +553,10 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
     // each plane's base
     const bool whole = (n2 & 15) == 0;
     const int o0 = (16 * TA + ib) * n2 + bcol;
-    if (whole && grp * 4 + 4 <= nzc) {
+    if (whole && nlive == 4) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+        float2 *Fc = Fg + (size_t)c * plane;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           Fc[o0 + r * n2] = make_float2(acc[2 * c][r] * unscale, acc[2 * c + 1][r] * unscale);
@@ -568,8 +564,8 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
     } else {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        if (grp * 4 + c >= nzc) continue;
-        float2 *Fc = F + (size_t)(grp * 4 + c) * plane;
+        if (c >= nlive) continue;
+        float2 *Fc = Fg + (size_t)c * plane;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int a = 16 * TA + ib + r;
@@ -579,12 +575,51 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
       }
     }
   } else {
-    float *P = part + ((size_t)grp * nslots + slot) * 2048;
+    float *Pw = Pg + (size_t)slot * 2048;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) P[m * 256 + (ib + r) * 16 + j] = acc[m][r] * unscale;
+      for (int r = 0; r < 4; ++r) Pw[m * 256 + (ib + r) * 16 + j] = acc[m][r] * unscale;
   }
+}
+
+__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3(
+    const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
+    const float2 *__restrict__ G, float2 *__restrict__ F, float *__restrict__ part,
+    const int *__restrict__ ksc, int n, int ntile, int nslots, int nsamp, int nzc, float K2) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int it = blockIdx.x * kGridWaves + wave;
+  const int grp = blockIdx.y;
+  if (it >= nitems) return;
+  const size_t plane = (size_t)(2 * n) * (2 * n);
+  grid_x3_item(items[it], ent, G + (size_t)(4 * grp) * nsamp, F + (size_t)(4 * grp) * plane,
+               part + (size_t)grp * nslots * 2048, ksc[0], n, ntile, nsamp,
+               min(4, nzc - 4 * grp), K2, wave, lane);
+}
+
+// One group of 4 complex slices of one plan in a batched gridding launch.
+struct GridJob {
+  const int4 *ent;
+  const float2 *G;  // the group's spectra
+  float2 *F;        // the group's first grid plane
+  float *part;      // the group's partial tiles
+  const int *ksc;   // the plan's scale exponent
+  int n, ntile, nsamp, nlive;
+  float K2;
+  int pad[3];
+};
+
+// The float32 gridding of several plans (groups) in one launch: wave w takes
+// work item wi[w] of job wj[w] (the items of all jobs, longest first).
+__global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid_x3_batch(
+    const GridJob *__restrict__ jobs, const int4 *__restrict__ wi, const int *__restrict__ wj,
+    int nwork) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * kGridWaves + wave;
+  if (w >= nwork) return;
+  const GridJob jb = jobs[wj[w]];
+  grid_x3_item(wi[w], jb.ent, jb.G, jb.F, jb.part, jb.ksc[0], jb.n, jb.ntile, jb.nsamp, jb.nlive,
+               jb.K2, wave, lane);
 }
 
 // Sums the partial tiles of heavily loaded tiles in chunk order.
@@ -929,6 +964,7 @@ struct tomo_fr_plan {
   DevBuf ipart;      // (list entries) x 4 complex: tile partials of fr_forward (first use)
   int64_t nent = 0;  // list entries, padding included
   DevBuf span;  // (2n,) int2: columns [x, y) of grid row a that hold samples
+  std::vector<int4> items_h;  // the gridding work items (host copy, for batches)
   DevBuf gmax, gsc;  // float32 plans: per-row spectrum maxima, gridding scale exponent
   int ntile = 0, nitems = 0, nred = 0, nslots = 0;
   DevBuf G, F, R;
@@ -1096,6 +1132,7 @@ int build_tables(tomo_fr_plan *P, const std::vector<float> &theta) {
   // raster order (runs of neighbouring tiles per XCD, for L2 reuse of the
   // spectra) measured 17% slower: the load imbalance outweighs the reuse
   std::stable_sort(items.begin(), items.end(), [](const int4 &x, const int4 &y) { return x.z > y.z; });
+  P->items_h = items;
   P->nitems = (int)items.size();
   P->nred = (int)red.size();
   P->nslots = nslots;
@@ -1328,6 +1365,9 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
   return TOMO_OK;
 }
 
+template <typename T>
+static int fr_tail_after_grid(tomo_fr_plan *P, T *obj, hipStream_t st);
+
 // Gridding, the two pruned FFT passes and the output stage, from G; T is the
 // storage type of G, F, R and obj.
 template <typename T>
@@ -1357,6 +1397,17 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
     }
   }
   TOMO_LAUNCH_CHECK();
+  return fr_tail_after_grid<T>(P, obj, st);
+}
+
+// The stages after the gridding: the partial-tile sums, the two pruned FFT
+// passes and the output stage.
+template <typename T>
+static int fr_tail_after_grid(tomo_fr_plan *P, T *obj, hipStream_t st) {
+  const int n = (int)P->n, nz = (int)P->nz, nzc = (int)P->nzc;
+  const int ng = (int)P->ngrp;
+  const int n2 = 2 * n;
+  cpx<T> *F = P->F.as<cpx<T>>(), *R = P->R.as<cpx<T>>();
   if (P->nred > 0) {
     ProfScope ps_("k_fr_grid_reduce", st);
     hipLaunchKernelGGL(k_fr_grid_reduce<T>, dim3(P->nred, ng), dim3(256), 0, st,
@@ -1403,8 +1454,7 @@ static int fr_backprojection_t(tomo_fr_plan *P, T *obj, const T *data, hipStream
 }
 
 template <typename T>
-static int fr_filter_backprojection_t(tomo_fr_plan *P, tomo_flt_plan *F, T *obj, const T *data,
-                                      hipStream_t st) {
+static int fr_head(tomo_fr_plan *P, tomo_flt_plan *F, const T *data, hipStream_t st) {
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   {
@@ -1417,7 +1467,14 @@ static int fr_filter_backprojection_t(tomo_fr_plan *P, tomo_flt_plan *F, T *obj,
     });
   }
   TOMO_LAUNCH_CHECK();
-  return fr_tail<T>(P, obj, st);
+  return TOMO_OK;
+}
+
+template <typename T>
+static int fr_filter_backprojection_t(tomo_fr_plan *P, tomo_flt_plan *F, T *obj, const T *data,
+                                      hipStream_t st) {
+  const int rc = fr_head<T>(P, F, data, st);
+  return rc ? rc : fr_tail<T>(P, obj, st);
 }
 
 extern "C" int fr_backprojection(tomo_fr_plan *P, void *obj, const void *data, void *stream) {
@@ -1474,6 +1531,130 @@ extern "C" int fr_filter_backprojection_prepared(tomo_fr_plan *P, tomo_flt_plan 
     default:
       return fr_filter_backprojection_t(P, F, (float *)obj, (const float *)data, st);
   }
+}
+
+struct tomo_fr_batch {
+  std::vector<tomo_fr_plan *> plans;
+  DevBuf jobs, wi, wj;
+  int nwork = 0;
+};
+
+extern "C" void fr_batch_destroy(tomo_fr_batch *B) {
+  if (!B) return;
+  B->jobs.release();
+  B->wi.release();
+  B->wj.release();
+  delete B;
+}
+
+extern "C" int fr_batch_create(tomo_fr_batch **batch, tomo_fr_plan *const *plans, int count) {
+  clear_error();
+  TOMO_REQUIRE(batch != nullptr, "fr_batch_create: batch pointer is NULL");
+  *batch = nullptr;
+  TOMO_REQUIRE(plans != nullptr && count >= 1, "fr_batch_create: need at least one plan");
+  for (int r = 0; r < count; ++r) {
+    TOMO_REQUIRE(plans[r] != nullptr, "fr_batch_create: plan %d is NULL", r);
+    TOMO_REQUIRE(plans[r]->dtype == TOMO_F32, "fr_batch_create: plan %d is not float32", r);
+    for (int q = 0; q < r; ++q)
+      TOMO_REQUIRE(plans[q] != plans[r], "fr_batch_create: plan %d appears twice", r);
+  }
+  auto *B = new (std::nothrow) tomo_fr_batch();
+  if (!B) {
+    set_error("fr_batch_create: host allocation failed");
+    return TOMO_ENOMEM;
+  }
+  B->plans.assign(plans, plans + count);
+  std::vector<GridJob> jobs;
+  std::vector<std::pair<int4, int>> work;
+  for (tomo_fr_plan *P : B->plans) {
+    const size_t plane = (size_t)(2 * P->n) * (2 * P->n);
+    for (int g = 0; g < (int)P->ngrp; ++g) {
+      GridJob jb{};
+      jb.ent = P->ent.as<int4>();
+      jb.G = P->G.as<float2>() + (size_t)(4 * g) * P->nsamp;
+      jb.F = P->F.as<float2>() + (size_t)(4 * g) * plane;
+      jb.part = P->part.as<float>() + (size_t)g * P->nslots * 2048;
+      jb.ksc = P->gsc.as<int>();
+      jb.n = (int)P->n;
+      jb.ntile = P->ntile;
+      jb.nsamp = (int)P->nsamp;
+      jb.nlive = (int)std::min<int64_t>(4, P->nzc - 4 * g);
+      jb.K2 = P->K2;
+      for (const int4 &it : P->items_h) work.push_back({it, (int)jobs.size()});
+      jobs.push_back(jb);
+    }
+  }
+  // longest items first over the whole batch (the results do not depend on
+  // the order)
+  std::stable_sort(work.begin(), work.end(),
+                   [](const std::pair<int4, int> &x, const std::pair<int4, int> &y) {
+                     return x.first.z > y.first.z;
+                   });
+  std::vector<int4> wi(work.size());
+  std::vector<int> wj(work.size());
+  for (size_t k = 0; k < work.size(); ++k) wi[k] = work[k].first, wj[k] = work[k].second;
+  B->nwork = (int)work.size();
+  int rc = B->jobs.alloc(sizeof(GridJob) * jobs.size());
+  if (!rc) rc = B->wi.alloc(sizeof(int4) * std::max<size_t>(1, wi.size()));
+  if (!rc) rc = B->wj.alloc(sizeof(int) * std::max<size_t>(1, wj.size()));
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpy(B->jobs.p, jobs.data(), sizeof(GridJob) * jobs.size(), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess && !wi.empty())
+    e = hipMemcpy(B->wi.p, wi.data(), sizeof(int4) * wi.size(), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess && !wj.empty())
+    e = hipMemcpy(B->wj.p, wj.data(), sizeof(int) * wj.size(), hipMemcpyHostToDevice);
+  if (!rc && e != hipSuccess) {
+    set_error("fr_batch_create: copy failed: %s", hipGetErrorString(e));
+    rc = TOMO_EHIP;
+  }
+  if (rc) {
+    fr_batch_destroy(B);
+    return rc;
+  }
+  *batch = B;
+  return TOMO_OK;
+}
+
+extern "C" int fr_batch_filter_backprojection(tomo_fr_batch *B, tomo_flt_plan *const *flts,
+                                              void *const *objs, const void *const *datas,
+                                              void *stream) {
+  clear_error();
+  TOMO_REQUIRE(B != nullptr && flts != nullptr && objs != nullptr && datas != nullptr,
+               "fr_batch_filter_backprojection: NULL argument");
+  const int count = (int)B->plans.size();
+  for (int r = 0; r < count; ++r) {
+    const tomo_fr_plan *P = B->plans[r];
+    const tomo_flt_plan *F = flts[r];
+    TOMO_REQUIRE(F != nullptr && objs[r] != nullptr && datas[r] != nullptr,
+                 "fr_batch_filter_backprojection: NULL array or filter plan for request %d", r);
+    TOMO_REQUIRE(F->n == P->n && F->nproj == P->nproj && F->nz == P->nz,
+                 "fr_batch_filter_backprojection: filter plan %d does not match its plan", r);
+    TOMO_REQUIRE(F->H2.p != nullptr && F->prepared,
+                 "fr_batch_filter_backprojection: filter plan %d is not prepared (flt_prepare)",
+                 r);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  for (int r = 0; r < count; ++r) {
+    tomo_fr_plan *P = B->plans[r];
+    int rc = fr_head<float>(P, flts[r], (const float *)datas[r], st);
+    if (rc) return rc;
+    ProfScope ps_("k_fr_gscale", st);
+    hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, st, P->gmax.as<float>(),
+                       (int)(P->nzc * P->nproj), P->gsc.as<int>());
+  }
+  TOMO_LAUNCH_CHECK();
+  if (B->nwork > 0) {
+    ProfScope ps_("k_fr_grid_x3_batch", st);
+    hipLaunchKernelGGL(k_fr_grid_x3_batch, dim3((unsigned)ceil_div(B->nwork, kGridWaves)),
+                       dim3(64 * kGridWaves), 0, st, B->jobs.as<GridJob>(), B->wi.as<int4>(),
+                       B->wj.as<int>(), B->nwork);
+  }
+  TOMO_LAUNCH_CHECK();
+  for (int r = 0; r < count; ++r) {
+    const int rc = fr_tail_after_grid<float>(B->plans[r], (float *)objs[r], st);
+    if (rc) return rc;
+  }
+  return TOMO_OK;
 }
 
 extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *stream) {
