@@ -158,7 +158,10 @@ int fr_filter_backprojection_prepared(tomo_fr_plan *plan, tomo_flt_plan *flt, vo
  * longest first over the whole batch, so that the small requests of a
  * mixed batch fill the machine beside the large ones.  Each result equals
  * that of fr_filter_backprojection_prepared on the same plan bit for bit.
- * The plans' scratch is used; plans must not run elsewhere meanwhile.
+ * The call is ordered on `stream` (it starts after the work queued there
+ * before it, and work queued after it waits for all of it; capturable into a
+ * graph); inside, the per-request stages fan out over 4 streams the batch
+ * owns.  The plans' scratch is used; plans must not run elsewhere meanwhile.
  *   batch  out: the batch handle (fr_batch_destroy frees it, not the plans)
  *   plans  host array of `count` fr plans, float32, distinct
  *   flts   host array of prepared filter plans (flt_prepare), one per plan

@@ -12,10 +12,11 @@ different streams never share memory.
 
 With ``grid='batched'`` (float32) the requests share ONE gridding launch
 instead: ``fr_batch_filter_backprojection`` runs every request's filter +
-spectrum stage, then grids all requests together from one work list sorted
-longest first (the small requests' tiles fill the machine beside the large
-ones' instead of each small request launching a grid that leaves most CUs
-idle), then every request's FFT and output stages, all on one stream.
+spectrum stage (fanned out over 4 streams the batch owns), then grids all
+requests together from one work list (the largest requests' items first, each
+request's items together; the small requests' short items fill the tail
+instead of each small request launching a grid that leaves most CUs idle),
+then every request's FFT and output stages (fanned out again).
 
 This is the request-level scheduler the reference does not have (it runs one
 volume per process); the per-request arithmetic is exactly the chunk step,

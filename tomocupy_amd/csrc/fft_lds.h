@@ -476,24 +476,30 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
   }
 }
 
-// ---- transforms of a compile-time power-of-two size -------------------------
-// The same Stockham passes with N, the radix sequence (radix 16 passes, then
-// one radix 8/4/2 pass for the remaining factor, as FftPlan::init picks), the
-// sub-transform lengths, the twiddle steps and the interleave 2^LGH all
-// constants.  With one pad slot per 16 elements, the R reads of a butterfly
-// (stride N/R, a multiple of 16) and its R writes (stride Ns, a multiple of 16,
-// or consecutive when Ns = 1) sit at constant offsets from one base address,
-// so a pass costs one address computation per butterfly instead of one per
-// element.
+// ---- transforms of a compile-time size ---------------------------------------
+// The same Stockham passes with N, the radix sequence, the sub-transform
+// lengths, the twiddle steps and the interleave 2^LGH all constants, for
+// N = 2^k or 3 * 2^k with 2^k >= 256.  The radices are those FftPlan::radices
+// picks: radix 16 passes, one radix 8/4/2 pass for the remaining power of
+// two, then the radix-3 pass, so every sub-transform length Ns is a power of
+// two.  With one pad slot per 16 elements, the R reads of a butterfly
+// (stride N/R, a multiple of 16) and its R writes (stride Ns, a multiple of
+// 16, or consecutive when Ns = 1) sit at constant offsets from one base
+// address, so a pass costs one address computation per butterfly instead of
+// one per element.
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 
 template <int N>
-struct Pow2Plan {
-  static_assert(N >= 256 && (N & (N - 1)) == 0, "power of two >= 256");
-  static constexpr int n16 = ilog2c(N) / 4;
-  static constexpr int rest = 1 << (ilog2c(N) % 4);
-  static constexpr int passes = n16 + (rest > 1 ? 1 : 0);
-  static constexpr int radix(int s) { return s < n16 ? 16 : rest; }
+struct StaticPlan {
+  static constexpr int three = N % 3 == 0 ? 1 : 0;
+  static constexpr int P2 = three ? N / 3 : N;
+  static_assert(P2 >= 256 && (P2 & (P2 - 1)) == 0, "2^k or 3 * 2^k with 2^k >= 256");
+  static constexpr int n16 = ilog2c(P2) / 4;
+  static constexpr int rest = 1 << (ilog2c(P2) % 4);
+  static constexpr int passes = n16 + (rest > 1 ? 1 : 0) + three;
+  static constexpr int radix(int s) {
+    return s < n16 ? 16 : (s == n16 && rest > 1) ? rest : 3;
+  }
 };
 
 template <int R, int Ns, int N, int DIR, int EPT, int LGH>
@@ -536,24 +542,29 @@ __device__ __forceinline__ void fft_pass_c(float2 *buf, const float2 *__restrict
   __syncthreads();
 }
 
-// lds_fft for a compile-time size N (a power of two >= 256); `tw` is the
-// twiddle table of size N.  Same layout, requirements and barriers as lds_fft.
+// lds_fft for a compile-time size N (StaticPlan); `tw` is the twiddle table
+// of size N.  Same layout, requirements and barriers as lds_fft.
 template <int N, int DIR, int EPT, int LGH, int S = 0, int Ns = 1>
 __device__ __forceinline__ void lds_fft_c(float2 *buf, const float2 *__restrict__ tw, int tid,
                                           int nt) {
-  if constexpr (S < Pow2Plan<N>::passes) {
-    constexpr int R = Pow2Plan<N>::radix(S);
+  if constexpr (S < StaticPlan<N>::passes) {
+    constexpr int R = StaticPlan<N>::radix(S);
     fft_pass_c<R, Ns, N, DIR, EPT, LGH>(buf, tw, tid, nt);
     lds_fft_c<N, DIR, EPT, LGH, S + 1, Ns * R>(buf, tw, tid, nt);
   }
 }
 
-// Calls f(std::integral_constant<int, N>) for the power-of-two sizes with a
-// compiled specialisation, f(std::integral_constant<int, 0>) otherwise.
+// Calls f(std::integral_constant<int, N>) for the sizes with a compiled
+// specialisation (2^k and 3 * 2^k), f(std::integral_constant<int, 0>)
+// otherwise.
 template <typename F>
 inline void with_static_size(int N, F &&f) {
   switch (N) {
     case 512: f(std::integral_constant<int, 512>()); break;
+    case 768: f(std::integral_constant<int, 768>()); break;
+    case 1536: f(std::integral_constant<int, 1536>()); break;
+    case 3072: f(std::integral_constant<int, 3072>()); break;
+    case 6144: f(std::integral_constant<int, 6144>()); break;
     case 1024: f(std::integral_constant<int, 1024>()); break;
     case 2048: f(std::integral_constant<int, 2048>()); break;
     case 4096: f(std::integral_constant<int, 4096>()); break;

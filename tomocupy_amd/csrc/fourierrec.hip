@@ -747,9 +747,9 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft(const cpx<T> 
 
 // ---- adjoint (forward projection) kernels --------------------------------
 
-// NS as k_fr_colfft: a compiled power-of-two 2n (1024 threads, col_lg(NS)
-// columns per workgroup) or 0 / -1 / -2 for the run-time plans of fft_level
-// 0 / 1 / 2.
+// NS as k_fr_colfft: a compiled 2n = 2^k or 3 * 2^k (up to 1024 threads,
+// col_lg(NS) columns per workgroup) or 0 / -1 / -2 for the run-time plans of
+// fft_level 0 / 1 / 2.
 template <int NS>
 __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft_adj(const float *__restrict__ obj, float2 *__restrict__ R,
                                 const float *__restrict__ phix, const float *__restrict__ phiy,
@@ -1307,13 +1307,14 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
     set_error("fr_create: n=%lld needs more than 512 threads per FFT workgroup", (long long)n);
     return fail(TOMO_EINVAL);
   }
-  // size-specialised kernels where 2n is a compiled power of two (and the
-  // column grouping matches the specialisation)
+  // size-specialised kernels where 2n is a compiled size, 2^k or 3 * 2^k (and
+  // the column grouping matches the specialisation)
   P->nsRow = fft_generic ? 0 : (int)n2;
   P->nsCol = (!fft_generic && lg == col_lg((int)n2)) ? (int)n2 : 0;
   with_static_size(P->nsRow, [&](auto ns) { P->nsRow = decltype(ns)::value; });
   with_static_size(P->nsCol, [&](auto ns) { P->nsCol = decltype(ns)::value; });
-  if (P->nsCol > 0) P->colThreadsStatic = (int)((n2 << lg) / kFftEPT);
+  // 16 values per thread (15 with the radix-3 pass of the 3 * 2^k sizes)
+  if (P->nsCol > 0) P->colThreadsStatic = fft_threads(P->fft2.d, n2 << lg, kFftEPT);
   // column-blocked R for 16-bit plans only: measured at n=2048 the column
   // pass gains 40-50 us per chunk of 8 slices, the row pass loses 22 us
   // (16-byte segments) in half precision and 43 us (32-byte) in float32
@@ -1533,10 +1534,19 @@ extern "C" int fr_filter_backprojection_prepared(tomo_fr_plan *P, tomo_flt_plan 
   }
 }
 
+// A batch: the plans, the gridding work list over all of them, and
+// kBatchStreams internal streams (default priority) over which the
+// per-request stages fan out from the caller's stream and join back, through
+// events, so that the call stays stream-ordered and capturable.
+constexpr int kBatchStreams = 4;
+
 struct tomo_fr_batch {
   std::vector<tomo_fr_plan *> plans;
+  std::vector<int> lane;  // internal stream of each request (longest first, least loaded)
   DevBuf jobs, wi, wj;
   int nwork = 0;
+  hipStream_t st[kBatchStreams] = {};
+  hipEvent_t fork = nullptr, mid = nullptr, done[kBatchStreams] = {};
 };
 
 extern "C" void fr_batch_destroy(tomo_fr_batch *B) {
@@ -1544,6 +1554,12 @@ extern "C" void fr_batch_destroy(tomo_fr_batch *B) {
   B->jobs.release();
   B->wi.release();
   B->wj.release();
+  for (int i = 0; i < kBatchStreams; ++i) {
+    if (B->st[i]) (void)hipStreamDestroy(B->st[i]);
+    if (B->done[i]) (void)hipEventDestroy(B->done[i]);
+  }
+  if (B->fork) (void)hipEventDestroy(B->fork);
+  if (B->mid) (void)hipEventDestroy(B->mid);
   delete B;
 }
 
@@ -1564,9 +1580,32 @@ extern "C" int fr_batch_create(tomo_fr_batch **batch, tomo_fr_plan *const *plans
     return TOMO_ENOMEM;
   }
   B->plans.assign(plans, plans + count);
+  // requests by decreasing cost (spectra + grid planes, in elements)
+  std::vector<double> cost(count);
+  for (int r = 0; r < count; ++r) {
+    const tomo_fr_plan *P = B->plans[r];
+    cost[r] = (double)P->nz * P->nproj * P->n + (double)P->nzc * 4.0 * P->n * P->n;
+  }
+  std::vector<int> order(count);
+  for (int r = 0; r < count; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  B->lane.assign(count, 0);
+  double load[kBatchStreams] = {};
+  for (int r : order) {
+    int k = 0;
+    for (int i = 1; i < kBatchStreams; ++i)
+      if (load[i] < load[k]) k = i;
+    B->lane[r] = k;
+    load[k] += cost[r];
+  }
+  // the gridding work: each job's (plan, group's) items stay together, in the
+  // plan's longest-first order, jobs of the largest requests first; so
+  // consecutive workgroups share a plan's spectra in L2 as in the per-plan
+  // launch, and the small requests' short items fill the tail
   std::vector<GridJob> jobs;
   std::vector<std::pair<int4, int>> work;
-  for (tomo_fr_plan *P : B->plans) {
+  for (int r : order) {
+    tomo_fr_plan *P = B->plans[r];
     const size_t plane = (size_t)(2 * P->n) * (2 * P->n);
     for (int g = 0; g < (int)P->ngrp; ++g) {
       GridJob jb{};
@@ -1584,12 +1623,6 @@ extern "C" int fr_batch_create(tomo_fr_batch **batch, tomo_fr_plan *const *plans
       jobs.push_back(jb);
     }
   }
-  // longest items first over the whole batch (the results do not depend on
-  // the order)
-  std::stable_sort(work.begin(), work.end(),
-                   [](const std::pair<int4, int> &x, const std::pair<int4, int> &y) {
-                     return x.first.z > y.first.z;
-                   });
   std::vector<int4> wi(work.size());
   std::vector<int> wj(work.size());
   for (size_t k = 0; k < work.size(); ++k) wi[k] = work[k].first, wj[k] = work[k].second;
@@ -1607,11 +1640,52 @@ extern "C" int fr_batch_create(tomo_fr_batch **batch, tomo_fr_plan *const *plans
     set_error("fr_batch_create: copy failed: %s", hipGetErrorString(e));
     rc = TOMO_EHIP;
   }
+  for (int i = 0; i < kBatchStreams && !rc; ++i) {
+    e = hipStreamCreateWithFlags(&B->st[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B->done[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      set_error("fr_batch_create: stream or event creation failed: %s", hipGetErrorString(e));
+      rc = TOMO_EHIP;
+    }
+  }
+  if (!rc) {
+    e = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B->mid, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      set_error("fr_batch_create: event creation failed: %s", hipGetErrorString(e));
+      rc = TOMO_EHIP;
+    }
+  }
   if (rc) {
     fr_batch_destroy(B);
     return rc;
   }
   *batch = B;
+  return TOMO_OK;
+}
+
+// The caller's stream `st` -> every internal stream (after `ev` on st).
+static int batch_fork(tomo_fr_batch *B, hipStream_t st, hipEvent_t ev) {
+  hipError_t e = hipEventRecord(ev, st);
+  for (int i = 0; i < kBatchStreams && e == hipSuccess; ++i) e = hipStreamWaitEvent(B->st[i], ev, 0);
+  if (e != hipSuccess) {
+    set_error("fr_batch_filter_backprojection: fork failed: %s", hipGetErrorString(e));
+    return TOMO_EHIP;
+  }
+  return TOMO_OK;
+}
+
+// Every internal stream -> the caller's stream.
+static int batch_join(tomo_fr_batch *B, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < kBatchStreams && e == hipSuccess; ++i) {
+    e = hipEventRecord(B->done[i], B->st[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, B->done[i], 0);
+  }
+  if (e != hipSuccess) {
+    set_error("fr_batch_filter_backprojection: join failed: %s", hipGetErrorString(e));
+    return TOMO_EHIP;
+  }
   return TOMO_OK;
 }
 
@@ -1634,15 +1708,21 @@ extern "C" int fr_batch_filter_backprojection(tomo_fr_batch *B, tomo_flt_plan *c
                  r);
   }
   hipStream_t st = (hipStream_t)stream;
-  for (int r = 0; r < count; ++r) {
+  // filter + spectra of every request, fanned out over the internal streams
+  int rc = batch_fork(B, st, B->fork);
+  for (int r = 0; r < count && !rc; ++r) {
     tomo_fr_plan *P = B->plans[r];
-    int rc = fr_head<float>(P, flts[r], (const float *)datas[r], st);
-    if (rc) return rc;
-    ProfScope ps_("k_fr_gscale", st);
-    hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, st, P->gmax.as<float>(),
+    hipStream_t s = B->st[B->lane[r]];
+    rc = fr_head<float>(P, flts[r], (const float *)datas[r], s);
+    if (rc) break;
+    ProfScope ps_("k_fr_gscale", s);
+    hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, s, P->gmax.as<float>(),
                        (int)(P->nzc * P->nproj), P->gsc.as<int>());
   }
+  if (!rc) rc = batch_join(B, st);
+  if (rc) return rc;
   TOMO_LAUNCH_CHECK();
+  // one gridding launch for all requests, on the caller's stream
   if (B->nwork > 0) {
     ProfScope ps_("k_fr_grid_x3_batch", st);
     hipLaunchKernelGGL(k_fr_grid_x3_batch, dim3((unsigned)ceil_div(B->nwork, kGridWaves)),
@@ -1650,11 +1730,12 @@ extern "C" int fr_batch_filter_backprojection(tomo_fr_batch *B, tomo_flt_plan *c
                        B->wj.as<int>(), B->nwork);
   }
   TOMO_LAUNCH_CHECK();
-  for (int r = 0; r < count; ++r) {
-    const int rc = fr_tail_after_grid<float>(B->plans[r], (float *)objs[r], st);
-    if (rc) return rc;
-  }
-  return TOMO_OK;
+  // partial-tile sums, FFT passes and output stage, fanned out again
+  rc = batch_fork(B, st, B->mid);
+  for (int r = 0; r < count && !rc; ++r)
+    rc = fr_tail_after_grid<float>(B->plans[r], (float *)objs[r], B->st[B->lane[r]]);
+  if (!rc) rc = batch_join(B, st);
+  return rc;
 }
 
 extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *stream) {
