@@ -544,14 +544,109 @@ __device__ __forceinline__ void fft_pass_c(float2 *buf, const float2 *__restrict
 
 // lds_fft for a compile-time size N (StaticPlan); `tw` is the twiddle table
 // of size N.  Same layout, requirements and barriers as lds_fft.
-template <int N, int DIR, int EPT, int LGH, int S = 0, int Ns = 1>
+// Passes S .. SEnd-1 only (Ns: the sub-transform length entering pass S).
+template <int N, int DIR, int EPT, int LGH, int S = 0, int Ns = 1,
+          int SEnd = StaticPlan<N>::passes>
 __device__ __forceinline__ void lds_fft_c(float2 *buf, const float2 *__restrict__ tw, int tid,
                                           int nt) {
-  if constexpr (S < StaticPlan<N>::passes) {
+  if constexpr (S < SEnd) {
     constexpr int R = StaticPlan<N>::radix(S);
     fft_pass_c<R, Ns, N, DIR, EPT, LGH>(buf, tw, tid, nt);
-    lds_fft_c<N, DIR, EPT, LGH, S + 1, Ns * R>(buf, tw, tid, nt);
+    lds_fft_c<N, DIR, EPT, LGH, S + 1, Ns * R, SEnd>(buf, tw, tid, nt);
   }
+}
+
+// ---- compile-time transforms with global input and output --------------
+// fft_c_global runs the transform of lds_fft_c, but its first pass takes its
+// inputs from ld(i, t) (element i of interleaved sequence t, straight into
+// registers) and its last pass hands each output to st(i, t, value) instead
+// of writing it to LDS: two LDS sweeps and four barriers fewer than staging
+// the input in LDS and reading the result back.  The caller must not use
+// `buf` concurrently (it is written without a barrier first) and gets no
+// barrier at the end.  Requires at least two passes.
+
+// Pass 0 (Ns = 1, no twiddles) from ld into LDS.
+template <int R, int N, int DIR, int EPT, int LGH, typename Ld>
+__device__ __forceinline__ void fft_pass_c_first(float2 *buf, int tid, int nt, Ld ld) {
+  constexpr int MAXB = EPT / R;
+  constexpr int H = 1 << LGH;
+  constexpr int nbPer = N / R;
+  constexpr int nb = nbPer << LGH;
+  static_assert(MAXB >= 1 && nbPer % 16 == 0, "layout");
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> LGH;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = ld(j + r * nbPer, t);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> LGH;
+      butterfly<R, DIR>(v[q]);
+      const int base = (lds_pad(j * R) << LGH) + t;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * H] = v[q][out_slot<R>(r)];
+    }
+  }
+}
+
+// The last pass (sub-transform length Ns) from LDS to st.
+template <int R, int Ns, int N, int DIR, int EPT, int LGH, typename St>
+__device__ __forceinline__ void fft_pass_c_last(const float2 *buf, const float2 *__restrict__ tw,
+                                                int tid, int nt, St st) {
+  constexpr int MAXB = EPT / R;
+  constexpr int H = 1 << LGH;
+  constexpr int nbPer = N / R;
+  constexpr int nb = nbPer << LGH;
+  constexpr int twstep = N / (Ns * R);
+  static_assert(MAXB >= 1 && nbPer % 16 == 0 && Ns > 1 && Ns % 16 == 0, "layout");
+  constexpr int rstride = (nbPer + nbPer / 16) << LGH;
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int g = tid + q * nt;
+    if (g < nb) {
+      const int t = g & (H - 1), j = g >> LGH;
+      const int base = (lds_pad(j) << LGH) + t;
+      float2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = buf[base + r * rstride];
+      const int k = j % Ns;
+      twiddle<R, DIR>(v, tw, (unsigned)(k * twstep));
+      butterfly<R, DIR>(v);
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) st(o + r * Ns, t, v[out_slot<R>(r)]);
+    }
+  }
+}
+
+// First pass from ld, the rest in LDS (ends with a barrier, result in buf).
+template <int N, int DIR, int EPT, int LGH, typename Ld>
+__device__ __forceinline__ void fft_c_from_global(float2 *buf, const float2 *__restrict__ tw,
+                                                  int tid, int nt, Ld ld) {
+  constexpr int R0 = StaticPlan<N>::radix(0);
+  fft_pass_c_first<R0, N, DIR, EPT, LGH>(buf, tid, nt, ld);
+  __syncthreads();
+  lds_fft_c<N, DIR, EPT, LGH, 1, R0>(buf, tw, tid, nt);
+}
+
+template <int N, int DIR, int EPT, int LGH, typename Ld, typename St>
+__device__ __forceinline__ void fft_c_global(float2 *buf, const float2 *__restrict__ tw, int tid,
+                                             int nt, Ld ld, St st) {
+  using SP = StaticPlan<N>;
+  static_assert(SP::passes >= 2, "two passes at least");
+  constexpr int R0 = SP::radix(0);
+  constexpr int RL = SP::radix(SP::passes - 1);
+  fft_pass_c_first<R0, N, DIR, EPT, LGH>(buf, tid, nt, ld);
+  __syncthreads();
+  lds_fft_c<N, DIR, EPT, LGH, 1, R0, SP::passes - 1>(buf, tw, tid, nt);
+  fft_pass_c_last<RL, N / RL, N, DIR, EPT, LGH>(buf, tw, tid, nt, st);
 }
 
 // Calls f(std::integral_constant<int, N>) for the sizes with a compiled

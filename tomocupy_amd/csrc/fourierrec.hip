@@ -81,6 +81,18 @@ __device__ __forceinline__ void store_spectrum(cpx<T> *out, const float2 *sm, in
   }
 }
 
+// The gmax part of store_spectrum for a thread's running maximum m (every
+// thread must call it).
+template <typename T>
+__device__ __forceinline__ void store_max(float m, float *gmax, size_t row) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (gmax) {
+      m = block_max(m);
+      if (threadIdx.x == 0) gmax[row] = m;
+    }
+  }
+}
+
 template <typename T, int NS>
 __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx<T> *__restrict__ G,
                                                  FftDesc fd, int n, int nproj, int nz, int nzc,
@@ -94,15 +106,27 @@ __global__ __launch_bounds__(512) void k_fr_prep(const T *__restrict__ data, cpx
   const int z2 = c + nzc;
   const T *r2 = data + ((size_t)(z2 < nz ? z2 : c) * nproj + p) * n;
   const bool has2 = z2 < nz;
-  stage_lds<8, cpx<T>>(
-      n, tid, nt, [&](int j) { return cpx<T>{r1[j], has2 ? r2[j] : T(0)}; },
-      [&](int j, cpx<T> v) { sm[lds_pad(j)] = cscale(tof2(v), shift_sign(j)); });
-  __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd.tw, tid, nt);
-  else
+  if constexpr (NS > 0) {
+    float m = 0.f;
+    fft_c_global<NS, -1, kFftEPT, 0>(
+        sm, fd.tw, tid, nt,
+        [&](int j, int) {
+          return cscale(tof2(cpx<T>{r1[j], has2 ? r2[j] : T(0)}), shift_sign(j));
+        },
+        [&](int k, int, float2 x) {
+          const float2 v = cscale(x, shift_sign(k) * scale);
+          m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+          stc(out + k, v);
+        });
+    store_max<T>(m, gmax, (size_t)c * nproj + p);
+  } else {
+    stage_lds<8, cpx<T>>(
+        n, tid, nt, [&](int j) { return cpx<T>{r1[j], has2 ? r2[j] : T(0)}; },
+        [&](int j, cpx<T> v) { sm[lds_pad(j)] = cscale(tof2(v), shift_sign(j)); });
+    __syncthreads();
     lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
-  store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
+    store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
+  }
 }
 
 // Fused FBP filter + pair packing + shifted detector FFT (the fused form of
@@ -128,21 +152,25 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
   const int zb = has2 ? z2 : c;
   const T *r1 = data + ((size_t)c * nproj + p) * n;
   const T *r2 = data + ((size_t)zb * nproj + p) * n;
-  stage_lds<8, cpx<T>>(
-      n2, tid, nt,
-      [&](int j) { return j < n ? cpx<T>{r1[j], has2 ? r2[j] : T(0)} : cpx<T>{T(0), T(0)}; },
-      [&](int j, cpx<T> v) { sm[lds_pad(j)] = tof2(v); });
   if (tid == 0) {
     edge[0] = ldr(r1);
     edge[1] = ldr(r1 + n - 1);
     edge[2] = has2 ? ldr(r2) : 0.f;
     edge[3] = has2 ? ldr(r2 + n - 1) : 0.f;
   }
-  __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<2 * NS, -1, kFftEPT, 0>(sm, fd2.tw, tid, nt);
-  else
+  if constexpr (NS > 0) {
+    // (the barriers inside make edge[] visible before its use below)
+    fft_c_from_global<2 * NS, -1, kFftEPT, 0>(sm, fd2.tw, tid, nt, [&](int j, int) {
+      return j < n ? tof2(cpx<T>{r1[j], has2 ? r2[j] : T(0)}) : make_float2(0.f, 0.f);
+    });
+  } else {
+    stage_lds<8, cpx<T>>(
+        n2, tid, nt,
+        [&](int j) { return j < n ? cpx<T>{r1[j], has2 ? r2[j] : T(0)} : cpx<T>{T(0), T(0)}; },
+        [&](int j, cpx<T> v) { sm[lds_pad(j)] = tof2(v); });
+    __syncthreads();
     lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd2, 0, tid, nt);
+  }
   const float2 *Ha = H2 + (size_t)c * n2;
   const float2 *Hb = H2 + (size_t)zb * n2;
   for (int k = tid; k <= n; k += nt) {
@@ -171,11 +199,22 @@ __global__ __launch_bounds__(512) void k_fr_filtprep(
     sm[lds_pad(j)] = cscale(y, shift_sign(j));
   }
   __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd1.tw, tid, nt);
-  else
+  if constexpr (NS > 0) {
+    // the n-point pass from LDS, its last stage straight to the spectrum
+    constexpr int RL = StaticPlan<NS>::radix(StaticPlan<NS>::passes - 1);
+    lds_fft_c<NS, -1, kFftEPT, 0, 0, 1, StaticPlan<NS>::passes - 1>(sm, fd1.tw, tid, nt);
+    float m = 0.f;
+    fft_pass_c_last<RL, NS / RL, NS, -1, kFftEPT, 0>(sm, fd1.tw, tid, nt,
+                                                     [&](int k, int, float2 x) {
+      const float2 v = cscale(x, shift_sign(k) * scale);
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+      stc(out + k, v);
+    });
+    store_max<T>(m, gmax, (size_t)c * nproj + p);
+  } else {
     lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd1, 0, tid, nt);
-  store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
+    store_spectrum(out, sm, n, scale, tid, nt, gmax, (size_t)c * nproj + p);
+  }
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -665,32 +704,56 @@ __global__ __launch_bounds__(512) void k_fr_rowfft(const cpx<T> *__restrict__ F,
   const cpx<T> *src = F + ((size_t)c * n2 + a) * n2;
   // only the row's occupied span was gridded; the rest of the row is zero
   const int2 sp = span[a];
-  stage_lds<8, cpx<T>>(
-      n2, tid, nt,
-      [&](int b) { return (b >= sp.x && b < sp.y) ? src[b] : cpx<T>{T(0), T(0)}; },
-      [&](int b, cpx<T> v) {
-        const float2 x = tof2(v);
-        sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
-      });
-  __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<NS, 1, kFftEPT, 0>(sm, fd.tw, tid, nt);
-  else
-    lds_fft<1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
   const int h = n / 2;
-  if (lgB > 0) {
-    const int B = 1 << lgB;
-    cpx<T> *dst = R + (size_t)c * n2 * n + (size_t)a * B;
-    for (int v = tid; v < n; v += nt) {
-      const float2 x = sm[lds_pad(v + h)];
-      stc(dst + (size_t)(v >> lgB) * n2 * B + (v & (B - 1)),
-          ((v + h) & 1) ? make_float2(-x.x, -x.y) : x);
-    }
+  // compiled sizes, float32: the first pass loads the row and the last one
+  // stores R (measured at n = 2048: 161 -> 146 us per chunk of 8 slices; the
+  // 16-bit column-blocked R went from 164 to 209 us that way, so 16-bit plans
+  // stage through LDS)
+  if constexpr (NS > 0 && std::is_same<T, float>::value) {
+    fft_c_global<NS, 1, kFftEPT, 0>(
+        sm, fd.tw, tid, nt,
+        [&](int b, int) {
+          const float2 x = (b >= sp.x && b < sp.y) ? tof2(src[b]) : make_float2(0.f, 0.f);
+          return (b & 1) ? make_float2(-x.x, -x.y) : x;
+        },
+        [&](int o, int, float2 x) {
+          const int v = o - h;
+          if (v < 0 || v >= n) return;
+          if (o & 1) x = make_float2(-x.x, -x.y);
+          if (lgB > 0)
+            stc(R + (size_t)c * n2 * n + (size_t)a * (1 << lgB) +
+                    (size_t)(v >> lgB) * n2 * (1 << lgB) + (v & ((1 << lgB) - 1)),
+                x);
+          else
+            stc(R + ((size_t)c * n2 + a) * n + v, x);
+        });
   } else {
-    cpx<T> *dst = R + ((size_t)c * n2 + a) * n;
-    for (int v = tid; v < n; v += nt) {
-      const float2 x = sm[lds_pad(v + h)];
-      stc(dst + v, ((v + h) & 1) ? make_float2(-x.x, -x.y) : x);
+    stage_lds<8, cpx<T>>(
+        n2, tid, nt,
+        [&](int b) { return (b >= sp.x && b < sp.y) ? src[b] : cpx<T>{T(0), T(0)}; },
+        [&](int b, cpx<T> v) {
+          const float2 x = tof2(v);
+          sm[lds_pad(b)] = (b & 1) ? make_float2(-x.x, -x.y) : x;
+        });
+    __syncthreads();
+    if constexpr (NS > 0)
+      lds_fft_c<NS, 1, kFftEPT, 0>(sm, fd.tw, tid, nt);
+    else
+      lds_fft<1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
+    if (lgB > 0) {
+      const int B = 1 << lgB;
+      cpx<T> *dst = R + (size_t)c * n2 * n + (size_t)a * B;
+      for (int v = tid; v < n; v += nt) {
+        const float2 x = sm[lds_pad(v + h)];
+        stc(dst + (size_t)(v >> lgB) * n2 * B + (v & (B - 1)),
+            ((v + h) & 1) ? make_float2(-x.x, -x.y) : x);
+      }
+    } else {
+      cpx<T> *dst = R + ((size_t)c * n2 + a) * n;
+      for (int v = tid; v < n; v += nt) {
+        const float2 x = sm[lds_pad(v + h)];
+        stc(dst + v, ((v + h) & 1) ? make_float2(-x.x, -x.y) : x);
+      }
     }
   }
 }
@@ -713,35 +776,61 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft(const cpx<T> 
   const int n2 = 2 * n, v0 = cb * CB;
   const cpx<T> *src = R + (size_t)c * n2 * n;
   const cpx<T> *blk = src + (size_t)v0 * n2;  // block cb of the column-blocked layout
-  stage_lds<16, cpx<T>>(
-      CB * n2, tid, nt,
-      [&](int idx) {
-        const int a = idx >> lgCB, t = idx & (CB - 1);
-        if (blocked) return blk[idx];
-        return (v0 + t < n) ? src[(size_t)a * n + v0 + t] : cpx<T>{T(0), T(0)};
-      },
-      [&](int idx, cpx<T> v) {
-        const int a = idx >> lgCB, t = idx & (CB - 1);
-        const float2 x = tof2(v);
-        sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
-      });
-  __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<NS, 1, kFftEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
-  else
-    lds_fft<1, (NS < 0 ? kFftEPT : kColEPT), fft_lv(NS)>(sm, fd, lgCB, tid, nt);
   const int z2 = c + nzc;
   const int h = n / 2;
-  for (int idx = tid; idx < CB * n; idx += nt) {
-    const int ty = idx >> lgCB, t = idx & (CB - 1);
-    const int tx = v0 + t;
-    if (tx >= n) continue;
-    const int u = ty + 1 + h;
-    float2 x = sm[(lds_pad(u) << lgCB) + t];
-    const int2 ci = circ[ty];
-    const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
-    str(obj + ((size_t)c * n + ty) * n + tx, x.x * ph);
-    if (z2 < nz) str(obj + ((size_t)z2 * n + ty) * n + tx, x.y * ph);
+  // compiled sizes, float32: the first pass loads R and the last one stores
+  // the slices (n = 2048: 205 -> 187 us per chunk of 8 slices; 16-bit plans,
+  // 129 -> 145 us, stage through LDS)
+  if constexpr (NS > 0 && std::is_same<T, float>::value) {
+    constexpr int LG = col_lg(NS);  // == lgCB (the plan picks NS only then)
+    fft_c_global<NS, 1, kFftEPT, LG>(
+        sm, fd.tw, tid, nt,
+        [&](int a, int t) {
+          float2 x;
+          if (blocked)
+            x = tof2(blk[(a << LG) + t]);
+          else
+            x = (v0 + t < n) ? tof2(src[(size_t)a * n + v0 + t]) : make_float2(0.f, 0.f);
+          return (a & 1) ? make_float2(-x.x, -x.y) : x;
+        },
+        [&](int u, int t, float2 x) {
+          const int ty = u - 1 - h, tx = v0 + t;
+          if (ty < 0 || ty >= n || tx >= n) return;
+          const int2 ci = circ[ty];
+          const float ph =
+              (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
+          str(obj + ((size_t)c * n + ty) * n + tx, x.x * ph);
+          if (z2 < nz) str(obj + ((size_t)z2 * n + ty) * n + tx, x.y * ph);
+        });
+  } else {
+    stage_lds<16, cpx<T>>(
+        CB * n2, tid, nt,
+        [&](int idx) {
+          const int a = idx >> lgCB, t = idx & (CB - 1);
+          if (blocked) return blk[idx];
+          return (v0 + t < n) ? src[(size_t)a * n + v0 + t] : cpx<T>{T(0), T(0)};
+        },
+        [&](int idx, cpx<T> v) {
+          const int a = idx >> lgCB, t = idx & (CB - 1);
+          const float2 x = tof2(v);
+          sm[(lds_pad(a) << lgCB) + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
+        });
+    __syncthreads();
+    if constexpr (NS > 0)
+      lds_fft_c<NS, 1, kFftEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
+    else
+      lds_fft<1, (NS < 0 ? kFftEPT : kColEPT), fft_lv(NS)>(sm, fd, lgCB, tid, nt);
+    for (int idx = tid; idx < CB * n; idx += nt) {
+      const int ty = idx >> lgCB, t = idx & (CB - 1);
+      const int tx = v0 + t;
+      if (tx >= n) continue;
+      const int u = ty + 1 + h;
+      float2 x = sm[(lds_pad(u) << lgCB) + t];
+      const int2 ci = circ[ty];
+      const float ph = (tx >= ci.x && tx < ci.y) ? phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f) : 0.f;
+      str(obj + ((size_t)c * n + ty) * n + tx, x.x * ph);
+      if (z2 < nz) str(obj + ((size_t)z2 * n + ty) * n + tx, x.y * ph);
+    }
   }
 }
 
