@@ -852,6 +852,28 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft_adj(const flo
   const int CB = 1 << lgCB;
   const int n2 = 2 * n, v0 = cb * CB, h = n / 2;
   const int z2 = c + nzc;
+  float2 *dst = R + (size_t)c * n2 * n;
+  if constexpr (NS > 0) {
+    // compiled sizes: first pass from obj (rows outside [1+h, n+h] are
+    // zero), last pass straight to R
+    constexpr int LG = col_lg(NS);
+    fft_c_global<NS, -1, kFftEPT, LG>(
+        sm, fd.tw, tid, nt,
+        [&](int u, int t) {
+          const int ty = u - 1 - h, tx = v0 + t;
+          if (ty < 0 || ty >= n || tx >= n) return make_float2(0.f, 0.f);
+          const int2 ci = circ[ty];
+          if (tx < ci.x || tx >= ci.y) return make_float2(0.f, 0.f);
+          const float ph = phix[tx] * phiy[ty] * ((u & 1) ? -1.f : 1.f);
+          return make_float2(obj[((size_t)c * n + ty) * n + tx] * ph,
+                             (z2 < nz) ? obj[((size_t)z2 * n + ty) * n + tx] * ph : 0.f);
+        },
+        [&](int a, int t, float2 x) {
+          if (v0 + t >= n) return;
+          dst[(size_t)a * n + v0 + t] = (a & 1) ? make_float2(-x.x, -x.y) : x;
+        });
+    return;
+  }
   for (int idx = tid; idx < CB * lds_padded_len(n2); idx += nt) sm[idx] = make_float2(0.f, 0.f);
   __syncthreads();
   stage_lds<8, float2>(
@@ -873,11 +895,7 @@ __global__ __launch_bounds__(NS > 0 ? 1024 : 512) void k_fr_colfft_adj(const flo
         sm[(lds_pad(u) << lgCB) + t] = make_float2(v.x * ph, v.y * ph);
       });
   __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<NS, -1, kFftEPT, col_lg(NS)>(sm, fd.tw, tid, nt);
-  else
-    lds_fft<-1, (NS < 0 ? kFftEPT : kColEPT), fft_lv(NS)>(sm, fd, lgCB, tid, nt);
-  float2 *dst = R + (size_t)c * n2 * n;
+  lds_fft<-1, (NS < 0 ? kFftEPT : kColEPT), fft_lv(NS)>(sm, fd, lgCB, tid, nt);
   for (int idx = tid; idx < CB * n2; idx += nt) {
     const int a = idx >> lgCB, t = idx & (CB - 1);
     if (v0 + t >= n) continue;
@@ -895,15 +913,23 @@ __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict_
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n2 = 2 * n, h = n / 2;
   const float2 *src = R + ((size_t)c * n2 + a) * n;
+  if constexpr (NS > 0) {
+    float2 *dst = F + ((size_t)c * n2 + a) * n2;
+    fft_c_global<NS, -1, kFftEPT, 0>(
+        sm, fd.tw, tid, nt,
+        [&](int v, int) {
+          const float2 x = (v >= h && v < h + n) ? src[v - h] : make_float2(0.f, 0.f);
+          return (v & 1) ? make_float2(-x.x, -x.y) : x;
+        },
+        [&](int b, int, float2 x) { dst[b] = (b & 1) ? make_float2(-x.x, -x.y) : x; });
+    return;
+  }
   stage_lds<8, float2>(
       n2, tid, nt,
       [&](int v) { return (v >= h && v < h + n) ? src[v - h] : make_float2(0.f, 0.f); },
       [&](int v, float2 x) { sm[lds_pad(v)] = (v & 1) ? make_float2(-x.x, -x.y) : x; });
   __syncthreads();
-  if constexpr (NS > 0)
-    lds_fft_c<NS, -1, kFftEPT, 0>(sm, fd.tw, tid, nt);
-  else
-    lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
+  lds_fft<-1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
   float2 *dst = F + ((size_t)c * n2 + a) * n2;
   for (int b = tid; b < n2; b += nt) {
     const float2 x = sm[lds_pad(b)];
