@@ -60,6 +60,9 @@ def parse():
     ap.add_argument('--rotate-proj-order', type=int, default=1,
                     help='steps form: interpolation order of the rotation (0..5)')
     ap.add_argument('--filter', default='ramp')
+    ap.add_argument('--fft', default='static', choices=['static', 'generic'],
+                    help='pipeline form: compiled transforms where the size has them (static) '
+                         'or the run-time radix plans everywhere (generic)')
     ap.add_argument('--center', type=float, default=None, help='default n/2 + 0.5')
     ap.add_argument('--distinct', type=int, default=8,
                     help='distinct synthetic slices, tiled over nz')
@@ -530,7 +533,7 @@ class Pipeline:
                                               fw_filter='sym16', fw_level=7,
                                               vo_all_dim=a.vo_all_dim))
                 self.bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, dtype,
-                                                  a.filter))
+                                                  a.filter, rec_options={'fft': a.fft}))
                 self.sinos.append(torch.empty((a.chunk, a.nproj, a.n), dtype=td, device='cuda'))
         self.rec = torch.empty((nchunks * a.chunk, a.n, a.n), dtype=td, device='cuda')
         torch.cuda.synchronize()
@@ -753,6 +756,7 @@ def main():
                                   center),
                    'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk,
                    'streams': a.streams, 'stripe': a.stripe, 'filter': a.filter,
+                   'fft': a.fft,
                    'timed_with_kernel_events': 'dominant kernel only' if dom else False},
         'roofline': roof, 'cpu_baseline': cpu, 'accuracy': acc, 'half': half,
         'conveyor': conv, 'roofline_overlapped_events': roof_ov,

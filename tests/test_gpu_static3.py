@@ -1,4 +1,6 @@
-"""Compile-time transforms of the 3 * 2^k sizes (2n = 768, 1536, 3072, 6144):
+"""Compile-time transforms of the sizes 2^k 3^a 5^b (2n = 768, 1536, 3072,
+6144; 1280, 2304, 2560, 3840, 4608, 5120, with the n-point spectrum kernels
+compiled where n is such a size too and run-time otherwise):
 the backprojection, the fused filter + backprojection and the forward
 projection against the FP64 oracle (per-slice rel-L2 <= 1e-5), and the
 compiled kernels and the run-time radix plans of the same plan shape
@@ -25,7 +27,9 @@ def _rel(got, ref):
 
 
 @pytest.mark.parametrize("n,nproj,nz", [(384, 40, 2), (768, 30, 3), (1536, 16, 2),
-                                        (3072, 8, 1)])
+                                        (3072, 8, 1), (640, 30, 2), (1152, 20, 3),
+                                        (1280, 16, 2), (1920, 12, 2), (2304, 8, 2),
+                                        (2560, 8, 1)])
 def test_backprojection_static3(n, nproj, nz):
     from tomocupy_amd import FourierRec
     print('n', n, flush=True)
@@ -43,7 +47,7 @@ def test_backprojection_static3(n, nproj, nz):
     rec.free()
 
 
-@pytest.mark.parametrize("n", [384, 768, 1536])
+@pytest.mark.parametrize("n", [384, 768, 1536, 640, 1280, 2304])
 def test_filter_backprojection_static3(n):
     from tomocupy_amd import BackprojFunctions
     nproj, nz = 24, 3
@@ -66,7 +70,7 @@ def test_filter_backprojection_static3(n):
     bf.free()
 
 
-@pytest.mark.parametrize("n", [384, 768])
+@pytest.mark.parametrize("n", [384, 768, 640, 1152])
 def test_forward_static3(n):
     from tomocupy_amd import FourierRec
     nproj, nz = 20, 2
@@ -86,7 +90,9 @@ def test_forward_static3(n):
 
 
 @pytest.mark.parametrize("n,dtype", [(768, 'float32'), (1024, 'float32'), (1536, 'float32'),
-                                     (3072, 'float32'), (768, 'float16'), (1536, 'float16')])
+                                     (3072, 'float32'), (768, 'float16'), (1536, 'float16'),
+                                     (1280, 'float32'), (2304, 'float32'), (1920, 'float32'),
+                                     (1280, 'float16')])
 def test_compiled_equals_runtime_plans(n, dtype):
     from tomocupy_amd import BackprojFunctions
     from tomocupy_amd.backproj_functions import padded_length

@@ -479,26 +479,30 @@ __device__ void lds_fft(float2 *buf, const FftDesc &d, int lgH, int tid, int nt)
 // ---- transforms of a compile-time size ---------------------------------------
 // The same Stockham passes with N, the radix sequence, the sub-transform
 // lengths, the twiddle steps and the interleave 2^LGH all constants, for
-// N = 2^k or 3 * 2^k with 2^k >= 256.  The radices are those FftPlan::radices
+// N = 2^k 3^a 5^b with 2^k >= 256.  The radices are those FftPlan::radices
 // picks: radix 16 passes, one radix 8/4/2 pass for the remaining power of
-// two, then the radix-3 pass, so every sub-transform length Ns is a power of
-// two.  With one pad slot per 16 elements, the R reads of a butterfly
+// two, then the radix-3 and radix-5 passes, so every sub-transform length Ns
+// is 1 or a multiple of 16 (its index splits are divisions by constants).  With one pad slot per 16 elements, the R reads of a butterfly
 // (stride N/R, a multiple of 16) and its R writes (stride Ns, a multiple of
 // 16, or consecutive when Ns = 1) sit at constant offsets from one base
 // address, so a pass costs one address computation per butterfly instead of
 // one per element.
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 
+constexpr int count_factor(int x, int p) { return x % p == 0 ? 1 + count_factor(x / p, p) : 0; }
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
 template <int N>
 struct StaticPlan {
-  static constexpr int three = N % 3 == 0 ? 1 : 0;
-  static constexpr int P2 = three ? N / 3 : N;
-  static_assert(P2 >= 256 && (P2 & (P2 - 1)) == 0, "2^k or 3 * 2^k with 2^k >= 256");
+  static constexpr int n3 = count_factor(N, 3), n5 = count_factor(N, 5);
+  static constexpr int P2 = N / (ipow(3, n3) * ipow(5, n5));
+  static_assert(P2 >= 256 && (P2 & (P2 - 1)) == 0, "2^k 3^a 5^b with 2^k >= 256");
   static constexpr int n16 = ilog2c(P2) / 4;
   static constexpr int rest = 1 << (ilog2c(P2) % 4);
-  static constexpr int passes = n16 + (rest > 1 ? 1 : 0) + three;
+  static constexpr int n2p = n16 + (rest > 1 ? 1 : 0);  // passes of the power of two
+  static constexpr int passes = n2p + n3 + n5;
   static constexpr int radix(int s) {
-    return s < n16 ? 16 : (s == n16 && rest > 1) ? rest : 3;
+    return s < n16 ? 16 : s < n2p ? rest : s < n2p + n3 ? 3 : 5;
   }
 };
 
@@ -523,7 +527,7 @@ __device__ __forceinline__ void fft_pass_c(float2 *buf, const float2 *__restrict
       const int base = (lds_pad(j) << LGH) + t;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = buf[base + r * rstride];
-      if constexpr (Ns > 1) twiddle<R, DIR>(v[q], tw, (unsigned)((j & (Ns - 1)) * twstep));
+      if constexpr (Ns > 1) twiddle<R, DIR>(v[q], tw, (unsigned)((j % Ns) * twstep));
       butterfly<R, DIR>(v[q]);
     }
   }
@@ -533,7 +537,7 @@ __device__ __forceinline__ void fft_pass_c(float2 *buf, const float2 *__restrict
     const int g = tid + q * nt;
     if (g < nb) {
       const int t = g & (H - 1), j = g >> LGH;
-      const int k = j & (Ns - 1);
+      const int k = j % Ns;
       const int base = (lds_pad((j - k) * R + k) << LGH) + t;
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[base + r * wstride] = v[q][out_slot<R>(r)];
@@ -650,15 +654,22 @@ __device__ __forceinline__ void fft_c_global(float2 *buf, const float2 *__restri
 }
 
 // Calls f(std::integral_constant<int, N>) for the sizes with a compiled
-// specialisation (2^k and 3 * 2^k), f(std::integral_constant<int, 0>)
-// otherwise.
+// specialisation (2^k; 3, 9, 5 and 15 times a power of two: the detector
+// widths 2^k, 3 2^k, 1152/2304, 1280/2560 and 1920), f(std::integral_constant
+// <int, 0>) otherwise.
 template <typename F>
 inline void with_static_size(int N, F &&f) {
   switch (N) {
     case 512: f(std::integral_constant<int, 512>()); break;
     case 768: f(std::integral_constant<int, 768>()); break;
+    case 1280: f(std::integral_constant<int, 1280>()); break;
     case 1536: f(std::integral_constant<int, 1536>()); break;
+    case 2304: f(std::integral_constant<int, 2304>()); break;
+    case 2560: f(std::integral_constant<int, 2560>()); break;
     case 3072: f(std::integral_constant<int, 3072>()); break;
+    case 3840: f(std::integral_constant<int, 3840>()); break;
+    case 4608: f(std::integral_constant<int, 4608>()); break;
+    case 5120: f(std::integral_constant<int, 5120>()); break;
     case 6144: f(std::integral_constant<int, 6144>()); break;
     case 1024: f(std::integral_constant<int, 1024>()); break;
     case 2048: f(std::integral_constant<int, 2048>()); break;
@@ -668,6 +679,13 @@ inline void with_static_size(int N, F &&f) {
     case -2: f(std::integral_constant<int, -2>()); break;  // run-time sizes, fft_level 2
     default: f(std::integral_constant<int, 0>()); break;
   }
+}
+
+// True if N has a compiled specialisation.
+inline bool static_fft_size(int N) {
+  bool ok = false;
+  with_static_size(N, [&](auto ns) { ok = decltype(ns)::value > 0; });
+  return ok;
 }
 
 // Maps a flattened workgroup id to a logical one such that the workgroups the

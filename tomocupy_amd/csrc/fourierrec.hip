@@ -1422,23 +1422,35 @@ extern "C" int fr_create_opts(tomo_fr_plan **plan, int64_t n, int64_t nproj, int
     set_error("fr_create: n=%lld needs more than 512 threads per FFT workgroup", (long long)n);
     return fail(TOMO_EINVAL);
   }
-  // size-specialised kernels where 2n is a compiled size, 2^k or 3 * 2^k (and
-  // the column grouping matches the specialisation)
-  P->nsRow = fft_generic ? 0 : (int)n2;
-  P->nsCol = (!fft_generic && lg == col_lg((int)n2)) ? (int)n2 : 0;
-  with_static_size(P->nsRow, [&](auto ns) { P->nsRow = decltype(ns)::value; });
-  with_static_size(P->nsCol, [&](auto ns) { P->nsCol = decltype(ns)::value; });
-  // 16 values per thread (15 with the radix-3 pass of the 3 * 2^k sizes)
-  if (P->nsCol > 0) P->colThreadsStatic = fft_threads(P->fft2.d, n2 << lg, kFftEPT);
+  // size-specialised kernels where 2n is a compiled size (with_static_size):
+  // the row pass, and the column pass with col_lg(2n) columns per workgroup
+  // (up to 1024 threads at 16 values each, 15 with radix-3/5 passes); the
+  // spectrum kernels where n is one too.  Otherwise the run-time-plan
+  // kernels of the size's transform level (0, -1, -2).
+  const int lev = fft_level(n);
+  const int rt = lev == 2 ? -2 : (lev == 1 ? -1 : 0);
+  const bool st2 = !fft_generic && static_fft_size((int)n2);
+  P->nsRow = st2 ? (int)n2 : rt;
+  P->nsCol = rt;
+  if (st2) {
+    const int lgs = col_lg((int)n2);
+    const size_t lds_s = (size_t)(1 << lgs) * col1;
+    const int thr = fft_threads(P->fft2.d, (int64_t)n2 << lgs, kFftEPT);
+    if (lds_s <= maxlds && thr > 0 && thr <= 1024) {
+      P->nsCol = (int)n2;
+      CB = 1 << lgs;
+      lg = lgs;
+      P->colCB = CB;
+      P->lgCB = lg;
+      P->colLds = lds_s;
+      P->colThreadsStatic = thr;
+    }
+  }
   // column-blocked R for 16-bit plans only: measured at n=2048 the column
   // pass gains 40-50 us per chunk of 8 slices, the row pass loses 22 us
   // (16-byte segments) in half precision and 43 us (32-byte) in float32
   P->lgB = (dtype != TOMO_F32 && n % (1 << lg) == 0) ? lg : 0;
-  P->nsPrep = fft_generic ? 0 : (int)n;
-  with_static_size(P->nsPrep, [&](auto ns) { P->nsPrep = decltype(ns)::value; });
-  if (P->nsRow == 0 || 2 * P->nsPrep != P->nsRow) P->nsPrep = 0;
-  if (fft_general(n))  // run-time kernels with the level-1 or level-2 passes
-    P->nsRow = P->nsCol = P->nsPrep = fft_level(n) == 2 ? -2 : -1;
+  P->nsPrep = (st2 && static_fft_size((int)n)) ? (int)n : rt;
   hipError_t ea = hipSuccess;
   auto lds_attr = [&](const void *k, size_t bytes) {
     if (ea == hipSuccess)
