@@ -60,6 +60,7 @@ struct DevBuf {
 __host__ __device__ inline float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__host__ __device__ inline float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 __host__ __device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }

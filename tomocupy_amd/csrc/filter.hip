@@ -205,7 +205,13 @@ __global__ __launch_bounds__(1024) void k_flt_edges64(const double *__restrict__
 }
 
 // H2[k] = FFT_2n(h2)[k] / (2n), h2 = h on lags (-n, n) wrapped to length 2n
-// (h2[n] = 0); FP64 segment sums as above, rounded once.
+// (h2[n] = 0); FP64 segment sums as above, rounded once.  Also, at
+// H2 + nz 2n, the spectra for a differenced input: with d[j] = z[j] - z[j-1]
+// (circular, length 2n), D[k] = Z[k] (1 - W^k), W = exp(-2 pi i/2n), so
+// Z[k] H2[k] = D[k] H2[k] / (1 - W^k) for k > 0; slot 0 keeps H2[0] (the DC
+// term is formed from the row sum).  A high-pass filter applied to a smooth
+// row this way rounds relative to the small differences instead of the
+// row's large low frequencies.
 __global__ __launch_bounds__(1024) void k_flt_H2_64(const double *__restrict__ h,
                                                     float2 *__restrict__ H2, int ne, int n) {
   __shared__ double pre[kSeg][64], pim[kSeg][64];
@@ -240,7 +246,18 @@ __global__ __launch_bounds__(1024) void k_flt_H2_64(const double *__restrict__ h
     sr += pre[q][lane];
     si += pim[q][lane];
   }
-  H2[(size_t)z * n2 + k] = make_float2((float)(sr / n2), (float)(si / n2));
+  sr /= n2;
+  si /= n2;
+  H2[(size_t)z * n2 + k] = make_float2((float)sr, (float)si);
+  double gr = sr, gi = si;
+  if (k > 0) {
+    // 1 - W^k = 2 sin^2(pi k/2n) + i sin(pi k/n)
+    const double sh = sinpi((double)k / (double)n2), s1 = sinpi((double)k / (double)n);
+    const double dr = 2.0 * sh * sh, di = s1, den = dr * dr + di * di;
+    gr = (sr * dr + si * di) / den;
+    gi = (si * dr - sr * di) / den;
+  }
+  H2[((size_t)gridDim.y + z) * n2 + k] = make_float2((float)gr, (float)gi);
 }
 
 }  // namespace
@@ -465,7 +482,8 @@ extern "C" int flt_create(tomo_flt_plan **plan, int64_t n, int64_t ne, int64_t n
   // fused 2n path (needs 2n <= 8192 and ne >= 2n so that lags |d| < n fit)
   if (2 * n <= 8192 && ne >= 2 * n && ne <= 16384 && !rc) {
     rc = P->fft2n.init((int)(2 * n));
-    if (!rc) rc = P->H2.alloc(sizeof(float2) * nz * 2 * n);
+    // H2, then the differenced-input spectra H2 / (1 - W^k) (see k_flt_H2_64)
+    if (!rc) rc = P->H2.alloc(sizeof(float2) * 2 * nz * 2 * n);
     if (!rc) rc = P->HL.alloc(sizeof(float) * nz * n);
     if (!rc) rc = P->HR.alloc(sizeof(float) * nz * n);
     if (!rc) rc = P->P.alloc(sizeof(double) * nz * (ne + 1));

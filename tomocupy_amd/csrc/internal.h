@@ -15,7 +15,9 @@ struct tomo_flt_plan {
   size_t lds = 0;
   // fused path: per slice, the spectrum of the impulse response wrapped to
   // length 2n, and the edge-replication responses HL/HR (see filter.hip)
-  tomo::DevBuf H2, HL, HR, P, Hd;  // Hd: FP64 impulse response (nz, ne)
+  // H2: (2, nz, 2n) complex, the wrapped filter spectra and the same for a
+  // differenced input; Hd: FP64 impulse response (nz, ne)
+  tomo::DevBuf H2, HL, HR, P, Hd;
   int prepared = 0;  // H2/HL/HR hold the kernels of the last flt_prepare
 };
 
