@@ -36,6 +36,13 @@ namespace {
 constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
 constexpr int kGridChunk = 512; // default max list entries per work item (multiple of 16)
 constexpr int kGridBatch = 64;  // list entries a gridding wave stages at a time
+// The four 16-lane groups of a gridding wave read staged entries 8 apart in
+// the same instruction; one pad int4 (entries) or two pad float2 (spectra)
+// after every 8 put the four addresses on different LDS banks.
+constexpr int kSeLen = kGridBatch + kGridBatch / 8;
+constexpr int kSgLen = kGridBatch + kGridBatch / 4;
+__device__ __forceinline__ int se_at(int e) { return e + (e >> 3); }
+__device__ __forceinline__ int sg_at(int e) { return e + 2 * (e >> 3); }
 constexpr int kColEPT = 32;     // FFT values per thread in the column pass
 // log2 of the columns a column-pass workgroup transforms for 2n = N: as many
 // as fit 512 threads at kColEPT values each (and the LDS), at most 16
@@ -412,8 +419,8 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-  __shared__ int4 se[kGridWaves][2][kGridBatch];
-  __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
+  __shared__ int4 se[kGridWaves][2][kSeLen];
+  __shared__ float2 sg[kGridWaves][2][4][kSgLen];
   const int eend = ebeg + cnt;
   int4 re = make_int4(-1, kPadStarts, 0, 0);
   cpx<T> rg[4];
@@ -427,9 +434,9 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
   if (cnt > 0) fetch(ebeg);
   int buf = 0;
   for (int bb = ebeg; bb < eend; bb += kGridBatch, buf ^= 1) {
-    se[wave][buf][lane] = re;
+    se[wave][buf][se_at(lane)] = re;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = tof2(rg[c]);
+    for (int c = 0; c < 4; ++c) sg[wave][buf][c][sg_at(lane)] = tof2(rg[c]);
     __builtin_amdgcn_wave_barrier();
     if (bb + kGridBatch < eend) fetch(bb + kGridBatch);
     const int ngr = min(kGridBatch, eend - bb) >> 5;
@@ -438,14 +445,14 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
       // 32q + 8 g4 + u (u < 8) to both operands
       float e0v[8], e1v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) gl.pair(se[wave][buf][32 * q + 8 * g4 + u], e0v[u], e1v[u]);
+      for (int u = 0; u < 8; ++u) gl.pair(se[wave][buf][se_at(32 * q + 8 * g4 + u)], e0v[u], e1v[u]);
       const typename M::v8 b = M::pack8(e0v);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         float ar[8], ai[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const float2 v = sg[wave][buf][c][32 * q + 8 * g4 + u];
+          const float2 v = sg[wave][buf][c][sg_at(32 * q + 8 * g4 + u)];
           ar[u] = e1v[u] * v.x;
           ai[u] = e1v[u] * v.y;
         }
@@ -578,8 +585,8 @@ __device__ __forceinline__ void grid_x3_item(const int4 item, const int4 *__rest
   floatx4 acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
-  __shared__ int4 se[kGridWaves][2][kGridBatch];
-  __shared__ float2 sg[kGridWaves][2][4][kGridBatch];
+  __shared__ int4 se[kGridWaves][2][kSeLen];
+  __shared__ float2 sg[kGridWaves][2][4][kSgLen];
   const int eend = ebeg + cnt;
   int4 re = make_int4(-1, kPadStarts, 0, 0);
   float2 rg[4];
@@ -593,9 +600,9 @@ __device__ __forceinline__ void grid_x3_item(const int4 item, const int4 *__rest
   if (cnt > 0) fetch(ebeg);
   int buf = 0;
   for (int bb = ebeg; bb < eend; bb += kGridBatch, buf ^= 1) {
-    se[wave][buf][lane] = re;
+    se[wave][buf][se_at(lane)] = re;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sg[wave][buf][c][lane] = rg[c];
+    for (int c = 0; c < 4; ++c) sg[wave][buf][c][sg_at(lane)] = rg[c];
     __builtin_amdgcn_wave_barrier();
     if (bb + kGridBatch < eend) fetch(bb + kGridBatch);
     const int ngr = min(kGridBatch, eend - bb) >> 5;
@@ -604,7 +611,7 @@ __device__ __forceinline__ void grid_x3_item(const int4 item, const int4 *__rest
       // operands, so the K order inside the instruction does not matter
       float e0v[8], e1v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) gl.pair(se[wave][buf][32 * q + 8 * g4 + u], e0v[u], e1v[u]);
+      for (int u = 0; u < 8; ++u) gl.pair(se[wave][buf][se_at(32 * q + 8 * g4 + u)], e0v[u], e1v[u]);
       half8_t bh, bl;
       split8(e0v, bh, bl);
 #pragma unroll
@@ -612,7 +619,7 @@ __device__ __forceinline__ void grid_x3_item(const int4 item, const int4 *__rest
         float vr[8], vi[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const float2 v = sg[wave][buf][c][32 * q + 8 * g4 + u];
+          const float2 v = sg[wave][buf][c][sg_at(32 * q + 8 * g4 + u)];
           vr[u] = v.x;
           vi[u] = v.y;
         }

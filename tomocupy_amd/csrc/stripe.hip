@@ -359,24 +359,43 @@ __global__ __launch_bounds__(256) void k_fw_cols(
 
 // Notch coefficients of the stripe band: X_k = (sum over the analysis row
 // blocks, in order, of the partial DFTs) * gk_k, gk_k = c_k g_k / my.  One
-// thread per (column, k, slice).
-template <int KM>
-__global__ __launch_bounds__(256) void k_fw_xsum(const float2 *__restrict__ part,
-                                                 const float *__restrict__ gk,
-                                                 float2 *__restrict__ X, int W, int nrb) {
-  const int w = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, z = blockIdx.z;
-  if (w >= W) return;
-  const float2 *p = part + ((size_t)z * nrb * KM + k) * W + w;
-  const size_t stride = (size_t)KM * W;
+// launch for every level (the synthesis needs them only after the deepest
+// analysis): the levels' blocks are concatenated, level j owning blocks
+// [blk0, blk0 + ceil(W/256) * nk * nz); one thread per (column, k, slice).
+constexpr int kXsumMaxLevels = 16;
+struct XsumLevel {
+  const float2 *part;
+  const float *gk;
+  float2 *X;
+  int W, nrb, nk, KM, blk0, pad;
+};
+struct XsumArgs {
+  XsumLevel lv[kXsumMaxLevels];
+  int nlev, nz;
+};
+
+__global__ __launch_bounds__(256) void k_fw_xsum(const XsumArgs a) {
+  int j = 0;
+  while (j + 1 < a.nlev && (int)blockIdx.x >= a.lv[j + 1].blk0) ++j;
+  const XsumLevel &l = a.lv[j];
+  const int nwb = (l.W + 255) / 256;
+  int b = (int)blockIdx.x - l.blk0;
+  const int wb = b % nwb;
+  b /= nwb;
+  const int k = b % l.nk, z = b / l.nk;
+  const int w = wb * 256 + threadIdx.x;
+  if (w >= l.W) return;
+  const float2 *p = l.part + ((size_t)z * l.nrb * l.KM + k) * l.W + w;
+  const size_t stride = (size_t)l.KM * l.W;
   float sr = 0.f, si = 0.f;
 #pragma unroll 8
-  for (int q = 0; q < nrb; ++q) {
+  for (int q = 0; q < l.nrb; ++q) {
     const float2 v = p[q * stride];
     sr += v.x;
     si += v.y;
   }
-  const float s = gk[k];
-  X[((size_t)z * KM + k) * W + w] = make_float2(sr * s, si * s);
+  const float s = l.gk[k];
+  l.X[((size_t)z * l.KM + k) * l.W + w] = make_float2(sr * s, si * s);
 }
 
 // Synthesis along angles of the correction: lo = sfb(d, 0), hi = sfb(D, 0)
@@ -733,15 +752,32 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
                              l.H, l.Wo, l.Ho, l.tw.as<float2>(), l.part.as<float2>(), l.nk);
         });
       });
-      if (l.nk > 0) {
-        constexpr int KM = decltype(kmc)::value;
-        ProfScope ps_("k_fw_xsum", st);
-        hipLaunchKernelGGL(k_fw_xsum<KM>, dim3((unsigned)ceil_div(l.Wo, 256), l.nk, nz),
-                           dim3(256), 0, st, l.part.as<float2>(), l.gk.as<float>(),
-                           l.X.as<float2>(), l.Wo, l.nrb);
-      }
     });
     TOMO_LAUNCH_CHECK();
+  }
+  {
+    XsumArgs xa{};
+    xa.nz = nz;
+    int blk = 0;
+    for (int k = 0; k < level; ++k) {
+      const FwLevel &l = P->lv[k];
+      if (l.nk == 0) continue;
+      XsumLevel &x = xa.lv[xa.nlev++];
+      x.part = l.part.as<float2>();
+      x.gk = l.gk.as<float>();
+      x.X = l.X.as<float2>();
+      x.W = l.Wo;
+      x.nrb = l.nrb;
+      x.nk = l.nk;
+      x.KM = l.KM;
+      x.blk0 = blk;
+      blk += (int)ceil_div(l.Wo, 256) * l.nk * nz;
+    }
+    if (xa.nlev > 0) {
+      ProfScope ps_("k_fw_xsum", st);
+      hipLaunchKernelGGL(k_fw_xsum, dim3((unsigned)blk), dim3(256), 0, st, xa);
+      TOMO_LAUNCH_CHECK();
+    }
   }
   // synthesis of the correction, deepest level first; `cur` is the correction
   // of the level below (pitch curH x curW), NULL (zero) at the deepest level
