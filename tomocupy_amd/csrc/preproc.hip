@@ -13,31 +13,52 @@ using namespace tomo;
 
 namespace {
 
+// Sum of frames [f0, f1) of pixel i (frame stride zn) as conv(value), four
+// independent FP64 partial sums so a thread's loads are in flight together.
+// The terms are integers (dark) or float32 quotients (flat) well inside
+// FP64's 53 bits, so every partial sum and their total are exact and the
+// result does not depend on the order.
+template <typename TI, typename F>
+__device__ __forceinline__ double frame_sum(const TI *__restrict__ p, int f0, int f1, int zn,
+                                            int i, F conv) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int f = f0;
+  for (; f + 4 <= f1; f += 4) {
+    const double a = conv(p[(size_t)f * zn + i]), b = conv(p[(size_t)(f + 1) * zn + i]);
+    const double c = conv(p[(size_t)(f + 2) * zn + i]), d = conv(p[(size_t)(f + 3) * zn + i]);
+    s0 += a;
+    s1 += b;
+    s2 += c;
+    s3 += d;
+  }
+  for (; f < f1; ++f) s0 += conv(p[(size_t)f * zn + i]);
+  return (s0 + s1) + (s2 + s3);
+}
+
 // dark0 = mean(dark); flat means (one, or two halves for flat_linear)
 // (proc_functions.py:55-72).  Means accumulate in FP64.  TI is the type of
-// the counts: uint16_t (raw frames) or float (binned frames).
+// the counts: uint16_t (raw frames) or float (binned frames).  One thread
+// per pixel, 64-thread workgroups so that the launch spreads over all CUs.
+constexpr int kMeansThreads = 64;
+
 template <typename TI>
-__global__ void k_pre_means(const TI *__restrict__ dark, const TI *__restrict__ flat,
-                            float *__restrict__ dark0, float *__restrict__ flatA,
-                            float *__restrict__ flatB, int zn, int ndark, int nflat,
-                            float bright, int flat_linear) {
+__global__ __launch_bounds__(kMeansThreads) void k_pre_means(const TI *__restrict__ dark,
+                                                  const TI *__restrict__ flat,
+                                                  float *__restrict__ dark0,
+                                                  float *__restrict__ flatA,
+                                                  float *__restrict__ flatB, int zn, int ndark,
+                                                  int nflat, float bright, int flat_linear) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= zn) return;
-  double sd = 0.0;
-  for (int f = 0; f < ndark; ++f) sd += (double)dark[(size_t)f * zn + i];
-  const float d0 = (float)(sd / ndark);
-  dark0[i] = d0;
+  auto raw = [](TI v) { return (double)v; };
+  auto rel = [bright](TI v) { return (double)((float)v / bright); };
+  dark0[i] = (float)(frame_sum(dark, 0, ndark, zn, i, raw) / ndark);
   if (!flat_linear) {
-    double sf = 0.0;
-    for (int f = 0; f < nflat; ++f) sf += (double)((float)flat[(size_t)f * zn + i] / bright);
-    flatA[i] = (float)(sf / nflat);
+    flatA[i] = (float)(frame_sum(flat, 0, nflat, zn, i, rel) / nflat);
   } else {
     const int h = nflat / 2;
-    double sa = 0.0, sb = 0.0;
-    for (int f = 0; f < h; ++f) sa += (double)((float)flat[(size_t)f * zn + i] / bright);
-    for (int f = h + 1; f < nflat; ++f) sb += (double)((float)flat[(size_t)f * zn + i] / bright);
-    flatA[i] = (float)(sa / h);
-    flatB[i] = (float)(sb / (nflat - h - 1));
+    flatA[i] = (float)(frame_sum(flat, 0, h, zn, i, rel) / h);
+    flatB[i] = (float)(frame_sum(flat, h + 1, nflat, zn, i, rel) / (nflat - h - 1));
   }
 }
 
@@ -120,9 +141,10 @@ static int launch_means(tomo_pre_plan *P, const TI *dark, const TI *flat, double
   const int zn = (int)(P->nz * P->n);
   {
     ProfScope ps_("k_pre_means", st);
-    hipLaunchKernelGGL(k_pre_means<TI>, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st, dark,
-                       flat, P->dark0.as<float>(), P->flatA.as<float>(), P->flatB.as<float>(), zn,
-                       (int)P->ndark, (int)P->nflat, (float)bright_ratio, flat_linear);
+    hipLaunchKernelGGL(k_pre_means<TI>, dim3((unsigned)ceil_div(zn, kMeansThreads)),
+                       dim3(kMeansThreads), 0, st, dark, flat, P->dark0.as<float>(),
+                       P->flatA.as<float>(), P->flatB.as<float>(), zn, (int)P->ndark,
+                       (int)P->nflat, (float)bright_ratio, flat_linear);
   }
   TOMO_LAUNCH_CHECK();
   return TOMO_OK;
@@ -327,9 +349,9 @@ extern "C" int pre_process_noalloc(int64_t nproj, int64_t nz, int64_t n, int64_t
   float *d0 = sc, *fa = sc + kNoAllocMaxZN, *fb = sc + 2 * kNoAllocMaxZN;
   hipStream_t st = (hipStream_t)stream;
   const int zn = (int)(nz * n);
-  hipLaunchKernelGGL(k_pre_means<uint16_t>, dim3((unsigned)ceil_div(zn, 256)), dim3(256), 0, st,
-                     dark, flat, d0, fa, fb, zn, (int)ndark, (int)nflat, (float)bright_ratio,
-                     flat_linear);
+  hipLaunchKernelGGL(k_pre_means<uint16_t>, dim3((unsigned)ceil_div(zn, kMeansThreads)),
+                     dim3(kMeansThreads), 0, st, dark, flat, d0, fa, fb, zn, (int)ndark,
+                     (int)nflat, (float)bright_ratio, flat_linear);
   TOMO_LAUNCH_CHECK();
   const dim3 grid((unsigned)nproj, (unsigned)nz);
   if (out_dtype == TOMO_F16)
