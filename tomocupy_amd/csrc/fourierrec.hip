@@ -503,10 +503,11 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_grid16(
 // Scale exponent of the float32 gridding: kA = 13 - floor(log2(max |G|))
 // over the per-row maxima of the spectra, so every weighted spectrum value
 // times 2^kA stays below 2^14 (0 for an all-zero chunk).
-__global__ __launch_bounds__(256) void k_fr_gscale(const float *__restrict__ gmax, int count,
-                                                   int *__restrict__ ksc) {
+__global__ __launch_bounds__(1024) void k_fr_gscale(const float *__restrict__ gmax, int count,
+                                                    int *__restrict__ ksc) {
   float m = 0.f;
-  for (int i = threadIdx.x; i < count; i += 256) m = fmaxf(m, gmax[i]);
+#pragma unroll 4
+  for (int i = threadIdx.x; i < count; i += 1024) m = fmaxf(m, gmax[i]);
   m = block_max(m);
   if (threadIdx.x == 0) {
     int k = 0;
@@ -721,24 +722,24 @@ template <typename T>
 __global__ void k_fr_grid_reduce(const int4 *__restrict__ red, const float *__restrict__ part,
                                  cpx<T> *__restrict__ F, int n, int ntile, int nslots, int nzc) {
   const int4 r = red[blockIdx.x];
-  const int grp = blockIdx.y;
+  const int grp = blockIdx.y, c = blockIdx.z;  // one complex slice of the group
+  if (grp * 4 + c >= nzc) return;
   const int cell = threadIdx.x;  // 256 threads = 16 x 16 cells
   const int n2 = 2 * n;
   const int TA = r.x / ntile, TB = r.x - TA * ntile;
   const int a = 16 * TA + (cell >> 4), b = 16 * TB + (cell & 15);
-  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float re = 0.f, im = 0.f;
+  // (unrolled so that the loads of several slots are in flight together; the
+  // additions stay in slot order)
+  const float *P = part + ((size_t)grp * nslots + r.y) * 2048 + 2 * c * 256 + cell;
+#pragma unroll 4
   for (int q = 0; q < r.z; ++q) {
-    const float *P = part + ((size_t)grp * nslots + r.y + q) * 2048;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] += P[m * 256 + cell];
+    re += P[(size_t)q * 2048];
+    im += P[(size_t)q * 2048 + 256];
   }
   if (a >= n2 || b >= n2) return;
   const size_t plane = (size_t)n2 * n2;
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-    if (grp * 4 + c < nzc)
-      stc(F + (size_t)(grp * 4 + c) * plane + (size_t)a * n2 + b,
-          make_float2(acc[2 * c], acc[2 * c + 1]));
+  stc(F + (size_t)(grp * 4 + c) * plane + (size_t)a * n2 + b, make_float2(re, im));
 }
 
 template <typename T, int NS>
@@ -1564,7 +1565,7 @@ static int fr_tail(tomo_fr_plan *P, T *obj, hipStream_t st) {
     if constexpr (std::is_same<T, float>::value) {
       {
         ProfScope ps1_("k_fr_gscale", st);
-        hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, st, P->gmax.as<float>(),
+        hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(1024), 0, st, P->gmax.as<float>(),
                            nzc * (int)P->nproj, P->gsc.as<int>());
       }
       ProfScope ps_("k_fr_grid_x3", st);
@@ -1593,7 +1594,7 @@ static int fr_tail_after_grid(tomo_fr_plan *P, T *obj, hipStream_t st) {
   cpx<T> *F = P->F.as<cpx<T>>(), *R = P->R.as<cpx<T>>();
   if (P->nred > 0) {
     ProfScope ps_("k_fr_grid_reduce", st);
-    hipLaunchKernelGGL(k_fr_grid_reduce<T>, dim3(P->nred, ng), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_fr_grid_reduce<T>, dim3(P->nred, ng, 4), dim3(256), 0, st,
                        P->red.as<int4>(), P->part.as<float>(), F, n, P->ntile, P->nslots, nzc);
   }
   TOMO_LAUNCH_CHECK();
@@ -1898,7 +1899,7 @@ extern "C" int fr_batch_filter_backprojection(tomo_fr_batch *B, tomo_flt_plan *c
     rc = fr_head<float>(P, flts[r], (const float *)datas[r], s);
     if (rc) break;
     ProfScope ps_("k_fr_gscale", s);
-    hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(256), 0, s, P->gmax.as<float>(),
+    hipLaunchKernelGGL(k_fr_gscale, dim3(1), dim3(1024), 0, s, P->gmax.as<float>(),
                        (int)(P->nzc * P->nproj), P->gsc.as<int>());
   }
   if (!rc) rc = batch_join(B, st);
