@@ -67,6 +67,25 @@ int tomo_prof_read(const char *kernel, double *total_ms, int64_t *launches);
  */
 int tomo_dot_f64(const float *a, const float *b, int64_t n, double *out, void *stream);
 
+/*
+ * The vector updates of one CG iteration of the differentiation form's
+ * least-squares solver (no counterpart in the reference), each one pass over
+ * device memory with the step sizes read on the device (no host wait):
+ *
+ * tomo_cg_step: alpha = rr[0] / pq[0] (0 when pq[0] <= 0), rounded to
+ * float32; x += alpha p; r -= alpha q; then out[0] = sum_i r[i]^2 in FP64 in
+ * exactly the order of tomo_dot_f64 (so out[0] equals tomo_dot_f64(r, r)
+ * bit for bit).  out: at least 1025 doubles, as for tomo_dot_f64; it must not
+ * alias rr or pq.
+ *
+ * tomo_cg_direction: beta = rr_new[0] / rr[0] (0 when rr[0] <= 0), rounded
+ * to float32; p = r + beta p.
+ */
+int tomo_cg_step(float *x, float *r, const float *p, const float *q, int64_t n,
+                 const double *rr, const double *pq, double *out, void *stream);
+int tomo_cg_direction(float *p, const float *r, int64_t n, const double *rr,
+                      const double *rr_new, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* USFFT backprojector                                                      */
 /* ------------------------------------------------------------------------ */

@@ -12,7 +12,8 @@ Checks, on the GPU against the FP64 oracle:
   with the FP64 oracle operators: iterate rel-L2 < 1e-4 after 5 iterations;
   objective within 2% at every iteration (float32 rounding of the operator
   slowly de-orthogonalises the search directions; the FP64 run has the same
-  start and data).
+  start and data);
+* the fused CG vector updates against the same updates in torch.
 """
 import numpy as np
 import pytest
@@ -160,3 +161,54 @@ def test_dot_f64_matches_numpy(n):
     assert got.dtype == torch.float64
     assert abs(float(got) - ref) <= 1e-13 * scale
     assert float(got) == float(again)
+
+
+@pytest.mark.parametrize("n", [1, 5000, 3000001])
+def test_cg_updates_match_torch(n):
+    """The fused CG vector updates (tomo_cg_step, tomo_cg_direction) against
+    the same updates in torch with the float32 step sizes: x and r within one
+    rounding (one fused multiply-add against a multiply and an add), the
+    returned r.r bitwise equal to tomo_dot_f64 of the updated r, and the
+    zero-denominator rule (alpha = 0 when p.q <= 0, beta = 0 when r.r <= 0)."""
+    from tomocupy_amd import _lib
+    from tomocupy_amd.autograd import _dot
+    L = _lib.lib()
+    st = _lib.stream_ptr(None)
+    rng = np.random.default_rng(n + 11)
+    x, r, p, q = (torch.from_numpy(rng.standard_normal(n).astype(np.float32)).cuda()
+                  for _ in range(4))
+    rr = torch.zeros(1025, dtype=torch.float64, device='cuda')
+    pq = torch.zeros(1025, dtype=torch.float64, device='cuda')
+    out = torch.zeros(1025, dtype=torch.float64, device='cuda')
+    rr[0], pq[0] = 3.5, 1.75
+    for case in ('step', 'zero'):
+        if case == 'zero':
+            pq[0] = -1.0
+        alpha = torch.where(pq[0] > 0, rr[0] / torch.where(pq[0] > 0, pq[0], 1.0),
+                            torch.zeros((), dtype=torch.float64, device='cuda')).float()
+        xe = x + alpha * p
+        re = r - alpha * q
+        _lib.check(L.tomo_cg_step(x.data_ptr(), r.data_ptr(), p.data_ptr(), q.data_ptr(), n,
+                                  rr.data_ptr(), pq.data_ptr(), out.data_ptr(), st),
+                   'tomo_cg_step')
+        torch.cuda.synchronize()
+        tol = 2 * torch.finfo(torch.float32).eps
+        assert torch.allclose(x, xe, rtol=tol, atol=tol * float(alpha.abs()) * 8)
+        assert torch.allclose(r, re, rtol=tol, atol=tol * float(alpha.abs()) * 8)
+        assert float(out[0]) == float(_dot(r, r))
+        if case == 'zero':
+            assert torch.equal(x, xe) and torch.equal(r, re)
+    # direction: beta = rr_new / rr
+    rnew = torch.zeros(1025, dtype=torch.float64, device='cuda')
+    for rr0, rn0 in ((2.0, 0.5), (0.0, 0.5)):
+        rr[0], rnew[0] = rr0, rn0
+        beta = np.float32(rn0 / rr0) if rr0 > 0 else np.float32(0.0)
+        pe = r + float(beta) * p
+        _lib.check(L.tomo_cg_direction(p.data_ptr(), r.data_ptr(), n, rr.data_ptr(),
+                                       rnew.data_ptr(), st), 'tomo_cg_direction')
+        torch.cuda.synchronize()
+        eps = torch.finfo(torch.float32).eps
+        assert torch.allclose(p, pe, rtol=2 * eps,
+                              atol=4 * eps * float(beta) * float(p.abs().max()))
+        if rr0 == 0.0:
+            assert torch.equal(p, r)

@@ -38,6 +38,8 @@ SIGNATURES = {
     'tomo_prof_read': (_c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(_c_i64)]),
     'tomo_dot_f64': (_c_int, [_c_p, _c_p, _c_i64, _c_p, _c_p]),
+    'tomo_cg_step': (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
+    'tomo_cg_direction': (_c_int, [_c_p, _c_p, _c_i64, _c_p, _c_p, _c_p]),
     'fr_create': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_p]),
     'fr_create_opts': (_c_int, [ctypes.POINTER(_c_p), _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_int,
                                 _c_int, _c_p]),

@@ -81,7 +81,8 @@ def cgls(rec, data, lam=0.0, niter=50, x0=None, callback=None):
     ``rec`` is a :class:`FourierRec` plan, ``data`` the measured sinograms
     ``(nz, nproj, n)``.  Returns the images ``(nz, n, n)``.  Inner products are
     accumulated in float64 on the GPU and the step sizes stay on the GPU, so an
-    iteration does not wait for the host.  ``callback(k, x, rr)`` is called
+    iteration does not wait for the host; the vector updates are two fused
+    passes (``tomo_cg_step``: x, r and the new r.r; ``tomo_cg_direction``).  ``callback(k, x, rr)`` is called
     after every iteration with the normal-equation residual norm squared ``rr``
     (a 0-d float64 tensor).
     """
@@ -102,19 +103,30 @@ def cgls(rec, data, lam=0.0, niter=50, x0=None, callback=None):
         x = x0.detach().clone().to(torch.float32)
         r = _apply_backprojection(rec, data) - normal(x)
     p = r.clone()
-    work = torch.empty(1025, dtype=torch.float64, device=data.device)
-    rr = _dot(r, r, work)
-    zero = torch.zeros((), dtype=torch.float64, device=data.device)
+    x = x.contiguous()
+    r = r.contiguous()
+    nel = r.numel()
+    # FP64 scalars and partial sums on the device (tomo_dot_f64 layout):
+    # pq, and rr / rr_new alternating between two buffers
+    wpq = torch.empty(1025, dtype=torch.float64, device=data.device)
+    wrr = [torch.empty(1025, dtype=torch.float64, device=data.device) for _ in range(2)]
+    L = _lib.lib()
+    st = _lib.stream_ptr(None)
+    _lib.check(L.tomo_dot_f64(r.data_ptr(), r.data_ptr(), nel, wrr[0].data_ptr(), st),
+               'tomo_dot_f64')
+    cur = 0
     for k in range(niter):
-        q = normal(p)
-        pq = _dot(p, q, work)
-        alpha = torch.where(pq > 0, rr / torch.where(pq > 0, pq, 1.0), zero).to(torch.float32)
-        x.addcmul_(p, alpha)
-        r.addcmul_(q, alpha, value=-1.0)
-        rr_new = _dot(r, r, work)
+        q = normal(p).contiguous()
+        _lib.check(L.tomo_dot_f64(p.data_ptr(), q.data_ptr(), nel, wpq.data_ptr(), st),
+                   'tomo_dot_f64')
+        # alpha = rr / pq; x += alpha p; r -= alpha q; rr_new = r.r
+        _lib.check(L.tomo_cg_step(x.data_ptr(), r.data_ptr(), p.data_ptr(), q.data_ptr(), nel,
+                                  wrr[cur].data_ptr(), wpq.data_ptr(), wrr[1 - cur].data_ptr(),
+                                  st), 'tomo_cg_step')
         if callback is not None:
-            callback(k, x, rr_new)
-        beta = torch.where(rr > 0, rr_new / torch.where(rr > 0, rr, 1.0), zero).to(torch.float32)
-        rr = rr_new
-        p.mul_(beta).add_(r)
+            callback(k, x, wrr[1 - cur][0].clone())
+        # beta = rr_new / rr; p = r + beta p
+        _lib.check(L.tomo_cg_direction(p.data_ptr(), r.data_ptr(), nel, wrr[cur].data_ptr(),
+                                       wrr[1 - cur].data_ptr(), st), 'tomo_cg_direction')
+        cur = 1 - cur
     return x

@@ -191,7 +191,73 @@ __global__ __launch_bounds__(256) void k_dot_final(double *__restrict__ out) {
   if (threadIdx.x == 0) out[0] = sd[0];
 }
 
+// CG step: x += alpha p, r -= alpha q, then the FP64 partial sums of r.r over
+// the same elements, in the same order and tree as k_dot_partial.
+__global__ __launch_bounds__(256) void k_cg_step(float *__restrict__ x, float *__restrict__ r,
+                                                 const float *__restrict__ p,
+                                                 const float *__restrict__ q, int64_t n,
+                                                 const double *__restrict__ rr,
+                                                 const double *__restrict__ pq,
+                                                 double *__restrict__ part) {
+  __shared__ double sd[256];
+  const double den = pq[0];
+  const float alpha = den > 0.0 ? (float)(rr[0] / den) : 0.f;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)kDotBlocks * 256) {
+    x[i] = fmaf(alpha, p[i], x[i]);
+    const float ri = fmaf(-alpha, q[i], r[i]);
+    r[i] = ri;
+    s = fma((double)ri, (double)ri, s);
+  }
+  sd[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sd[threadIdx.x] += sd[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sd[0];
+}
+
+// CG direction: p = r + beta p.
+__global__ __launch_bounds__(256) void k_cg_direction(float *__restrict__ p,
+                                                      const float *__restrict__ r, int64_t n,
+                                                      const double *__restrict__ rr,
+                                                      const double *__restrict__ rr_new) {
+  const double den = rr[0];
+  const float beta = den > 0.0 ? (float)(rr_new[0] / den) : 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    p[i] = fmaf(beta, p[i], r[i]);
+}
+
 }  // namespace
+
+extern "C" int tomo_cg_step(float *x, float *r, const float *p, const float *q, int64_t n,
+                            const double *rr, const double *pq, double *out, void *stream) {
+  tomo::clear_error();
+  TOMO_REQUIRE(x && r && p && q && rr && pq && out, "tomo_cg_step: NULL array");
+  TOMO_REQUIRE(n >= 0, "tomo_cg_step: n < 0");
+  TOMO_REQUIRE(out != rr && out != pq, "tomo_cg_step: out aliases rr or pq");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cg_step, dim3(kDotBlocks), dim3(256), 0, st, x, r, p, q, n, rr, pq,
+                     out + 1);
+  TOMO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, out);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
+
+extern "C" int tomo_cg_direction(float *p, const float *r, int64_t n, const double *rr,
+                                 const double *rr_new, void *stream) {
+  tomo::clear_error();
+  TOMO_REQUIRE(p && r && rr && rr_new, "tomo_cg_direction: NULL array");
+  TOMO_REQUIRE(n >= 0, "tomo_cg_direction: n < 0");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cg_direction, dim3(kDotBlocks), dim3(256), 0, st, p, r, n, rr, rr_new);
+  TOMO_LAUNCH_CHECK();
+  return TOMO_OK;
+}
 
 extern "C" int tomo_dot_f64(const float *a, const float *b, int64_t n, double *out,
                             void *stream) {
