@@ -756,6 +756,8 @@ int fw_run(tomo_pre_plan *P, const WaveletTaps *wv, double sigma, int level, con
     TOMO_LAUNCH_CHECK();
   }
   {
+    TOMO_REQUIRE(level <= kXsumMaxLevels, "pre_remove_stripe_fw: more than %d levels",
+                 kXsumMaxLevels);
     XsumArgs xa{};
     xa.nz = nz;
     int blk = 0;
