@@ -36,6 +36,9 @@ namespace {
 constexpr int kGridWaves = 4;   // waves (= work items) per gridding workgroup
 constexpr int kGridChunk = 512; // default max list entries per work item (multiple of 16)
 constexpr int kGridBatch = 64;  // list entries a gridding wave stages at a time
+// device memory for the interpolation's tile partials (fr_forward): as many
+// groups of 4 complex slices per launch as fit
+constexpr int64_t kInterpBytes = (int64_t)512 << 20;
 // The four 16-lane groups of a gridding wave read staged entries 8 apart in
 // the same instruction; one pad int4 (entries) or two pad float2 (spectra)
 // after every 8 put the four addresses on different LDS banks.
@@ -1005,13 +1008,14 @@ __global__ __launch_bounds__(512) void k_fr_rowfft_adj(const float2 *__restrict_
 // so the result is deterministic.
 __global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
     const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
-    const float2 *__restrict__ F, float4 *__restrict__ ipart, int n, int ntile, int nzc,
-    float K2) {
+    const float2 *__restrict__ F, float4 *__restrict__ ipart, int64_t nent, int n, int ntile,
+    int nzc, float K2) {
   __shared__ float4 ft[kGridWaves][16 * 16 * 2];  // [row][col][slice pair]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int it = blockIdx.x * kGridWaves + wave;
-  const int grp = blockIdx.y;
+  const int grp = blockIdx.y;  // group of 4 complex slices; its own slab of ipart
   if (it >= nitems) return;
+  ipart += (size_t)grp * 2 * nent;
   const int4 item = items[it];
   const int tile = item.x, ebeg = item.y, cnt = item.z;
   const int n2 = 2 * n;
@@ -1079,10 +1083,12 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
 // list order).
 __global__ __launch_bounds__(256) void k_fr_interp_sum(const int4 *__restrict__ ient,
                                                        const float4 *__restrict__ ipart,
-                                                       float2 *__restrict__ G, int nsamp,
-                                                       int grp, int nzc) {
+                                                       int64_t nent, float2 *__restrict__ G,
+                                                       int nsamp, int nzc) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int grp = blockIdx.y;
   if (s >= nsamp) return;
+  ipart += (size_t)grp * 2 * nent;
   const int4 q = ient[s];
   const int idx[4] = {q.x, q.y, q.z, q.w};
   float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1929,8 +1935,13 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
     set_error("fr_forward: the forward projection is implemented for float32 plans only");
     return TOMO_EUNSUPPORTED;
   }
-  if (!P->ipart.p) {  // tile partials of the interpolation, allocated on the first call
-    const int rc0 = P->ipart.alloc(2 * sizeof(float4) * (size_t)std::max<int64_t>(1, P->nent));
+  // tile partials of the interpolation, allocated on the first call: one slab
+  // per group of 4 complex slices, for up to kInterpGroups groups per launch
+  const int64_t nentp = std::max<int64_t>(1, P->nent);
+  const int gb = (int)std::max<int64_t>(
+      1, std::min<int64_t>(P->ngrp, kInterpBytes / (2 * (int64_t)sizeof(float4) * nentp)));
+  if (!P->ipart.p) {
+    const int rc0 = P->ipart.alloc(2 * sizeof(float4) * (size_t)nentp * gb);
     if (rc0) return rc0;
   }
   float *data = (float *)data_;
@@ -1961,20 +1972,22 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
     });
   }
   TOMO_LAUNCH_CHECK();
-  for (int g = 0; g < ng; ++g) {
+  for (int g = 0; g < ng; g += gb) {
+    const int ngl = std::min(gb, ng - g);  // groups in this launch
     {
       ProfScope ps_("k_fr_interp_tiles", st);
-      hipLaunchKernelGGL(k_fr_interp_tiles, dim3((unsigned)ceil_div(P->nitems, kGridWaves), 1, 1),
+      hipLaunchKernelGGL(k_fr_interp_tiles,
+                         dim3((unsigned)ceil_div(P->nitems, kGridWaves), (unsigned)ngl, 1),
                          dim3(64 * kGridWaves), 0, st, P->items.as<int4>(), P->nitems,
                          P->ent.as<int4>(), P->F.as<float2>() + (size_t)g * 4 * n2 * n2,
-                         P->ipart.as<float4>(), n, P->ntile, nzc - 4 * g, P->K2);
+                         P->ipart.as<float4>(), nentp, n, P->ntile, nzc - 4 * g, P->K2);
     }
     TOMO_LAUNCH_CHECK();
     {
       ProfScope ps_("k_fr_interp_sum", st);
-      hipLaunchKernelGGL(k_fr_interp_sum, dim3((unsigned)ceil_div(P->nsamp, 256)), dim3(256), 0,
-                         st, P->ient.as<int4>(), P->ipart.as<float4>(),
-                         P->G.as<float2>() + (size_t)g * 4 * P->nsamp, (int)P->nsamp, 0,
+      hipLaunchKernelGGL(k_fr_interp_sum, dim3((unsigned)ceil_div(P->nsamp, 256), (unsigned)ngl),
+                         dim3(256), 0, st, P->ient.as<int4>(), P->ipart.as<float4>(), nentp,
+                         P->G.as<float2>() + (size_t)g * 4 * P->nsamp, (int)P->nsamp,
                          nzc - 4 * g);
     }
     TOMO_LAUNCH_CHECK();
