@@ -1106,26 +1106,40 @@ __global__ __launch_bounds__(256) void k_fr_interp_sum(const int4 *__restrict__ 
     if (grp * 4 + c < nzc) G[(size_t)(grp * 4 + c) * nsamp + s] = v[c];
 }
 
-template <int LV>
-__global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ G, float *__restrict__ data, FftDesc fd,
-                              int n, int nproj, int nz, int nzc, float scale) {
+// Adjoint of k_fr_prep: the shifted inverse n-point DFT of each spectrum row
+// back to the two real slices of the pair.  NS as k_fr_prep: compiled sizes
+// read the spectrum straight into the first pass and store from the last.
+template <int NS>
+__global__ __launch_bounds__(512) void k_fr_prep_adj(const float2 *__restrict__ G,
+                                                     float *__restrict__ data, FftDesc fd, int n,
+                                                     int nproj, int nz, int nzc, float scale) {
   extern __shared__ float2 sm[];
   const int p = blockIdx.x, c = blockIdx.y;
   if (c >= nzc) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const float2 *in = G + ((size_t)c * nproj + p) * n;
-  stage_lds<8, float2>(
-      n, tid, nt, [&](int k) { return in[k]; },
-      [&](int k, float2 v) { sm[lds_pad(k)] = cscale(v, shift_sign(k) * scale); });
-  __syncthreads();
-  lds_fft<1, kFftEPT, LV>(sm, fd, 0, tid, nt);
   const int z2 = c + nzc;
   float *r1 = data + ((size_t)c * nproj + p) * n;
   float *r2 = data + ((size_t)z2 * nproj + p) * n;
-  for (int j = tid; j < n; j += nt) {
-    const float2 x = cscale(sm[lds_pad(j)], shift_sign(j));
-    r1[j] = x.x;
-    if (z2 < nz) r2[j] = x.y;
+  if constexpr (NS > 0) {
+    fft_c_global<NS, 1, kFftEPT, 0>(
+        sm, fd.tw, tid, nt, [&](int k, int) { return cscale(in[k], shift_sign(k) * scale); },
+        [&](int j, int, float2 x) {
+          x = cscale(x, shift_sign(j));
+          r1[j] = x.x;
+          if (z2 < nz) r2[j] = x.y;
+        });
+  } else {
+    stage_lds<8, float2>(
+        n, tid, nt, [&](int k) { return in[k]; },
+        [&](int k, float2 v) { sm[lds_pad(k)] = cscale(v, shift_sign(k) * scale); });
+    __syncthreads();
+    lds_fft<1, kFftEPT, fft_lv(NS)>(sm, fd, 0, tid, nt);
+    for (int j = tid; j < n; j += nt) {
+      const float2 x = cscale(sm[lds_pad(j)], shift_sign(j));
+      r1[j] = x.x;
+      if (z2 < nz) r2[j] = x.y;
+    }
   }
 }
 
@@ -1950,7 +1964,6 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
   const int n = (int)P->n, nproj = (int)P->nproj, nz = (int)P->nz, nzc = (int)P->nzc;
   const int ng = (int)P->ngrp;
   const int n2 = 2 * n;
-  const int level = fft_level(n);
   {
     ProfScope ps_("k_fr_colfft_adj", st);
     with_static_size(P->nsCol, [&](auto ns) {
@@ -1994,11 +2007,10 @@ extern "C" int fr_forward(tomo_fr_plan *P, void *data_, const void *obj_, void *
   }
   {
     ProfScope ps_("k_fr_prep_adj", st);
-    with_fft_level(level, [&](auto lv) {
-      constexpr int LV = decltype(lv)::value;
-      hipLaunchKernelGGL(k_fr_prep_adj<LV>, dim3(nproj, ng * 4), dim3(P->prepThreads),
-                         P->prepLds, st, P->G.as<float2>(), data, P->fft1.d, n, nproj, nz, nzc,
-                         P->scale1);
+    with_static_size(P->nsPrep, [&](auto ns) {
+      hipLaunchKernelGGL(k_fr_prep_adj<decltype(ns)::value>, dim3(nproj, ng * 4),
+                         dim3(P->prepThreads), P->prepLds, st, P->G.as<float2>(), data,
+                         P->fft1.d, n, nproj, nz, nzc, P->scale1);
     });
   }
   TOMO_LAUNCH_CHECK();
