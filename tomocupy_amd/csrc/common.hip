@@ -226,9 +226,22 @@ __global__ __launch_bounds__(256) void k_cg_direction(float *__restrict__ p,
                                                       const double *__restrict__ rr_new) {
   const double den = rr[0];
   const float beta = den > 0.0 ? (float)(rr_new[0] / den) : 0.f;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * 256)
-    p[i] = fmaf(beta, p[i], r[i]);
+  const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x, stride = (int64_t)gridDim.x * 256;
+  // 16-byte accesses for the aligned bulk, then the tail element by element
+  const bool al = (((uintptr_t)p | (uintptr_t)r) & 15) == 0;
+  const int64_t n4 = al ? n / 4 : 0;
+  float4 *p4 = reinterpret_cast<float4 *>(p);
+  const float4 *r4 = reinterpret_cast<const float4 *>(r);
+  for (int64_t i = gid; i < n4; i += stride) {
+    float4 a = p4[i];
+    const float4 b = r4[i];
+    a.x = fmaf(beta, a.x, b.x);
+    a.y = fmaf(beta, a.y, b.y);
+    a.z = fmaf(beta, a.z, b.z);
+    a.w = fmaf(beta, a.w, b.w);
+    p4[i] = a;
+  }
+  for (int64_t i = 4 * n4 + gid; i < n; i += stride) p[i] = fmaf(beta, p[i], r[i]);
 }
 
 }  // namespace
