@@ -39,6 +39,7 @@ constexpr int kGridBatch = 64;  // list entries a gridding wave stages at a time
 // device memory for the interpolation's tile partials (fr_forward): as many
 // groups of 4 complex slices per launch as fit
 constexpr int64_t kInterpBytes = (int64_t)512 << 20;
+constexpr int kInterpPitch = 17;  // LDS row pitch (float4) of the interpolation tile
 // The four 16-lane groups of a gridding wave read staged entries 8 apart in
 // the same instruction; one pad int4 (entries) or two pad float2 (spectra)
 // after every 8 put the four addresses on different LDS banks.
@@ -1010,7 +1011,12 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
     const int4 *__restrict__ items, int nitems, const int4 *__restrict__ ent,
     const float2 *__restrict__ F, float4 *__restrict__ ipart, int64_t nent, int n, int ntile,
     int nzc, float K2) {
-  __shared__ float4 ft[kGridWaves][16 * 16 * 2];  // [row][col][slice pair]
+  // per wave: the tile's 256 cells in two planes (complex slices 0-1, 2-3),
+  // cell (r, c) at r * kInterpPitch + c.  The lanes read unrelated cells; in
+  // one plane, with the pad after each row, the cells of one column fall on
+  // different 16-byte bank slots (with the two slice pairs of a cell side by
+  // side, a read used only half of the slots: 3x the bank-conflict cycles)
+  __shared__ float4 ft[kGridWaves][2][16 * kInterpPitch];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int it = blockIdx.x * kGridWaves + wave;
   const int grp = blockIdx.y;  // group of 4 complex slices; its own slab of ipart
@@ -1021,7 +1027,8 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
   const int n2 = 2 * n;
   const int TA = tile / ntile, TB = tile - TA * ntile;
   const size_t plane = (size_t)n2 * n2;
-  float4 *T = ft[wave];
+  float4 *T0 = ft[wave][0], *T1 = ft[wave][1];
+  auto at = [](int r, int c) { return r * kInterpPitch + c; };
   // stage the tile: 256 cells x 4 slices, 16 values per lane
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -1035,8 +1042,8 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
       if (grp * 4 + c < nzc && a < n2 && bcol < n2)
         v[c] = F[(size_t)(grp * 4 + c) * plane + (size_t)a * n2 + bcol];
     }
-    T[2 * cell] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
-    T[2 * cell + 1] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+    T0[at(r, cc)] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+    T1[at(r, cc)] = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
   }
   __builtin_amdgcn_wave_barrier();
   const int eend = ebeg + cnt;
@@ -1063,7 +1070,8 @@ __global__ __launch_bounds__(64 * kGridWaves) void k_fr_interp_tiles(
       for (int i0 = 0; i0 < 9; ++i0) {
         const int cc = min(max(cs + i0, 0), 15);
         const float w = ey[i1] * ex[i0];
-        const float4 x = T[2 * (r * 16 + cc)], y = T[2 * (r * 16 + cc) + 1];
+        const int o = at(r, cc);
+        const float4 x = T0[o], y = T1[o];
         acc0.x = fmaf(w, x.x, acc0.x);
         acc0.y = fmaf(w, x.y, acc0.y);
         acc0.z = fmaf(w, x.z, acc0.z);
