@@ -52,6 +52,7 @@ def parse():
     ap.add_argument('--streams', type=int, default=3)
     ap.add_argument('--dtype', default='float32', choices=['float32', 'float16', 'bfloat16'])
     ap.add_argument('--stripe', default='fw', choices=['fw', 'none', 'ti', 'vo-all'])
+    ap.add_argument('--fw-level', type=int, default=7, help='Fourier-wavelet levels (sym16, sigma 1)')
     ap.add_argument('--vo-all-dim', type=int, default=1, choices=[1, 2])
     ap.add_argument('--phase-batch', type=int, default=1,
                     help='steps form: projections per phase-retrieval launch')
@@ -88,11 +89,11 @@ def parse():
     return ap.parse_args()
 
 
-def fw_levels(a, L=32, level=7):
+def fw_levels(a, L=32):
     """Band shapes (H, W, Ho, Wo, Hs, Ws) of the sym16 Fourier-wavelet levels."""
     H, W = a.nproj + a.nproj // 8, a.n
     out = []
-    for _ in range(level):
+    for _ in range(a.fw_level):
         Ho, Wo = (H + L - 1) // 2, (W + L - 1) // 2
         out.append((H, W, Ho, Wo, 2 * Ho - L + 2, 2 * Wo - L + 2))
         H, W = Ho, Wo
@@ -277,7 +278,7 @@ def _cpu_pair(arg):
     from oracle import filter_fp64 as ff
     from oracle import fourierrec_fp64 as fr
     from oracle import preproc_fp64 as pp
-    sino = pp.proc_chain(data, dark, flat, stripe=a.stripe)
+    sino = pp.proc_chain(data, dark, flat, stripe=a.stripe, fw_level=a.fw_level)
     n = data.shape[-1]
     ne = 4 * n
     filt = ff.fbp_filter_center(sino, ff.calc_filter(ne, a.filter), ne, center)
@@ -530,7 +531,7 @@ class Pipeline:
             with torch.cuda.stream(st):
                 self.pfs.append(ProcFunctions(a.nproj, a.chunk, a.n, ndark, nflat, dtype,
                                               remove_stripe_method=a.stripe, fw_sigma=1.0,
-                                              fw_filter='sym16', fw_level=7,
+                                              fw_filter='sym16', fw_level=a.fw_level,
                                               vo_all_dim=a.vo_all_dim))
                 self.bfs.append(BackprojFunctions(a.n, a.nproj, a.chunk, th, center, dtype,
                                                   a.filter, rec_options={'fft': a.fft}))
@@ -576,7 +577,8 @@ def conveyor(a, torch, theta, h_data, h_dark, h_flat, center, dtype):
     from tomocupy_amd.rec import GPURec
     g = GPURec(a.n, a.nproj, a.nz, a.chunk, torch.from_numpy(theta).cuda(), center,
                h_dark.shape[0], h_flat.shape[0], dtype, fbp_filter=a.filter,
-               remove_stripe_method=a.stripe, fw_sigma=1.0, fw_filter='sym16', fw_level=7)
+               remove_stripe_method=a.stripe, fw_sigma=1.0, fw_filter='sym16',
+               fw_level=a.fw_level)
 
     def read_chunk(k):
         return h_data, h_dark, h_flat
@@ -756,7 +758,7 @@ def main():
                                   center),
                    'n': a.n, 'nproj': a.nproj, 'nz': a.nz, 'chunk': a.chunk,
                    'streams': a.streams, 'stripe': a.stripe, 'filter': a.filter,
-                   'fft': a.fft,
+                   'fw_level': a.fw_level, 'fft': a.fft,
                    'timed_with_kernel_events': 'dominant kernel only' if dom else False},
         'roofline': roof, 'cpu_baseline': cpu, 'accuracy': acc, 'half': half,
         'conveyor': conv, 'roofline_overlapped_events': roof_ov,
